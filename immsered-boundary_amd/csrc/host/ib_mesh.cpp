@@ -1,0 +1,831 @@
+// ib_mesh.cpp — host AMR + immersed-boundary mesh builder.
+//
+// Produces the flattened level set of ibgpu.h (ibg_mesh_soa) for one or more
+// airfoil cases.  The rules are those of the reference's initial refinement
+// and IB setup; each step cites the reference code whose behaviour it
+// restates.  The reference builds the same sets incrementally across MPI
+// ranks (AMR.cpp:6-209); here every case is built at once on the host, and
+// the raw index order is this builder's own (canonical keys are what match).
+#include "ib_mesh.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+std::string g_last_error;
+
+enum { NORMAL = IBG_NORMAL, DMGHOST = IBG_DMGHOST, RES1 = IBG_RES1,
+       RES2 = IBG_RES2, PRO = IBG_PRO };
+
+inline int64_t pack_key(int ix, int iy, int iz) {
+  // 21 bits per coordinate, offset so that ghosts (>= -4) stay positive
+  return ((int64_t)(ix + 8) << 42) | ((int64_t)(iy + 8) << 21) | (int64_t)iz;
+}
+
+inline uint64_t spread2(uint32_t v) {
+  uint64_t x = v & 0xffffffffu;
+  x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+  x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+  x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+  x = (x | (x << 2)) & 0x3333333333333333ull;
+  x = (x | (x << 1)) & 0x5555555555555555ull;
+  return x;
+}
+inline uint64_t spread3(uint32_t v) {
+  uint64_t x = v & 0x1fffff;
+  x = (x | x << 32) & 0x1f00000000ffffull;
+  x = (x | x << 16) & 0x1f0000ff0000ffull;
+  x = (x | x << 8) & 0x100f00f00f00f00full;
+  x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+  x = (x | x << 2) & 0x1249249249249249ull;
+  return x;
+}
+inline uint64_t morton(int dim, int ix, int iy, int iz) {
+  uint32_t x = (uint32_t)(ix + 8), y = (uint32_t)(iy + 8), z = (uint32_t)iz;
+  if (dim == 2) return spread2(x) | (spread2(y) << 1);
+  return spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
+}
+
+struct LCell {
+  int ix, iy, iz;
+  int8_t kind;
+};
+
+struct Level {
+  std::vector<LCell> cells;
+  std::unordered_map<int64_t, int> map;
+  int nzl = 1;          // z cells on this level (periodic)
+  double h[3];
+  bool twod = true;     // refined from the coarser level in x,y only
+  void rebuild_map() {
+    map.clear();
+    map.reserve(cells.size() * 2);
+    for (int i = 0; i < (int)cells.size(); ++i)
+      map[pack_key(cells[i].ix, cells[i].iy, cells[i].iz)] = i;
+  }
+  int find(int ix, int iy, int iz) const {
+    int zz = ((iz % nzl) + nzl) % nzl;   // z is periodic (Userfunction_cy_naca15.cpp:176)
+    auto it = map.find(pack_key(ix, iy, zz));
+    return it == map.end() ? -1 : it->second;
+  }
+};
+
+struct Patch {
+  double pc[3], nv[3], area;
+  int body;
+};
+
+// CtoFPair with level-local indices, resolved to global after sorting
+struct LPair {
+  int ci;            // level-local coarse cell (level l)
+  int fi[8];         // level-local sons (level l+1), -1 where absent
+  int layer;         // 0: f_res_ghost[0], 1: f_res_ghost[1], 2: f_pro_ghost
+};
+
+struct CaseMesh {
+  ibm_case cs;
+  std::vector<Level> lv;
+  std::vector<std::vector<LPair>> pairs;   // per coarse level
+  std::vector<Patch> patches;
+  std::vector<std::vector<double>> segxy;  // per body: points
+  std::vector<int> seg_body, seg_first;    // segment -> body, point index
+  std::vector<double> seg_cx, seg_cy, seg_nx, seg_ny, seg_len;
+};
+
+void build_segments(CaseMesh &m) {
+  const ibm_case &c = m.cs;
+  int off = 0;
+  for (int b = 0; b < c.nbody; ++b) {
+    int n = c.body_npts[b];
+    for (int i = 0; i + 1 < n; ++i) {
+      double x0 = c.body_xy[2 * (off + i)], y0 = c.body_xy[2 * (off + i) + 1];
+      double x1 = c.body_xy[2 * (off + i + 1)], y1 = c.body_xy[2 * (off + i + 1) + 1];
+      double dx = x1 - x0, dy = y1 - y0;
+      double len = std::sqrt(dx * dx + dy * dy);
+      if (len <= 0) continue;
+      // Outward normal: the upper surface (even body index) is given LE->TE
+      // with the fluid on its left; the lower surface (odd) on its right.
+      double nx = -dy / len, ny = dx / len;
+      if (b % 2 == 1) { nx = -nx; ny = -ny; }
+      m.seg_cx.push_back(0.5 * (x0 + x1));
+      m.seg_cy.push_back(0.5 * (y0 + y1));
+      m.seg_nx.push_back(nx);
+      m.seg_ny.push_back(ny);
+      m.seg_len.push_back(len);
+      m.seg_body.push_back(b);
+    }
+    off += n;
+  }
+}
+
+// nearest segment center to (x,y).  A section has a few hundred segments, so
+// a straight scan (vectorised, threads over cells) beats any search structure.
+int nearest_segment(const CaseMesh &m, double x, double y) {
+  const int ns = (int)m.seg_cx.size();
+  int best = -1;
+  double bd = 1e300;
+  const double *cx = m.seg_cx.data(), *cy = m.seg_cy.data();
+  for (int s = 0; s < ns; ++s) {
+    double dx = x - cx[s], dy = y - cy[s];
+    double d = dx * dx + dy * dy;
+    if (d < bd) { bd = d; best = s; }
+  }
+  return best;
+}
+
+void cell_center(const CaseMesh &m, const Level &L, const LCell &c, double *xyz) {
+  // ComptCellParams MeshOp.H:4-9: boxcenter = (i+0.5)*dh
+  xyz[0] = m.cs.origin[0] + ((double)c.ix + 0.5) * L.h[0];
+  xyz[1] = m.cs.origin[1] + ((double)c.iy + 0.5) * L.h[1];
+  xyz[2] = m.cs.origin[2] + ((double)c.iz + 0.5) * L.h[2];
+}
+
+// Closest patch and signed distance: BoxtoWall of the closest patch center,
+// signdis = nv . (boxcenter - pc) (Body.cpp:791-821).
+void closest_patch(const CaseMesh &m, const double *xyz, int *patch, double *signdis) {
+  int s = nearest_segment(m, xyz[0], xyz[1]);
+  if (s < 0) { *patch = -1; *signdis = 9999.0; return; }
+  int sp = std::max(m.cs.span_patches, 1);
+  double dz = m.cs.span / sp;
+  int zl = (int)std::floor((xyz[2] - m.cs.origin[2]) / dz);
+  zl = std::min(std::max(zl, 0), sp - 1);
+  int p = s * sp + zl;
+  const Patch &P = m.patches[p];
+  *patch = p;
+  *signdis = P.nv[0] * (xyz[0] - P.pc[0]) + P.nv[1] * (xyz[1] - P.pc[1]) +
+             P.nv[2] * (xyz[2] - P.pc[2]);
+}
+
+void build_patches(CaseMesh &m) {
+  int sp = std::max(m.cs.span_patches, 1);
+  double dz = m.cs.span / sp;
+  for (size_t s = 0; s < m.seg_cx.size(); ++s) {
+    for (int k = 0; k < sp; ++k) {
+      Patch P;
+      P.pc[0] = m.seg_cx[s]; P.pc[1] = m.seg_cy[s];
+      P.pc[2] = m.cs.origin[2] + (k + 0.5) * dz;
+      P.nv[0] = m.seg_nx[s]; P.nv[1] = m.seg_ny[s]; P.nv[2] = 0.0;
+      P.area = m.seg_len[s] * dz;
+      P.body = m.seg_body[s];
+      m.patches.push_back(P);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- levels
+void build_levels(CaseMesh &m) {
+  const ibm_case &c = m.cs;
+  const int L = c.nlevels;
+  const bool d3 = (c.dim == 3);
+  m.lv.resize(L);
+  m.pairs.resize(L);
+  for (int l = 0; l < L; ++l) {
+    Level &lv = m.lv[l];
+    lv.twod = (l == 0) ? false : (!d3 || l >= c.twod_from);
+    double rz = 1.0;
+    if (l > 0) { rz = m.lv[l - 1].h[2] / (lv.twod ? 1.0 : 2.0); }
+    // dh[i] (mpiparams.cpp:134-160): x,y halve per level, z by power ratio
+    lv.h[0] = c.dh0[0] / std::pow(2.0, l);
+    lv.h[1] = c.dh0[1] / std::pow(2.0, l);
+    lv.h[2] = (l == 0) ? c.dh0[2] : rz;
+    lv.nzl = (l == 0) ? c.nz : m.lv[l - 1].nzl * (lv.twod ? 1 : 2);
+  }
+  // Level 0: background grid plus two layers (ighost, Constants.H:7) of
+  // domain ghosts around x and y; z is periodic (no z ghosts).
+  Level &l0 = m.lv[0];
+  for (int k = 0; k < c.nz; ++k)
+    for (int j = -2; j < c.ny + 2; ++j)
+      for (int i = -2; i < c.nx + 2; ++i) {
+        bool in = (i >= 0 && i < c.nx && j >= 0 && j < c.ny);
+        l0.cells.push_back({i, j, k, (int8_t)(in ? NORMAL : DMGHOST)});
+      }
+  l0.rebuild_map();
+
+  for (int l = 0; l + 1 < L; ++l) {
+    Level &lc = m.lv[l];
+    Level &lf = m.lv[l + 1];
+    const int n = (int)lc.cells.size();
+    std::vector<double> sd(n);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+      double xyz[3]; int p;
+      cell_center(m, lc, lc.cells[i], xyz);
+      closest_patch(m, xyz, &p, &sd[i]);
+    }
+    // tag = 0: band rule of TagtheBox (Userfunction_cy_naca15.cpp:589-600)
+    std::vector<int8_t> tag(n, -1);
+    for (int i = 0; i < n; ++i) {
+      if (lc.cells[i].kind != NORMAL) continue;
+      if (sd[i] > c.band_lo[l] && sd[i] < c.band_hi[l]) tag[i] = 0;
+    }
+    // Proper nesting: a tagged cell keeps its tag only when every cell within
+    // two cells of it (x,y) is a normal cell of this level, so that the new
+    // prolongation ring and its gradient stencil lie on normal cells.
+    std::vector<int8_t> keep(n, 0);
+    for (int i = 0; i < n; ++i) {
+      if (tag[i] != 0) continue;
+      const LCell &cc = lc.cells[i];
+      bool ok = true;
+      for (int dj = -2; dj <= 2 && ok; ++dj)
+        for (int di = -2; di <= 2 && ok; ++di) {
+          int a = lc.find(cc.ix + di, cc.iy + dj, cc.iz);
+          if (a < 0 || lc.cells[a].kind != NORMAL) ok = false;
+        }
+      keep[i] = ok;
+    }
+    for (int i = 0; i < n; ++i) if (tag[i] == 0 && !keep[i]) tag[i] = -1;
+    // TagRefineGhost AMRGhost_refine.cpp:30-98: tag 1 for tagged cells with
+    // an untagged 26-neighbour (which become prolongation cells, tag -2),
+    // tag 2 for tagged cells with an untagged cell two cells away on an axis.
+    const int zr = d3 ? 1 : 0;
+    for (int i = 0; i < n; ++i) {
+      if (tag[i] < 0 || tag[i] == -2) continue;
+      if (tag[i] != 0 && tag[i] != 1) continue;
+      const LCell &cc = lc.cells[i];
+      for (int dk = -zr; dk <= zr; ++dk)
+        for (int dj = -1; dj <= 1; ++dj)
+          for (int di = -1; di <= 1; ++di) {
+            int a = lc.find(cc.ix + di, cc.iy + dj, cc.iz + dk);
+            if (a < 0) continue;
+            if (tag[a] == -1 || tag[a] == -2) {
+              tag[a] = -2;
+              if (lc.cells[a].kind != DMGHOST) tag[i] = 1;
+            }
+          }
+    }
+    for (int i = 0; i < n; ++i) {
+      if (tag[i] != 0) continue;
+      const LCell &cc = lc.cells[i];
+      const int dd[6][3] = {{-1,0,0},{1,0,0},{0,-1,0},{0,1,0},{0,0,-1},{0,0,1}};
+      for (int q = 0; q < (d3 ? 6 : 4); ++q) {
+        int a = lc.find(cc.ix + dd[q][0], cc.iy + dd[q][1], cc.iz + dd[q][2]);
+        if (a < 0) continue;
+        int a2 = lc.find(cc.ix + 2 * dd[q][0], cc.iy + 2 * dd[q][1], cc.iz + 2 * dd[q][2]);
+        if (a2 < 0) continue;
+        if ((tag[a2] == -1 || tag[a2] == -2) && lc.cells[a2].kind != DMGHOST) tag[i] = 2;
+      }
+    }
+    // the prolongation ring must be made of normal cells
+    for (int i = 0; i < n; ++i)
+      if (tag[i] == -2 && lc.cells[i].kind != NORMAL)
+        throw std::runtime_error("refinement ring reaches a ghost cell on level " + std::to_string(l));
+    // MakeSmallBox AMRLevel.cpp:413: sons of tagged and ring cells
+    const int kz = lf.twod ? 1 : 2;
+    const int zf = lf.twod ? 1 : 2;
+    std::vector<LPair> &pl = m.pairs[l];
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int i = 0; i < n; ++i) {
+        bool tagged = (tag[i] >= 0);
+        bool ring = (tag[i] == -2);
+        if ((pass == 0 && !tagged) || (pass == 1 && !ring)) continue;
+        const LCell &cc = lc.cells[i];
+        LPair pr;
+        pr.ci = i;
+        for (int q = 0; q < 8; ++q) pr.fi[q] = -1;
+        for (int si = 0; si < 2; ++si)
+          for (int sj = 0; sj < 2; ++sj)
+            for (int sk = 0; sk < kz; ++sk) {
+              int fx = 2 * cc.ix + si, fy = 2 * cc.iy + sj, fz = cc.iz * zf + sk;
+              pr.fi[si * 4 + sj * 2 + sk] = (int)lf.cells.size();
+              lf.cells.push_back({fx, fy, fz, (int8_t)(pass == 0 ? NORMAL : PRO)});
+            }
+        if (pass == 1) { pr.layer = 2; pl.push_back(pr); }
+        else if (tag[i] == 1) { pr.layer = 0; pl.push_back(pr); }
+        else if (tag[i] == 2) { pr.layer = 1; pl.push_back(pr); }
+      }
+    }
+    lf.rebuild_map();
+    // coarse cells: tag 1 / 2 become restriction ghosts, tag 0 are removed
+    // (RemoveGhostandNormal_Refine AMRGhostcells.cpp:265-330)
+    std::vector<int> newidx(n, -1);
+    std::vector<LCell> kept;
+    kept.reserve(n);
+    for (int i = 0; i < n; ++i) {
+      if (tag[i] == 0) continue;
+      LCell cc = lc.cells[i];
+      if (tag[i] == 1) cc.kind = RES1;
+      if (tag[i] == 2) cc.kind = RES2;
+      newidx[i] = (int)kept.size();
+      kept.push_back(cc);
+    }
+    lc.cells.swap(kept);
+    lc.rebuild_map();
+    for (auto &pr : pl) pr.ci = newidx[pr.ci];
+    // pairs of the previous coarser level reference this level's sons; a son
+    // removed here would break a restriction pair
+    if (l > 0) {
+      for (auto &pr : m.pairs[l - 1])
+        for (int q = 0; q < 8; ++q)
+          if (pr.fi[q] >= 0) {
+            int nidx = newidx[pr.fi[q]];
+            if (nidx < 0) throw std::runtime_error("a restriction son was removed by refinement");
+            pr.fi[q] = nidx;
+          }
+    }
+  }
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- handle
+struct ibm_handle {
+  ibg_mesh_soa soa;
+  int ncases = 0, L = 0, dim = 2;
+  std::vector<int32_t> blk_start, blk_nnormal, blk_twod;
+  std::vector<double> blk_h;
+  std::vector<int32_t> key, nb, cell_faces, ibidx;
+  std::vector<int8_t> kind;
+  std::vector<double> signdis, center;
+  std::vector<int32_t> blk_face_start, face_cells;
+  std::vector<int8_t> face_dir;
+  std::vector<int32_t> dmg_cell, dmg_ref, dmg_type;
+  std::vector<int32_t> res_ci, res_fi, res_blk, pro_ci, pro_fi, pro_blk;
+  std::vector<int32_t> ib_cell, ib_intp, ib_body;
+  std::vector<double> ib_coef, ib_hgdis, ib_nv, ib_attach;
+  std::vector<int32_t> pat_intp, pat_case;
+  std::vector<double> pat_coef, pat_hgdis, pat_nv, pat_pc, pat_area;
+  std::vector<int32_t> body_case;
+  std::vector<double> body_center;
+  std::vector<ibm_case> cases;
+};
+
+namespace {
+
+// Image-point search (Body.cpp:1044-1183 for IB cells, Body.cpp:1527-1575 for
+// patches): march from `start` along nv in steps of one cell in the dominant
+// direction until the 2x2(x2) bracketing cells are neither solid nor IB cells
+// (WALLINTPFLAG_NOTINFECTED, Body.H:17), then the multilinear weights of
+// Mesh::ComptHgIntpCoef (Mesh.cpp:1403-1442).
+
+struct Finder {
+  const Level *lv;
+  int gbase;                       // global index of level-local cell 0
+  const std::vector<int> *perm;    // level-local -> position in sorted block
+  int find(int ix, int iy, int iz) const {
+    int a = lv->find(ix, iy, iz);
+    return a < 0 ? -1 : gbase + (*perm)[a];
+  }
+};
+
+bool search_image(const Finder &F, const ibm_handle &H, const double *h,
+                  const double *origin, const double *start, const double *nv,
+                  const std::vector<int8_t> &solid_g, const std::vector<int32_t> &ibidx_g,
+                  int32_t *intp, double *coef, double *hgdis_out, int *closecell) {
+  double dk[3];
+  for (int d = 0; d < 3; ++d) dk[d] = nv[d] / h[d];
+  double dmax = std::max(std::fabs(dk[0]), std::fabs(dk[1]));
+  dmax = std::max(dmax, std::fabs(dk[2]));
+  double dxyz[3] = {nv[0] / dmax, nv[1] / dmax, nv[2] / dmax};
+  double steplen = std::sqrt(dxyz[0] * dxyz[0] + dxyz[1] * dxyz[1] + dxyz[2] * dxyz[2]);
+  const double ca[3] = {0.0, 1.0, 0.0};
+  const double cb[3] = {1.0, -1.0, 1.0};
+  double hgdis = 0.0;
+  for (int it = 0; it < 64; ++it) {
+    hgdis += steplen;
+    double pt[3];
+    for (int d = 0; d < 3; ++d) pt[d] = start[d] + nv[d] * hgdis;
+    int ik[3];
+    for (int d = 0; d < 3; ++d) ik[d] = (int)std::floor((pt[d] - origin[d]) / h[d]);
+    int cc = F.find(ik[0], ik[1], ik[2]);
+    if (cc < 0) return false;
+    if (H.kind[cc] == IBG_DMGHOST) return false;
+    const double *cen = &H.center[3 * (size_t)cc];
+    double dkt[3];
+    for (int d = 0; d < 3; ++d) dkt[d] = (pt[d] - cen[d]) / h[d];
+    if (H.dim == 2) dkt[2] = 0.0;   // one z layer: the image point is in it
+    int hgdir[3];
+    for (int d = 0; d < 3; ++d) hgdir[d] = dkt[d] > 0.0 ? 1 : 0;
+    bool ok = true;
+    for (int pi = 0; pi < 2; ++pi)
+      for (int pj = 0; pj < 2; ++pj)
+        for (int pk = 0; pk < 2; ++pk) {
+          int nx = pi + hgdir[0], ny = pj + hgdir[1], nz = pk + hgdir[2];
+          int a = H.nb[(size_t)cc * 27 + nx * 9 + ny * 3 + nz];
+          int q = pi * 4 + pj * 2 + pk;
+          intp[q] = a;
+          if (a < 0) { ok = false; continue; }
+          if (solid_g[a] || ibidx_g[a] >= 0) ok = false;
+          coef[q] = (ca[nx] + cb[nx] * std::fabs(dkt[0])) *
+                    (ca[ny] + cb[ny] * std::fabs(dkt[1])) *
+                    (ca[nz] + cb[nz] * std::fabs(dkt[2]));
+        }
+    if (ok) { *hgdis_out = hgdis; *closecell = cc; return true; }
+  }
+  return false;
+}
+
+void build_all(ibm_handle &H, std::vector<CaseMesh> &cms) {
+  const int nc = (int)cms.size();
+  const int L = cms[0].cs.nlevels;
+  const int dim = cms[0].cs.dim;
+  H.ncases = nc; H.L = L; H.dim = dim;
+  const int nblk = nc * L;
+  H.blk_start.assign(nblk + 1, 0);
+  H.blk_nnormal.assign(nblk, 0);
+  H.blk_twod.assign(nblk, 0);
+  H.blk_h.assign((size_t)nblk * 3, 0.0);
+  // sorted order per block: normal cells in Morton order, then ghosts by
+  // (kind, Morton)
+  std::vector<std::vector<std::vector<int>>> order(nc), perm(nc);
+  int64_t total = 0;
+  for (int c = 0; c < nc; ++c) {
+    order[c].resize(L); perm[c].resize(L);
+    for (int l = 0; l < L; ++l) {
+      const Level &lv = cms[c].lv[l];
+      int n = (int)lv.cells.size();
+      std::vector<int> &o = order[c][l];
+      o.resize(n);
+      for (int i = 0; i < n; ++i) o[i] = i;
+      std::vector<uint64_t> mk(n);
+      for (int i = 0; i < n; ++i) mk[i] = morton(dim, lv.cells[i].ix, lv.cells[i].iy, lv.cells[i].iz);
+      std::sort(o.begin(), o.end(), [&](int a, int b) {
+        int ka = lv.cells[a].kind == NORMAL ? 0 : lv.cells[a].kind;
+        int kb = lv.cells[b].kind == NORMAL ? 0 : lv.cells[b].kind;
+        if (ka != kb) return ka < kb;
+        return mk[a] < mk[b];
+      });
+      perm[c][l].resize(n);
+      for (int i = 0; i < n; ++i) perm[c][l][o[i]] = i;
+      int b = c * L + l;
+      int nn = 0;
+      for (int i = 0; i < n; ++i) nn += (lv.cells[i].kind == NORMAL);
+      H.blk_nnormal[b] = nn;
+      H.blk_twod[b] = lv.twod ? 1 : 0;
+      for (int d = 0; d < 3; ++d) H.blk_h[(size_t)b * 3 + d] = lv.h[d];
+      H.blk_start[b] = (int32_t)total;
+      total += n;
+    }
+  }
+  if (total > 2000000000LL) throw std::runtime_error("mesh exceeds int32 cell indices");
+  H.blk_start[nblk] = (int32_t)total;
+  const size_t N = (size_t)total;
+  H.key.assign(N * 3, 0);
+  H.kind.assign(N, 0);
+  H.nb.assign(N * 27, -1);
+  H.cell_faces.assign(N * 6, -1);
+  H.signdis.assign(N, 9999.0);
+  H.ibidx.assign(N, -1);
+  H.center.assign(N * 3, 0.0);
+  std::vector<int32_t> cpatch(N, -1);
+  std::vector<int8_t> solid(N, 0);
+  const int dmax = (dim == 3) ? 3 : 2;
+
+  for (int c = 0; c < nc; ++c) {
+    CaseMesh &cm = cms[c];
+    for (int l = 0; l < L; ++l) {
+      const Level &lv = cm.lv[l];
+      const int b = c * L + l;
+      const int base = H.blk_start[b];
+      const std::vector<int> &o = order[c][l];
+      const std::vector<int> &pm = perm[c][l];
+#pragma omp parallel for schedule(static)
+      for (long p = 0; p < (long)o.size(); ++p) {
+        const LCell &cc = lv.cells[o[p]];
+        size_t g = base + p;
+        H.key[g * 3] = cc.ix; H.key[g * 3 + 1] = cc.iy; H.key[g * 3 + 2] = cc.iz;
+        H.kind[g] = cc.kind;
+        cell_center(cm, lv, cc, &H.center[g * 3]);
+        int pt; double sd;
+        closest_patch(cm, &H.center[g * 3], &pt, &sd);
+        H.signdis[g] = sd;
+        cpatch[g] = pt;
+        solid[g] = (sd < 0.0) ? 1 : 0;   // DistanceSign_boxtowall Body.cpp:810-817
+        // Box::neib[3][3][3]: same-level neighbours, z periodic
+        for (int di = 0; di < 3; ++di)
+          for (int dj = 0; dj < 3; ++dj)
+            for (int dk = 0; dk < 3; ++dk) {
+              int a = lv.find(cc.ix + di - 1, cc.iy + dj - 1, cc.iz + dk - 1);
+              H.nb[g * 27 + di * 9 + dj * 3 + dk] = (a < 0) ? -1 : base + pm[a];
+            }
+      }
+    }
+  }
+  // faces (Mesh::ConstructBoxFace Mesh.cpp:18-83): every normal cell gets its
+  // 2*dim faces; a face is shared with the neighbour on that side, bi[0] is
+  // the low side.  2D builds no z faces (ComputeFaceFlux skips fnv==2,
+  // NS_Solver.H:728-730).
+  H.blk_face_start.assign(nblk + 1, 0);
+  for (int c = 0; c < nc; ++c)
+    for (int l = 0; l < L; ++l) {
+      int b = c * L + l;
+      H.blk_face_start[b] = (int32_t)(H.face_cells.size() / 2);
+      int s = H.blk_start[b], e = s + H.blk_nnormal[b];
+      for (int d = 0; d < dmax; ++d) {
+        for (int g = s; g < e; ++g) {
+          for (int sd = 0; sd < 2; ++sd) {
+            int off[3] = {1, 1, 1};
+            off[d] = sd == 0 ? 0 : 2;
+            int a = H.nb[(size_t)g * 27 + off[0] * 9 + off[1] * 3 + off[2]];
+            if (a < 0) throw std::runtime_error("normal cell without a face neighbour");
+            if (H.cell_faces[(size_t)g * 6 + d * 2 + sd] >= 0) continue;
+            int f = (int)(H.face_cells.size() / 2);
+            int lo = sd == 0 ? a : g, hi = sd == 0 ? g : a;
+            H.face_cells.push_back(lo);
+            H.face_cells.push_back(hi);
+            H.face_dir.push_back((int8_t)d);
+            H.cell_faces[(size_t)g * 6 + d * 2 + sd] = f;
+            H.cell_faces[(size_t)a * 6 + d * 2 + (1 - sd)] = f;
+          }
+        }
+      }
+    }
+  H.blk_face_start[nblk] = (int32_t)(H.face_cells.size() / 2);
+
+  // domain ghosts (Domain::FaceProperty Domain.H:372-446): the first matching
+  // domain face in the order x-, x+, y-, y+ (Userfunction_cy_naca15.cpp:
+  // 283-346), refcell one step inward (two for the outer layer).  Corner
+  // ghosts come after edge ghosts so that their reference cell is filled
+  // first.
+  for (int c = 0; c < nc; ++c) {
+    const ibm_case &cs = cms[c].cs;
+    int b = c * L + 0;
+    int s = H.blk_start[b], e = H.blk_start[b + 1];
+    const int ftype[4] = {-13, -12, -50, -50};
+    const int fdir[4][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}};
+    for (int pass = 0; pass < 2; ++pass)
+      for (int g = s; g < e; ++g) {
+        if (H.kind[g] != IBG_DMGHOST) continue;
+        int ix = H.key[(size_t)g * 3], iy = H.key[(size_t)g * 3 + 1];
+        bool xo = (ix < 0 || ix >= cs.nx), yo = (iy < 0 || iy >= cs.ny);
+        bool corner = xo && yo;
+        if ((pass == 0) == corner) continue;
+        int j;
+        if (ix < 0) j = 0; else if (ix >= cs.nx) j = 1; else if (iy < 0) j = 2; else j = 3;
+        auto inface = [&](int x, int y) {
+          if (j == 0) return x < 0;
+          if (j == 1) return x >= cs.nx;
+          if (j == 2) return y < 0;
+          return y >= cs.ny;
+        };
+        int ref = H.nb[(size_t)g * 27 + (1 + fdir[j][0]) * 9 + (1 + fdir[j][1]) * 3 + 1];
+        if (ref >= 0 && inface(H.key[(size_t)ref * 3], H.key[(size_t)ref * 3 + 1]))
+          ref = H.nb[(size_t)ref * 27 + (1 + fdir[j][0]) * 9 + (1 + fdir[j][1]) * 3 + 1];
+        if (ref < 0) throw std::runtime_error("domain ghost without a reference cell");
+        H.dmg_cell.push_back(g);
+        H.dmg_ref.push_back(ref);
+        H.dmg_type.push_back(ftype[j]);
+      }
+  }
+  // coarse/fine pairs, global indices: per coarse level f_res_ghost[0],
+  // f_res_ghost[1], then f_pro_ghost
+  for (int c = 0; c < nc; ++c)
+    for (int l = 0; l + 1 < L; ++l) {
+      const int bc = c * L + l, bf = c * L + l + 1;
+      for (int layer = 0; layer < 3; ++layer)
+        for (const LPair &pr : cms[c].pairs[l]) {
+          if (pr.layer != layer) continue;
+          int gci = H.blk_start[bc] + perm[c][l][pr.ci];
+          int fi[8];
+          for (int q = 0; q < 8; ++q)
+            fi[q] = pr.fi[q] < 0 ? -1 : H.blk_start[bf] + perm[c][l + 1][pr.fi[q]];
+          if (layer < 2) {
+            H.res_ci.push_back(gci);
+            for (int q = 0; q < 8; ++q) H.res_fi.push_back(fi[q]);
+            H.res_blk.push_back(bc);
+          } else {
+            H.pro_ci.push_back(gci);
+            for (int q = 0; q < 8; ++q) H.pro_fi.push_back(fi[q]);
+            H.pro_blk.push_back(bc);
+          }
+        }
+    }
+  // immersed boundary on the finest level of each case
+  int patch_base = 0;
+  for (int c = 0; c < nc; ++c) {
+    CaseMesh &cm = cms[c];
+    const ibm_case &cs = cm.cs;
+    const int lf = L - 1;
+    const int b = c * L + lf;
+    const int s = H.blk_start[b], e = H.blk_start[b + 1];
+    const Level &lv = cm.lv[lf];
+    Finder F{&lv, s, &perm[c][lf]};
+    for (int bi = 0; bi < cs.nbody; ++bi) {
+      H.body_case.push_back(c);
+      for (int d = 0; d < 3; ++d) H.body_center.push_back(cs.body_center[d]);
+    }
+    if (cs.nbody > 0) {
+      // InfectWallBox_Init_Negative Body.cpp:1016-1082: non-solid cells with a
+      // solid neighbour (2D: in-plane 3x3; 3D: 27) and their non-solid
+      // neighbours are IB cells.
+      std::vector<char> inf(e - s, 0);
+      auto nbrs = [&](int g, int q) -> int {
+        return H.nb[(size_t)g * 27 + q];
+      };
+      std::vector<int> ql;
+      for (int di = 0; di < 3; ++di)
+        for (int dj = 0; dj < 3; ++dj)
+          for (int dk = 0; dk < 3; ++dk)
+            if (dim == 3 || dk == 1) ql.push_back(di * 9 + dj * 3 + dk);
+      for (int g = s; g < e; ++g) {
+        if (H.kind[g] == IBG_DMGHOST || solid[g]) continue;
+        bool hit = false;
+        for (int q : ql) {
+          int a = nbrs(g, q);
+          if (a >= 0 && solid[a] && H.kind[a] != IBG_DMGHOST) { hit = true; break; }
+        }
+        if (!hit) continue;
+        inf[g - s] = 1;
+        for (int q : ql) {
+          int a = nbrs(g, q);
+          if (a >= 0 && !solid[a] && H.kind[a] != IBG_DMGHOST) inf[a - s] = 1;
+        }
+      }
+      for (int g = s; g < e; ++g) {
+        if (!inf[g - s]) continue;
+        if (H.kind[g] != IBG_NORMAL)
+          throw std::runtime_error("an immersed-boundary cell lies on a ghost of the finest level");
+        H.ibidx[g] = (int32_t)H.ib_cell.size();
+        H.ib_cell.push_back(g);
+      }
+      const double *h = &H.blk_h[(size_t)b * 3];
+      for (size_t k = 0; k < H.ib_cell.size(); ++k) {
+        int g = H.ib_cell[k];
+        if (g < s || g >= e) continue;
+        const Patch &P = cm.patches[cpatch[g]];
+        double sd = H.signdis[g];
+        const double *cen = &H.center[(size_t)g * 3];
+        double att[3];
+        for (int d = 0; d < 3; ++d) att[d] = cen[d] - P.nv[d] * sd;   // Body.cpp:1073
+        int32_t intp[8]; double coef[8], hgdis; int cc;
+        for (int q = 0; q < 8; ++q) { intp[q] = -1; coef[q] = 0.0; }
+        if (!search_image(F, H, h, cs.origin, cen, P.nv, solid, H.ibidx, intp, coef, &hgdis, &cc))
+          throw std::runtime_error("no valid image point for an IB cell");
+        for (int q = 0; q < 8; ++q) { H.ib_intp.push_back(intp[q]); H.ib_coef.push_back(coef[q]); }
+        H.ib_hgdis.push_back(hgdis);
+        for (int d = 0; d < 3; ++d) { H.ib_nv.push_back(P.nv[d]); H.ib_attach.push_back(att[d]); }
+        H.ib_body.push_back(P.body + (int)(H.body_case.size() - cs.nbody));
+      }
+      // wall-force patches, FindPatchHangingNode Body.cpp:1506-1575
+      for (size_t p = 0; p < cm.patches.size(); ++p) {
+        const Patch &P = cm.patches[p];
+        int32_t intp[8]; double coef[8], hgdis; int cc;
+        for (int q = 0; q < 8; ++q) { intp[q] = -1; coef[q] = 0.0; }
+        if (!search_image(F, H, h, cs.origin, P.pc, P.nv, solid, H.ibidx, intp, coef, &hgdis, &cc))
+          throw std::runtime_error("no valid image point for a surface patch");
+        for (int q = 0; q < 8; ++q) { H.pat_intp.push_back(intp[q]); H.pat_coef.push_back(coef[q]); }
+        H.pat_hgdis.push_back(hgdis);
+        for (int d = 0; d < 3; ++d) { H.pat_nv.push_back(P.nv[d]); H.pat_pc.push_back(P.pc[d]); }
+        H.pat_area.push_back(P.area);
+        H.pat_case.push_back(c);
+      }
+      patch_base += (int)cm.patches.size();
+    }
+  }
+  (void)patch_base;
+}
+
+void fill_soa(ibm_handle &H) {
+  ibg_mesh_soa &m = H.soa;
+  std::memset(&m, 0, sizeof(m));
+  m.dim = H.dim; m.ncases = H.ncases; m.nlevels = H.L;
+  m.ncells = H.blk_start.back();
+  m.blk_start = H.blk_start.data(); m.blk_nnormal = H.blk_nnormal.data();
+  m.blk_h = H.blk_h.data(); m.blk_twod = H.blk_twod.data();
+  m.key = H.key.data(); m.kind = H.kind.data(); m.nb = H.nb.data();
+  m.cell_faces = H.cell_faces.data(); m.signdis = H.signdis.data(); m.ibidx = H.ibidx.data();
+  m.nfaces = (int32_t)H.face_dir.size();
+  m.blk_face_start = H.blk_face_start.data(); m.face_cells = H.face_cells.data();
+  m.face_dir = H.face_dir.data();
+  m.ndmg = (int32_t)H.dmg_cell.size();
+  m.dmg_cell = H.dmg_cell.data(); m.dmg_ref = H.dmg_ref.data(); m.dmg_type = H.dmg_type.data();
+  m.nres = (int32_t)H.res_ci.size();
+  m.res_ci = H.res_ci.data(); m.res_fi = H.res_fi.data(); m.res_blk = H.res_blk.data();
+  m.npro = (int32_t)H.pro_ci.size();
+  m.pro_ci = H.pro_ci.data(); m.pro_fi = H.pro_fi.data(); m.pro_blk = H.pro_blk.data();
+  m.nib = (int32_t)H.ib_cell.size();
+  m.ib_cell = H.ib_cell.data(); m.ib_intp = H.ib_intp.data(); m.ib_coef = H.ib_coef.data();
+  m.ib_hgdis = H.ib_hgdis.data(); m.ib_nv = H.ib_nv.data(); m.ib_attach = H.ib_attach.data();
+  m.ib_body = H.ib_body.data();
+  m.npatch = (int32_t)H.pat_area.size();
+  m.pat_intp = H.pat_intp.data(); m.pat_coef = H.pat_coef.data(); m.pat_hgdis = H.pat_hgdis.data();
+  m.pat_nv = H.pat_nv.data(); m.pat_pc = H.pat_pc.data(); m.pat_area = H.pat_area.data();
+  m.pat_case = H.pat_case.data();
+  m.nbody = (int32_t)H.body_case.size();
+  m.body_case = H.body_case.data(); m.body_center = H.body_center.data();
+}
+
+}  // namespace
+
+extern "C" {
+
+ibm_handle *ibm_build(const ibm_case *cases, int32_t ncases) {
+  try {
+    if (!cases || ncases < 1 || ncases > IBG_MAX_CASES) throw std::runtime_error("bad case count");
+    std::vector<CaseMesh> cms(ncases);
+    for (int c = 0; c < ncases; ++c) {
+      const ibm_case &cs = cases[c];
+      if (cs.nlevels < 1 || cs.nlevels > IBG_MAX_LEVELS) throw std::runtime_error("bad level count");
+      if (cs.nlevels != cases[0].nlevels || cs.dim != cases[0].dim)
+        throw std::runtime_error("cases of one batch must share dim and level count");
+      if (cs.dim == 2 && cs.nz != 1) throw std::runtime_error("2D cases have nz == 1");
+      if (cs.nbody > IBM_MAX_BODIES) throw std::runtime_error("too many bodies");
+      cms[c].cs = cs;
+      build_segments(cms[c]);
+      build_patches(cms[c]);
+      build_levels(cms[c]);
+    }
+    ibm_handle *H = new ibm_handle();
+    H->cases.assign(cases, cases + ncases);
+    build_all(*H, cms);
+    fill_soa(*H);
+    return H;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return nullptr;
+  }
+}
+
+const ibg_mesh_soa *ibm_mesh(const ibm_handle *h) { return h ? &h->soa : nullptr; }
+
+const char *ibm_last_error(void) { return g_last_error.c_str(); }
+
+void ibm_free(ibm_handle *h) { delete h; }
+
+int32_t ibm_cell_centers(const ibm_handle *h, double *xyz) {
+  if (!h || !xyz) return 1;
+  std::memcpy(xyz, h->center.data(), h->center.size() * sizeof(double));
+  return 0;
+}
+
+// FlowConstants.cpp:5-44 with the case globals of Userfunction_cy_naca15.cpp:
+// roe_ref = u_ref = L_ref = 1, mu_ref = 1/Re, T_ref = u_ref^2/(Ma^2 Rg_ref gama);
+// freestream (SetInitValues :356-371) u = 1, T = (1/Ma)^2/(gama Rg), roe = 1;
+// inlet total state (SetBDFaceVars :396-420), outlet facevar[1].p + backpressure.
+int32_t ibm_physics(const ibm_case *c, ibg_physics *o) {
+  if (!c || !o) return 1;
+  std::memset(o, 0, sizeof(*o));
+  const double Rg_ref = 287.4, u_ref = 1.0, roe_ref = 1.0, L_ref = 1.0;
+  o->gama = c->gama;
+  o->Ma = c->Ma;
+  o->Re = c->Re;
+  o->Pr = c->Pr;
+  const double T_ref = u_ref * u_ref / (c->Ma * c->Ma * Rg_ref * c->gama);
+  o->Rg = 1.0 / (c->gama * std::pow(c->Ma, 2));
+  o->Cv = o->Rg / (c->gama - 1.0);
+  o->Cp = o->Cv + o->Rg;
+  o->mu = 1.0;
+  o->qk = o->mu / (o->Re * o->Pr * (c->gama - 1) * std::pow(c->Ma, 2));
+  o->S_over_T_ref = 110.33333333 / T_ref;
+  double *iv = o->initvar;   // roe u v w p T e
+  iv[1] = 1.0; iv[2] = 0.0; iv[3] = 0.0;
+  double speed = std::pow(std::pow(iv[1], 2) + std::pow(iv[2], 2) + std::pow(iv[3], 2), 0.5);
+  iv[5] = std::pow(speed / c->Ma, 2) / (c->gama * o->Rg);
+  iv[0] = 1.0;
+  iv[4] = iv[0] * o->Rg * iv[5];
+  iv[6] = o->Cv * iv[5] + 0.5 * (iv[1] * iv[1] + iv[2] * iv[2] + iv[3] * iv[3]);
+  double ke = 0.5 * (iv[1] * iv[1] + iv[2] * iv[2] + iv[3] * iv[3]);
+  o->inlet_T0 = ke * iv[0] / o->Cp + iv[5];
+  o->inlet_p0 = iv[4] * std::pow(1.0 + 0.2 * std::pow(c->Ma, 2), 1.4 / 0.4);
+  o->inlet_angle = 0.0;
+  o->outlet_p = iv[4] + c->backpressure;
+  o->cfl = c->cfl;
+  o->dt_ratio = 1.0;
+  o->dt_init = c->dt_init;
+  const double mu_ref = 1.0 / c->Re;
+  const double p_ref = roe_ref * u_ref * u_ref, area_ref = L_ref * L_ref;
+  const double shear_ref = mu_ref * u_ref / L_ref;
+  o->p_ref_area = p_ref * area_ref;
+  o->shear_ref_area = shear_ref * area_ref;
+  // F_ref = 0.5 u^2 roe (L^2 * span) (Userfunction_cy_naca15.cpp:439)
+  o->force_ref = 0.5 * u_ref * u_ref * roe_ref * (L_ref * L_ref * c->span);
+  o->moment_ref = o->force_ref * L_ref;
+  return 0;
+}
+
+int32_t ibm_init_state(const ibm_handle *h, const ibg_physics *phys, double *fv) {
+  if (!h || !phys || !fv) return 1;
+  const int L = h->L;
+  for (int c = 0; c < h->ncases; ++c) {
+    int s = h->blk_start[c * L], e = h->blk_start[(c + 1) * L];
+    for (int g = s; g < e; ++g)
+      for (int v = 0; v < IBG_NFV; ++v) fv[(size_t)g * IBG_NFV + v] = phys[c].initvar[v];
+  }
+  return 0;
+}
+
+void ibm_naca4(double t, int32_t npts, double xc, double yc, double aoa_deg, double *xy) {
+  const double pi = 3.14159265358979323846;
+  const double a = -aoa_deg * pi / 180.0;   // nose up: rotate clockwise
+  const double ca = std::cos(a), sa = std::sin(a);
+  for (int s = 0; s < 2; ++s) {
+    for (int i = 0; i < npts; ++i) {
+      double x = 0.5 * (1.0 - std::cos(pi * i / (npts - 1)));
+      double yt = 5.0 * t * (0.2969 * std::sqrt(x) - 0.1260 * x - 0.3516 * x * x +
+                             0.2843 * x * x * x - 0.1036 * x * x * x * x);
+      double y = (s == 0) ? yt : -yt;
+      double px = x - 0.25, py = y;          // about the quarter chord
+      double rx = ca * px - sa * py, ry = sa * px + ca * py;
+      xy[2 * (s * npts + i)] = xc + rx;
+      xy[2 * (s * npts + i) + 1] = yc + ry;
+    }
+  }
+}
+
+}  // extern "C"

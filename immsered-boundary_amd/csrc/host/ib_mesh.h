@@ -1,0 +1,79 @@
+/*
+ * ib_mesh.h — host-side AMR + immersed-boundary mesh builder (libibhost.so).
+ *
+ * Builds, for one or more airfoil cases, the flattened level set that the
+ * residual engine consumes (ibg_mesh_soa in ibgpu.h): per-level normal and
+ * ghost cells, the 27-neighbour tables, faces, domain-boundary ghosts, the
+ * coarse/fine restriction and prolongation pairs, signed distances, the
+ * immersed-boundary cell list with image-point interpolation weights, and the
+ * surface patches with their own image points for the wall force.
+ *
+ * The construction rules follow the reference's initial refinement:
+ *   band tagging            Userfunction_cy_naca15.cpp:533-605 (rs < signdis < re)
+ *   restriction / pro rings AMRGhost_refine.cpp:30-98
+ *   domain ghosts           Domain.H:372-446
+ *   signed distance         Body.cpp:791-821
+ *   IB tagging              Body.cpp:1016-1096
+ *   image points + weights  Body.cpp:999-1190, Mesh.cpp:1330-1442
+ *   patch image points      Body.cpp:1506-1600
+ */
+#ifndef IB_MESH_H_
+#define IB_MESH_H_
+
+#include <stdint.h>
+#include "ibgpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IBM_MAX_BODIES 8
+
+typedef struct ibm_case {
+  int32_t dim;                  /* 2 or 3                                     */
+  int32_t nx, ny, nz;           /* base grid (2D: nz = 1)                     */
+  double dh0[3];                /* base cell size                             */
+  double origin[3];             /* physical position of the low corner        */
+  int32_t nlevels;              /* max_mesh_level                             */
+  int32_t twod_from;            /* levels >= twod_from refine x,y only        */
+  double band_lo[IBG_MAX_LEVELS];/* level l cells with band_lo<signdis<band_hi */
+  double band_hi[IBG_MAX_LEVELS];/*   are refined to l+1 (l = 0..nlevels-2)   */
+  /* bodies: closed airfoils split in two open polylines (upper, lower) as
+     the reference splits its bodies (Userfunction_cy_naca15.cpp:205-211)    */
+  int32_t nbody;
+  int32_t body_npts[IBM_MAX_BODIES];
+  const double *body_xy;        /* concatenated (x,y) points                  */
+  double body_center[3];        /* moment reference point                     */
+  double span;                  /* z extent of the surface patches            */
+  int32_t span_patches;         /* patches per segment along z (>= 1)         */
+  /* flow */
+  double gama, Ma, Re, Pr, cfl, backpressure, dt_init;
+} ibm_case;
+
+typedef struct ibm_handle ibm_handle;
+
+ibm_handle *ibm_build(const ibm_case *cases, int32_t ncases);
+const ibg_mesh_soa *ibm_mesh(const ibm_handle *h);
+/* physics constants of case c, FlowConstants.cpp:5-44 semantics            */
+int32_t ibm_physics(const ibm_case *c, ibg_physics *out);
+/* initial state: every cell set to the case's freestream (InitFlow
+   NS_Solver.H:432-446), AoS [ncells][7]                                    */
+int32_t ibm_init_state(const ibm_handle *h, const ibg_physics *phys,
+                       double *fv_aos);
+/* cell centers [ncells][3]                                                 */
+int32_t ibm_cell_centers(const ibm_handle *h, double *xyz);
+/* error text of the last failed build                                      */
+const char *ibm_last_error(void);
+void ibm_free(ibm_handle *h);
+
+/* NACA 4-digit symmetric section (closed trailing edge), npts cosine-spaced
+   points per surface, chord 1, quarter chord at (xc,yc), rotated nose-up
+   by aoa_deg.  Writes upper then lower surface, each LE -> TE, into xy
+   [2*npts][2].                                                             */
+void ibm_naca4(double t, int32_t npts, double xc, double yc, double aoa_deg,
+               double *xy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,164 @@
+"""Python mirror of the NS_Solver call surface over libibgpu.so (ctypes).
+
+Method names follow the reference's solver (NS_Solver.H:195-703,
+Body.H:723, AMR.H:1012, BCValues.H:51): TimeMarching, ComputeFaceFlux,
+ComptViscousStress, Comptdt, ResinDomain, IBConditions_Image,
+InterfaceExchange, BCBoxTreat, ComptWallForce.  Every call goes to the HIP
+engine; a missing library raises (there is no CPU path here).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _ffi
+
+ALPHA = {1: [1.0], 3: [0.1918, 0.4929, 1.0], 4: [0.1084, 0.2602, 0.5052, 1.0]}   # NS_Solver.H:202-231
+MODES = {"fp64": 0, "mixed": 1}
+
+
+class IBGError(RuntimeError):
+    def __init__(self, code, where, cell=-1):
+        super().__init__(f"{where} returned ibg_status {code} (first offending cell {cell})")
+        self.code = code
+        self.cell = cell
+
+
+def make_config(mode="fp64", nstage=3, dt_var_step=None, temporal_refine=True, limiter=0,
+                viscous=True, tile=0, nlevels=1):
+    cfg = _ffi.Config()
+    cfg.mode = MODES[mode]
+    cfg.nstage = nstage
+    for i, a in enumerate(ALPHA[nstage]):
+        cfg.alpha[i] = a
+    period = (1 << (nlevels - 1)) if temporal_refine else 1
+    cfg.dt_var_step = period if dt_var_step is None else dt_var_step
+    cfg.temporal_refine = 1 if temporal_refine else 0
+    cfg.limiter = limiter
+    cfg.viscous = 1 if viscous else 0
+    cfg.tile = tile
+    return cfg
+
+
+class Engine:
+    def __init__(self, mesh, mode="fp64", nstage=3, temporal_refine=True, limiter=0,
+                 viscous=True, tile=0, dt_var_step=None):
+        self.lib = _ffi.gpu_lib()
+        self.mesh = mesh
+        self.cfg = make_config(mode, nstage, dt_var_step, temporal_refine, limiter, viscous, tile,
+                               mesh.nlevels)
+        self.ctx = C.c_void_p()
+        self._chk(self.lib.ibg_create(C.byref(self.cfg), C.byref(self.ctx)), "ibg_create")
+        self._chk(self.lib.ibg_upload_mesh(self.ctx, mesh.soa_ptr, mesh.phys), "ibg_upload_mesh")
+
+    def _chk(self, code, where):
+        if code != 0:
+            cell = C.c_int64(-1)
+            c2 = C.c_int32(0)
+            try:
+                if self.ctx:
+                    self.lib.ibg_last_error(self.ctx, C.byref(cell), C.byref(c2))
+            except Exception:
+                pass
+            raise IBGError(code, where, cell.value)
+
+    # state ------------------------------------------------------------------
+    def set_state(self, fv):
+        fv = np.ascontiguousarray(fv, dtype=np.float64)
+        self._chk(self.lib.ibg_set_state(self.ctx, fv.ctypes.data_as(_ffi.f64p), fv.shape[0]),
+                  "ibg_set_state")
+
+    def get_state(self):
+        fv = np.zeros((self.mesh.ncells, _ffi.IBG_NFV))
+        self._chk(self.lib.ibg_get_state(self.ctx, fv.ctypes.data_as(_ffi.f64p), fv.shape[0]),
+                  "ibg_get_state")
+        return fv
+
+    def set_dt(self, dt):
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        self._chk(self.lib.ibg_set_dt(self.ctx, dt.ctypes.data_as(_ffi.f64p)), "ibg_set_dt")
+
+    def get_dt(self):
+        dt = np.zeros(self.mesh.ncases)
+        self._chk(self.lib.ibg_get_dt(self.ctx, dt.ctypes.data_as(_ffi.f64p)), "ibg_get_dt")
+        return dt
+
+    # reference call surface -------------------------------------------------
+    def InterfaceExchange(self, ts):
+        self._chk(self.lib.ibg_interface_exchange(self.ctx, ts), "ibg_interface_exchange")
+
+    def IBConditions_Image(self, motion=None):
+        self._chk(self.lib.ibg_ib_fill(self.ctx, motion), "ibg_ib_fill")
+
+    def BCBoxTreat(self, ts):
+        self._chk(self.lib.ibg_bc_fill(self.ctx, ts), "ibg_bc_fill")
+
+    def Comptdt(self, ts=0):
+        dt = np.zeros(self.mesh.ncases)
+        self._chk(self.lib.ibg_compute_dt(self.ctx, ts, dt.ctypes.data_as(_ffi.f64p)), "ibg_compute_dt")
+        return dt
+
+    def ComptViscousStress(self, ts):
+        self._chk(self.lib.ibg_viscous_flux(self.ctx, ts), "ibg_viscous_flux")
+
+    def Stage(self, ts, subt):
+        self._chk(self.lib.ibg_stage(self.ctx, ts, subt), "ibg_stage")
+
+    def TimeMarching(self, ts0, nsteps=1, use_graph=False):
+        self._chk(self.lib.ibg_time_march(self.ctx, ts0, nsteps, 1 if use_graph else 0),
+                  "ibg_time_march")
+
+    def ResinDomain(self):
+        n = self.mesh.ncases * 5
+        linf, l2 = np.zeros(n), np.zeros(n)
+        self._chk(self.lib.ibg_residual(self.ctx, linf.ctypes.data_as(_ffi.f64p),
+                                        l2.ctypes.data_as(_ffi.f64p)), "ibg_residual")
+        return linf, l2
+
+    def ComptWallForce(self, motion=None):
+        out = np.zeros(self.mesh.ncases * 6)
+        self._chk(self.lib.ibg_forces(self.ctx, motion, out.ctypes.data_as(_ffi.f64p)), "ibg_forces")
+        return out
+
+    # probes -----------------------------------------------------------------
+    def faceflux(self):
+        out = np.zeros((self.mesh.nfaces, 5))
+        self._chk(self.lib.ibg_get_faceflux(self.ctx, out.ctypes.data_as(_ffi.f64p), out.shape[0]),
+                  "ibg_get_faceflux")
+        return out
+
+    def vsflux(self):
+        out = np.zeros((self.mesh.nfaces, 5))
+        self._chk(self.lib.ibg_get_vsflux(self.ctx, out.ctypes.data_as(_ffi.f64p), out.shape[0]),
+                  "ibg_get_vsflux")
+        return out
+
+    def source(self):
+        out = np.zeros((self.mesh.ncells, 5))
+        self._chk(self.lib.ibg_get_source(self.ctx, out.ctypes.data_as(_ffi.f64p), out.shape[0]),
+                  "ibg_get_source")
+        return out
+
+    def timing(self):
+        tot, st = C.c_double(), C.c_double()
+        n = C.c_int64()
+        self.lib.ibg_last_timing(self.ctx, C.byref(tot), C.byref(st), C.byref(n))
+        return tot.value, st.value, n.value
+
+    def cell_stage_updates(self):
+        n = C.c_int64()
+        self.lib.ibg_cell_stage_updates(self.ctx, C.byref(n))
+        return n.value
+
+    def synchronize(self):
+        self._chk(self.lib.ibg_synchronize(self.ctx), "ibg_synchronize")
+
+    def close(self):
+        if self.ctx:
+            self.lib.ibg_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

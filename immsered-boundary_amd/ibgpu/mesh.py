@@ -1,0 +1,186 @@
+"""Case definitions and the host mesh wrapper.
+
+A case is an airfoil immersed in a Cartesian box [-10,10]^2 (x along the
+freestream).  The reference rotates its body for an angle of attack and keeps
+the freestream along +x (Userfunction_cy_naca15.cpp:205-216, 356-371); so do
+these cases.  Upper and lower surfaces are two bodies, as in the reference
+(Userfunction_cy_naca15.cpp:205-211).
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import _ffi
+
+
+@dataclass
+class CaseSpec:
+    name: str
+    dim: int = 2
+    nx: int = 64
+    ny: int = 64
+    nz: int = 1
+    lo: float = -10.0
+    hi: float = 10.0
+    span: float = 0.1            # z extent (2D: one cell of this depth)
+    nlevels: int = 4
+    twod_from: int = 1
+    bands: List[tuple] = field(default_factory=list)   # (lo, hi) per level < L-1
+    thickness: float = 0.12
+    npts: int = 100               # points per surface (cosine spaced)
+    aoa: float = 2.0
+    Ma: float = 0.5
+    Re: float = 5000.0
+    Pr: float = 0.72
+    gama: float = 1.4
+    cfl: float = 0.5
+    backpressure: float = 0.0
+    span_patches: int = 1
+
+    def to_ctypes(self):
+        c = _ffi.Case()
+        c.dim = self.dim
+        c.nx, c.ny, c.nz = self.nx, self.ny, self.nz
+        dx = (self.hi - self.lo) / self.nx
+        dy = (self.hi - self.lo) / self.ny
+        c.dh0[0], c.dh0[1], c.dh0[2] = dx, dy, self.span / self.nz
+        c.origin[0], c.origin[1], c.origin[2] = self.lo, self.lo, 0.0
+        c.nlevels = self.nlevels
+        c.twod_from = self.twod_from
+        for i in range(_ffi.IBG_MAX_LEVELS):
+            c.band_lo[i], c.band_hi[i] = (0.0, 0.0)
+        for i, (a, b) in enumerate(self.bands):
+            c.band_lo[i], c.band_hi[i] = a, b
+        xy = np.zeros((2 * self.npts, 2), dtype=np.float64)
+        host = _ffi.host_lib()
+        host.ibm_naca4(self.thickness, self.npts, 0.0, 0.0, self.aoa,
+                       xy.ctypes.data_as(_ffi.f64p))
+        self._xy = np.ascontiguousarray(xy)       # keep alive
+        c.nbody = 2
+        c.body_npts[0] = self.npts
+        c.body_npts[1] = self.npts
+        c.body_xy = self._xy.ctypes.data_as(_ffi.f64p)
+        c.body_center[0], c.body_center[1], c.body_center[2] = 0.0, 0.0, 0.5 * self.span
+        c.span = self.span
+        c.span_patches = self.span_patches
+        c.gama, c.Ma, c.Re, c.Pr = self.gama, self.Ma, self.Re, self.Pr
+        c.cfl = self.cfl
+        c.backpressure = self.backpressure
+        c.dt_init = 1.0e30
+        return c
+
+
+def config1(levels=4, base=64):
+    """Plain method, small: NACA0012, 2D, Mach 0.5, AoA 2, Re 5000."""
+    # refinement bands (signed distance, chord units) per level
+    bands = [(-0.6, 2.0), (-0.3, 1.0), (-0.15, 0.5), (-0.08, 0.25), (-0.04, 0.12)]
+    return CaseSpec(name="naca0012_m05_re5000", dim=2, nx=base, ny=base, nz=1,
+                    nlevels=levels, bands=bands[:levels - 1], thickness=0.12,
+                    npts=100, aoa=2.0, Ma=0.5, Re=5000.0)
+
+
+def config2(levels=6, base=256):
+    """Mid 2D case: NACA0015, Mach 0.3, Re 10000, 256^2 base + 5 AMR levels."""
+    bands = [(-0.3, 4.0), (-0.15, 2.5), (-0.08, 1.5), (-0.04, 0.9), (-0.02, 0.45)]
+    return CaseSpec(name="naca0015_m03_re10000", dim=2, nx=base, ny=base, nz=1,
+                    nlevels=levels, bands=bands[:levels - 1], thickness=0.15,
+                    npts=200, aoa=0.0, Ma=0.3, Re=10000.0)
+
+
+def tiny_case(levels=3, base=32, aoa=4.0, Ma=0.5, Re=2000.0, thickness=0.12):
+    """A few-thousand-cell case for parity tests that the oracle runs in seconds."""
+    bands = [(-1.0, 1.5), (-0.5, 0.6), (-0.25, 0.3), (-0.12, 0.15)]
+    return CaseSpec(name="tiny", dim=2, nx=base, ny=base, nz=1, lo=-4.0, hi=4.0,
+                    nlevels=levels, bands=bands[:levels - 1], thickness=thickness,
+                    npts=60, aoa=aoa, Ma=Ma, Re=Re)
+
+
+class Mesh:
+    """Owns an ibm_handle; exposes the flattened mesh as numpy views."""
+
+    def __init__(self, specs):
+        if isinstance(specs, CaseSpec):
+            specs = [specs]
+        self.specs = list(specs)
+        host = _ffi.host_lib()
+        arr = (_ffi.Case * len(self.specs))()
+        for i, s in enumerate(self.specs):
+            arr[i] = s.to_ctypes()
+        self._cases = arr
+        h = host.ibm_build(arr, len(self.specs))
+        if not h:
+            raise RuntimeError("mesh build failed: " + host.ibm_last_error().decode())
+        self._h = C.c_void_p(h)
+        self._host = host
+        self.soa_ptr = host.ibm_mesh(self._h)
+        self.soa = self.soa_ptr.contents
+        self.phys = (_ffi.Physics * len(self.specs))()
+        for i in range(len(self.specs)):
+            host.ibm_physics(C.byref(arr[i]), C.byref(self.phys[i]))
+        m = self.soa
+        self.dim, self.ncases, self.nlevels = m.dim, m.ncases, m.nlevels
+        self.ncells, self.nfaces = m.ncells, m.nfaces
+        nb = self.ncases * self.nlevels
+        v = np.ctypeslib.as_array
+        self.blk_start = v(m.blk_start, (nb + 1,)).copy()
+        self.blk_nnormal = v(m.blk_nnormal, (nb,)).copy()
+        self.blk_h = v(m.blk_h, (nb, 3)).copy()
+        self.blk_face_start = v(m.blk_face_start, (nb + 1,)).copy()
+        n = self.ncells
+        self.key = v(m.key, (n, 3))
+        self.kind = v(m.kind, (n,))
+        self.nb = v(m.nb, (n, 27))
+        self.cell_faces = v(m.cell_faces, (n, 6))
+        self.signdis = v(m.signdis, (n,))
+        self.ibidx = v(m.ibidx, (n,))
+        self.face_cells = v(m.face_cells, (self.nfaces, 2)) if self.nfaces else np.zeros((0, 2), np.int32)
+        self.face_dir = v(m.face_dir, (self.nfaces,)) if self.nfaces else np.zeros(0, np.int8)
+        self.dmg = (v(m.dmg_cell, (m.ndmg,)), v(m.dmg_ref, (m.ndmg,)), v(m.dmg_type, (m.ndmg,))) if m.ndmg else None
+        self.nres, self.npro, self.nib, self.npatch = m.nres, m.npro, m.nib, m.npatch
+        self.res_ci = v(m.res_ci, (m.nres,)) if m.nres else np.zeros(0, np.int32)
+        self.res_fi = v(m.res_fi, (m.nres, 8)) if m.nres else np.zeros((0, 8), np.int32)
+        self.pro_ci = v(m.pro_ci, (m.npro,)) if m.npro else np.zeros(0, np.int32)
+        self.pro_fi = v(m.pro_fi, (m.npro, 8)) if m.npro else np.zeros((0, 8), np.int32)
+        self.ib_cell = v(m.ib_cell, (m.nib,)) if m.nib else np.zeros(0, np.int32)
+        self.ib_intp = v(m.ib_intp, (m.nib, 8)) if m.nib else np.zeros((0, 8), np.int32)
+        self.ib_coef = v(m.ib_coef, (m.nib, 8)) if m.nib else np.zeros((0, 8))
+        self.ib_hgdis = v(m.ib_hgdis, (m.nib,)) if m.nib else np.zeros(0)
+        self.pat_intp = v(m.pat_intp, (m.npatch, 8)) if m.npatch else np.zeros((0, 8), np.int32)
+        self.pat_coef = v(m.pat_coef, (m.npatch, 8)) if m.npatch else np.zeros((0, 8))
+        self.centers = np.zeros((n, 3))
+        host.ibm_cell_centers(self._h, self.centers.ctypes.data_as(_ffi.f64p))
+
+    def block(self, case, level):
+        return case * self.nlevels + level
+
+    def level_of(self):
+        """level index per cell"""
+        lv = np.zeros(self.ncells, np.int32)
+        for b in range(self.ncases * self.nlevels):
+            lv[self.blk_start[b]:self.blk_start[b + 1]] = b % self.nlevels
+        return lv
+
+    def init_state(self):
+        fv = np.zeros((self.ncells, _ffi.IBG_NFV))
+        self._host.ibm_init_state(self._h, self.phys, fv.ctypes.data_as(_ffi.f64p))
+        return fv
+
+    def counts(self):
+        out = []
+        for c in range(self.ncases):
+            row = []
+            for l in range(self.nlevels):
+                b = self.block(c, l)
+                row.append(int(self.blk_nnormal[b]))
+            out.append(row)
+        return out
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                self._host.ibm_free(self._h)
+                self._h = None
+        except Exception:
+            pass

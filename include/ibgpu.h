@@ -1,0 +1,252 @@
+/*
+ * ibgpu.h — C ABI of the MI355X residual engine (libibgpu.so).
+ *
+ * The engine replaces, for one GPU holding the whole mesh, the per-stage
+ * residual path of the Cartesian-AMR immersed-boundary solver: MUSCL
+ * reconstruction with the van Leer limiter, the all-speed Roe inviscid flux,
+ * the viscous face flux, the image-point immersed-boundary ghost fill, the
+ * domain-boundary ghosts, the coarse/fine transfer and the low-storage
+ * multistage update over the AMR level set.
+ *
+ * Every entry point below names the reference interface it replaces
+ * (file:line in the reference tree).  All entry points are stream ordered on
+ * the context's HIP stream, are not re-entrant per context, and return an
+ * ibg_status.  No torch types cross this boundary: plain pointers and sizes.
+ *
+ * Data contract.  The host (a C++ NS_Solver-shaped class, or a Python/ctypes
+ * driver) hands over one flattened mesh, ibg_mesh_soa, whose arrays stay owned
+ * by the caller; the library copies what it needs into device memory.  Cell
+ * state crosses the boundary as the reference's AoS FlowVariables layout
+ * (roe,u,v,w,p,T,e as 7 doubles per cell, FlowVariables.H:320-326).
+ */
+#ifndef IBGPU_H_
+#define IBGPU_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IBG_MAX_LEVELS 12
+#define IBG_MAX_CASES 64
+#define IBG_NFV 7          /* doubles per FlowVariables (laminar) */
+
+/* ------------------------------------------------------------------ status */
+typedef enum ibg_status {
+  IBG_OK = 0,
+  IBG_ERR_ARG = 1,          /* bad argument / size mismatch                 */
+  IBG_ERR_HIP = 2,          /* HIP runtime error                            */
+  IBG_ERR_STATE = 3,        /* call out of order (e.g. no mesh uploaded)    */
+  /* numerical aborts of the reference, returned instead of MPI_Abort       */
+  IBG_ERR_NEG_DENSITY = 523, /* NS_Solver.cpp:888-894 (611 finest level)    */
+  IBG_ERR_NEG_T = 37,        /* NS_Solver.cpp:55-66, 139-155                 */
+  IBG_ERR_NAN_RESIDUAL = 268,/* NS_Solver.H:355-375                          */
+  IBG_ERR_LIMITER = 57,      /* Slopelimiter.H:54-57                         */
+  IBG_ERR_NEG_DT = 424,      /* NS_Solver.cpp:253-258                        */
+  IBG_ERR_NEG_HG_T = 1050    /* Body.cpp:1442-1451                           */
+} ibg_status;
+
+/* cell kinds on a level (Box.H:63-66 types plus the AMR pair role) */
+enum ibg_cell_kind {
+  IBG_NORMAL = 0,   /* marched leaf cell (m_box [0,num_without_ghost))       */
+  IBG_DMGHOST = 1,  /* domain-boundary ghost (Dmghost)                       */
+  IBG_RES1 = 2,     /* coarse cell under fine cells, f_res_ghost[0]          */
+  IBG_RES2 = 3,     /* coarse cell under fine cells, f_res_ghost[1]          */
+  IBG_PRO = 4       /* fine cell over coarse cells, son in f_pro_ghost       */
+};
+
+/* precision modes */
+enum ibg_mode {
+  IBG_MODE_FP64 = 0,   /* plain: every operation in FP64                     */
+  IBG_MODE_MIXED = 1   /* FP32 limiter/Roe/viscous arithmetic, FP64 state,
+                          accumulation, norms and wall interpolation          */
+};
+
+/* ------------------------------------------------------------- mesh (SoA)
+ * One or more cases (an ensemble) flattened into one cell index space.
+ * Cells of case c, level l occupy [blk_start[b], blk_start[b+1]) with
+ * b = c*nlevels + l; the first blk_nnormal[b] of them are normal cells
+ * (Morton order), the rest are ghosts.  Neighbour indices are global.
+ * Canonical keys: (case, level, ix, iy, iz).                              */
+typedef struct ibg_mesh_soa {
+  int32_t dim;                 /* 2 or 3                                      */
+  int32_t ncases;
+  int32_t nlevels;             /* max_mesh_level, same for every case         */
+  int32_t ncells;
+  const int32_t *blk_start;    /* [ncases*nlevels+1]                          */
+  const int32_t *blk_nnormal;  /* [ncases*nlevels]                            */
+  const double  *blk_h;        /* [ncases*nlevels][3] cell size dh[l]         */
+  const int32_t *blk_twod;     /* [ncases*nlevels] level refined in x,y only  */
+  /* per cell */
+  const int32_t *key;          /* [ncells][3]  (ix,iy,iz) on its level        */
+  const int8_t  *kind;         /* [ncells]     ibg_cell_kind                  */
+  const int32_t *nb;           /* [ncells][27] Box::neib[i][j][k] at i*9+j*3+k*/
+  const int32_t *cell_faces;   /* [ncells][6]  Box::faces[d][s] at d*2+s      */
+  const double  *signdis;      /* [ncells]     BoxtoWall::signdis             */
+  const int32_t *ibidx;        /* [ncells]     Mesh::infectbox (-1 = none)    */
+  /* faces, grouped per block (case,level); Face{bi[2],fnv} Face.H:9-15     */
+  int32_t nfaces;
+  const int32_t *blk_face_start; /* [ncases*nlevels+1]                        */
+  const int32_t *face_cells;   /* [nfaces][2] bi[0]=low side, bi[1]=high side */
+  const int8_t  *face_dir;     /* [nfaces]    fnv                             */
+  /* domain ghosts, Domainghost{cell,refcell,nbface} GhostPair.H:20-38       */
+  int32_t ndmg;
+  const int32_t *dmg_cell, *dmg_ref, *dmg_type;  /* type = BC code (-13,...)  */
+  /* coarse/fine pairs CtoFPair{ci, fi.son[2][2][2]} GhostPair.H:6-18        */
+  int32_t nres;                /* f_res_ghost[0] then [1], per coarse level   */
+  const int32_t *res_ci, *res_fi;   /* res_fi [nres][8], son[i][j][k] i*4+j*2+k*/
+  const int32_t *res_blk;      /* [nres] coarse block b                       */
+  int32_t npro;                /* f_pro_ghost                                 */
+  const int32_t *pro_ci, *pro_fi, *pro_blk;
+  /* immersed-boundary cells, IBCell/HGCell Box.H:331-395 (finest level)    */
+  int32_t nib;
+  const int32_t *ib_cell;      /* [nib]     IBCell::ci                        */
+  const int32_t *ib_intp;      /* [nib][8]  HGCell::intpcell                  */
+  const double  *ib_coef;      /* [nib][8]  HGCell::intpcoef                  */
+  const double  *ib_hgdis;     /* [nib]     HGCell::hgdis                     */
+  const double  *ib_nv;        /* [nib][3]  closest patch normal              */
+  const double  *ib_attach;    /* [nib][3]  HGCell::attach_pt                 */
+  const int32_t *ib_body;      /* [nib]     BoxtoWall::body                   */
+  /* surface patches used for the wall force, Surfpatch Box.H:497-544       */
+  int32_t npatch;
+  const int32_t *pat_intp;     /* [npatch][8] */
+  const double  *pat_coef;     /* [npatch][8] */
+  const double  *pat_hgdis;    /* [npatch]    */
+  const double  *pat_nv;       /* [npatch][3] */
+  const double  *pat_pc;       /* [npatch][3] */
+  const double  *pat_area;     /* [npatch]    */
+  const int32_t *pat_case;     /* [npatch]    */
+  /* bodies: rigid motion for the wall velocity, Body.H:709-721              */
+  int32_t nbody;
+  const int32_t *body_case;    /* [nbody]     */
+  const double  *body_center;  /* [nbody][3]  bodycenter                     */
+} ibg_mesh_soa;
+
+/* Nondimensional constants of one case, FlowConstants.cpp:5-44 and the case
+ * globals of Userfunction_cy_naca15.cpp:18-65, 356-425.                    */
+typedef struct ibg_physics {
+  double gama, Ma, Re, Pr, Rg, Cv, Cp, mu, qk, S_over_T_ref;
+  double initvar[IBG_NFV];     /* freestream FlowVariables                    */
+  double inlet_T0, inlet_p0;   /* facevar[0].T / .p  (total state at inlet)   */
+  double inlet_angle;          /* inlet_flow_angle                            */
+  double outlet_p;             /* facevar[1].p + backpressure                 */
+  double cfl, dt_ratio;        /* CFL, dt_ratio                               */
+  double dt_init;              /* dt before the first Comptdt                 */
+  double force_ref;            /* F_ref for C_L/C_D                           */
+  double moment_ref;           /* M_ref                                       */
+  double p_ref_area, shear_ref_area;  /* p_ref*area_ref, shear_ref*area_ref   */
+} ibg_physics;
+
+/* Rigid body motion of one body at the current step (Body.H:709-721).     */
+typedef struct ibg_body_motion {
+  double tra_vel[3];           /* free_tra_vel + force_tra_vel + frame_trans  */
+  double rot_vel[3];           /* force_rot_vel + free_rot_vel (about center) */
+} ibg_body_motion;
+
+typedef struct ibg_config {
+  int32_t mode;                /* ibg_mode                                    */
+  int32_t nstage;              /* sub_time_step: 1, 3 or 4 (NS_Solver.H:200)  */
+  double  alpha[4];            /* tp[] stage coefficients (NS_Solver.H:202-231)*/
+  int32_t dt_var_step;         /* Comptdt period (NS_Solver.H:661)            */
+  int32_t temporal_refine;     /* TEMPORAL_REFINE schedule (mpiparams.cpp:186)*/
+  int32_t limiter;             /* 0 van Leer (Slopelimiter.H:17-20), 1 van Albada */
+  int32_t viscous;             /* VISCOUSITY (NS_Solver.H:15)                 */
+  int32_t tile;                /* cells per stage-kernel tile (0 = default)   */
+} ibg_config;
+
+typedef struct ibg_ctx ibg_ctx;
+
+/* Library / context lifetime.  Replaces the NS_Solver constructor
+ * NS_Solver(Mesh&, Domain&, int) NS_Solver.H:195-237 (stage table, solver
+ * containers) and AMR::AllocateDataMem AMR.H:116-124.                      */
+ibg_status ibg_create(const ibg_config *cfg, ibg_ctx **out);
+ibg_status ibg_destroy(ibg_ctx *ctx);
+const char *ibg_version(void);
+
+/* Check every index the kernels will dereference (neighbour, stencil, pair,
+ * image-point and patch lists) without touching the GPU.  The reference
+ * asserts the same facts under -D DEBUG (NS_Solver.H:741, NS_Solver.cpp:421,
+ * TwoLevelOp.H:188, Mesh.cpp:1539).  ibg_upload_mesh runs it first.        */
+ibg_status ibg_check_mesh(const ibg_mesh_soa *mesh, const char **msg);
+
+/* Upload the flattened mesh (after construction and after every regrid).
+ * Replaces CheckDataContainer NS_Solver.H:549-569 (container sizing) and the
+ * pair lists consumed by AMR::InterfaceExchange AMRInterface.cpp:3-110.     */
+ibg_status ibg_upload_mesh(ibg_ctx *ctx, const ibg_mesh_soa *mesh,
+                           const ibg_physics *phys /* [ncases] */);
+
+/* State in/out as AoS FlowVariables [ncells][7].  Replaces direct access to
+ * Meshlevel::m_data Meshlevel.H:233 (and InitFlow NS_Solver.H:432-446).    */
+ibg_status ibg_set_state(ibg_ctx *ctx, const double *fv_aos, int64_t ncells);
+ibg_status ibg_get_state(ibg_ctx *ctx, double *fv_aos, int64_t ncells);
+
+/* Per-step ghost fills.
+ * BCValues::BCBoxTreat(Mesh&) BCValues.H:51-188                            */
+ibg_status ibg_bc_fill(ibg_ctx *ctx, int32_t ts);
+/* AMR::InterfaceExchange() AMRInterface.cpp:3-35 (restriction, periodic
+ * copy, prolongation on their sub-cycling schedules)                       */
+ibg_status ibg_interface_exchange(ibg_ctx *ctx, int32_t ts);
+/* Body::IBConditions_Image(Mesh&, vector<Body>&) Body.cpp:1347-1504.
+ * motion: [nbody] or NULL for stationary bodies.                           */
+ibg_status ibg_ib_fill(ibg_ctx *ctx, const ibg_body_motion *motion);
+
+/* Solver phases (NS_Solver public methods).
+ * NS_Solver::Comptdt() NS_Solver.cpp:196-280; dt_out [ncases] (may be NULL) */
+ibg_status ibg_compute_dt(ibg_ctx *ctx, int32_t ts, double *dt_out);
+/* NS_Solver::ComptOldConsVar() NS_Solver.H:392-424                         */
+ibg_status ibg_old_cons(ibg_ctx *ctx, int32_t ts);
+/* NS_Solver::ComptViscousStress() NS_Solver.cpp:365-629 (frozen per step) */
+ibg_status ibg_viscous_flux(ibg_ctx *ctx, int32_t ts);
+/* One stage: ComputeFaceFlux NS_Solver.H:704-832 + ComptSource
+ * NS_Solver.cpp:282-363 + UpdateConsVars NS_Solver.cpp:844-985 +
+ * ComptNewPrimVar NS_Solver.cpp:3-194 (+ periodic copy NS_Solver.H:698).   */
+ibg_status ibg_stage(ibg_ctx *ctx, int32_t ts, int32_t subt);
+/* NS_Solver::TimeMarching(AMR&) NS_Solver.H:657-702 (dt, U_old, BC, viscous,
+ * all stages, residual) for nsteps steps starting at ts0, each preceded by
+ * InterfaceExchange and IBConditions_Image as in main.cpp:84-99.  When
+ * use_graph is nonzero the per-step launch sequence is captured in a HIP
+ * graph and replayed.                                                      */
+ibg_status ibg_time_march(ibg_ctx *ctx, int32_t ts0, int32_t nsteps,
+                          int32_t use_graph);
+/* Residual of the last stage.  NS_Solver::ResinDomain() NS_Solver.H:264-378:
+ * linf[case*5+v] = max |source[0][v]| over the marched cells (unnormalised),
+ * l2[case*5+v]   = sqrt(sum source[0][v]^2) over the same cells.            */
+ibg_status ibg_residual(ibg_ctx *ctx, double *linf, double *l2);
+/* Body::ComptWallForce(Mesh&) Body.cpp:439-558, summed over the bodies of
+ * each case (Bodymove.cpp:49-55): out[case*6 + {Fx,Fy,Fz,Mx,My,Mz}] in the
+ * reference's dimensional units (force, moment about body 0 center).       */
+ibg_status ibg_forces(ibg_ctx *ctx, const ibg_body_motion *motion,
+                      double *out);
+
+/* Parity probes: copy device-side intermediate arrays to the host.
+ * faceflux / vsflux [nfaces][5]: NS_Solver::faceflux / vsflux NS_Solver.H:850-851
+ * source [ncells][5] (zero for non-marched / ghost cells): source[0]       */
+ibg_status ibg_get_faceflux(ibg_ctx *ctx, double *out, int64_t nfaces);
+ibg_status ibg_get_vsflux(ibg_ctx *ctx, double *out, int64_t nfaces);
+ibg_status ibg_get_source(ibg_ctx *ctx, double *out, int64_t ncells);
+ibg_status ibg_get_dt(ibg_ctx *ctx, double *dt /* [ncases] */);
+ibg_status ibg_set_dt(ibg_ctx *ctx, const double *dt /* [ncases] */);
+
+/* First offending cell of the last numerical abort (global cell index), or
+ * -1.  Replaces the printf + MPI_Abort diagnostics of NS_Solver.cpp:55-66,
+ * 888-894.                                                                  */
+ibg_status ibg_last_error(ibg_ctx *ctx, int64_t *cell, int32_t *code);
+
+/* Wait for all work of the context.                                        */
+ibg_status ibg_synchronize(ibg_ctx *ctx);
+
+/* Time (ms, HIP events on the context stream) of the last ibg_time_march:
+ * whole region and the per-stage kernel's average duration.                */
+ibg_status ibg_last_timing(ibg_ctx *ctx, double *total_ms,
+                           double *stage_kernel_avg_ms, int64_t *stage_launches);
+
+/* Counters for the metric: normal cells updated per step, summed over the
+ * steps of the last ibg_time_march, times the stage count (the cell-stage
+ * update count of NS_Solver.cpp:868-872, 948-953).                        */
+ibg_status ibg_cell_stage_updates(ibg_ctx *ctx, int64_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IBGPU_H_ */

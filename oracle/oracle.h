@@ -1,0 +1,86 @@
+/*
+ * oracle.h — FP64 CPU restatement of the reference's per-step residual path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP engine:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * it.  The product (libibgpu.so and its host code) never links or calls it.
+ *
+ * State is the reference's AoS FlowVariables (roe,u,v,w,p,T,e; 7 doubles per
+ * cell, FlowVariables.H:320-326) over the flattened mesh of ibgpu.h.  Every
+ * function follows the reference's arithmetic in the same operation order;
+ * each cites the lines it restates.
+ */
+#ifndef ORACLE_H_
+#define ORACLE_H_
+
+#include <stdint.h>
+#include "ibgpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_cfg {
+  int32_t nstage;            /* sub_time_step  NS_Solver.H:200            */
+  double alpha[4];           /* tp[]           NS_Solver.H:202-231        */
+  int32_t dt_var_step;       /* NS_Solver.H:661                            */
+  int32_t temporal_refine;   /* TEMPORAL_REFINE                            */
+  int32_t limiter;           /* 0 van Leer, 1 van Albada                   */
+  int32_t viscous;           /* VISCOUSITY                                 */
+} orc_cfg;
+
+/* schedule (mpiparams.cpp:186-212) */
+int32_t orc_marched(int32_t L, int32_t level, int32_t ts, int32_t temporal_refine);
+int32_t orc_transfer_due(int32_t L, int32_t coarse_level, int32_t ts, int32_t temporal_refine);
+double orc_step_num(int32_t L, int32_t level, int32_t temporal_refine);
+
+/* limiter on one variable (Slopelimiter.H:22-58 + NS_Solver.H:21-27)     */
+double orc_muscl_left(double q_i, double qi, double qi1, int32_t limiter);
+double orc_muscl_right(double q_i, double qi, double qi1, int32_t limiter);
+/* inviscid face flux from a 4-cell stencil of FlowVariables (7 doubles)
+   NS_Solver.H:51-113, 766-794                                            */
+void orc_roe_flux(const double *fLL, const double *fL, const double *fR,
+                  const double *fRR, int32_t dir, double area,
+                  const ibg_physics *ph, int32_t limiter, double *flux5);
+
+/* per-phase functions over the whole mesh; only marched blocks are touched */
+void orc_face_flux(const ibg_mesh_soa *m, const ibg_physics *ph,
+                   const orc_cfg *cfg, const double *fv, int32_t ts,
+                   double *faceflux);
+void orc_viscous_flux(const ibg_mesh_soa *m, const ibg_physics *ph,
+                      const orc_cfg *cfg, const double *fv, int32_t ts,
+                      double *vsflux);
+void orc_source(const ibg_mesh_soa *m, const orc_cfg *cfg,
+                const double *faceflux, const double *vsflux, int32_t ts,
+                double *source);
+/* U_old <- cons(W) (NS_Solver.H:392-424) */
+void orc_old_cons(const ibg_mesh_soa *m, const orc_cfg *cfg, const double *fv,
+                  int32_t ts, double *cv_old);
+/* UpdateConsVars + ComptNewPrimVar; returns 0 or the abort code; *bad_cell */
+int32_t orc_update(const ibg_mesh_soa *m, const ibg_physics *ph,
+                   const orc_cfg *cfg, const double *cv_old,
+                   const double *source, const double *dt, int32_t ts,
+                   int32_t subt, double *cv_new, double *fv, int64_t *bad_cell);
+int32_t orc_bc_fill(const ibg_mesh_soa *m, const ibg_physics *ph,
+                    const orc_cfg *cfg, double *fv, int32_t ts);
+void orc_interface_exchange(const ibg_mesh_soa *m, const ibg_physics *ph,
+                            const orc_cfg *cfg, double *fv, int32_t ts);
+int32_t orc_ib_fill(const ibg_mesh_soa *m, const ibg_physics *ph, double *fv,
+                    const ibg_body_motion *motion);
+int32_t orc_compute_dt(const ibg_mesh_soa *m, const ibg_physics *ph,
+                       const double *fv, double *dt /* [ncases] in/out */);
+void orc_residual(const ibg_mesh_soa *m, const orc_cfg *cfg,
+                  const double *source, int32_t ts, double *linf, double *l2);
+void orc_forces(const ibg_mesh_soa *m, const ibg_physics *ph, const double *fv,
+                const ibg_body_motion *motion, double *out /* [ncases*6] */);
+/* One full step of main.cpp:84-99: InterfaceExchange, IBConditions_Image,
+   TimeMarching.  fv, dt updated in place; source (may be NULL) receives the
+   last stage's source[0]; linf/l2 (may be NULL) the residual.             */
+int32_t orc_step(const ibg_mesh_soa *m, const ibg_physics *ph,
+                 const orc_cfg *cfg, double *fv, double *dt, int32_t ts,
+                 double *source, double *linf, double *l2, int64_t *bad_cell);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,140 @@
+"""ctypes binding of oracle/_build/liboracle.so (the FP64 CPU restatement).
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "immsered-boundary_amd")
+if PKG not in sys.path:
+    sys.path.insert(0, PKG)
+
+from ibgpu import _ffi  # noqa: E402
+
+f64p, i32p = _ffi.f64p, _ffi.i32p
+
+
+class OrcCfg(C.Structure):
+    _fields_ = [("nstage", C.c_int32), ("alpha", C.c_double * 4),
+                ("dt_var_step", C.c_int32), ("temporal_refine", C.c_int32),
+                ("limiter", C.c_int32), ("viscous", C.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+    if not os.path.exists(path):
+        raise OSError(f"{path} missing: run `python __graft_entry__.py build`")
+    L = C.CDLL(path)
+    M = C.POINTER(_ffi.MeshSoA)
+    P = C.POINTER(_ffi.Physics)
+    G = C.POINTER(OrcCfg)
+    BM = C.POINTER(_ffi.BodyMotion)
+    sig = {
+        "orc_marched": (C.c_int32, [C.c_int32] * 4),
+        "orc_transfer_due": (C.c_int32, [C.c_int32] * 4),
+        "orc_step_num": (C.c_double, [C.c_int32] * 3),
+        "orc_muscl_left": (C.c_double, [C.c_double] * 3 + [C.c_int32]),
+        "orc_muscl_right": (C.c_double, [C.c_double] * 3 + [C.c_int32]),
+        "orc_roe_flux": (None, [f64p, f64p, f64p, f64p, C.c_int32, C.c_double, P, C.c_int32, f64p]),
+        "orc_face_flux": (None, [M, P, G, f64p, C.c_int32, f64p]),
+        "orc_viscous_flux": (None, [M, P, G, f64p, C.c_int32, f64p]),
+        "orc_source": (None, [M, G, f64p, f64p, C.c_int32, f64p]),
+        "orc_old_cons": (None, [M, G, f64p, C.c_int32, f64p]),
+        "orc_update": (C.c_int32, [M, P, G, f64p, f64p, f64p, C.c_int32, C.c_int32, f64p, f64p,
+                                   C.POINTER(C.c_int64)]),
+        "orc_bc_fill": (C.c_int32, [M, P, G, f64p, C.c_int32]),
+        "orc_interface_exchange": (None, [M, P, G, f64p, C.c_int32]),
+        "orc_ib_fill": (C.c_int32, [M, P, f64p, BM]),
+        "orc_compute_dt": (C.c_int32, [M, P, f64p, f64p]),
+        "orc_residual": (None, [M, G, f64p, C.c_int32, f64p, f64p]),
+        "orc_forces": (None, [M, P, f64p, BM, f64p]),
+        "orc_step": (C.c_int32, [M, P, G, f64p, f64p, C.c_int32, f64p, f64p, f64p,
+                                 C.POINTER(C.c_int64)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def ptr(a):
+    return a.ctypes.data_as(f64p)
+
+
+def cfg(nstage=3, alpha=None, dt_var_step=1, temporal_refine=1, limiter=0, viscous=1):
+    c = OrcCfg()
+    c.nstage = nstage
+    if alpha is None:
+        alpha = {1: [1.0], 3: [0.1918, 0.4929, 1.0], 4: [0.1084, 0.2602, 0.5052, 1.0]}[nstage]
+    for i, a in enumerate(alpha):
+        c.alpha[i] = a
+    c.dt_var_step = dt_var_step
+    c.temporal_refine = temporal_refine
+    c.limiter = limiter
+    c.viscous = viscous
+    return c
+
+
+class Oracle:
+    """Whole-mesh oracle calls on a Mesh (ibgpu.mesh.Mesh)."""
+
+    def __init__(self, mesh, config=None):
+        self.m = mesh
+        self.L = lib()
+        # Comptdt period: marching_step[0] of the level schedule, as the engine
+        # defaults to (mpiparams.cpp:208-210)
+        self.cfg = config or cfg(dt_var_step=1 << (mesh.nlevels - 1))
+
+    def step(self, fv, dt, ts):
+        src = np.zeros((self.m.ncells, 5))
+        linf = np.zeros(self.m.ncases * 5)
+        l2 = np.zeros(self.m.ncases * 5)
+        bad = C.c_int64(-1)
+        code = self.L.orc_step(self.m.soa_ptr, self.m.phys, C.byref(self.cfg), ptr(fv), ptr(dt),
+                               ts, ptr(src), ptr(linf), ptr(l2), C.byref(bad))
+        return code, src, linf, l2
+
+    def face_flux(self, fv, ts):
+        out = np.zeros((self.m.nfaces, 5))
+        self.L.orc_face_flux(self.m.soa_ptr, self.m.phys, C.byref(self.cfg), ptr(fv), ts, ptr(out))
+        return out
+
+    def viscous_flux(self, fv, ts):
+        out = np.zeros((self.m.nfaces, 5))
+        self.L.orc_viscous_flux(self.m.soa_ptr, self.m.phys, C.byref(self.cfg), ptr(fv), ts, ptr(out))
+        return out
+
+    def source(self, ff, vs, ts):
+        out = np.zeros((self.m.ncells, 5))
+        self.L.orc_source(self.m.soa_ptr, C.byref(self.cfg), ptr(ff), ptr(vs), ts, ptr(out))
+        return out
+
+    def bc_fill(self, fv, ts):
+        return self.L.orc_bc_fill(self.m.soa_ptr, self.m.phys, C.byref(self.cfg), ptr(fv), ts)
+
+    def interface_exchange(self, fv, ts):
+        self.L.orc_interface_exchange(self.m.soa_ptr, self.m.phys, C.byref(self.cfg), ptr(fv), ts)
+
+    def ib_fill(self, fv, motion=None):
+        return self.L.orc_ib_fill(self.m.soa_ptr, self.m.phys, ptr(fv), motion)
+
+    def compute_dt(self, fv, dt):
+        return self.L.orc_compute_dt(self.m.soa_ptr, self.m.phys, ptr(fv), ptr(dt))
+
+    def forces(self, fv, motion=None):
+        out = np.zeros(self.m.ncases * 6)
+        self.L.orc_forces(self.m.soa_ptr, self.m.phys, ptr(fv), motion, ptr(out))
+        return out
