@@ -68,7 +68,7 @@ def build_oracle():
     bdir = os.path.join(ROOT, "oracle", "_build")
     os.makedirs(bdir, exist_ok=True)
     out = os.path.join(bdir, "liboracle.so")
-    _run(["gcc", "-O2", "-std=c99", "-fPIC", "-shared", f"-I{INC}",
+    _run(["gcc", "-O2", "-std=c99", "-fPIC", "-shared", "-fopenmp", f"-I{INC}",
           os.path.join(ROOT, "oracle", "oracle.c"), "-o", out + ".tmp", "-lm"])
     os.replace(out + ".tmp", out)
 

@@ -1,0 +1,55 @@
+// ibg_common.h — device-side types of the residual engine.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ibgpu.h"
+
+namespace ibg {
+
+struct __align__(16) F4 { float x, y, z, w; };
+struct __align__(32) D4 { double x, y, z, w; };
+
+template <typename R> struct V4;
+template <> struct V4<float> { using T = F4; };
+template <> struct V4<double> { using T = D4; };
+
+// Packed per-cell record of NV = D+2 values: 2D -> one 4-vector;
+// 3D -> a 4-vector (first four values) plus a scalar plane (fifth value).
+// One stencil cell is one 16-byte (FP32) or two 16-byte (FP64) loads.
+template <typename R, int D>
+struct Pack {
+  typename V4<R>::T *a;
+  R *e;   // 3D only
+};
+
+enum CellFlag : uint8_t {
+  F_UPDATE = 1,   // UpdateConsVars touches the cell
+  F_PRIM = 2,     // ComptNewPrimVar recovers W from U
+  F_FREE = 4,     // ComptNewPrimVar resets W to initvar
+  F_RESID = 8,    // counted by ResinDomain
+  F_FINEST = 16   // finest level with bodies (abort code 611 vs 523)
+};
+
+struct DevPhys {
+  double gama, Ma, Re, Rg, Cv, Cp, mu, qk, SoT;
+  double init[IBG_NFV];
+  double inlet_T0, inlet_p0, inlet_cos, inlet_sin, outlet_p;
+  double cfl, dt_ratio;
+  double force_ref, p_ref_area, shear_ref_area;
+};
+
+struct BlkInfo {
+  double h[3], invh[3], area[3], vol, step_num;
+  int case_id, level, finest, pad;
+};
+
+struct TileInfo {
+  int cell0, ncell, face0, nface, blk, pad0, pad1, pad2;
+};
+
+struct Motion {
+  double tra[3], rot[3], center[3];
+};
+
+}  // namespace ibg

@@ -169,4 +169,85 @@ __device__ __forceinline__ void roe_flux(const Prim<R, D> &t0, const Prim<R, D> 
   }
 }
 
+// Viscous face flux of ComptViscousStress (NS_Solver.cpp:426-581, laminar).
+// Stencil of the face between L (low side, the reference's abox) and Rr:
+// t[0..3] = L+t1, Rr+t1, L-t1, Rr-t1 and, in 3D, t[4..7] the same for t2,
+// where t1 < t2 are the two tangential axes.  Gradients: normal difference
+// (pfpkesai0) and the four-point tangential difference (pfpkesai1), times
+// the cell metric 1/dh (VarReflect with keisa = diag(1/dh)); Sutherland mu
+// (:531-533); tau and heat flux (:544-557); output A*[0, tau.n, (u.tau+q).n].
+template <typename R, int D>
+struct ViscPhys {
+  R Rg, Re, qk, SoT;
+  R invh[3];
+  R area[3];
+};
+
+template <typename R, int D>
+__device__ __forceinline__ void visc_flux(const Prim<R, D> &L, const Prim<R, D> &Rr,
+                                          const Prim<R, D> *t, int d, const ViscPhys<R, D> &vp,
+                                          R *out) {
+  auto T_of = [&](const Prim<R, D> &q) -> R { return q.p / (q.r * vp.Rg); };
+  const R TL = T_of(L), TR = T_of(Rr);
+  // g[q][k]: d(u,v,w,T)/d(kesai_k)
+  R g[4][3];
+  R Tt[8];
+  const int nt = (D == 3) ? 8 : 4;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) Tt[k] = (k < nt) ? T_of(t[k]) : R(0);
+  auto val = [&](const Prim<R, D> &q, int var, R T) -> R {
+    return var == 0 ? q.u : (var == 1 ? q.v : (var == 2 ? q.w : T));
+  };
+  const int t1 = (d == 0) ? 1 : 0;
+  const int t2 = (d == 2) ? 1 : 2;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (D == 2 && q == 2) { g[q][0] = g[q][1] = g[q][2] = R(0); continue; }
+    g[q][d] = val(Rr, q, TR) - val(L, q, TL);
+    g[q][t1] = R(0.25) * (val(t[0], q, Tt[0]) + val(t[1], q, Tt[1]) - val(t[2], q, Tt[2]) - val(t[3], q, Tt[3]));
+    if (D == 3)
+      g[q][t2] = R(0.25) * (val(t[4], q, Tt[4]) + val(t[5], q, Tt[5]) - val(t[6], q, Tt[6]) - val(t[7], q, Tt[7]));
+    else
+      g[q][2] = R(0);
+  }
+  const R dux = g[0][0] * vp.invh[0], duy = g[0][1] * vp.invh[1], duz = g[0][2] * vp.invh[2];
+  const R dvx = g[1][0] * vp.invh[0], dvy = g[1][1] * vp.invh[1], dvz = g[1][2] * vp.invh[2];
+  const R dwx = g[2][0] * vp.invh[0], dwy = g[2][1] * vp.invh[1], dwz = g[2][2] * vp.invh[2];
+  const R dTx = g[3][0] * vp.invh[0], dTy = g[3][1] * vp.invh[1], dTz = g[3][2] * vp.invh[2];
+  const R ftemp = (TL + TR) * R(0.5);
+  const R mu0 = r_sqrt(ftemp * ftemp * ftemp) * ((R(1.0) + vp.SoT) / (ftemp + vp.SoT));
+  const R qk0 = vp.qk * mu0;
+  const R Re = vp.Re;
+  const R divu = (D == 3) ? (dux + dvy + dwz) : (dux + dvy);
+  const R tauxx = R(2.0) * mu0 * (dux - divu / R(3.0)) / Re;
+  const R tauyy = R(2.0) * mu0 * (dvy - divu / R(3.0)) / Re;
+  const R tauxy = mu0 / Re * (duy + dvx);
+  const R fu = (L.u + Rr.u) * R(0.5), fv = (L.v + Rr.v) * R(0.5);
+  const R area = vp.area[d];
+  if (D == 2) {
+    if (d == 0) {
+      const R bx = fu * tauxx + fv * tauxy + qk0 * dTx;
+      out[0] = R(0); out[1] = tauxx * area; out[2] = tauxy * area; out[3] = bx * area;
+    } else {
+      const R by = fu * tauxy + fv * tauyy + qk0 * dTy;
+      out[0] = R(0); out[1] = tauxy * area; out[2] = tauyy * area; out[3] = by * area;
+    }
+  } else {
+    const R tauzz = R(2.0) * mu0 * (dwz - divu / R(3.0)) / Re;
+    const R tauxz = mu0 / Re * (duz + dwx);
+    const R tauyz = mu0 / Re * (dvz + dwy);
+    const R fw = (L.w + Rr.w) * R(0.5);
+    if (d == 0) {
+      const R bx = fu * tauxx + fv * tauxy + fw * tauxz + qk0 * dTx;
+      out[0] = R(0); out[1] = tauxx * area; out[2] = tauxy * area; out[3] = tauxz * area; out[4] = bx * area;
+    } else if (d == 1) {
+      const R by = fu * tauxy + fv * tauyy + fw * tauyz + qk0 * dTy;
+      out[0] = R(0); out[1] = tauxy * area; out[2] = tauyy * area; out[3] = tauyz * area; out[4] = by * area;
+    } else {
+      const R bz = fu * tauxz + fv * tauyz + fw * tauzz + qk0 * dTz;
+      out[0] = R(0); out[1] = tauxz * area; out[2] = tauyz * area; out[3] = tauzz * area; out[4] = bz * area;
+    }
+  }
+}
+
 }  // namespace ibg

@@ -1,0 +1,674 @@
+// ibg_kernels.h — HIP kernels of the residual engine (gfx950).
+//
+// Conventions: R is the arithmetic type of the flux kernels (double in the
+// plain mode, float in the mixed mode); D is 2 or 3; NV = D+2 values per
+// cell.  Every ghost fill, the CFL time step, the wall force and the
+// conserved state are FP64 in both modes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ibg_common.h"
+#include "ibg_flux.h"
+
+namespace ibg {
+
+constexpr int kTile = 256;   // cells per workgroup of the tiled kernels (max)
+
+// ------------------------------------------------------------ pack access
+template <typename R, int D>
+__device__ __forceinline__ Prim<R, D> pload(const Pack<R, D> &W, int c) {
+  const typename V4<R>::T q = W.a[c];
+  Prim<R, D> o;
+  o.r = q.x; o.u = q.y; o.v = q.z;
+  if (D == 2) { o.w = R(0); o.p = q.w; }
+  else { o.w = q.w; o.p = W.e[c]; }
+  return o;
+}
+
+template <typename R, int D>
+__device__ __forceinline__ void pstore(const Pack<R, D> &W, int c, R r, R u, R v, R w, R p) {
+  typename V4<R>::T q;
+  q.x = r; q.y = u; q.z = v;
+  if (D == 2) { q.w = p; }
+  else { q.w = w; W.e[c] = p; }
+  W.a[c] = q;
+}
+
+// NV-value records (conserved state, residuals): values 0..3 in a, 4 in e
+template <typename R, int D>
+__device__ __forceinline__ void vload(const Pack<R, D> &P, int c, R *o) {
+  const typename V4<R>::T q = P.a[c];
+  o[0] = q.x; o[1] = q.y; o[2] = q.z; o[3] = q.w;
+  if (D == 3) o[4] = P.e[c];
+}
+
+template <typename R, int D>
+__device__ __forceinline__ void vstore(const Pack<R, D> &P, int c, const R *v) {
+  typename V4<R>::T q;
+  q.x = v[0]; q.y = v[1]; q.z = v[2]; q.w = v[3];
+  P.a[c] = q;
+  if (D == 3) P.e[c] = v[4];
+}
+
+// FP64 master write, mirrored to the FP32 shadow in mixed mode
+template <int D>
+__device__ __forceinline__ void wstore64(const Pack<double, D> &W64, const Pack<float, D> &W32,
+                                         bool mixed, int c, double r, double u, double v, double w,
+                                         double p) {
+  pstore<double, D>(W64, c, r, u, v, w, p);
+  if (mixed) pstore<float, D>(W32, c, (float)r, (float)u, (float)v, (float)w, (float)p);
+}
+
+// error word: ((cell << 16) | code), the smallest cell wins
+__device__ __forceinline__ void report_error(unsigned long long *err, int cell, int code) {
+  unsigned long long v = ((unsigned long long)(unsigned)cell << 16) | (unsigned)code;
+  atomicMin(err, v);
+}
+
+// ------------------------------------------------------------ stage kernel
+template <typename R, int D>
+struct StageArgs {
+  const TileInfo *tiles;
+  const int4 *tst;           // tile face stencil LL,L,R,RR
+  const int8_t *tdir;        // tile face direction
+  const uint16_t *slots;     // [Nn][2D] tile-local face slot of each cell face
+  const uint8_t *flags;      // [Nn]
+  const BlkInfo *blk;
+  const DevPhys *phys;
+  const double *dt;          // [ncases]
+  Pack<R, D> W;              // stencil state (FP64 master or FP32 shadow)
+  Pack<double, D> W64;       // master, written every stage in the plain mode
+  Pack<double, D> U0;        // conserved state at the start of the step
+  Pack<R, D> Rv;             // frozen viscous residual
+  double alpha;
+  int limiter;
+  int last;                  // last stage: residual partials (+ W64 in mixed)
+  int mixed;
+  double *rpart;             // [ntiles][10]
+  double *src;               // optional probe [N][5] written at the last stage
+  unsigned long long *err;
+};
+
+// One workgroup per tile of Morton-consecutive normal cells of one
+// (case, level).  Phase 1: every face touching the tile — MUSCL + all-speed
+// Roe (NS_Solver.H:704-832) — into LDS.  Phase 2: ComptSource
+// (NS_Solver.cpp:282-363), UpdateConsVars (:844-985) and ComptNewPrimVar
+// (:3-194) for the tile's cells.
+template <typename R, int D>
+__global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
+  constexpr int NV = D + 2;
+  using V = typename V4<R>::T;
+  extern __shared__ __align__(32) unsigned char smem_raw[];
+  const TileInfo ti = a.tiles[blockIdx.x];
+  V *fl4 = reinterpret_cast<V *>(smem_raw);
+  R *fl1 = reinterpret_cast<R *>(smem_raw + sizeof(V) * (size_t)ti.nface);   // 3D 5th value
+  __shared__ double red[2 * 5][kTile / 64];
+
+  const BlkInfo &bi = a.blk[ti.blk];
+  const DevPhys &P = a.phys[bi.case_id];
+  PhysR<R> ph;
+  ph.gama = (R)P.gama; ph.Rg = (R)P.Rg; ph.Cv = (R)P.Cv; ph.Ma = (R)P.Ma;
+  const R areas[3] = {(R)bi.area[0], (R)bi.area[1], (R)bi.area[2]};
+
+  for (int f = threadIdx.x; f < ti.nface; f += blockDim.x) {
+    const int4 st = a.tst[ti.face0 + f];
+    const int dir = a.tdir[ti.face0 + f];
+    const Prim<R, D> q0 = pload<R, D>(a.W, st.x);
+    const Prim<R, D> q1 = pload<R, D>(a.W, st.y);
+    const Prim<R, D> q2 = pload<R, D>(a.W, st.z);
+    const Prim<R, D> q3 = pload<R, D>(a.W, st.w);
+    R flux[NV];
+    roe_flux<R, D>(q0, q1, q2, q3, dir, areas[dir], ph, a.limiter, flux);
+    V o;
+    o.x = flux[0]; o.y = flux[1]; o.z = flux[2]; o.w = flux[3];
+    fl4[f] = o;
+    if (D == 3) fl1[f] = flux[4];
+  }
+  __syncthreads();
+
+  const double level_ts = a.alpha * a.dt[bi.case_id] * bi.step_num;
+  double mx[NV], sq[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) { mx[k] = 0.0; sq[k] = 0.0; }
+  const int i = threadIdx.x;
+  if (i < ti.ncell) {
+    const int c = ti.cell0 + i;
+    const uint8_t fg = a.flags[c];
+    int sl[2 * D];
+    if (D == 2) {
+      const ushort4 s4 = *reinterpret_cast<const ushort4 *>(a.slots + (size_t)c * 4);
+      sl[0] = s4.x; sl[1] = s4.y; sl[2] = s4.z; sl[3] = s4.w;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 2 * D; ++q) sl[q] = a.slots[(size_t)c * 2 * D + q];
+    }
+    // ComptSource: per direction (F_lo - F_hi), plus the frozen viscous
+    // residual sum_d (Fv_hi - Fv_lo)
+    double s[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) s[k] = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const V lo = fl4[sl[2 * d]], hi = fl4[sl[2 * d + 1]];
+      s[0] += (double)lo.x - (double)hi.x;
+      s[1] += (double)lo.y - (double)hi.y;
+      s[2] += (double)lo.z - (double)hi.z;
+      s[3] += (double)lo.w - (double)hi.w;
+      if (D == 3) s[4] += (double)fl1[sl[2 * d]] - (double)fl1[sl[2 * d + 1]];
+    }
+    {
+      R rv[NV];
+      vload<R, D>(a.Rv, c, rv);
+#pragma unroll
+      for (int k = 0; k < NV; ++k) s[k] += (double)rv[k];
+    }
+    bool write = false;
+    double wr = 0, wu = 0, wv = 0, ww = 0, wp = 0;
+    if (fg & F_UPDATE) {
+      double U[NV];
+      vload<double, D>(a.U0, c, U);
+#pragma unroll
+      for (int k = 0; k < NV; ++k) U[k] = U[k] + s[k] * level_ts / bi.vol;
+      if (U[0] < 0.0) report_error(a.err, c, (fg & F_FINEST) ? 611 : 523);
+      if (fg & F_PRIM) {
+        // Get_Prim_Vars FlowVariables.H:50-57
+        wr = U[0];
+        wu = U[1] / U[0];
+        wv = U[2] / U[0];
+        double e;
+        if (D == 3) { ww = U[3] / U[0]; e = U[4] / U[0]; }
+        else e = U[3] / U[0];
+        const double ke = (D == 3) ? 0.5 * (wu * wu + wv * wv + ww * ww) : 0.5 * (wu * wu + wv * wv);
+        wp = (e - ke) * (P.gama - 1.0) * wr;
+        if (wp / (wr * P.Rg) < 0.0) report_error(a.err, c, 37);
+        write = true;
+      }
+    }
+    if (!(fg & F_PRIM) && (fg & F_FREE)) {
+      wr = P.init[0]; wu = P.init[1]; wv = P.init[2]; ww = P.init[3]; wp = P.init[4];
+      write = true;
+    }
+    if (write) {
+      if (!a.mixed || a.last) pstore<double, D>(a.W64, c, wr, wu, wv, ww, wp);
+      if (a.mixed) pstore<R, D>(a.W, c, (R)wr, (R)wu, (R)wv, (R)ww, (R)wp);
+    }
+    if (a.last) {
+      if (a.src) {
+        double *o = a.src + (size_t)c * 5;
+        o[0] = s[0]; o[1] = s[1]; o[2] = s[2];
+        if (D == 3) { o[3] = s[3]; o[4] = s[4]; } else { o[3] = 0.0; o[4] = s[3]; }
+      }
+      if (fg & F_RESID) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) { mx[k] = fabs(s[k]); sq[k] = s[k] * s[k]; }
+      }
+    }
+  }
+  if (a.last) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      for (int off = 32; off > 0; off >>= 1) {
+        mx[k] = fmax(mx[k], __shfl_xor(mx[k], off, 64));
+        sq[k] += __shfl_xor(sq[k], off, 64);
+      }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) { red[k][wid] = mx[k]; red[5 + k][wid] = sq[k]; }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int nw = (blockDim.x + 63) >> 6;
+      double *o = a.rpart + (size_t)blockIdx.x * 10;
+      for (int k = 0; k < 10; ++k) o[k] = 0.0;
+      for (int k = 0; k < NV; ++k) {
+        double m = 0.0, q = 0.0;
+        for (int w = 0; w < nw; ++w) { m = fmax(m, red[k][w]); q += red[5 + k][w]; }
+        const int kk = (D == 3) ? k : (k == 3 ? 4 : k);
+        o[kk] = m;
+        o[5 + kk] = q;
+      }
+    }
+  }
+}
+
+// ----------------------------------------- viscous flux + step preparation
+template <typename R, int D>
+struct ViscArgs {
+  const TileInfo *tiles;
+  const int4 *tst;           // L, R from .y, .z
+  const int4 *tvs;           // [nface][D-1] tangential stencil
+  const int8_t *tdir;
+  const uint16_t *slots;
+  const BlkInfo *blk;
+  const DevPhys *phys;
+  Pack<R, D> W;              // state the viscous flux reads
+  Pack<double, D> W64;
+  Pack<double, D> U0;
+  Pack<R, D> Rv;
+  int viscous;
+};
+
+// Once per step (ComptViscousStress NS_Solver.cpp:365-629 is frozen across
+// the stages, NS_Solver.H:669-699): viscous fluxes of the tile faces into
+// LDS, then per cell U_old = cons(W) (ComptOldConsVar NS_Solver.H:392-424)
+// and Rv = sum_d (Fv_hi - Fv_lo).
+template <typename R, int D>
+__global__ void __launch_bounds__(kTile) k_visc_prep(ViscArgs<R, D> a) {
+  constexpr int NV = D + 2;
+  using V = typename V4<R>::T;
+  extern __shared__ __align__(32) unsigned char smem_raw[];
+  const TileInfo ti = a.tiles[blockIdx.x];
+  V *fl4 = reinterpret_cast<V *>(smem_raw);
+  R *fl1 = reinterpret_cast<R *>(smem_raw + sizeof(V) * (size_t)ti.nface);
+  const BlkInfo &bi = a.blk[ti.blk];
+  const DevPhys &P = a.phys[bi.case_id];
+  if (a.viscous) {
+    ViscPhys<R, D> vp;
+    vp.Rg = (R)P.Rg; vp.Re = (R)P.Re; vp.qk = (R)P.qk; vp.SoT = (R)P.SoT;
+    for (int k = 0; k < 3; ++k) { vp.invh[k] = (R)bi.invh[k]; vp.area[k] = (R)bi.area[k]; }
+    for (int f = threadIdx.x; f < ti.nface; f += blockDim.x) {
+      const int4 st = a.tst[ti.face0 + f];
+      const int dir = a.tdir[ti.face0 + f];
+      const Prim<R, D> L = pload<R, D>(a.W, st.y);
+      const Prim<R, D> Rr = pload<R, D>(a.W, st.z);
+      Prim<R, D> t[(D == 3) ? 8 : 4];
+      const int4 v0 = a.tvs[(size_t)(ti.face0 + f) * (D - 1)];
+      t[0] = pload<R, D>(a.W, v0.x); t[1] = pload<R, D>(a.W, v0.y);
+      t[2] = pload<R, D>(a.W, v0.z); t[3] = pload<R, D>(a.W, v0.w);
+      if (D == 3) {
+        const int4 v1 = a.tvs[(size_t)(ti.face0 + f) * (D - 1) + 1];
+        t[4 % ((D == 3) ? 8 : 4)] = pload<R, D>(a.W, v1.x);
+        t[5 % ((D == 3) ? 8 : 4)] = pload<R, D>(a.W, v1.y);
+        t[6 % ((D == 3) ? 8 : 4)] = pload<R, D>(a.W, v1.z);
+        t[7 % ((D == 3) ? 8 : 4)] = pload<R, D>(a.W, v1.w);
+      }
+      R out[NV];
+      visc_flux<R, D>(L, Rr, t, dir, vp, out);
+      V o;
+      o.x = out[0]; o.y = out[1]; o.z = out[2]; o.w = out[3];
+      fl4[f] = o;
+      if (D == 3) fl1[f] = out[4];
+    }
+  }
+  __syncthreads();
+  const int i = threadIdx.x;
+  if (i >= ti.ncell) return;
+  const int c = ti.cell0 + i;
+  // U_old = cons(W64): Get_Cons_Vars FlowVariables.H:43-48, e = Cv T + |u|^2/2
+  const Prim<double, D> w = pload<double, D>(a.W64, c);
+  const double T = w.p / (w.r * P.Rg);
+  const double ke = (D == 3) ? 0.5 * (w.u * w.u + w.v * w.v + w.w * w.w) : 0.5 * (w.u * w.u + w.v * w.v);
+  const double e = P.Cv * T + ke;
+  double U[NV];
+  U[0] = w.r; U[1] = w.r * w.u; U[2] = w.r * w.v;
+  if (D == 3) { U[3] = w.r * w.w; U[4] = w.r * e; } else U[3] = w.r * e;
+  vstore<double, D>(a.U0, c, U);
+  double s[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) s[k] = 0.0;
+  if (a.viscous) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int lo = a.slots[(size_t)c * 2 * D + 2 * d], hi = a.slots[(size_t)c * 2 * D + 2 * d + 1];
+      const V flo = fl4[lo], fhi = fl4[hi];
+      s[0] += (double)fhi.x - (double)flo.x;
+      s[1] += (double)fhi.y - (double)flo.y;
+      s[2] += (double)fhi.z - (double)flo.z;
+      s[3] += (double)fhi.w - (double)flo.w;
+      if (D == 3) s[4] += (double)fl1[hi] - (double)fl1[lo];
+    }
+  }
+  R rv[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) rv[k] = (R)s[k];
+  vstore<R, D>(a.Rv, c, rv);
+}
+
+// ---------------------------------------------------------- ghost fills
+// BCBoxTreat BCValues.H:51-188: -13 pressure inlet (:191-206), -12 pressure
+// outlet (:208-214), -50 outflow (:216-220); entries (cell, refcell, type,
+// case).
+template <int D>
+__global__ void k_bc(int n, const int4 *dmg, Pack<double, D> W64, Pack<float, D> W32, int mixed,
+                     const DevPhys *phys) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int4 e = dmg[i];
+  const int g = e.x, ty = e.z;
+  const DevPhys &P = phys[e.w];
+  const Prim<double, D> q = pload<double, D>(W64, e.y);
+  if (ty == -50) {
+    wstore64<D>(W64, W32, mixed, g, q.r, q.u, q.v, q.w, q.p);
+  } else if (ty == -12) {
+    const double T = q.p / (q.r * P.Rg);
+    const double p = P.outlet_p;
+    wstore64<D>(W64, W32, mixed, g, p / (P.Rg * T), q.u, q.v, q.w, p);
+  } else if (ty == -13) {
+    const double speed0 = sqrt(q.u * q.u + q.v * q.v + q.w * q.w);
+    const double u = speed0 * P.inlet_cos, v = speed0 * P.inlet_sin;
+    const double T = P.inlet_T0 - (0.5 * (u * u + v * v)) / P.Cp;
+    const double p = P.inlet_p0 * pow((T / P.inlet_T0), 3.5);
+    wstore64<D>(W64, W32, mixed, g, p / (P.Rg * T), u, v, 0.0, p);
+  }
+}
+
+// PairRestriction[_2d] TwoLevelOp.H:217-231 with AverageCell[_2d] :45-87
+template <int D>
+__global__ void k_restrict(int n, const int *ci, const int *fi, Pack<double, D> W64,
+                           Pack<float, D> W32, int mixed, int twod, const DevPhys *phys,
+                           const int *cellcase) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = ci[i];
+  const DevPhys &P = phys[cellcase[c]];
+  double r = 0, ru = 0, rv = 0, rw = 0, re = 0;
+  for (int q = 0; q < 8; ++q) {
+    if (twod && (q & 1)) continue;
+    const Prim<double, D> s = pload<double, D>(W64, fi[(size_t)i * 8 + q]);
+    const double T = s.p / (s.r * P.Rg);
+    const double e = P.Cv * T + ((D == 3) ? 0.5 * (s.u * s.u + s.v * s.v + s.w * s.w) : 0.5 * (s.u * s.u + s.v * s.v));
+    r += s.r;
+    ru += s.u * s.r;
+    rv += s.v * s.r;
+    rw += s.w * s.r;
+    re += e * s.r;
+  }
+  const double u = ru / r, v = rv / r, w = rw / r, e = re / r;
+  const double rho = r * (twod ? 0.25 : 0.125);
+  const double ke = (D == 3) ? 0.5 * (u * u + v * v + w * w) : 0.5 * (u * u + v * v);
+  const double p = (e - ke) * (P.gama - 1.0) * rho;
+  wstore64<D>(W64, W32, mixed, c, rho, u, v, w, p);
+}
+
+// PairProlongation[_2d] TwoLevelOp.H:183-215: central gradients of the
+// conserved variables over the 6 face neighbours (cneib, :352-358), sons at
+// +-1/4 (fincrem).
+template <int D>
+__global__ void k_prolong(int n, const int *ci, const int *fi, const int *nb27, Pack<double, D> W64,
+                          Pack<float, D> W32, int mixed, int twod, const DevPhys *phys,
+                          const int *cellcase) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = ci[i];
+  const DevPhys &P = phys[cellcase[c]];
+  const int *nb = nb27 + (size_t)c * 27;
+  auto cons = [&](int a, double *U) {
+    const Prim<double, D> q = pload<double, D>(W64, a);
+    const double T = q.p / (q.r * P.Rg);
+    const double e = P.Cv * T + ((D == 3) ? 0.5 * (q.u * q.u + q.v * q.v + q.w * q.w) : 0.5 * (q.u * q.u + q.v * q.v));
+    U[0] = q.r; U[1] = q.r * q.u; U[2] = q.r * q.v; U[3] = q.r * q.w; U[4] = q.r * e;
+  };
+  double Uc[5], Ux0[5], Ux1[5], Uy0[5], Uy1[5], Uz0[5], Uz1[5];
+  cons(c, Uc);
+  cons(nb[0 * 9 + 1 * 3 + 1], Ux0); cons(nb[2 * 9 + 1 * 3 + 1], Ux1);
+  cons(nb[1 * 9 + 0 * 3 + 1], Uy0); cons(nb[1 * 9 + 2 * 3 + 1], Uy1);
+  cons(nb[1 * 9 + 1 * 3 + 0], Uz0); cons(nb[1 * 9 + 1 * 3 + 2], Uz1);
+  double gx[5], gy[5], gz[5];
+  for (int t = 0; t < 5; ++t) {
+    gx[t] = (Ux1[t] - Ux0[t]) * 0.5;
+    gy[t] = (Uy1[t] - Uy0[t]) * 0.5;
+    gz[t] = (Uz1[t] - Uz0[t]) * 0.5;
+  }
+  const double fincrem[2] = {-0.25, 0.25};
+  for (int si = 0; si < 2; ++si)
+    for (int sj = 0; sj < 2; ++sj)
+      for (int sk = 0; sk < (twod ? 1 : 2); ++sk) {
+        const int f = fi[(size_t)i * 8 + si * 4 + sj * 2 + sk];
+        double U[5];
+        for (int t = 0; t < 5; ++t) {
+          if (twod) U[t] = Uc[t] + fincrem[si] * gx[t] + fincrem[sj] * gy[t];
+          else U[t] = Uc[t] + fincrem[si] * gx[t] + fincrem[sj] * gy[t] + fincrem[sk] * gz[t];
+        }
+        const double r = U[0], u = U[1] / U[0], v = U[2] / U[0], w = U[3] / U[0], e = U[4] / U[0];
+        const double ke = (D == 3) ? 0.5 * (u * u + v * v + w * w) : 0.5 * (u * u + v * v);
+        const double p = (e - ke) * (P.gama - 1.0) * r;
+        wstore64<D>(W64, W32, mixed, f, r, u, v, w, p);
+      }
+}
+
+// Body::IBConditions_Image Body.cpp:1366-1496 (laminar): image state from
+// the bracketing cells (Mesh::IntpHGgcellVar Mesh.cpp:1532-1547), then
+// LinearDist (Body.cpp:183-204) for signdis >= 0, the mirror otherwise.  The
+// image cells of an IB cell are never IB cells (checked at upload), so one
+// pass equals the reference's compute-all-then-write-all.
+template <int D>
+__global__ void k_ib(int n, const int *cell, const int *intp, const double *coef,
+                     const double *hgdis, const double *attach, const int *body, const int *ibcase,
+                     const double *signdis, const Motion *mot, int have_motion,
+                     Pack<double, D> W64, Pack<float, D> W32, int mixed, const DevPhys *phys,
+                     unsigned long long *err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const DevPhys &P = phys[ibcase[i]];
+  double T = 0, p = 0, u = 0, v = 0, w = 0;
+  for (int q = 0; q < 8; ++q) {
+    const int a = intp[(size_t)i * 8 + q];
+    if (a < 0) continue;
+    const double cq = coef[(size_t)i * 8 + q];
+    const Prim<double, D> s = pload<double, D>(W64, a);
+    T += cq * (s.p / (s.r * P.Rg));
+    p += cq * s.p;
+    u += cq * s.u;
+    v += cq * s.v;
+    w += cq * s.w;
+  }
+  double pv[3] = {0.0, 0.0, 0.0};
+  if (have_motion) {
+    const Motion M = mot[body[i]];
+    const double *at = attach + (size_t)i * 3;
+    const double r1[3] = {at[0] - M.center[0], at[1] - M.center[1], at[2] - M.center[2]};
+    pv[0] = M.tra[0] + (M.rot[1] * r1[2] - M.rot[2] * r1[1]);
+    pv[1] = M.tra[1] + (M.rot[2] * r1[0] - M.rot[0] * r1[2]);
+    pv[2] = M.tra[2] + (M.rot[0] * r1[1] - M.rot[1] * r1[0]);
+  }
+  const int c = cell[i];
+  const double sd = signdis[c];
+  double ou, ov, ow;
+  if (sd < 0.0) {
+    ou = 2.0 * pv[0] - u; ov = 2.0 * pv[1] - v; ow = 2.0 * pv[2] - w;
+  } else {
+    const double rt = sd + hgdis[i];
+    const double r0 = sd / rt;
+    ou = pv[0] + (u - pv[0]) * r0;
+    ov = pv[1] + (v - pv[1]) * r0;
+    ow = pv[2] + (w - pv[2]) * r0;
+  }
+  if (T < 0.0) report_error(err, c, 1050);
+  wstore64<D>(W64, W32, mixed, c, p / (P.Rg * T), ou, ov, (D == 3) ? ow : 0.0, p);
+}
+
+// Comptdt NS_Solver.cpp:229-262 per tile: min over the tile's cells of
+// CFL V / sum_d (|u_d| + c) A_d (all three directions, also in 2D).
+template <int D>
+__global__ void __launch_bounds__(kTile) k_cfl(const TileInfo *tiles, Pack<double, D> W64,
+                                               const BlkInfo *blk, const DevPhys *phys,
+                                               double *cflpart, unsigned long long *err) {
+  __shared__ double red[kTile / 64];
+  const TileInfo ti = tiles[blockIdx.x];
+  const BlkInfo &bi = blk[ti.blk];
+  const DevPhys &P = phys[bi.case_id];
+  double d0 = 1.0e300;
+  if ((int)threadIdx.x < ti.ncell) {
+    const int c = ti.cell0 + threadIdx.x;
+    const Prim<double, D> q = pload<double, D>(W64, c);
+    const double a = sqrt(P.gama * P.Rg * (q.p / (q.r * P.Rg)));
+    const double c0 = (fabs(q.u) + a) * bi.area[0];
+    const double c1 = (fabs(q.v) + a) * bi.area[1];
+    const double c2 = (fabs(q.w) + a) * bi.area[2];
+    d0 = P.cfl * bi.vol / (c0 + c1 + c2);
+    if (d0 < 0.0) report_error(err, c, 424);
+  }
+  for (int off = 32; off > 0; off >>= 1) d0 = fmin(d0, __shfl_xor(d0, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = d0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < (int)((blockDim.x + 63) >> 6); ++w) m = fmin(m, red[w]);
+    cflpart[blockIdx.x] = m;
+  }
+}
+
+// dt = min(CFL_dt, dt*dt_ratio) per case, CFL_dt capped at 10000
+// (NS_Solver.cpp:198, 263)
+__global__ void k_dt_final(int ntiles, const TileInfo *tiles, const BlkInfo *blk,
+                           const double *cflpart, double *dt, const DevPhys *phys, int ncases) {
+  const int cs = blockIdx.x;
+  if (cs >= ncases) return;
+  __shared__ double red[256];
+  double m = 10000.0;
+  for (int t = threadIdx.x; t < ntiles; t += blockDim.x)
+    if (blk[tiles[t].blk].case_id == cs) m = fmin(m, cflpart[t]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = red[0];
+    for (int k = 1; k < (int)blockDim.x; ++k) v = fmin(v, red[k]);
+    const double d = dt[cs] * phys[cs].dt_ratio;
+    dt[cs] = (v < d) ? v : d;
+  }
+}
+
+__global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Body::ComptWallForce Body.cpp:460-555, one workgroup per case, summed in
+// a fixed order (deterministic).
+template <int D>
+__global__ void k_forces(int npatch, const int *pcase, const int *intp, const double *coef,
+                         const double *hgdis, const double *nv, const double *pc,
+                         const double *area, const double *ccenter, const Motion *mot,
+                         int have_motion, Pack<double, D> W64, const DevPhys *phys, double *out) {
+  const int c = blockIdx.x;
+  const DevPhys &P = phys[c];
+  __shared__ double acc[256][12];
+  double s[12];
+  for (int k = 0; k < 12; ++k) s[k] = 0.0;
+  const double *bc = ccenter + (size_t)c * 3;
+  for (int i = threadIdx.x; i < npatch; i += blockDim.x) {
+    if (pcase[i] != c) continue;
+    double p = 0, u = 0, v = 0, w = 0;
+    for (int q = 0; q < 8; ++q) {
+      const int a = intp[(size_t)i * 8 + q];
+      if (a < 0) continue;
+      const double cq = coef[(size_t)i * 8 + q];
+      const Prim<double, D> sp = pload<double, D>(W64, a);
+      p += cq * sp.p;
+      u += cq * sp.u;
+      v += cq * sp.v;
+      w += cq * sp.w;
+    }
+    const double *x = pc + (size_t)i * 3;
+    const double *n = nv + (size_t)i * 3;
+    double pv[3] = {0, 0, 0};
+    if (have_motion) {
+      const Motion M = mot[c];
+      const double r1[3] = {x[0] - M.center[0], x[1] - M.center[1], x[2] - M.center[2]};
+      pv[0] = M.tra[0] + (M.rot[1] * r1[2] - M.rot[2] * r1[1]);
+      pv[1] = M.tra[1] + (M.rot[2] * r1[0] - M.rot[0] * r1[2]);
+      pv[2] = M.tra[2] + (M.rot[0] * r1[1] - M.rot[1] * r1[0]);
+    }
+    const double du[3] = {u - pv[0], v - pv[1], w - pv[2]};
+    const double dn = du[0] * n[0] + du[1] * n[1] + du[2] * n[2];
+    const double A = area[i];
+    double df0[3], df1[3];
+    for (int k = 0; k < 3; ++k) {
+      df0[k] = ((du[k] - n[k] * dn) / hgdis[i]) * P.mu * A;
+      df1[k] = n[k] * (-p * A);
+    }
+    const double dx[3] = {x[0] - bc[0], x[1] - bc[1], x[2] - bc[2]};
+    const double dm0[3] = {dx[1] * df0[2] - dx[2] * df0[1], dx[2] * df0[0] - dx[0] * df0[2], dx[0] * df0[1] - dx[1] * df0[0]};
+    const double dm1[3] = {dx[1] * df1[2] - dx[2] * df1[1], dx[2] * df1[0] - dx[0] * df1[2], dx[0] * df1[1] - dx[1] * df1[0]};
+    for (int k = 0; k < 3; ++k) { s[k] += df1[k]; s[3 + k] += df0[k]; s[6 + k] += dm1[k]; s[9 + k] += dm0[k]; }
+  }
+  for (int k = 0; k < 12; ++k) acc[threadIdx.x][k] = s[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t[12];
+    for (int k = 0; k < 12; ++k) t[k] = 0.0;
+    for (int j = 0; j < (int)blockDim.x; ++j)
+      for (int k = 0; k < 12; ++k) t[k] += acc[j][k];
+    for (int k = 0; k < 3; ++k) {
+      out[c * 6 + k] = t[k] * P.p_ref_area + t[3 + k] * P.shear_ref_area;
+      out[c * 6 + 3 + k] = t[6 + k] * P.p_ref_area + t[9 + k] * P.shear_ref_area;
+    }
+  }
+}
+
+// residual finalize: per case, over the tiles of the last march
+__global__ void k_resid_final(int ntiles, const TileInfo *tiles, const BlkInfo *blk,
+                              const double *rpart, double *out, int ncases) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncases) return;
+  double m[5] = {0, 0, 0, 0, 0}, q[5] = {0, 0, 0, 0, 0};
+  for (int t = 0; t < ntiles; ++t) {
+    if (blk[tiles[t].blk].case_id != c) continue;
+    for (int k = 0; k < 5; ++k) {
+      m[k] = fmax(m[k], rpart[(size_t)t * 10 + k]);
+      q[k] += rpart[(size_t)t * 10 + 5 + k];
+    }
+  }
+  for (int k = 0; k < 5; ++k) { out[c * 10 + k] = m[k]; out[c * 10 + 5 + k] = sqrt(q[k]); }
+}
+
+// parity probes: inviscid / viscous flux of every face, [face][5]
+template <typename R, int D>
+__global__ void k_face_probe(int nface, const int4 *st, const int8_t *dir, const int16_t *fblk,
+                             Pack<R, D> W, const BlkInfo *blk, const DevPhys *phys, int lim,
+                             double *out) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nface) return;
+  constexpr int NV = D + 2;
+  const BlkInfo &bi = blk[fblk[f]];
+  const DevPhys &P = phys[bi.case_id];
+  PhysR<R> ph;
+  ph.gama = (R)P.gama; ph.Rg = (R)P.Rg; ph.Cv = (R)P.Cv; ph.Ma = (R)P.Ma;
+  const int4 s = st[f];
+  R flux[NV];
+  roe_flux<R, D>(pload<R, D>(W, s.x), pload<R, D>(W, s.y), pload<R, D>(W, s.z),
+                 pload<R, D>(W, s.w), dir[f], (R)bi.area[dir[f]], ph, lim, flux);
+  double *o = out + (size_t)f * 5;
+  if (D == 3) { for (int k = 0; k < 5; ++k) o[k] = flux[k]; }
+  else { o[0] = flux[0]; o[1] = flux[1]; o[2] = flux[2]; o[3] = 0.0; o[4] = flux[3]; }
+}
+
+template <typename R, int D>
+__global__ void k_visc_probe(int nface, const int4 *st, const int4 *vst, const int8_t *dir,
+                             const int16_t *fblk, Pack<R, D> W, const BlkInfo *blk,
+                             const DevPhys *phys, double *out) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nface) return;
+  constexpr int NV = D + 2;
+  const BlkInfo &bi = blk[fblk[f]];
+  const DevPhys &P = phys[bi.case_id];
+  ViscPhys<R, D> vp;
+  vp.Rg = (R)P.Rg; vp.Re = (R)P.Re; vp.qk = (R)P.qk; vp.SoT = (R)P.SoT;
+  for (int k = 0; k < 3; ++k) { vp.invh[k] = (R)bi.invh[k]; vp.area[k] = (R)bi.area[k]; }
+  const int4 s = st[f];
+  Prim<R, D> t[(D == 3) ? 8 : 4];
+  const int4 v0 = vst[(size_t)f * (D - 1)];
+  t[0] = pload<R, D>(W, v0.x); t[1] = pload<R, D>(W, v0.y); t[2] = pload<R, D>(W, v0.z); t[3] = pload<R, D>(W, v0.w);
+  if (D == 3) {
+    const int4 v1 = vst[(size_t)f * (D - 1) + 1];
+    t[4 % ((D == 3) ? 8 : 4)] = pload<R, D>(W, v1.x);
+    t[5 % ((D == 3) ? 8 : 4)] = pload<R, D>(W, v1.y);
+    t[6 % ((D == 3) ? 8 : 4)] = pload<R, D>(W, v1.z);
+    t[7 % ((D == 3) ? 8 : 4)] = pload<R, D>(W, v1.w);
+  }
+  R o5[NV];
+  visc_flux<R, D>(pload<R, D>(W, s.y), pload<R, D>(W, s.z), t, dir[f], vp, o5);
+  double *o = out + (size_t)f * 5;
+  if (D == 3) { for (int k = 0; k < 5; ++k) o[k] = o5[k]; }
+  else { o[0] = o5[0]; o[1] = o5[1]; o[2] = o5[2]; o[3] = 0.0; o[4] = o5[3]; }
+}
+
+// plain copy (bandwidth / counter calibration probe)
+__global__ void k_copy(const F4 *__restrict__ a, F4 *__restrict__ b, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+}  // namespace ibg

@@ -153,6 +153,9 @@ GPU_FUNCS = {
     "ibg_last_error": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
     "ibg_synchronize": (C.c_int, [C.c_void_p]),
     "ibg_last_timing": (C.c_int, [C.c_void_p, f64p, f64p, C.POINTER(C.c_int64)]),
+    "ibg_bench_stage": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, f64p,
+                                  C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "ibg_bench_copy": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, f64p]),
     "ibg_cell_stage_updates": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
 }
 

@@ -144,6 +144,18 @@ class Engine:
         self.lib.ibg_last_timing(self.ctx, C.byref(tot), C.byref(st), C.byref(n))
         return tot.value, st.value, n.value
 
+    def bench_stage(self, ts, subt, reps):
+        ms = C.c_double()
+        nc, nf = C.c_int64(), C.c_int64()
+        self._chk(self.lib.ibg_bench_stage(self.ctx, ts, subt, reps, C.byref(ms), C.byref(nc),
+                                           C.byref(nf)), "ibg_bench_stage")
+        return ms.value, nc.value, nf.value
+
+    def bench_copy(self, nbytes, reps=20):
+        ms = C.c_double()
+        self._chk(self.lib.ibg_bench_copy(self.ctx, int(nbytes), reps, C.byref(ms)), "ibg_bench_copy")
+        return ms.value
+
     def cell_stage_updates(self):
         n = C.c_int64()
         self.lib.ibg_cell_stage_updates(self.ctx, C.byref(n))

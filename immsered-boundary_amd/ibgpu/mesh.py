@@ -83,7 +83,7 @@ def config1(levels=4, base=64):
 
 def config2(levels=6, base=256):
     """Mid 2D case: NACA0015, Mach 0.3, Re 10000, 256^2 base + 5 AMR levels."""
-    bands = [(-0.3, 4.0), (-0.15, 2.5), (-0.08, 1.5), (-0.04, 0.9), (-0.02, 0.45)]
+    bands = [(-0.3, 6.0), (-0.15, 4.0), (-0.08, 2.5), (-0.04, 1.6), (-0.02, 1.0)]
     return CaseSpec(name="naca0015_m03_re10000", dim=2, nx=base, ny=base, nz=1,
                     nlevels=levels, bands=bands[:levels - 1], thickness=0.15,
                     npts=200, aoa=0.0, Ma=0.3, Re=10000.0)
@@ -95,6 +95,40 @@ def tiny_case(levels=3, base=32, aoa=4.0, Ma=0.5, Re=2000.0, thickness=0.12):
     return CaseSpec(name="tiny", dim=2, nx=base, ny=base, nz=1, lo=-4.0, hi=4.0,
                     nlevels=levels, bands=bands[:levels - 1], thickness=thickness,
                     npts=60, aoa=aoa, Ma=Ma, Re=Re)
+
+
+def tiny3d_case(levels=3, base=24, nz=4, aoa=4.0, Ma=0.5, Re=2000.0):
+    """Quasi-span 3D case (periodic span, levels refine x,y only) for parity tests."""
+    bands = [(-1.0, 1.5), (-0.5, 0.6), (-0.25, 0.3)]
+    return CaseSpec(name="tiny3d", dim=3, nx=base, ny=base, nz=nz, lo=-4.0, hi=4.0, span=0.4,
+                    nlevels=levels, twod_from=1, bands=bands[:levels - 1], thickness=0.12,
+                    npts=40, aoa=aoa, Ma=Ma, Re=Re, span_patches=nz)
+
+
+def config3(levels=5, base=128, nz=8):
+    """3D quasi-span: NACA0012 extruded over a periodic span of 0.1 chord,
+    128x128x8 base, 4 AMR levels (x,y refinement), Mach 0.6, AoA 4, Re 5e4."""
+    bands = [(-0.3, 3.0), (-0.15, 1.5), (-0.08, 0.8), (-0.04, 0.4)]
+    return CaseSpec(name="naca0012_3d_m06_re5e4", dim=3, nx=base, ny=base, nz=nz, span=0.1,
+                    nlevels=levels, twod_from=1, bands=bands[:levels - 1], thickness=0.12,
+                    npts=100, aoa=4.0, Ma=0.6, Re=50000.0, span_patches=nz)
+
+
+def ensemble_cases(n=32, seed=20261202, base=128, levels=5):
+    """Batched ensemble: NACA 4-digit sections, thickness U[0.09,0.18], AoA
+    U[-4,8] deg, Mach U[0.3,0.7], Re 1e4 (seeded)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        t = float(rng.uniform(0.09, 0.18))
+        a = float(rng.uniform(-4.0, 8.0))
+        m = float(rng.uniform(0.3, 0.7))
+        w = float(rng.uniform(0.6, 1.4))   # uneven refinement extent
+        bands = [(-0.3, 4.0 * w), (-0.15, 2.5 * w), (-0.08, 1.5 * w), (-0.04, 0.8 * w)]
+        out.append(CaseSpec(name=f"ens{i:02d}", dim=2, nx=base, ny=base, nz=1, nlevels=levels,
+                            bands=bands[:levels - 1], thickness=t, npts=100, aoa=a, Ma=m,
+                            Re=10000.0))
+    return out
 
 
 class Mesh:

@@ -241,6 +241,19 @@ ibg_status ibg_synchronize(ibg_ctx *ctx);
 ibg_status ibg_last_timing(ibg_ctx *ctx, double *total_ms,
                            double *stage_kernel_avg_ms, int64_t *stage_launches);
 
+/* Measurement probe with no counterpart in the reference: launch the stage
+ * kernel of step ts, stage subt, reps times back to back on the context
+ * stream between two HIP events.  avg_ms = elapsed / reps; cells and faces
+ * are the normal cells and tile faces one launch processes.  The state is
+ * advanced by the repeated stages (use it after the run being measured).   */
+ibg_status ibg_bench_stage(ibg_ctx *ctx, int32_t ts, int32_t subt, int32_t reps,
+                           double *avg_ms, int64_t *cells, int64_t *faces);
+
+/* Measurement probe with no counterpart in the reference: a plain device
+ * copy of nbytes (read nbytes, write nbytes) repeated reps times on the
+ * context stream; avg_ms per copy.  Calibrates HBM counters and bandwidth. */
+ibg_status ibg_bench_copy(ibg_ctx *ctx, int64_t nbytes, int32_t reps, double *avg_ms);
+
 /* Counters for the metric: normal cells updated per step, summed over the
  * steps of the last ibg_time_march, times the stage count (the cell-stage
  * update count of NS_Solver.cpp:868-872, 948-953).                        */

@@ -193,6 +193,7 @@ void orc_face_flux(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *
       if (!orc_marched(L, l, ts, cfg->temporal_refine)) continue;
       int b = blk_of(m, c, l);
       const double *h = &m->blk_h[3 * b];
+#pragma omp parallel for schedule(static)
       for (int f = m->blk_face_start[b]; f < m->blk_face_start[b + 1]; ++f) {
         int d = m->face_dir[f];
         int dn[3] = {0, 0, 0};
@@ -223,6 +224,7 @@ void orc_viscous_flux(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cf
       int b = blk_of(m, c, l);
       const double *h = &m->blk_h[3 * b];
       const double k0 = 1.0 / h[0], k1 = 1.0 / h[1], k2 = 1.0 / h[2];
+#pragma omp parallel for schedule(static)
       for (int f = m->blk_face_start[b]; f < m->blk_face_start[b + 1]; ++f) {
         int d = m->face_dir[f];
         int abox = m->face_cells[2 * f];
@@ -294,6 +296,7 @@ void orc_source(const ibg_mesh_soa *m, const orc_cfg *cfg, const double *ff,
       if (!orc_marched(L, l, ts, cfg->temporal_refine)) continue;
       int b = blk_of(m, c, l);
       int s = m->blk_start[b], e = s + m->blk_nnormal[b];
+#pragma omp parallel for schedule(static)
       for (int g = s; g < e; ++g) {
         double sinc[3][5];
         memset(sinc, 0, sizeof(sinc));
@@ -340,6 +343,7 @@ void orc_old_cons(const ibg_mesh_soa *m, const orc_cfg *cfg, const double *fv, i
       if (!orc_marched(L, l, ts, cfg->temporal_refine)) continue;
       int b = blk_of(m, c, l);
       int s = m->blk_start[b], e = s + m->blk_nnormal[b];
+#pragma omp parallel for schedule(static)
       for (int g = s; g < e; ++g) get_cons(&fv[(size_t)g * NF], &cv_old[(size_t)g * 5]);
     }
 }
@@ -353,6 +357,7 @@ int32_t orc_update(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *
                    int32_t subt, double *cv_new, double *fv, int64_t *bad_cell) {
   const int L = m->nlevels;
   int32_t code = 0;
+  int64_t badc = -1;
   for (int c = 0; c < m->ncases; ++c) {
     const ibg_physics *P = &ph[c];
     const double dt0 = cfg->alpha[subt] * dt[c];
@@ -364,29 +369,40 @@ int32_t orc_update(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *
       const double level_ts = dt0 * orc_step_num(L, l, cfg->temporal_refine);
       const int finest = (l == L - 1) && (m->nbody > 0);
       int s = m->blk_start[b], e = s + m->blk_nnormal[b];
+#pragma omp parallel for schedule(static)
       for (int g = s; g < e; ++g) {
         const int solid = m->signdis[g] < 0.0;
         const int ib = m->ibidx[g] >= 0;
         if (!finest || (!solid && !ib)) {
           for (int v = 0; v < 5; ++v)
             cv_new[(size_t)g * 5 + v] = cv_old[(size_t)g * 5 + v] + source[(size_t)g * 5 + v] * level_ts / vol;
-          if (cv_new[(size_t)g * 5] < 0.0 && !code) {
-            code = finest ? 611 : 523;
-            if (bad_cell) *bad_cell = g;
+          if (cv_new[(size_t)g * 5] < 0.0) {
+#pragma omp critical(orc_err)
+            if (!code || g < badc) {
+              code = finest ? 611 : 523;
+              badc = g;
+            }
           }
         }
       }
+#pragma omp parallel for schedule(static)
       for (int g = s; g < e; ++g) {
         const int ib = m->ibidx[g] >= 0;
         double *w = &fv[(size_t)g * NF];
         if (!finest) {
           if (m->signdis[g] > 0.0) get_prim(w, &cv_new[(size_t)g * 5], P);
           else memcpy(w, P->initvar, sizeof(double) * NF);
-          if (w[TT] < 0.0 && !code) { code = 37; if (bad_cell) *bad_cell = g; }
+          if (w[TT] < 0.0) {
+#pragma omp critical(orc_err)
+            if (!code || g < badc) { code = 37; badc = g; }
+          }
         } else {
           if (m->signdis[g] > 0.0 && !ib) {
             get_prim(w, &cv_new[(size_t)g * 5], P);
-            if (w[TT] < 0.0 && !code) { code = 37; if (bad_cell) *bad_cell = g; }
+            if (w[TT] < 0.0) {
+#pragma omp critical(orc_err)
+              if (!code || g < badc) { code = 37; badc = g; }
+            }
           } else if (!ib) {
             memcpy(w, P->initvar, sizeof(double) * NF);
           }
@@ -394,6 +410,7 @@ int32_t orc_update(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *
       }
     }
   }
+  if (bad_cell && code) *bad_cell = badc;
   return code;
 }
 
