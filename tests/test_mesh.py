@@ -1,0 +1,217 @@
+"""Invariants of the host AMR + IB mesh builder (CPU only), on canonical keys.
+
+Each check restates a rule of the reference's mesh in terms of keys, not of
+the builder's index order:
+  neighbours   nb[c][i][j][k] has key(c) + (i-1, j-1, k-1), z periodic
+               (Box::neib, Box.H:117)
+  faces        every normal cell has its 2*dim faces, bi[0] is the low side,
+               each face is listed once (Mesh::ConstructBoxFace Mesh.cpp:18-83)
+  ghosts       two ghost layers make every MUSCL/viscous stencil complete
+  pairs        prolongation/restriction sons are the 2x2(x2) children of the
+               coarse key; restriction layer 1 touches an uncovered coarse
+               cell, layer 2 is two cells from one on an axis
+               (AMRGhost_refine.cpp:30-98)
+  domain       ghosts take the first matching face x-, x+, y-, y+ and copy
+               the first interior cell inward (Domain.H:372-446)
+  IB           IB cells are the non-solid cells next to a solid cell and
+               their non-solid neighbours (Body.cpp:1016-1082); every image
+               point is bracketed by non-solid, non-IB cells whose weights
+               sum to 1 and reproduce the image point's coordinates
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import ibgpu
+from ibgpu import _ffi
+
+
+@pytest.fixture(scope="module", params=["tiny", "tiny3d", "config1"])
+def mesh(request):
+    spec = {"tiny": ibgpu.tiny_case, "tiny3d": ibgpu.tiny3d_case, "config1": ibgpu.config1}[request.param]()
+    return ibgpu.Mesh(spec)
+
+
+def level_of(m):
+    return m.level_of()
+
+
+def test_neighbour_keys(mesh):
+    m = mesh
+    lv = level_of(m)
+    rng = np.random.default_rng(0)
+    cells = rng.choice(m.ncells, size=min(2000, m.ncells), replace=False)
+    nzl = {}
+    for b in range(m.nlevels):
+        s, e = m.blk_start[b], m.blk_start[b + 1]
+        nzl[b] = int(m.key[s:e, 2].max()) + 1
+    for c in cells:
+        for q in range(27):
+            a = m.nb[c, q]
+            if a < 0:
+                continue
+            di, dj, dk = q // 9 - 1, (q // 3) % 3 - 1, q % 3 - 1
+            assert lv[a] == lv[c]
+            assert m.key[a, 0] == m.key[c, 0] + di and m.key[a, 1] == m.key[c, 1] + dj
+            assert m.key[a, 2] == (m.key[c, 2] + dk) % nzl[lv[c]]
+
+
+def test_faces(mesh):
+    m = mesh
+    D = m.dim
+    normal = np.nonzero(m.kind == 0)[0]
+    lo, hi, d = m.face_cells[:, 0], m.face_cells[:, 1], m.face_dir
+    # low side + unit step along d = high side
+    e = np.eye(3, dtype=np.int64)[d]
+    k = m.key.astype(np.int64)
+    nzl = np.array([k[m.blk_start[b]:m.blk_start[b + 1], 2].max() + 1 for b in range(m.nlevels)])
+    lv = level_of(m)
+    zz = (k[lo, 2] + e[:, 2]) % nzl[lv[lo]]
+    assert np.all(k[lo, 0] + e[:, 0] == k[hi, 0])
+    assert np.all(k[lo, 1] + e[:, 1] == k[hi, 1])
+    assert np.all(zz == k[hi, 2])
+    # unique faces, every normal cell has 2*dim of them
+    pairs = set(zip(lo.tolist(), hi.tolist(), d.tolist()))
+    assert len(pairs) == m.nfaces
+    assert np.all(m.cell_faces[normal][:, :2 * D] >= 0)
+    assert np.all((m.kind[lo] == 0) | (m.kind[hi] == 0))
+
+
+def test_stencils_complete(mesh):
+    m = mesh
+    _lib = _ffi.gpu_lib()
+    msg = C.c_char_p()
+    assert _lib.ibg_check_mesh(m.soa_ptr, C.byref(msg)) == 0, msg.value
+    # two ghost layers: every normal cell reaches 2 cells along each axis
+    normal = np.nonzero(m.kind == 0)[0]
+    for d in range(m.dim):
+        for sgn in (0, 2):
+            off = [1, 1, 1]
+            off[d] = sgn
+            q = off[0] * 9 + off[1] * 3 + off[2]
+            a = m.nb[normal, q]
+            assert np.all(a >= 0)
+            assert np.all(m.nb[a, q] >= 0)
+
+
+def test_pairs_are_children(mesh):
+    m = mesh
+    L = m.nlevels
+    for ci, fi, twod in [(m.res_ci, m.res_fi, None), (m.pro_ci, m.pro_fi, None)]:
+        for i in range(len(ci)):
+            c = ci[i]
+            b = np.searchsorted(m.blk_start, c, side="right") - 1
+            tw = m.soa.blk_twod[b + 1]
+            for q in range(8):
+                si, sj, sk = q // 4, (q // 2) % 2, q % 2
+                f = fi[i, q]
+                if tw and sk == 1:
+                    assert f == -1
+                    continue
+                assert f >= 0
+                assert m.key[f, 0] == 2 * m.key[c, 0] + si
+                assert m.key[f, 1] == 2 * m.key[c, 1] + sj
+                assert m.key[f, 2] == (m.key[c, 2] if tw else 2 * m.key[c, 2] + sk)
+    # kinds: restriction coarse cells are RES1/RES2, prolongation sons PRO
+    assert set(np.unique(m.kind[m.res_ci])) <= {2, 3}
+    assert np.all(m.kind[m.pro_fi[m.pro_fi >= 0]] == 4)
+
+
+def test_restriction_layers(mesh):
+    m = mesh
+    # a coarse cell is "covered" when it has sons: RES1 has an uncovered
+    # normal 26-neighbour; RES2 has none but one two cells away on an axis
+    covered = np.zeros(m.ncells, bool)
+    covered[m.res_ci] = True
+    zr = 1 if m.dim == 3 else 0
+    for i, c in enumerate(m.res_ci):
+        near = False
+        for q in range(27):
+            di, dj, dk = q // 9 - 1, (q // 3) % 3 - 1, q % 3 - 1
+            if abs(dk) > zr:
+                continue
+            a = m.nb[c, q]
+            if a >= 0 and m.kind[a] == 0:
+                near = True
+        if m.kind[c] == 2:
+            assert near
+        else:
+            assert not near
+            far = False
+            for d in range(2):
+                for sgn in (-1, 1):
+                    off = [1, 1, 1]
+                    off[d] += sgn
+                    a = m.nb[c, off[0] * 9 + off[1] * 3 + off[2]]
+                    a2 = m.nb[a, off[0] * 9 + off[1] * 3 + off[2]] if a >= 0 else -1
+                    if a2 >= 0 and m.kind[a2] == 0:
+                        far = True
+            assert far
+
+
+def test_domain_ghosts(mesh):
+    m = mesh
+    if m.dmg is None:
+        return
+    cells, refc, types = m.dmg
+    spec = m.specs[0]
+    for g, r, t in zip(cells, refc, types):
+        ix, iy = m.key[g, 0], m.key[g, 1]
+        if ix < 0:
+            assert t == -13 and m.key[r, 0] == 0 and m.key[r, 1] == iy
+        elif ix >= spec.nx:
+            assert t == -12 and m.key[r, 0] == spec.nx - 1 and m.key[r, 1] == iy
+        elif iy < 0:
+            assert t == -50 and m.key[r, 1] == 0 and m.key[r, 0] == ix
+        else:
+            assert t == -50 and m.key[r, 1] == spec.ny - 1 and m.key[r, 0] == ix
+
+
+def test_ib_cells_and_image_points(mesh):
+    m = mesh
+    L = m.nlevels
+    b = L - 1
+    s, e = m.blk_start[b], m.blk_start[b + 1]
+    solid = m.signdis < 0
+    ib = m.ibidx >= 0
+    ql = [q for q in range(27) if m.dim == 3 or q % 3 == 1]
+    # rule of Body.cpp:1016-1082
+    expect = np.zeros(m.ncells, bool)
+    for c in range(s, e):
+        if m.kind[c] == 1 or solid[c]:
+            continue
+        if any(m.nb[c, q] >= 0 and solid[m.nb[c, q]] and m.kind[m.nb[c, q]] != 1 for q in ql):
+            expect[c] = True
+            for q in ql:
+                a = m.nb[c, q]
+                if a >= 0 and not solid[a] and m.kind[a] != 1:
+                    expect[a] = True
+    assert np.array_equal(expect[s:e], ib[s:e])
+    nv = np.ctypeslib.as_array(m.soa.ib_nv, (m.nib, 3))
+    for i in range(m.nib):
+        c = m.ib_cell[i]
+        img = m.centers[c] + nv[i] * m.ib_hgdis[i]
+        used = [q for q in range(8) if m.dim == 3 or q % 2 == 1]
+        cells = m.ib_intp[i, used]
+        w = m.ib_coef[i, used]
+        assert np.all(cells >= 0)
+        assert not solid[cells].any() and not ib[cells].any()
+        assert w.sum() == pytest.approx(1.0, abs=1e-14)
+        assert np.allclose((w[:, None] * m.centers[cells]).sum(axis=0)[:m.dim], img[:m.dim], atol=1e-12)
+
+
+def test_level_counts_and_nesting():
+    m = ibgpu.Mesh(ibgpu.tiny_case(levels=4))
+    counts = m.counts()[0]
+    assert all(c > 0 for c in counts)
+    # every fine normal cell lies under a coarse RES cell or under a removed one:
+    # its parent key is never a coarse normal cell
+    lv = m.level_of()
+    for l in range(1, m.nlevels):
+        fine = np.nonzero((lv == l) & (m.kind == 0))[0]
+        cs, ce = m.blk_start[l - 1], m.blk_start[l]
+        normal_keys = {tuple(k) for k in m.key[cs:ce][m.kind[cs:ce] == 0][:, :2]}
+        for f in fine[::5]:
+            parent = (m.key[f, 0] // 2, m.key[f, 1] // 2)
+            assert parent not in normal_keys
