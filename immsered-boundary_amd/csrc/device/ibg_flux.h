@@ -169,6 +169,113 @@ __device__ __forceinline__ void roe_flux(const Prim<R, D> &t0, const Prim<R, D> 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Mixed-mode (FP32) form of the same face flux.  The arithmetic is the same
+// expression set, evaluated with the hardware reciprocal and square root
+// (v_rcp_f32, v_sqrt_f32, ~1 ulp) and with the van Leer limiter in its
+// one-reciprocal form: for r = dqii/dqi > 0,
+//   qi + 0.5 dqi phi(r) = qi + dqi dqii / (dqi + dqii),
+// and qi otherwise (the reference's 1e10/1e6 guards only matter when
+// |dqi| < 1e-10 |dqii|, where both forms give qi to within 1e-10 dqii).
+// The error of this mode against the FP64 oracle is measured, not assumed.
+__device__ __forceinline__ float f_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float f_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+__device__ __forceinline__ float fast_left(float q_i, float qi, float qi1, int lim) {
+  const float dqi = qi - q_i, dqii = qi1 - qi;
+  const float pr = dqi * dqii;
+  if (!(pr > 0.0f)) return qi;
+  if (lim == 1) return qi + 0.5f * pr * (dqi + dqii) * f_rcp(dqi * dqi + dqii * dqii);
+  return qi + pr * f_rcp(dqi + dqii);
+}
+
+__device__ __forceinline__ float fast_right(float q_i, float qi, float qi1, int lim) {
+  const float dqi = qi - q_i, dqii = qi1 - qi;
+  const float pr = dqi * dqii;
+  if (!(pr > 0.0f)) return qi;
+  if (lim == 1) return qi - 0.5f * pr * (dqi + dqii) * f_rcp(dqi * dqi + dqii * dqii);
+  return qi - pr * f_rcp(dqi + dqii);
+}
+
+template <int D>
+__device__ __forceinline__ void roe_flux_fast(const Prim<float, D> &t0, const Prim<float, D> &t1,
+                                              const Prim<float, D> &t2, const Prim<float, D> &t3,
+                                              int dir, float area, const PhysR<float> &ph, int lim,
+                                              float *flux) {
+  const float gama = ph.gama, Rg = ph.Rg, Cv = ph.Cv;
+  const float lr = fast_left(t0.r, t1.r, t2.r, lim), rr = fast_right(t1.r, t2.r, t3.r, lim);
+  const float lu = fast_left(t0.u, t1.u, t2.u, lim), ru = fast_right(t1.u, t2.u, t3.u, lim);
+  const float lvv = fast_left(t0.v, t1.v, t2.v, lim), rvv = fast_right(t1.v, t2.v, t3.v, lim);
+  float lw = 0.0f, rw = 0.0f;
+  if (D == 3) { lw = fast_left(t0.w, t1.w, t2.w, lim); rw = fast_right(t1.w, t2.w, t3.w, lim); }
+  const float lp = fast_left(t0.p, t1.p, t2.p, lim), rp = fast_right(t1.p, t2.p, t3.p, lim);
+  const float ilr = f_rcp(lr), irr = f_rcp(rr);
+  const float iRg = f_rcp(Rg);
+  const float lT = lp * ilr * iRg, rT = rp * irr * iRg;
+  const float lke = (D == 3) ? 0.5f * (lu * lu + lvv * lvv + lw * lw) : 0.5f * (lu * lu + lvv * lvv);
+  const float rke = (D == 3) ? 0.5f * (ru * ru + rvv * rvv + rw * rw) : 0.5f * (ru * ru + rvv * rvv);
+  const float le = Cv * lT + lke, re = Cv * rT + rke;
+  const float lv = dir == 0 ? lu : (dir == 1 ? lvv : lw);
+  const float rv = dir == 0 ? ru : (dir == 1 ? rvv : rw);
+  const float kx = dir == 0 ? 1.0f : 0.0f, ky = dir == 1 ? 1.0f : 0.0f, kz = dir == 2 ? 1.0f : 0.0f;
+  // central part 1/2 (F_L + F_R) A
+  const float ha = 0.5f * area;
+  const float p0 = (lr * lv + rr * rv) * ha;
+  const float p1 = (lr * lu * lv + lp * kx + rr * ru * rv + rp * kx) * ha;
+  const float p2 = (lr * lvv * lv + lp * ky + rr * rvv * rv + rp * ky) * ha;
+  const float p3 = (lr * lw * lv + lp * kz + rr * rw * rv + rp * kz) * ha;
+  const float p4 = ((lp + le * lr) * lv + (rp + re * rr) * rv) * ha;
+  const float droe = rr - lr, dp = rp - lp;
+  const float du0 = ru - lu, du1 = rvv - lvv, du2 = rw - lw;
+  const float dubar = rv - lv;
+  // Roe average
+  const float r2lroe = f_sqrt(rr * ilr);
+  const float r2l0 = f_rcp(1.0f + r2lroe);
+  const float h_left = le + lp * ilr, h_right = re + rp * irr;
+  const float a_roe = f_sqrt(lr * rr);
+  const float a_u = (lu + ru * r2lroe) * r2l0;
+  const float a_v = (lvv + rvv * r2lroe) * r2l0;
+  const float a_w = (D == 3) ? (lw + rw * r2lroe) * r2l0 : 0.0f;
+  const float a_h = (h_left + h_right * r2lroe) * r2l0;
+  const float q2 = (D == 3) ? (a_u * a_u + a_v * a_v + a_w * a_w) : (a_u * a_u + a_v * a_v);
+  const float csquare = (gama - 1.0f) * (a_h - 0.5f * q2);
+  const float a_c = f_sqrt(csquare);
+  const float a_ubar = dir == 0 ? a_u : (dir == 1 ? a_v : a_w);
+  const float icsq = f_rcp(csquare);
+  const float namda3 = fabsf(a_ubar);
+  const float coef12 = 0.5f * icsq * area;
+  const float roe0cxdubar = a_roe * a_c * dubar;
+  const float faa = area * namda3;
+  const float al0 = faa * (droe - dp * icsq);
+  // all-speed eigenvalues (NS_Solver.H:66-77)
+  const float T_a = (lT + rT) * 0.5f;
+  const float V_a = (lv + rv) * 0.5f;
+  float f_Ma = fabsf(V_a) * f_rcp(f_sqrt(gama * Rg * T_a));
+  f_Ma = f_Ma < ph.Ma ? ph.Ma : f_Ma;
+  f_Ma = f_Ma > 1.0f ? 1.0f : f_Ma;
+  const float s0 = (1.0f - f_Ma) * (1.0f - f_Ma);
+  const float namda0 = f_sqrt(4.0f * csquare * f_Ma + s0 * a_ubar * a_ubar);
+  const float namda1 = fabsf(0.5f * ((1.0f + f_Ma) * a_ubar + namda0));
+  const float namda2 = fabsf(0.5f * ((1.0f + f_Ma) * a_ubar - namda0));
+  const float al1 = coef12 * namda1 * (dp + roe0cxdubar);
+  const float al2 = coef12 * namda2 * (dp - roe0cxdubar);
+  const float al3 = al0 + al1 + al2;
+  const float al4 = a_c * (al1 - al2);
+  const float al5 = faa * (a_roe * (du0 - kx * dubar));
+  const float al6 = faa * (a_roe * (du1 - ky * dubar));
+  const float al7 = (D == 3) ? faa * (a_roe * (du2 - kz * dubar)) : 0.0f;
+  const float ig1 = f_rcp(gama - 1.0f);
+  flux[0] = p0 - al3;
+  flux[1] = p1 - (a_u * al3 + kx * al4 + al5);
+  flux[2] = p2 - (a_v * al3 + ky * al4 + al6);
+  if (D == 3) {
+    flux[3] = p3 - (a_w * al3 + kz * al4 + al7);
+    flux[4] = p4 - (a_h * al3 + a_ubar * al4 + a_u * al5 + a_v * al6 + a_w * al7 - csquare * ig1 * al0);
+  } else {
+    flux[3] = p4 - (a_h * al3 + a_ubar * al4 + a_u * al5 + a_v * al6 - csquare * ig1 * al0);
+  }
+}
+
 // Viscous face flux of ComptViscousStress (NS_Solver.cpp:426-581, laminar).
 // Stencil of the face between L (low side, the reference's abox) and Rr:
 // t[0..3] = L+t1, Rr+t1, L-t1, Rr-t1 and, in 3D, t[4..7] the same for t2,
@@ -179,51 +286,78 @@ __device__ __forceinline__ void roe_flux(const Prim<R, D> &t0, const Prim<R, D> 
 template <typename R, int D>
 struct ViscPhys {
   R Rg, Re, qk, SoT;
-  R invh[3];
-  R area[3];
+  R ih0, ih1, ih2;   // 1/dh
+  R ar0, ar1, ar2;   // face areas
 };
+
+template <typename R> __device__ __forceinline__ R v_div(R a, R b) { return a / b; }
+template <> __device__ __forceinline__ float v_div<float>(float a, float b) { return a * f_rcp(b); }
+template <typename R> __device__ __forceinline__ R v_sqrt(R a) { return r_sqrt(a); }
+template <> __device__ __forceinline__ float v_sqrt<float>(float a) { return f_sqrt(a); }
+
+// four-point tangential difference pfpkesai1 (NS_Solver.H:153-154)
+template <typename R>
+__device__ __forceinline__ R pf1(R up1, R up2, R dn1, R dn2) {
+  return R(0.25) * (up1 + up2 - dn1 - dn2);
+}
 
 template <typename R, int D>
 __device__ __forceinline__ void visc_flux(const Prim<R, D> &L, const Prim<R, D> &Rr,
                                           const Prim<R, D> *t, int d, const ViscPhys<R, D> &vp,
                                           R *out) {
-  auto T_of = [&](const Prim<R, D> &q) -> R { return q.p / (q.r * vp.Rg); };
-  const R TL = T_of(L), TR = T_of(Rr);
-  // g[q][k]: d(u,v,w,T)/d(kesai_k)
-  R g[4][3];
-  R Tt[8];
-  const int nt = (D == 3) ? 8 : 4;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) Tt[k] = (k < nt) ? T_of(t[k]) : R(0);
-  auto val = [&](const Prim<R, D> &q, int var, R T) -> R {
-    return var == 0 ? q.u : (var == 1 ? q.v : (var == 2 ? q.w : T));
-  };
+  // T = p/(roe Rg) of every stencil cell (Get_T_ideal_gas)
+  const R TL = v_div(L.p, L.r * vp.Rg), TR = v_div(Rr.p, Rr.r * vp.Rg);
+  const R T0 = v_div(t[0].p, t[0].r * vp.Rg), T1 = v_div(t[1].p, t[1].r * vp.Rg);
+  const R T2 = v_div(t[2].p, t[2].r * vp.Rg), T3 = v_div(t[3].p, t[3].r * vp.Rg);
+  // normal differences (pfpkesai0) and the first tangential axis
+  const R nu = Rr.u - L.u, nv = Rr.v - L.v, nw = Rr.w - L.w, nT = TR - TL;
+  const R au = pf1(t[0].u, t[1].u, t[2].u, t[3].u);
+  const R av = pf1(t[0].v, t[1].v, t[2].v, t[3].v);
+  const R aw = (D == 3) ? pf1(t[0].w, t[1].w, t[2].w, t[3].w) : R(0);
+  const R aT = pf1(T0, T1, T2, T3);
+  R bu = R(0), bv = R(0), bw = R(0), bT = R(0);
+  if (D == 3) {
+    const R T4 = v_div(t[4 % (D == 3 ? 8 : 4)].p, t[4 % (D == 3 ? 8 : 4)].r * vp.Rg);
+    const R T5 = v_div(t[5 % (D == 3 ? 8 : 4)].p, t[5 % (D == 3 ? 8 : 4)].r * vp.Rg);
+    const R T6 = v_div(t[6 % (D == 3 ? 8 : 4)].p, t[6 % (D == 3 ? 8 : 4)].r * vp.Rg);
+    const R T7 = v_div(t[7 % (D == 3 ? 8 : 4)].p, t[7 % (D == 3 ? 8 : 4)].r * vp.Rg);
+    const Prim<R, D> &s4 = t[4 % (D == 3 ? 8 : 4)], &s5 = t[5 % (D == 3 ? 8 : 4)];
+    const Prim<R, D> &s6 = t[6 % (D == 3 ? 8 : 4)], &s7 = t[7 % (D == 3 ? 8 : 4)];
+    bu = pf1(s4.u, s5.u, s6.u, s7.u);
+    bv = pf1(s4.v, s5.v, s6.v, s7.v);
+    bw = pf1(s4.w, s5.w, s6.w, s7.w);
+    bT = pf1(T4, T5, T6, T7);
+  }
+  // gradients in kesai: normal along d, first tangential t1, second t2
+  R gu[3], gv[3], gw[3], gT[3];
   const int t1 = (d == 0) ? 1 : 0;
   const int t2 = (d == 2) ? 1 : 2;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    if (D == 2 && q == 2) { g[q][0] = g[q][1] = g[q][2] = R(0); continue; }
-    g[q][d] = val(Rr, q, TR) - val(L, q, TL);
-    g[q][t1] = R(0.25) * (val(t[0], q, Tt[0]) + val(t[1], q, Tt[1]) - val(t[2], q, Tt[2]) - val(t[3], q, Tt[3]));
-    if (D == 3)
-      g[q][t2] = R(0.25) * (val(t[4], q, Tt[4]) + val(t[5], q, Tt[5]) - val(t[6], q, Tt[6]) - val(t[7], q, Tt[7]));
-    else
-      g[q][2] = R(0);
+  for (int k = 0; k < 3; ++k) {
+    const bool isn = (k == d), is1 = (k == t1), is2 = (D == 3) && (k == t2);
+    gu[k] = isn ? nu : (is1 ? au : (is2 ? bu : R(0)));
+    gv[k] = isn ? nv : (is1 ? av : (is2 ? bv : R(0)));
+    gw[k] = isn ? nw : (is1 ? aw : (is2 ? bw : R(0)));
+    gT[k] = isn ? nT : (is1 ? aT : (is2 ? bT : R(0)));
   }
-  const R dux = g[0][0] * vp.invh[0], duy = g[0][1] * vp.invh[1], duz = g[0][2] * vp.invh[2];
-  const R dvx = g[1][0] * vp.invh[0], dvy = g[1][1] * vp.invh[1], dvz = g[1][2] * vp.invh[2];
-  const R dwx = g[2][0] * vp.invh[0], dwy = g[2][1] * vp.invh[1], dwz = g[2][2] * vp.invh[2];
-  const R dTx = g[3][0] * vp.invh[0], dTy = g[3][1] * vp.invh[1], dTz = g[3][2] * vp.invh[2];
+  if (D == 2) { gw[0] = gw[1] = gw[2] = R(0); }
+  // VarReflect with keisa = diag(1/dh)
+  const R dux = gu[0] * vp.ih0, duy = gu[1] * vp.ih1, duz = gu[2] * vp.ih2;
+  const R dvx = gv[0] * vp.ih0, dvy = gv[1] * vp.ih1, dvz = gv[2] * vp.ih2;
+  const R dwx = gw[0] * vp.ih0, dwy = gw[1] * vp.ih1, dwz = gw[2] * vp.ih2;
+  const R dTx = gT[0] * vp.ih0, dTy = gT[1] * vp.ih1, dTz = gT[2] * vp.ih2;
   const R ftemp = (TL + TR) * R(0.5);
-  const R mu0 = r_sqrt(ftemp * ftemp * ftemp) * ((R(1.0) + vp.SoT) / (ftemp + vp.SoT));
+  const R mu0 = v_sqrt(ftemp * ftemp * ftemp) * v_div(R(1.0) + vp.SoT, ftemp + vp.SoT);
   const R qk0 = vp.qk * mu0;
   const R Re = vp.Re;
   const R divu = (D == 3) ? (dux + dvy + dwz) : (dux + dvy);
-  const R tauxx = R(2.0) * mu0 * (dux - divu / R(3.0)) / Re;
-  const R tauyy = R(2.0) * mu0 * (dvy - divu / R(3.0)) / Re;
-  const R tauxy = mu0 / Re * (duy + dvx);
+  const R tauxx = v_div(R(2.0) * mu0 * (dux - v_div(divu, R(3.0))), Re);
+  const R tauyy = v_div(R(2.0) * mu0 * (dvy - v_div(divu, R(3.0))), Re);
+  const R tauxy = v_div(mu0, Re) * (duy + dvx);
   const R fu = (L.u + Rr.u) * R(0.5), fv = (L.v + Rr.v) * R(0.5);
-  const R area = vp.area[d];
+  // (x*1 + 0 + 0 is exact; written as a sum so that no per-lane table is built)
+  const R area = (d == 0 ? R(1) : R(0)) * vp.ar0 + (d == 1 ? R(1) : R(0)) * vp.ar1 +
+                 (d == 2 ? R(1) : R(0)) * vp.ar2;
   if (D == 2) {
     if (d == 0) {
       const R bx = fu * tauxx + fv * tauxy + qk0 * dTx;
@@ -233,9 +367,9 @@ __device__ __forceinline__ void visc_flux(const Prim<R, D> &L, const Prim<R, D> 
       out[0] = R(0); out[1] = tauxy * area; out[2] = tauyy * area; out[3] = by * area;
     }
   } else {
-    const R tauzz = R(2.0) * mu0 * (dwz - divu / R(3.0)) / Re;
-    const R tauxz = mu0 / Re * (duz + dwx);
-    const R tauyz = mu0 / Re * (dvz + dwy);
+    const R tauzz = v_div(R(2.0) * mu0 * (dwz - v_div(divu, R(3.0))), Re);
+    const R tauxz = v_div(mu0, Re) * (duz + dwx);
+    const R tauyz = v_div(mu0, Re) * (dvz + dwy);
     const R fw = (L.w + Rr.w) * R(0.5);
     if (d == 0) {
       const R bx = fu * tauxx + fv * tauxy + fw * tauxz + qk0 * dTx;

@@ -10,6 +10,8 @@
 #include "ibg_common.h"
 #include "ibg_flux.h"
 
+#include <type_traits>
+
 namespace ibg {
 
 constexpr int kTile = 256;   // cells per workgroup of the tiled kernels (max)
@@ -118,7 +120,10 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
     const Prim<R, D> q2 = pload<R, D>(a.W, st.z);
     const Prim<R, D> q3 = pload<R, D>(a.W, st.w);
     R flux[NV];
-    roe_flux<R, D>(q0, q1, q2, q3, dir, areas[dir], ph, a.limiter, flux);
+    if constexpr (std::is_same<R, float>::value)
+      roe_flux_fast<D>(q0, q1, q2, q3, dir, areas[dir], ph, a.limiter, flux);
+    else
+      roe_flux<R, D>(q0, q1, q2, q3, dir, areas[dir], ph, a.limiter, flux);
     V o;
     o.x = flux[0]; o.y = flux[1]; o.z = flux[2]; o.w = flux[3];
     fl4[f] = o;
@@ -126,7 +131,9 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
   }
   __syncthreads();
 
+  constexpr bool kMixed = std::is_same<R, float>::value;
   const double level_ts = a.alpha * a.dt[bi.case_id] * bi.step_num;
+  const double inv_vol = 1.0 / bi.vol;
   double mx[NV], sq[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) { mx[k] = 0.0; sq[k] = 0.0; }
@@ -167,20 +174,34 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
     if (fg & F_UPDATE) {
       double U[NV];
       vload<double, D>(a.U0, c, U);
+      if constexpr (kMixed) {
+        const double sc = level_ts * inv_vol;
 #pragma unroll
-      for (int k = 0; k < NV; ++k) U[k] = U[k] + s[k] * level_ts / bi.vol;
+        for (int k = 0; k < NV; ++k) U[k] = U[k] + s[k] * sc;
+      } else {
+        // cv_old + source * level_ts / v  (NS_Solver.cpp:870)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) U[k] = U[k] + s[k] * level_ts / bi.vol;
+      }
       if (U[0] < 0.0) report_error(a.err, c, (fg & F_FINEST) ? 611 : 523);
       if (fg & F_PRIM) {
         // Get_Prim_Vars FlowVariables.H:50-57
         wr = U[0];
-        wu = U[1] / U[0];
-        wv = U[2] / U[0];
         double e;
-        if (D == 3) { ww = U[3] / U[0]; e = U[4] / U[0]; }
-        else e = U[3] / U[0];
+        if constexpr (kMixed) {
+          const double ir = 1.0 / U[0];
+          wu = U[1] * ir;
+          wv = U[2] * ir;
+          if (D == 3) { ww = U[3] * ir; e = U[4] * ir; } else e = U[3] * ir;
+        } else {
+          wu = U[1] / U[0];
+          wv = U[2] / U[0];
+          if (D == 3) { ww = U[3] / U[0]; e = U[4] / U[0]; } else e = U[3] / U[0];
+        }
         const double ke = (D == 3) ? 0.5 * (wu * wu + wv * wv + ww * ww) : 0.5 * (wu * wu + wv * wv);
         wp = (e - ke) * (P.gama - 1.0) * wr;
-        if (wp / (wr * P.Rg) < 0.0) report_error(a.err, c, 37);
+        // T = p/(roe Rg) < 0  (NS_Solver.cpp:55-66), Rg > 0
+        if ((wp < 0.0) != (wr < 0.0)) report_error(a.err, c, 37);
         write = true;
       }
     }
@@ -189,8 +210,8 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
       write = true;
     }
     if (write) {
-      if (!a.mixed || a.last) pstore<double, D>(a.W64, c, wr, wu, wv, ww, wp);
-      if (a.mixed) pstore<R, D>(a.W, c, (R)wr, (R)wu, (R)wv, (R)ww, (R)wp);
+      if (!kMixed || a.last) pstore<double, D>(a.W64, c, wr, wu, wv, ww, wp);
+      if (kMixed) pstore<R, D>(a.W, c, (R)wr, (R)wu, (R)wv, (R)ww, (R)wp);
     }
     if (a.last) {
       if (a.src) {
@@ -267,7 +288,8 @@ __global__ void __launch_bounds__(kTile) k_visc_prep(ViscArgs<R, D> a) {
   if (a.viscous) {
     ViscPhys<R, D> vp;
     vp.Rg = (R)P.Rg; vp.Re = (R)P.Re; vp.qk = (R)P.qk; vp.SoT = (R)P.SoT;
-    for (int k = 0; k < 3; ++k) { vp.invh[k] = (R)bi.invh[k]; vp.area[k] = (R)bi.area[k]; }
+    vp.ih0 = (R)bi.invh[0]; vp.ih1 = (R)bi.invh[1]; vp.ih2 = (R)bi.invh[2];
+    vp.ar0 = (R)bi.area[0]; vp.ar1 = (R)bi.area[1]; vp.ar2 = (R)bi.area[2];
     for (int f = threadIdx.x; f < ti.nface; f += blockDim.x) {
       const int4 st = a.tst[ti.face0 + f];
       const int dir = a.tdir[ti.face0 + f];
@@ -309,9 +331,17 @@ __global__ void __launch_bounds__(kTile) k_visc_prep(ViscArgs<R, D> a) {
 #pragma unroll
   for (int k = 0; k < NV; ++k) s[k] = 0.0;
   if (a.viscous) {
+    int sl[2 * D];
+    if (D == 2) {
+      const ushort4 s4 = *reinterpret_cast<const ushort4 *>(a.slots + (size_t)c * 4);
+      sl[0] = s4.x; sl[1] = s4.y; sl[2] = s4.z; sl[3] = s4.w;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 2 * D; ++q) sl[q] = a.slots[(size_t)c * 2 * D + q];
+    }
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const int lo = a.slots[(size_t)c * 2 * D + 2 * d], hi = a.slots[(size_t)c * 2 * D + 2 * d + 1];
+      const int lo = sl[2 * d], hi = sl[2 * d + 1];
       const V flo = fl4[lo], fhi = fl4[hi];
       s[0] += (double)fhi.x - (double)flo.x;
       s[1] += (double)fhi.y - (double)flo.y;
@@ -646,7 +676,8 @@ __global__ void k_visc_probe(int nface, const int4 *st, const int4 *vst, const i
   const DevPhys &P = phys[bi.case_id];
   ViscPhys<R, D> vp;
   vp.Rg = (R)P.Rg; vp.Re = (R)P.Re; vp.qk = (R)P.qk; vp.SoT = (R)P.SoT;
-  for (int k = 0; k < 3; ++k) { vp.invh[k] = (R)bi.invh[k]; vp.area[k] = (R)bi.area[k]; }
+  vp.ih0 = (R)bi.invh[0]; vp.ih1 = (R)bi.invh[1]; vp.ih2 = (R)bi.invh[2];
+  vp.ar0 = (R)bi.area[0]; vp.ar1 = (R)bi.area[1]; vp.ar2 = (R)bi.area[2];
   const int4 s = st[f];
   Prim<R, D> t[(D == 3) ? 8 : 4];
   const int4 v0 = vst[(size_t)f * (D - 1)];
