@@ -37,6 +37,25 @@ struct DevPhys {
   double inlet_T0, inlet_p0, inlet_cos, inlet_sin, outlet_p;
   double cfl, dt_ratio;
   double force_ref, p_ref_area, shear_ref_area;
+  // SA closure: freestream and inlet working variable, turbulent heat
+  // conduction denominator Re*Pr_tur*(gama-1)*Ma^2 (NS_Solver.cpp:540),
+  // Pr^(1/3) of the wall-temperature relation (FlowConstants.cpp:44)
+  double nut_inf, nut_inlet, qkt_den, pr_cuberoot;
+};
+
+// Every copy of the cell state that a once-per-step ghost fill must keep
+// consistent: the FP64 master and the stencil buffers the stages rotate
+// through.  Plain mode: stencil buffer 0 is the master; d1, d2 are extra
+// FP64 buffers.  Mixed mode: f0 (home), f1, f2 are FP32 buffers.  The SA
+// working variable has the same set of copies (nm, nd*, nf*; nm == nullptr
+// when the closure is off).
+template <int D>
+struct StateOut {
+  Pack<double, D> m, d1, d2;
+  Pack<float, D> f0, f1, f2;
+  double *nm, *nd1, *nd2;
+  float *nf0, *nf1, *nf2;
+  int nd, nf;
 };
 
 struct BlkInfo {

@@ -9,9 +9,15 @@
 // one launch over a prefix.
 //
 // State is a packed per-cell record: W64 = (roe,u,v,p) [2D] or
-// (roe,u,v,w)+p [3D] in FP64 is the master copy; the mixed mode keeps an
-// FP32 shadow W32 that feeds the flux stencils.  T and e are derived
-// (T = p/(roe Rg), e = Cv T + |u|^2/2, FlowVariables.H:23,33).
+// (roe,u,v,w)+p [3D] in FP64 is the master copy.  The flux stencils read
+// one of three stencil buffers S0, S1, S2 (FP64 in the plain mode, where S0
+// is the master itself; FP32 in the mixed mode).  Stage k reads one buffer
+// and writes another (S0 -> S1 -> S2 -> ... -> S0, the last stage always
+// back into S0), so no workgroup of a launch sees a cell another workgroup
+// has already advanced.  Ghost fills write the master and every buffer.  T
+// and e are derived (T = p/(roe Rg), e = Cv T + |u|^2/2, FlowVariables.H:
+// 23,33).  The SA working variable, when the closure is on, has the same
+// master + stencil buffer structure.
 //
 // Hot kernel: k_stage (ibg_kernels.h), one workgroup per tile of TILE
 // Morton-consecutive normal cells; face fluxes live only in LDS.
@@ -49,10 +55,17 @@ struct ibg_ctx {
   std::vector<int64_t> upd_per_lmin;
   int ndmg_edge = 0, ndmg_corner = 0;
   // device buffers (packed records)
-  void *W64a = nullptr, *W32a = nullptr, *U0a = nullptr, *Rva = nullptr;
-  double *W64e = nullptr, *U0e = nullptr;
-  float *W32e = nullptr;
+  void *W64a = nullptr, *Rva = nullptr;
+  double *W64e = nullptr;
   void *Rve = nullptr;
+  int nbuf = 3;                                  // stencil buffers in use
+  void *Sa[3] = {nullptr, nullptr, nullptr};     // stencil buffers (type R)
+  void *Se[3] = {nullptr, nullptr, nullptr};
+  bool turb = false;
+  double *nu64 = nullptr;                        // SA working variable, master
+  void *NUs[3] = {nullptr, nullptr, nullptr};    // its stencil buffers (type R)
+  void *mutf = nullptr;                          // [Nn] (type R)
+  double *nuprobe = nullptr, *mprobe = nullptr;
   uint8_t *flags = nullptr;
   int *cellcase = nullptr, *nb27 = nullptr;
   TileInfo *tiles = nullptr;
@@ -67,7 +80,8 @@ struct ibg_ctx {
   int4 *dmg = nullptr;
   int *res_ci = nullptr, *res_fi = nullptr, *pro_ci = nullptr, *pro_fi = nullptr;
   int *ib_cell = nullptr, *ib_intp = nullptr, *ib_body = nullptr, *ib_case = nullptr;
-  double *ib_coef = nullptr, *ib_hgdis = nullptr, *ib_attach = nullptr, *signdis = nullptr;
+  double *ib_coef = nullptr, *ib_hgdis = nullptr, *ib_attach = nullptr, *ib_nv = nullptr,
+         *signdis = nullptr;
   int *pat_intp = nullptr, *pat_case = nullptr;
   double *pat_coef = nullptr, *pat_hgdis = nullptr, *pat_nv = nullptr, *pat_pc = nullptr,
          *pat_area = nullptr, *case_center = nullptr;
@@ -84,14 +98,29 @@ struct ibg_ctx {
   std::string errmsg;
 
   template <int D> Pack<double, D> w64() const { return Pack<double, D>{(D4 *)W64a, W64e}; }
-  template <int D> Pack<float, D> w32() const { return Pack<float, D>{(F4 *)W32a, W32e}; }
-  template <int D> Pack<double, D> u0() const { return Pack<double, D>{(D4 *)U0a, U0e}; }
   template <typename R, int D> Pack<R, D> rv() const {
     return Pack<R, D>{(typename V4<R>::T *)Rva, (R *)Rve};
   }
-  template <typename R, int D> Pack<R, D> wstencil() const {
-    if (std::is_same<R, float>::value) return Pack<R, D>{(typename V4<R>::T *)W32a, (R *)W32e};
-    return Pack<R, D>{(typename V4<R>::T *)W64a, (R *)W64e};
+  template <typename R, int D> Pack<R, D> sbuf(int k) const {
+    return Pack<R, D>{(typename V4<R>::T *)Sa[k], (R *)Se[k]};
+  }
+  template <typename R, int D> Pack<R, D> wstencil() const { return sbuf<R, D>(0); }
+  template <typename R> R *nubuf(int k) const { return (R *)NUs[k]; }
+  template <int D> StateOut<D> sout() const {
+    StateOut<D> o{};
+    o.m = w64<D>();
+    o.nm = turb ? nu64 : nullptr;
+    if (!mixed) {
+      o.nd = nbuf - 1;
+      if (nbuf > 1) { o.d1 = sbuf<double, D>(1); o.nd1 = (double *)NUs[1]; }
+      if (nbuf > 2) { o.d2 = sbuf<double, D>(2); o.nd2 = (double *)NUs[2]; }
+    } else {
+      o.nf = nbuf;
+      o.f0 = sbuf<float, D>(0); o.nf0 = (float *)NUs[0];
+      if (nbuf > 1) { o.f1 = sbuf<float, D>(1); o.nf1 = (float *)NUs[1]; }
+      if (nbuf > 2) { o.f2 = sbuf<float, D>(2); o.nf2 = (float *)NUs[2]; }
+    }
+    return o;
   }
 };
 
@@ -156,41 +185,74 @@ inline int transfer_due(const ibg_ctx *c, int cl, int ts) {
 }
 inline unsigned grid(int n, int b) { return (unsigned)((n + b - 1) / b); }
 
-size_t tile_smem(const ibg_ctx *c, size_t rsize) {
-  return (size_t)c->max_tile_faces * (4 * rsize + (c->D == 3 ? rsize : 0));
+// LDS of the tiled kernels: face fluxes (4 values + the 3D fifth), then for
+// the SA closure either the two (u,v,w,nu) face-cell records (stage) or the
+// face flux of the working variable (viscous pass).
+size_t tile_smem(const ibg_ctx *c, size_t rsize, bool stage) {
+  size_t per = 4 * rsize + (c->D == 3 ? rsize : 0);
+  if (c->turb) per += stage ? 8 * rsize : rsize;
+  return (size_t)c->max_tile_faces * per;
+}
+
+// Stencil buffers of stage k: S0 -> S1 -> S2 -> S1 ... with the last stage
+// writing S0; a single stage writes S1 and is copied back.
+void stage_bufs(int nstage, int subt, int *in, int *out) {
+  int cur = 0;
+  for (int k = 0;; ++k) {
+    const int o = (k == nstage - 1 && nstage > 1) ? 0 : (cur == 1 ? 2 : 1);
+    if (k == subt) { *in = cur; *out = o; return; }
+    cur = o;
+  }
 }
 
 template <typename R, int D>
-ibg_status launch_stage(ibg_ctx *c, int ntile, int subt, bool last, double *src) {
+ibg_status launch_stage(ibg_ctx *c, int ntile, int ncell, int subt, bool last, double *src,
+                        double *nusrc) {
   if (ntile <= 0) return IBG_OK;
+  int bin = 0, bout = 0;
+  stage_bufs(c->cfg.nstage, subt, &bin, &bout);
   StageArgs<R, D> a;
   a.tiles = c->tiles; a.tst = c->tst; a.tdir = c->tdir; a.slots = c->slots;
   a.flags = c->flags; a.blk = c->blk; a.phys = c->phys; a.dt = c->dt;
-  a.W = c->wstencil<R, D>();
+  a.Win = c->sbuf<R, D>(bin);
+  a.Wout = c->sbuf<R, D>(bout);
   a.W64 = c->w64<D>();
-  a.U0 = c->u0<D>();
   a.Rv = c->rv<R, D>();
+  a.turb = c->turb ? 1 : 0;
+  a.nuin = c->nubuf<R>(bin);
+  a.nuout = c->nubuf<R>(bout);
+  a.nu64 = c->nu64;
+  a.mutf = (const R *)c->mutf;
+  a.signdis = c->signdis;
   a.alpha = c->cfg.alpha[subt];
   a.limiter = c->cfg.limiter;
   a.last = last ? 1 : 0;
-  a.mixed = c->mixed ? 1 : 0;
   a.rpart = c->rpart;
   a.src = src;
+  a.nusrc = nusrc;
   a.err = c->err;
-  hipLaunchKernelGGL((k_stage<R, D>), dim3(ntile), dim3(c->tile), tile_smem(c, sizeof(R)), c->s, a);
+  hipLaunchKernelGGL((k_stage<R, D>), dim3(ntile), dim3(c->tile), tile_smem(c, sizeof(R), true), c->s, a);
   HIPCHK(hipGetLastError());
+  if (c->cfg.nstage == 1) {
+    // single-stage scheme: the marched cells go back into the home buffer
+    hipLaunchKernelGGL((k_copy_back<R, D>), dim3(grid(ncell, 256)), dim3(256), 0, c->s, ncell,
+                       c->sbuf<R, D>(bout), c->sbuf<R, D>(0), c->nubuf<R>(bout),
+                       c->turb ? c->nubuf<R>(0) : (R *)nullptr);
+    HIPCHK(hipGetLastError());
+  }
   return IBG_OK;
 }
 
-ibg_status run_stage(ibg_ctx *c, int ts, int subt, double *src) {
-  const int nt = c->tile_end[lmin_of(c, ts)];
+ibg_status run_stage(ibg_ctx *c, int ts, int subt, double *src, double *nusrc) {
+  const int lm = lmin_of(c, ts);
+  const int nt = c->tile_end[lm], ncell = c->cell_end[lm];
   const bool last = (subt == c->cfg.nstage - 1);
   if (last) c->last_ntiles = nt;
   if (c->D == 2)
-    return c->mixed ? launch_stage<float, 2>(c, nt, subt, last, src)
-                    : launch_stage<double, 2>(c, nt, subt, last, src);
-  return c->mixed ? launch_stage<float, 3>(c, nt, subt, last, src)
-                  : launch_stage<double, 3>(c, nt, subt, last, src);
+    return c->mixed ? launch_stage<float, 2>(c, nt, ncell, subt, last, src, nusrc)
+                    : launch_stage<double, 2>(c, nt, ncell, subt, last, src, nusrc);
+  return c->mixed ? launch_stage<float, 3>(c, nt, ncell, subt, last, src, nusrc)
+                  : launch_stage<double, 3>(c, nt, ncell, subt, last, src, nusrc);
 }
 
 template <typename R, int D>
@@ -199,12 +261,13 @@ ibg_status launch_visc_prep(ibg_ctx *c, int ntile) {
   ViscArgs<R, D> a;
   a.tiles = c->tiles; a.tst = c->tst; a.tvs = c->tvs; a.tdir = c->tdir; a.slots = c->slots;
   a.blk = c->blk; a.phys = c->phys;
-  a.W = c->wstencil<R, D>();
-  a.W64 = c->w64<D>();
-  a.U0 = c->u0<D>();
+  a.W = c->sbuf<R, D>(0);
   a.Rv = c->rv<R, D>();
+  a.nu = c->nubuf<R>(0);
+  a.mutf = (R *)c->mutf;
   a.viscous = c->cfg.viscous;
-  hipLaunchKernelGGL((k_visc_prep<R, D>), dim3(ntile), dim3(c->tile), tile_smem(c, sizeof(R)), c->s, a);
+  a.turb = c->turb ? 1 : 0;
+  hipLaunchKernelGGL((k_visc_prep<R, D>), dim3(ntile), dim3(c->tile), tile_smem(c, sizeof(R), false), c->s, a);
   HIPCHK(hipGetLastError());
   return IBG_OK;
 }
@@ -220,13 +283,12 @@ template <int D>
 ibg_status launch_bc(ibg_ctx *c) {
   if (c->ndmg_edge > 0) {
     hipLaunchKernelGGL((k_bc<D>), dim3(grid(c->ndmg_edge, 256)), dim3(256), 0, c->s, c->ndmg_edge,
-                       c->dmg, c->w64<D>(), c->w32<D>(), (int)c->mixed, c->phys);
+                       c->dmg, c->sout<D>(), c->phys);
     HIPCHK(hipGetLastError());
   }
   if (c->ndmg_corner > 0) {
     hipLaunchKernelGGL((k_bc<D>), dim3(grid(c->ndmg_corner, 256)), dim3(256), 0, c->s,
-                       c->ndmg_corner, c->dmg + c->ndmg_edge, c->w64<D>(), c->w32<D>(),
-                       (int)c->mixed, c->phys);
+                       c->ndmg_corner, c->dmg + c->ndmg_edge, c->sout<D>(), c->phys);
     HIPCHK(hipGetLastError());
   }
   return IBG_OK;
@@ -246,8 +308,7 @@ ibg_status launch_exchange(ibg_ctx *c, int ts) {
     if (n > 0) {
       hipLaunchKernelGGL((k_restrict<D>), dim3(grid(n, 256)), dim3(256), 0, c->s, n,
                          c->res_ci + c->res_off[l], c->res_fi + (size_t)8 * c->res_off[l],
-                         c->w64<D>(), c->w32<D>(), (int)c->mixed, c->blk_twod[l + 1], c->phys,
-                         c->cellcase);
+                         c->sout<D>(), c->blk_twod[l + 1], c->phys, c->cellcase);
       HIPCHK(hipGetLastError());
     }
   }
@@ -257,8 +318,7 @@ ibg_status launch_exchange(ibg_ctx *c, int ts) {
     if (n > 0) {
       hipLaunchKernelGGL((k_prolong<D>), dim3(grid(n, 256)), dim3(256), 0, c->s, n,
                          c->pro_ci + c->pro_off[l], c->pro_fi + (size_t)8 * c->pro_off[l],
-                         c->nb27, c->w64<D>(), c->w32<D>(), (int)c->mixed, c->blk_twod[l + 1],
-                         c->phys, c->cellcase);
+                         c->nb27, c->sout<D>(), c->blk_twod[l + 1], c->phys, c->cellcase);
       HIPCHK(hipGetLastError());
     }
   }
@@ -273,9 +333,9 @@ template <int D>
 ibg_status launch_ib(ibg_ctx *c) {
   if (c->nib > 0) {
     hipLaunchKernelGGL((k_ib<D>), dim3(grid(c->nib, 256)), dim3(256), 0, c->s, c->nib, c->ib_cell,
-                       c->ib_intp, c->ib_coef, c->ib_hgdis, c->ib_attach, c->ib_body, c->ib_case,
-                       c->signdis, c->motion, c->have_motion, c->w64<D>(), c->w32<D>(),
-                       (int)c->mixed, c->phys, c->err);
+                       c->ib_intp, c->ib_coef, c->ib_hgdis, c->ib_attach, c->ib_nv, c->ib_body,
+                       c->ib_case, c->signdis, c->motion, c->have_motion, c->sout<D>(), c->phys,
+                       c->err);
     HIPCHK(hipGetLastError());
   }
   return IBG_OK;
@@ -309,7 +369,7 @@ ibg_status run_step(ibg_ctx *c, int ts) {
   if ((st = run_bc(c, ts))) return st;
   if ((st = run_visc_prep(c, ts))) return st;
   for (int k = 0; k < c->cfg.nstage; ++k)
-    if ((st = run_stage(c, ts, k, nullptr))) return st;
+    if ((st = run_stage(c, ts, k, nullptr, nullptr))) return st;
   return IBG_OK;
 }
 
@@ -447,6 +507,8 @@ ibg_status ibg_create(const ibg_config *cfg, ibg_ctx **out) {
   if (!cfg || !out) return IBG_ERR_ARG;
   if (!(cfg->nstage == 1 || cfg->nstage == 3 || cfg->nstage == 4)) return IBG_ERR_ARG;
   if (cfg->mode != IBG_MODE_FP64 && cfg->mode != IBG_MODE_MIXED) return IBG_ERR_ARG;
+  if (cfg->turbulence != 0 && cfg->turbulence != 1) return IBG_ERR_ARG;
+  if (cfg->turbulence && !cfg->viscous) return IBG_ERR_ARG;
   const int tile = cfg->tile > 0 ? cfg->tile : kTile;
   if (tile % 64 != 0 || tile > kTile) return IBG_ERR_ARG;
   int ndev = 0;
@@ -457,6 +519,8 @@ ibg_status ibg_create(const ibg_config *cfg, ibg_ctx **out) {
   ibg_ctx *c = new ibg_ctx();
   c->cfg = *cfg;
   c->mixed = (cfg->mode == IBG_MODE_MIXED);
+  c->turb = (cfg->turbulence != 0);
+  c->nbuf = (cfg->nstage == 1) ? 2 : 3;
   c->tile = tile;
   // default stream priority (no priority streams)
   if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) { delete c; return IBG_ERR_HIP; }
@@ -488,6 +552,27 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
     }
   }
   if (m->ncases > IBG_MAX_CASES || m->nlevels > IBG_MAX_LEVELS) return IBG_ERR_ARG;
+  if (c->turb) {
+    // the SA source takes its face neighbours from the face list: check that
+    // they are the Box::neib neighbours ComptMutSource reads (NS_Solver.cpp:
+    // 662-666), and that the wall model has its normals
+    if (m->nib > 0 && !m->ib_nv) { c->errmsg = "SA closure needs ib_nv"; return IBG_ERR_ARG; }
+    for (int b = 0; b < m->ncases * m->nlevels; ++b)
+      for (int g = m->blk_start[b]; g < m->blk_start[b] + m->blk_nnormal[b]; ++g)
+        for (int d = 0; d < m->dim; ++d) {
+          int e[3] = {1, 1, 1};
+          e[d] = 0;
+          const int lo = m->nb[(size_t)g * 27 + e[0] * 9 + e[1] * 3 + e[2]];
+          e[d] = 2;
+          const int hi = m->nb[(size_t)g * 27 + e[0] * 9 + e[1] * 3 + e[2]];
+          const int flo = m->cell_faces[(size_t)g * 6 + d * 2], fhi = m->cell_faces[(size_t)g * 6 + d * 2 + 1];
+          if (m->face_cells[2 * flo] != lo || m->face_cells[2 * fhi + 1] != hi) {
+            c->errmsg = "face list and neighbour table disagree";
+            fprintf(stderr, "ibgpu: mesh rejected: %s\n", c->errmsg.c_str());
+            return IBG_ERR_ARG;
+          }
+        }
+  }
   (void)hipStreamSynchronize(c->s);
   free_all(c);
   const int D = m->dim, NV = D + 2, L = m->nlevels, nc = m->ncases, nblk = nc * L;
@@ -559,6 +644,9 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
     P.inlet_cos = cos(p.inlet_angle); P.inlet_sin = sin(p.inlet_angle);
     P.outlet_p = p.outlet_p; P.cfl = p.cfl; P.dt_ratio = p.dt_ratio;
     P.force_ref = p.force_ref; P.p_ref_area = p.p_ref_area; P.shear_ref_area = p.shear_ref_area;
+    P.nut_inf = p.nut_inf; P.nut_inlet = p.nut_inlet;
+    P.qkt_den = p.Re * p.Pr_tur * (p.gama - 1) * pow(p.Ma, 2);
+    P.pr_cuberoot = pow(p.Pr, 1.0 / 3.0);
   }
   // ---- per-cell arrays (device order)
   std::vector<int16_t> cellblk(N);
@@ -728,7 +816,8 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
   const int nib = m->nib;
   c->nib = nib;
   std::vector<int> ib_cell(nib), ib_intp((size_t)nib * 8), ib_body(nib), ib_case(nib);
-  std::vector<double> ib_coef((size_t)nib * 8), ib_hgdis(nib), ib_att((size_t)nib * 3);
+  std::vector<double> ib_coef((size_t)nib * 8), ib_hgdis(nib), ib_att((size_t)nib * 3),
+      ib_nvh((size_t)nib * 3, 0.0);
   for (int i = 0; i < nib; ++i) {
     const int g = m->ib_cell[i];
     ib_cell[i] = h2d[g];
@@ -736,6 +825,8 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
     ib_case[i] = hblk[g] / L;
     ib_hgdis[i] = m->ib_hgdis[i];
     for (int k = 0; k < 3; ++k) ib_att[(size_t)i * 3 + k] = m->ib_attach[(size_t)i * 3 + k];
+    if (m->ib_nv)
+      for (int k = 0; k < 3; ++k) ib_nvh[(size_t)i * 3 + k] = m->ib_nv[(size_t)i * 3 + k];
     for (int q = 0; q < 8; ++q) {
       int a = m->ib_intp[(size_t)i * 8 + q];
       double w = m->ib_coef[(size_t)i * 8 + q];
@@ -779,16 +870,28 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
 #define AL(dst, n) if ((st = dalloc(c, &dst, n))) return st
   const size_t rs = c->mixed ? sizeof(float) : sizeof(double);
   if ((st = dalloc_bytes(c, &c->W64a, sizeof(D4) * (size_t)N))) return st;
-  if ((st = dalloc_bytes(c, &c->U0a, sizeof(D4) * (size_t)Nn))) return st;
   if ((st = dalloc_bytes(c, &c->Rva, 4 * rs * (size_t)Nn))) return st;
   if (D == 3) {
     AL(c->W64e, N);
-    AL(c->U0e, Nn);
     if ((st = dalloc_bytes(c, &c->Rve, rs * (size_t)Nn))) return st;
   }
-  if (c->mixed) {
-    if ((st = dalloc_bytes(c, &c->W32a, sizeof(F4) * (size_t)N))) return st;
-    if (D == 3) AL(c->W32e, N);
+  // stencil buffers: S0 is the master in the plain mode
+  for (int k = 0; k < 3; ++k) { c->Sa[k] = nullptr; c->Se[k] = nullptr; c->NUs[k] = nullptr; }
+  c->nu64 = nullptr; c->mutf = nullptr; c->nuprobe = nullptr; c->mprobe = nullptr;
+  if (c->turb) {
+    AL(c->nu64, N);
+    if ((st = dalloc_bytes(c, &c->mutf, rs * (size_t)Nn))) return st;
+    AL(c->nuprobe, N);
+    AL(c->mprobe, (size_t)std::max(F, 1));
+  }
+  for (int k = 0; k < c->nbuf; ++k) {
+    if (k == 0 && !c->mixed) {
+      c->Sa[0] = c->W64a; c->Se[0] = c->W64e; c->NUs[0] = c->nu64;
+      continue;
+    }
+    if ((st = dalloc_bytes(c, &c->Sa[k], 4 * rs * (size_t)N))) return st;
+    if (D == 3 && (st = dalloc_bytes(c, &c->Se[k], rs * (size_t)N))) return st;
+    if (c->turb && (st = dalloc_bytes(c, &c->NUs[k], rs * (size_t)N))) return st;
   }
   UP(c->flags, flags);
   UP(c->cellcase, cellcase);
@@ -808,7 +911,7 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
   UP(c->dmg, dmg);
   UP(c->res_ci, res_ci); UP(c->res_fi, res_fi); UP(c->pro_ci, pro_ci); UP(c->pro_fi, pro_fi);
   UP(c->ib_cell, ib_cell); UP(c->ib_intp, ib_intp); UP(c->ib_body, ib_body); UP(c->ib_case, ib_case);
-  UP(c->ib_coef, ib_coef); UP(c->ib_hgdis, ib_hgdis); UP(c->ib_attach, ib_att);
+  UP(c->ib_coef, ib_coef); UP(c->ib_hgdis, ib_hgdis); UP(c->ib_attach, ib_att); UP(c->ib_nv, ib_nvh);
   UP(c->signdis, sd);
   UP(c->pat_intp, pat_intp); UP(c->pat_case, pat_case); UP(c->pat_coef, pat_coef);
   UP(c->pat_hgdis, pat_hgdis); UP(c->pat_nv, pat_nv); UP(c->pat_pc, pat_pc); UP(c->pat_area, pat_area);
@@ -819,8 +922,8 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
   UP(c->dt, dt0);
   AL(c->cflpart, (size_t)std::max(c->ntiles, 1));
   AL(c->err, 1);
-  AL(c->rpart, (size_t)std::max(c->ntiles, 1) * 10);
-  AL(c->resid, (size_t)nc * 10);
+  AL(c->rpart, (size_t)std::max(c->ntiles, 1) * kRpart);
+  AL(c->resid, (size_t)nc * kRpart);
   AL(c->probe, (size_t)std::max(N, F) * 5);
 #undef UP
 #undef AL
@@ -859,12 +962,47 @@ ibg_status ibg_set_state(ibg_ctx *c, const double *fv, int64_t n) {
       if (c->D == 3) e32[i] = (float)q[4];
     }
   }
+  HIPCHK(hipStreamSynchronize(c->s));
   HIPCHK(hipMemcpy(c->W64a, a.data(), sizeof(D4) * N, hipMemcpyHostToDevice));
   if (c->D == 3) HIPCHK(hipMemcpy(c->W64e, e.data(), sizeof(double) * N, hipMemcpyHostToDevice));
-  if (c->mixed) {
-    HIPCHK(hipMemcpy(c->W32a, a32.data(), sizeof(F4) * N, hipMemcpyHostToDevice));
-    if (c->D == 3) HIPCHK(hipMemcpy(c->W32e, e32.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+  for (int k = 0; k < c->nbuf; ++k) {
+    if (c->Sa[k] == c->W64a) continue;
+    if (c->mixed) {
+      HIPCHK(hipMemcpy(c->Sa[k], a32.data(), sizeof(F4) * N, hipMemcpyHostToDevice));
+      if (c->D == 3) HIPCHK(hipMemcpy(c->Se[k], e32.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    } else {
+      HIPCHK(hipMemcpy(c->Sa[k], a.data(), sizeof(D4) * N, hipMemcpyHostToDevice));
+      if (c->D == 3) HIPCHK(hipMemcpy(c->Se[k], e.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    }
   }
+  return IBG_OK;
+}
+
+ibg_status ibg_set_nut(ibg_ctx *c, const double *nut, int64_t n) {
+  if (!c || !nut || !c->have_mesh || n != c->N) return IBG_ERR_ARG;
+  if (!c->turb) return IBG_ERR_STATE;
+  const int N = c->N;
+  std::vector<double> a(N);
+  std::vector<float> a32(N);
+  for (int i = 0; i < N; ++i) { a[i] = nut[c->d2h[i]]; a32[i] = (float)a[i]; }
+  HIPCHK(hipStreamSynchronize(c->s));
+  HIPCHK(hipMemcpy(c->nu64, a.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+  for (int k = 0; k < c->nbuf; ++k) {
+    if (c->NUs[k] == (void *)c->nu64) continue;
+    if (c->mixed) HIPCHK(hipMemcpy(c->NUs[k], a32.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    else HIPCHK(hipMemcpy(c->NUs[k], a.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+  }
+  return IBG_OK;
+}
+
+ibg_status ibg_get_nut(ibg_ctx *c, double *nut, int64_t n) {
+  if (!c || !nut || !c->have_mesh || n != c->N) return IBG_ERR_ARG;
+  if (!c->turb) return IBG_ERR_STATE;
+  const int N = c->N;
+  std::vector<double> a(N);
+  HIPCHK(hipStreamSynchronize(c->s));
+  HIPCHK(hipMemcpy(a.data(), c->nu64, sizeof(double) * N, hipMemcpyDeviceToHost));
+  for (int i = 0; i < N; ++i) nut[c->d2h[i]] = a[i];
   return IBG_OK;
 }
 
@@ -945,7 +1083,7 @@ ibg_status ibg_stage(ibg_ctx *c, int32_t ts, int32_t subt) {
   if (!c || !c->have_mesh) return IBG_ERR_STATE;
   if (subt < 0 || subt >= c->cfg.nstage) return IBG_ERR_ARG;
   const bool last = (subt == c->cfg.nstage - 1);
-  return run_stage(c, ts, subt, last ? c->probe : nullptr);
+  return run_stage(c, ts, subt, last ? c->probe : nullptr, last ? c->nuprobe : nullptr);
 }
 
 ibg_status ibg_time_march(ibg_ctx *c, int32_t ts0, int32_t nsteps, int32_t use_graph) {
@@ -1005,14 +1143,30 @@ ibg_status ibg_residual(ibg_ctx *c, double *linf, double *l2) {
   hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(64), 0, c->s, c->last_ntiles, c->tiles, c->blk,
                      c->rpart, c->resid, c->ncases);
   HIPCHK(hipGetLastError());
-  std::vector<double> h((size_t)c->ncases * 10);
+  std::vector<double> h((size_t)c->ncases * kRpart);
   HIPCHK(hipMemcpyAsync(h.data(), c->resid, h.size() * 8, hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   for (int cs = 0; cs < c->ncases; ++cs)
     for (int k = 0; k < 5; ++k) {
-      if (linf) linf[cs * 5 + k] = h[cs * 10 + k];
-      if (l2) l2[cs * 5 + k] = h[cs * 10 + 5 + k];
+      if (linf) linf[cs * 5 + k] = h[cs * kRpart + k];
+      if (l2) l2[cs * 5 + k] = h[cs * kRpart + 5 + k];
     }
+  return IBG_OK;
+}
+
+ibg_status ibg_residual_nut(ibg_ctx *c, double *linf, double *l2) {
+  if (!c || !c->have_mesh) return IBG_ERR_STATE;
+  if (!c->turb) return IBG_ERR_STATE;
+  hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(64), 0, c->s, c->last_ntiles, c->tiles, c->blk,
+                     c->rpart, c->resid, c->ncases);
+  HIPCHK(hipGetLastError());
+  std::vector<double> h((size_t)c->ncases * kRpart);
+  HIPCHK(hipMemcpyAsync(h.data(), c->resid, h.size() * 8, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  for (int cs = 0; cs < c->ncases; ++cs) {
+    if (linf) linf[cs] = h[cs * kRpart + 10];
+    if (l2) l2[cs] = h[cs * kRpart + 11];
+  }
   return IBG_OK;
 }
 
@@ -1039,11 +1193,11 @@ ibg_status ibg_forces(ibg_ctx *c, const ibg_body_motion *mo, double *out) {
   if (c->D == 2)
     hipLaunchKernelGGL((k_forces<2>), dim3(c->ncases), dim3(256), 0, c->s, c->npatch, c->pat_case,
                        c->pat_intp, c->pat_coef, c->pat_hgdis, c->pat_nv, c->pat_pc, c->pat_area,
-                       c->case_center, c->motion, hm, c->w64<2>(), c->phys, c->resid);
+                       c->case_center, c->motion, hm, c->w64<2>(), c->phys, (int)c->turb, c->resid);
   else
     hipLaunchKernelGGL((k_forces<3>), dim3(c->ncases), dim3(256), 0, c->s, c->npatch, c->pat_case,
                        c->pat_intp, c->pat_coef, c->pat_hgdis, c->pat_nv, c->pat_pc, c->pat_area,
-                       c->case_center, c->motion, hm, c->w64<3>(), c->phys, c->resid);
+                       c->case_center, c->motion, hm, c->w64<3>(), c->phys, (int)c->turb, c->resid);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, c->resid, sizeof(double) * 6 * c->ncases, hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
@@ -1060,7 +1214,7 @@ void launch_probes(ibg_ctx *c, bool visc) {
   if (visc)
     hipLaunchKernelGGL((k_visc_probe<R, D>), dim3(grid(F, 256)), dim3(256), 0, c->s, F, c->face_st,
                        c->face_vst, c->face_dir, c->face_blk, c->wstencil<R, D>(), c->blk, c->phys,
-                       c->probe);
+                       c->turb ? c->nubuf<R>(0) : (const R *)nullptr, c->probe, c->mprobe);
   else
     hipLaunchKernelGGL((k_face_probe<R, D>), dim3(grid(F, 256)), dim3(256), 0, c->s, F, c->face_st,
                        c->face_dir, c->face_blk, c->wstencil<R, D>(), c->blk, c->phys,
@@ -1099,6 +1253,28 @@ ibg_status ibg_get_source(ibg_ctx *c, double *out, int64_t n) {
   HIPCHK(hipMemcpy(h.data(), c->probe, h.size() * 8, hipMemcpyDeviceToHost));
   for (int i = 0; i < c->N; ++i)
     for (int k = 0; k < 5; ++k) out[(size_t)c->d2h[i] * 5 + k] = (i < c->Nn) ? h[(size_t)i * 5 + k] : 0.0;
+  return IBG_OK;
+}
+
+ibg_status ibg_get_mutflux(ibg_ctx *c, double *out, int64_t nf) {
+  if (!c || !c->have_mesh || !out || nf != c->F) return IBG_ERR_ARG;
+  if (!c->turb) return IBG_ERR_STATE;
+  std::vector<double> tmp((size_t)c->F * 5);
+  ibg_status st = face_probe(c, tmp.data(), true);
+  if (st) return st;
+  std::vector<double> h((size_t)c->F);
+  HIPCHK(hipMemcpy(h.data(), c->mprobe, h.size() * 8, hipMemcpyDeviceToHost));
+  for (int hf = 0; hf < c->F; ++hf) out[hf] = h[(size_t)c->fh2d[hf]];
+  return IBG_OK;
+}
+
+ibg_status ibg_get_nut_source(ibg_ctx *c, double *out, int64_t n) {
+  if (!c || !c->have_mesh || !out || n != c->N) return IBG_ERR_ARG;
+  if (!c->turb) return IBG_ERR_STATE;
+  HIPCHK(hipStreamSynchronize(c->s));
+  std::vector<double> h((size_t)c->N);
+  HIPCHK(hipMemcpy(h.data(), c->nuprobe, h.size() * 8, hipMemcpyDeviceToHost));
+  for (int i = 0; i < c->N; ++i) out[(size_t)c->d2h[i]] = (i < c->Nn) ? h[(size_t)i] : 0.0;
   return IBG_OK;
 }
 
@@ -1149,7 +1325,7 @@ ibg_status ibg_bench_stage(ibg_ctx *c, int32_t ts, int32_t subt, int32_t reps, d
   ibg_status st;
   HIPCHK(hipEventRecord(c->ev0, c->s));
   for (int r = 0; r < reps; ++r)
-    if ((st = run_stage(c, ts, subt, nullptr))) return st;
+    if ((st = run_stage(c, ts, subt, nullptr, nullptr))) return st;
   HIPCHK(hipEventRecord(c->ev1, c->s));
   HIPCHK(hipEventSynchronize(c->ev1));
   float ms = 0.f;

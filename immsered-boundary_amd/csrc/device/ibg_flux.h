@@ -288,12 +288,42 @@ struct ViscPhys {
   R Rg, Re, qk, SoT;
   R ih0, ih1, ih2;   // 1/dh
   R ar0, ar1, ar2;   // face areas
+  R qkt;             // Re*Pr_tur*(gama-1)*Ma^2 (SA closure)
 };
+
+// Spalart-Allmaras constants, Turbulence.cpp:3-19
+struct SA {
+  static constexpr double cb1 = 0.1355;
+  static constexpr double segma = 2.0 / 3.0;
+  static constexpr double cb2 = 0.622;
+  static constexpr double kappa = 0.41;
+  static constexpr double cw2 = 0.3;
+  static constexpr double cw3 = 2.0;
+  static constexpr double cv1 = 7.1;
+  static constexpr double ct3 = 1.2;
+  static constexpr double ct4 = 0.5;
+  static constexpr double cw1 = cb1 / (kappa * kappa) + (1.0 + cb2) / segma;
+  static constexpr double cv1three = cv1 * cv1 * cv1;
+};
+
+// Viseddy2mut (Turbulence.cpp:23-30): mu_t = roe fv1 nu, X = nu/(mu/roe)
+// clamped at 0, fv1 = X^3/(X^3 + cv1^3).
+template <typename R>
+__device__ __forceinline__ R sa_mut(R nu, R roe, R mu);
 
 template <typename R> __device__ __forceinline__ R v_div(R a, R b) { return a / b; }
 template <> __device__ __forceinline__ float v_div<float>(float a, float b) { return a * f_rcp(b); }
 template <typename R> __device__ __forceinline__ R v_sqrt(R a) { return r_sqrt(a); }
 template <> __device__ __forceinline__ float v_sqrt<float>(float a) { return f_sqrt(a); }
+
+template <typename R>
+__device__ __forceinline__ R sa_mut(R nu, R roe, R mu) {
+  R bigx = v_div(nu, v_div(mu, roe));
+  if (bigx < R(0.0)) bigx = R(0.0);
+  const R b3 = bigx * bigx * bigx;
+  const R fv1 = v_div(b3, b3 + R(SA::cv1three));
+  return roe * fv1 * nu;
+}
 
 // four-point tangential difference pfpkesai1 (NS_Solver.H:153-154)
 template <typename R>
@@ -301,10 +331,16 @@ __device__ __forceinline__ R pf1(R up1, R up2, R dn1, R dn2) {
   return R(0.25) * (up1 + up2 - dn1 - dn2);
 }
 
+// With turb set, the SA terms of NS_Solver.cpp:534-542: the face eddy
+// viscosity mu_t = Viseddy2mut(nu_f, roe_f, mu) joins mu and the heat
+// conduction, and mflux receives the face flux of the working variable
+// (nu_f + mu/roe_f) (d nu / d n) / sigma * A (the face keisa is the unit
+// axis, so the dot product picks the normal derivative).
 template <typename R, int D>
 __device__ __forceinline__ void visc_flux(const Prim<R, D> &L, const Prim<R, D> &Rr,
                                           const Prim<R, D> *t, int d, const ViscPhys<R, D> &vp,
-                                          R *out) {
+                                          R *out, bool turb = false, R nuL = R(0), R nuR = R(0),
+                                          R *mflux = nullptr) {
   // T = p/(roe Rg) of every stencil cell (Get_T_ideal_gas)
   const R TL = v_div(L.p, L.r * vp.Rg), TR = v_div(Rr.p, Rr.r * vp.Rg);
   const R T0 = v_div(t[0].p, t[0].r * vp.Rg), T1 = v_div(t[1].p, t[1].r * vp.Rg);
@@ -347,17 +383,28 @@ __device__ __forceinline__ void visc_flux(const Prim<R, D> &L, const Prim<R, D> 
   const R dwx = gw[0] * vp.ih0, dwy = gw[1] * vp.ih1, dwz = gw[2] * vp.ih2;
   const R dTx = gT[0] * vp.ih0, dTy = gT[1] * vp.ih1, dTz = gT[2] * vp.ih2;
   const R ftemp = (TL + TR) * R(0.5);
-  const R mu0 = v_sqrt(ftemp * ftemp * ftemp) * v_div(R(1.0) + vp.SoT, ftemp + vp.SoT);
-  const R qk0 = vp.qk * mu0;
+  R mu0 = v_sqrt(ftemp * ftemp * ftemp) * v_div(R(1.0) + vp.SoT, ftemp + vp.SoT);
+  R qk0 = vp.qk * mu0;
+  // (x*1 + 0 + 0 is exact; written as a sum so that no per-lane table is built)
+  const R area = (d == 0 ? R(1) : R(0)) * vp.ar0 + (d == 1 ? R(1) : R(0)) * vp.ar1 +
+                 (d == 2 ? R(1) : R(0)) * vp.ar2;
+  if (turb) {
+    const R ih = (d == 0 ? R(1) : R(0)) * vp.ih0 + (d == 1 ? R(1) : R(0)) * vp.ih1 +
+                 (d == 2 ? R(1) : R(0)) * vp.ih2;
+    const R fvis = (nuL + nuR) * R(0.5);
+    const R froe = (L.r + Rr.r) * R(0.5);
+    const R faceroexvis = fvis + v_div(mu0, froe);
+    const R mut0 = sa_mut(fvis, froe, mu0);
+    mu0 += mut0;
+    qk0 += v_div(mut0, vp.qkt);
+    *mflux = v_div(faceroexvis * ((nuR - nuL) * ih), R(SA::segma)) * area;
+  }
   const R Re = vp.Re;
   const R divu = (D == 3) ? (dux + dvy + dwz) : (dux + dvy);
   const R tauxx = v_div(R(2.0) * mu0 * (dux - v_div(divu, R(3.0))), Re);
   const R tauyy = v_div(R(2.0) * mu0 * (dvy - v_div(divu, R(3.0))), Re);
   const R tauxy = v_div(mu0, Re) * (duy + dvx);
   const R fu = (L.u + Rr.u) * R(0.5), fv = (L.v + Rr.v) * R(0.5);
-  // (x*1 + 0 + 0 is exact; written as a sum so that no per-lane table is built)
-  const R area = (d == 0 ? R(1) : R(0)) * vp.ar0 + (d == 1 ? R(1) : R(0)) * vp.ar1 +
-                 (d == 2 ? R(1) : R(0)) * vp.ar2;
   if (D == 2) {
     if (d == 0) {
       const R bx = fu * tauxx + fv * tauxy + qk0 * dTx;
@@ -382,6 +429,97 @@ __device__ __forceinline__ void visc_flux(const Prim<R, D> &L, const Prim<R, D> 
       out[0] = R(0); out[1] = tauxz * area; out[2] = tauyz * area; out[3] = tauzz * area; out[4] = bz * area;
     }
   }
+}
+
+template <typename R> __device__ __forceinline__ R r_exp(R x);
+template <> __device__ __forceinline__ double r_exp<double>(double x) { return exp(x); }
+template <> __device__ __forceinline__ float r_exp<float>(float x) { return expf(x); }
+template <typename R> __device__ __forceinline__ R r_pow(R x, R y);
+template <> __device__ __forceinline__ double r_pow<double>(double x, double y) { return pow(x, y); }
+template <> __device__ __forceinline__ float r_pow<float>(float x, float y) { return powf(x, y); }
+
+// ComptMutSource (NS_Solver.cpp:648-808) for one non-solid cell: central
+// differences over the face neighbours (nl/nr: working variable, ul..wr:
+// velocity, low and high neighbour per axis), vorticity, Sutherland mu,
+// fv1/fv2, the modified vorticity S~ (floored at 0.3|Omega|sqrt2), ft2, r, g,
+// fw, then advection + production - destruction + cb2 diffusion.  mutf is
+// the frozen face-flux term sum(dirsign * mutflux)/V/Re of the step.  The
+// frame velocity of the reference is zero here.
+template <typename R, int D>
+__device__ __forceinline__ R sa_source(R u, R v, R w, R T, R roe, R nu, R sd, const R *nl,
+                                       const R *nr, const R *ul, const R *ur, const R *vl,
+                                       const R *vr, const R *wl, const R *wr, R ih0, R ih1, R ih2,
+                                       R SoT, R Re, R mutf) {
+  const R h = R(0.5);
+  const R dvisx = ((nr[0] - nl[0]) * h) * ih0;
+  const R dvisy = ((nr[1] - nl[1]) * h) * ih1;
+  const R dvisz = (D == 3) ? ((nr[2] - nl[2]) * h) * ih2 : R(0);
+  const R duy = ((ur[1] - ul[1]) * h) * ih1;
+  const R dvx = ((vr[0] - vl[0]) * h) * ih0;
+  R duz = R(0), dvz = R(0), dwx = R(0), dwy = R(0);
+  if (D == 3) {
+    duz = ((ur[2] - ul[2]) * h) * ih2;
+    dvz = ((vr[2] - vl[2]) * h) * ih2;
+    dwx = ((wr[0] - wl[0]) * h) * ih0;
+    dwy = ((wr[1] - wl[1]) * h) * ih1;
+  }
+  const R ox = R(0.5) * (dwy - dvz), oy = R(0.5) * (duz - dwx), oz = R(0.5) * (dvx - duy);
+  const R mu0 = v_sqrt(T * T * T) * v_div(R(1.0) + SoT, T + SoT);
+  const R vis0 = v_div(mu0, roe);
+  const R bigx = v_div(nu, vis0);
+  const R b3 = bigx * bigx * bigx;
+  const R fv1 = v_div(b3, b3 + R(SA::cv1three));
+  const R fv2 = R(1.0) - v_div(bigx, R(1.0) + bigx * fv1);
+  R segdis, vis2d;
+  if (r_abs(sd) < R(0.00000001)) {
+    segdis = R(1.0);
+    vis2d = R(1.0);
+  } else {
+    segdis = R(SA::kappa) * sd;
+    vis2d = v_div(nu, sd);
+  }
+  segdis *= segdis;
+  const R olen = v_sqrt(R(2.0)) * v_sqrt(ox * ox + oy * oy + oz * oz);
+  R bigs = olen + v_div(v_div(nu * fv2, segdis), Re);
+  bigs = (R(0.3) * olen < bigs) ? bigs : R(0.3) * olen;
+  const R ft2 = R(SA::ct3) * r_exp(-R(SA::ct4) * bigx * bigx);
+  R rr = v_div(v_div(nu, bigs * segdis), Re);
+  rr = (R(10.0) < rr) ? R(10.0) : rr;
+  const R rsix = r_pow(rr, R(6));
+  const R g = rr + R(SA::cw2) * (rsix - rr);
+  const R gsix = r_pow(g, R(6));
+  const R cw3six = R(64.0);
+  const R fw = g * r_pow(v_div(R(1.0) + cw3six, gsix + cw3six), R(1.0 / 6.0));
+  R src = -(u * dvisx + v * dvisy + w * dvisz);
+  src += R(SA::cb1) * (R(1.0) - ft2) * bigs * nu;
+  src -= v_div((R(SA::cw1) * fw - v_div(R(SA::cb1) * ft2, R(SA::kappa * SA::kappa))) * (vis2d * vis2d), Re);
+  src += v_div(v_div(R(SA::cb2) * (dvisx * dvisx + dvisy * dvisy + dvisz * dvisz), R(SA::segma)), Re);
+  src += mutf;
+  return src;
+}
+
+// Wall-function fits of Turbulence.cpp:32-55: y+ as a function of the local
+// Reynolds number and u+ as a function of y+ (piecewise fits in sqrt and the
+// fourth root; FP64 in both modes, as all wall interpolation).
+__device__ __forceinline__ double sa_yplus(double re) {
+  const double sq = sqrt(re);
+  if (re <= 9.0) return 1.002 * sq - 0.001223;
+  if (re <= 37.0) return 1.272 * sq - 0.9881 * sqrt(sq) + 0.9054;
+  if (re <= 98.0) return 0.2715 * sq + 1.144 * sqrt(sq) + 0.04718 * re;
+  if (re <= 250.0) return 0.04921 * sq + 1.576 * sqrt(sq) + 0.05582 * re;
+  if (re <= 790.0) return 0.5382 * sq + 0.2478 * sqrt(sq) + 0.04593 * re;
+  if (re <= 2800.0) return 1.105 * sq - 1.782 * sqrt(sq) + 0.03933 * re;
+  return 2.643 * sq - 10.52 * sqrt(sq) + 0.03207 * re;
+}
+
+__device__ __forceinline__ double sa_uplus(double y) {
+  if (y < 3.0) return y;
+  const double lg = log(y), sq = sqrt(y), qu = sqrt(sq);
+  if (y < 8.5) return -6.645 * lg + 25.59 * sq - 21.6 * qu - 1.864 * y;
+  if (y < 50.0) return 12.04 * lg - 4.079 * sq - 4.504 * qu + 0.1715 * y;
+  if (y < 300.0) return 3.031 * lg - 0.6252 * sq + 2.639 * qu + 0.007553 * y;
+  if (y < 1000.0) return -3.535 * lg - 1.104 * sq + 13.89 * qu + 0.004011 * y;
+  return 16.66 * lg + 1.408 * sq - 23.82 * qu - 0.002909 * y;
 }
 
 }  // namespace ibg

@@ -52,13 +52,27 @@ __device__ __forceinline__ void vstore(const Pack<R, D> &P, int c, const R *v) {
   if (D == 3) P.e[c] = v[4];
 }
 
-// FP64 master write, mirrored to the FP32 shadow in mixed mode
+// Ghost-fill write: the FP64 master and every stencil copy (see StateOut).
 template <int D>
-__device__ __forceinline__ void wstore64(const Pack<double, D> &W64, const Pack<float, D> &W32,
-                                         bool mixed, int c, double r, double u, double v, double w,
-                                         double p) {
-  pstore<double, D>(W64, c, r, u, v, w, p);
-  if (mixed) pstore<float, D>(W32, c, (float)r, (float)u, (float)v, (float)w, (float)p);
+__device__ __forceinline__ void put_all(const StateOut<D> &o, int c, double r, double u, double v,
+                                        double w, double p) {
+  pstore<double, D>(o.m, c, r, u, v, w, p);
+  if (o.nd > 0) pstore<double, D>(o.d1, c, r, u, v, w, p);
+  if (o.nd > 1) pstore<double, D>(o.d2, c, r, u, v, w, p);
+  if (o.nf > 0) pstore<float, D>(o.f0, c, (float)r, (float)u, (float)v, (float)w, (float)p);
+  if (o.nf > 1) pstore<float, D>(o.f1, c, (float)r, (float)u, (float)v, (float)w, (float)p);
+  if (o.nf > 2) pstore<float, D>(o.f2, c, (float)r, (float)u, (float)v, (float)w, (float)p);
+}
+
+template <int D>
+__device__ __forceinline__ void put_nu(const StateOut<D> &o, int c, double nu) {
+  if (!o.nm) return;
+  o.nm[c] = nu;
+  if (o.nd > 0) o.nd1[c] = nu;
+  if (o.nd > 1) o.nd2[c] = nu;
+  if (o.nf > 0) o.nf0[c] = (float)nu;
+  if (o.nf > 1) o.nf1[c] = (float)nu;
+  if (o.nf > 2) o.nf2[c] = (float)nu;
 }
 
 // error word: ((cell << 16) | code), the smallest cell wins
@@ -78,33 +92,52 @@ struct StageArgs {
   const BlkInfo *blk;
   const DevPhys *phys;
   const double *dt;          // [ncases]
-  Pack<R, D> W;              // stencil state (FP64 master or FP32 shadow)
-  Pack<double, D> W64;       // master, written every stage in the plain mode
-  Pack<double, D> U0;        // conserved state at the start of the step
+  Pack<R, D> Win;            // stencil state this stage reads
+  Pack<R, D> Wout;           // stencil state this stage writes (another buffer)
+  Pack<double, D> W64;       // FP64 master: the state at the start of the step
+                             // until the last stage writes it (mixed mode)
   Pack<R, D> Rv;             // frozen viscous residual
+  // SA closure (turb != 0): working variable in the same buffer roles
+  const R *nuin;
+  R *nuout;
+  double *nu64;
+  const R *mutf;             // [Nn] frozen face-flux term of the step
+  const double *signdis;
+  int turb;
   double alpha;
   int limiter;
   int last;                  // last stage: residual partials (+ W64 in mixed)
-  int mixed;
-  double *rpart;             // [ntiles][10]
+  double *rpart;             // [ntiles][kRpart]
   double *src;               // optional probe [N][5] written at the last stage
+  double *nusrc;             // optional probe [N] (SA source, last stage)
   unsigned long long *err;
 };
 
+constexpr int kRpart = 12;   // per tile: max[5], sumsq[5], SA max, SA sumsq
+
 // One workgroup per tile of Morton-consecutive normal cells of one
 // (case, level).  Phase 1: every face touching the tile — MUSCL + all-speed
-// Roe (NS_Solver.H:704-832) — into LDS.  Phase 2: ComptSource
-// (NS_Solver.cpp:282-363), UpdateConsVars (:844-985) and ComptNewPrimVar
-// (:3-194) for the tile's cells.
+// Roe (NS_Solver.H:704-832) — into LDS (with the SA closure, also the
+// velocity and working variable of both face cells).  Phase 2: ComptSource
+// (NS_Solver.cpp:282-363), ComptMutSource (:632-840), UpdateConsVars
+// (:844-985) and ComptNewPrimVar (:3-194) for the tile's cells.
+//
+// Win and Wout are different buffers, so no workgroup reads a state another
+// one has already advanced within the launch.  U_old is cons(W64): the
+// master holds the step-start state until the last stage rewrites it, and
+// each cell's master entry is read and written by the same thread.
 template <typename R, int D>
 __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
   constexpr int NV = D + 2;
+  constexpr bool kMixed = std::is_same<R, float>::value;
   using V = typename V4<R>::T;
   extern __shared__ __align__(32) unsigned char smem_raw[];
   const TileInfo ti = a.tiles[blockIdx.x];
   V *fl4 = reinterpret_cast<V *>(smem_raw);
   R *fl1 = reinterpret_cast<R *>(smem_raw + sizeof(V) * (size_t)ti.nface);   // 3D 5th value
-  __shared__ double red[2 * 5][kTile / 64];
+  V *sL = reinterpret_cast<V *>(smem_raw + (sizeof(V) + (D == 3 ? sizeof(R) : 0)) * (size_t)ti.nface);
+  V *sR = sL + ti.nface;   // (u,v,w,nu) of the low / high cell of each face (SA)
+  __shared__ double red[kRpart][kTile / 64];
 
   const BlkInfo &bi = a.blk[ti.blk];
   const DevPhys &P = a.phys[bi.case_id];
@@ -115,12 +148,12 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
   for (int f = threadIdx.x; f < ti.nface; f += blockDim.x) {
     const int4 st = a.tst[ti.face0 + f];
     const int dir = a.tdir[ti.face0 + f];
-    const Prim<R, D> q0 = pload<R, D>(a.W, st.x);
-    const Prim<R, D> q1 = pload<R, D>(a.W, st.y);
-    const Prim<R, D> q2 = pload<R, D>(a.W, st.z);
-    const Prim<R, D> q3 = pload<R, D>(a.W, st.w);
+    const Prim<R, D> q0 = pload<R, D>(a.Win, st.x);
+    const Prim<R, D> q1 = pload<R, D>(a.Win, st.y);
+    const Prim<R, D> q2 = pload<R, D>(a.Win, st.z);
+    const Prim<R, D> q3 = pload<R, D>(a.Win, st.w);
     R flux[NV];
-    if constexpr (std::is_same<R, float>::value)
+    if constexpr (kMixed)
       roe_flux_fast<D>(q0, q1, q2, q3, dir, areas[dir], ph, a.limiter, flux);
     else
       roe_flux<R, D>(q0, q1, q2, q3, dir, areas[dir], ph, a.limiter, flux);
@@ -128,15 +161,22 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
     o.x = flux[0]; o.y = flux[1]; o.z = flux[2]; o.w = flux[3];
     fl4[f] = o;
     if (D == 3) fl1[f] = flux[4];
+    if (a.turb) {
+      V l, r;
+      l.x = q1.u; l.y = q1.v; l.z = q1.w; l.w = a.nuin[st.y];
+      r.x = q2.u; r.y = q2.v; r.z = q2.w; r.w = a.nuin[st.z];
+      sL[f] = l;
+      sR[f] = r;
+    }
   }
   __syncthreads();
 
-  constexpr bool kMixed = std::is_same<R, float>::value;
   const double level_ts = a.alpha * a.dt[bi.case_id] * bi.step_num;
   const double inv_vol = 1.0 / bi.vol;
   double mx[NV], sq[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) { mx[k] = 0.0; sq[k] = 0.0; }
+  double nmx = 0.0, nsq = 0.0;
   const int i = threadIdx.x;
   if (i < ti.ncell) {
     const int c = ti.cell0 + i;
@@ -169,11 +209,43 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
 #pragma unroll
       for (int k = 0; k < NV; ++k) s[k] += (double)rv[k];
     }
+    // ComptMutSource for a non-solid updated cell (zero otherwise: the
+    // reference never writes mutsource of solid cells)
+    double snu = 0.0;
+    if (a.turb && (fg & F_UPDATE) && a.signdis[c] >= 0.0) {
+      const Prim<R, D> me = pload<R, D>(a.Win, c);
+      R nl[3], nr[3], ul[3], ur[3], vl[3], vr[3], wl[3], wr[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (d < D) {
+          const V lo = sL[sl[2 * d]], hi = sR[sl[2 * d + 1]];
+          ul[d] = lo.x; vl[d] = lo.y; wl[d] = lo.z; nl[d] = lo.w;
+          ur[d] = hi.x; vr[d] = hi.y; wr[d] = hi.z; nr[d] = hi.w;
+        } else {
+          ul[d] = ur[d] = vl[d] = vr[d] = wl[d] = wr[d] = nl[d] = nr[d] = R(0);
+        }
+      }
+      const R T = v_div(me.p, me.r * (R)P.Rg);
+      snu = (double)sa_source<R, D>(me.u, me.v, me.w, T, me.r, a.nuin[c], (R)a.signdis[c], nl, nr,
+                                    ul, ur, vl, vr, wl, wr, (R)bi.invh[0], (R)bi.invh[1],
+                                    (R)bi.invh[2], (R)P.SoT, (R)P.Re, a.mutf[c]);
+    }
     bool write = false;
     double wr = 0, wu = 0, wv = 0, ww = 0, wp = 0;
+    double nu_new = 0.0;
     if (fg & F_UPDATE) {
+      // U_old = cons(W64): Get_Cons_Vars FlowVariables.H:43-48,
+      // e = Cv T + |u|^2/2, T = p/(roe Rg)
       double U[NV];
-      vload<double, D>(a.U0, c, U);
+      {
+        const Prim<double, D> w0 = pload<double, D>(a.W64, c);
+        const double T0 = w0.p / (w0.r * P.Rg);
+        const double ke0 = (D == 3) ? 0.5 * (w0.u * w0.u + w0.v * w0.v + w0.w * w0.w)
+                                    : 0.5 * (w0.u * w0.u + w0.v * w0.v);
+        const double e0 = P.Cv * T0 + ke0;
+        U[0] = w0.r; U[1] = w0.r * w0.u; U[2] = w0.r * w0.v;
+        if (D == 3) { U[3] = w0.r * w0.w; U[4] = w0.r * e0; } else U[3] = w0.r * e0;
+      }
       if constexpr (kMixed) {
         const double sc = level_ts * inv_vol;
 #pragma unroll
@@ -182,6 +254,11 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
         // cv_old + source * level_ts / v  (NS_Solver.cpp:870)
 #pragma unroll
         for (int k = 0; k < NV; ++k) U[k] = U[k] + s[k] * level_ts / bi.vol;
+      }
+      if (a.turb) {
+        // viseddy = max(c_vis_old + mutsource * level_ts, 0) (:874-876, :955-957)
+        nu_new = a.nu64[c] + snu * level_ts;
+        nu_new = (nu_new < 0.0) ? 0.0 : nu_new;
       }
       if (U[0] < 0.0) report_error(a.err, c, (fg & F_FINEST) ? 611 : 523);
       if (fg & F_PRIM) {
@@ -206,12 +283,18 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
       }
     }
     if (!(fg & F_PRIM) && (fg & F_FREE)) {
+      // m_data = initvar, the working variable included
       wr = P.init[0]; wu = P.init[1]; wv = P.init[2]; ww = P.init[3]; wp = P.init[4];
+      nu_new = P.nut_inf;
       write = true;
     }
     if (write) {
-      if (!kMixed || a.last) pstore<double, D>(a.W64, c, wr, wu, wv, ww, wp);
-      if (kMixed) pstore<R, D>(a.W, c, (R)wr, (R)wu, (R)wv, (R)ww, (R)wp);
+      pstore<R, D>(a.Wout, c, (R)wr, (R)wu, (R)wv, (R)ww, (R)wp);
+      if (kMixed && a.last) pstore<double, D>(a.W64, c, wr, wu, wv, ww, wp);
+    }
+    if (a.turb && ((fg & F_UPDATE) || (fg & F_FREE))) {
+      a.nuout[c] = (R)nu_new;
+      if (kMixed && a.last) a.nu64[c] = nu_new;
     }
     if (a.last) {
       if (a.src) {
@@ -219,9 +302,12 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
         o[0] = s[0]; o[1] = s[1]; o[2] = s[2];
         if (D == 3) { o[3] = s[3]; o[4] = s[4]; } else { o[3] = 0.0; o[4] = s[3]; }
       }
+      if (a.nusrc) a.nusrc[c] = snu;
       if (fg & F_RESID) {
 #pragma unroll
         for (int k = 0; k < NV; ++k) { mx[k] = fabs(s[k]); sq[k] = s[k] * s[k]; }
+        nmx = fabs(snu);
+        nsq = snu * snu;
       }
     }
   }
@@ -233,16 +319,22 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
         sq[k] += __shfl_xor(sq[k], off, 64);
       }
     }
+    for (int off = 32; off > 0; off >>= 1) {
+      nmx = fmax(nmx, __shfl_xor(nmx, off, 64));
+      nsq += __shfl_xor(nsq, off, 64);
+    }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) {
 #pragma unroll
       for (int k = 0; k < NV; ++k) { red[k][wid] = mx[k]; red[5 + k][wid] = sq[k]; }
+      red[10][wid] = nmx;
+      red[11][wid] = nsq;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
       const int nw = (blockDim.x + 63) >> 6;
-      double *o = a.rpart + (size_t)blockIdx.x * 10;
-      for (int k = 0; k < 10; ++k) o[k] = 0.0;
+      double *o = a.rpart + (size_t)blockIdx.x * kRpart;
+      for (int k = 0; k < kRpart; ++k) o[k] = 0.0;
       for (int k = 0; k < NV; ++k) {
         double m = 0.0, q = 0.0;
         for (int w = 0; w < nw; ++w) { m = fmax(m, red[k][w]); q += red[5 + k][w]; }
@@ -250,6 +342,10 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
         o[kk] = m;
         o[5 + kk] = q;
       }
+      double m = 0.0, q = 0.0;
+      for (int w = 0; w < nw; ++w) { m = fmax(m, red[10][w]); q += red[11][w]; }
+      o[10] = m;
+      o[11] = q;
     }
   }
 }
@@ -264,17 +360,18 @@ struct ViscArgs {
   const uint16_t *slots;
   const BlkInfo *blk;
   const DevPhys *phys;
-  Pack<R, D> W;              // state the viscous flux reads
-  Pack<double, D> W64;
-  Pack<double, D> U0;
+  Pack<R, D> W;              // step-start stencil state
   Pack<R, D> Rv;
-  int viscous;
+  const R *nu;               // SA working variable (turb)
+  R *mutf;                   // [Nn] frozen face-flux term (turb)
+  int viscous, turb;
 };
 
 // Once per step (ComptViscousStress NS_Solver.cpp:365-629 is frozen across
 // the stages, NS_Solver.H:669-699): viscous fluxes of the tile faces into
-// LDS, then per cell U_old = cons(W) (ComptOldConsVar NS_Solver.H:392-424)
-// and Rv = sum_d (Fv_hi - Fv_lo).
+// LDS, then per cell Rv = sum_d (Fv_hi - Fv_lo).  With the SA closure the
+// face flux of the working variable (mutflux, :534-542) is reduced the same
+// way into sum_d (mutflux_hi - mutflux_lo) / V / Re (:791-800).
 template <typename R, int D>
 __global__ void __launch_bounds__(kTile) k_visc_prep(ViscArgs<R, D> a) {
   constexpr int NV = D + 2;
@@ -283,6 +380,7 @@ __global__ void __launch_bounds__(kTile) k_visc_prep(ViscArgs<R, D> a) {
   const TileInfo ti = a.tiles[blockIdx.x];
   V *fl4 = reinterpret_cast<V *>(smem_raw);
   R *fl1 = reinterpret_cast<R *>(smem_raw + sizeof(V) * (size_t)ti.nface);
+  R *mfl = reinterpret_cast<R *>(smem_raw + (sizeof(V) + (D == 3 ? sizeof(R) : 0)) * (size_t)ti.nface);
   const BlkInfo &bi = a.blk[ti.blk];
   const DevPhys &P = a.phys[bi.case_id];
   if (a.viscous) {
@@ -290,6 +388,7 @@ __global__ void __launch_bounds__(kTile) k_visc_prep(ViscArgs<R, D> a) {
     vp.Rg = (R)P.Rg; vp.Re = (R)P.Re; vp.qk = (R)P.qk; vp.SoT = (R)P.SoT;
     vp.ih0 = (R)bi.invh[0]; vp.ih1 = (R)bi.invh[1]; vp.ih2 = (R)bi.invh[2];
     vp.ar0 = (R)bi.area[0]; vp.ar1 = (R)bi.area[1]; vp.ar2 = (R)bi.area[2];
+    vp.qkt = (R)P.qkt_den;
     for (int f = threadIdx.x; f < ti.nface; f += blockDim.x) {
       const int4 st = a.tst[ti.face0 + f];
       const int dir = a.tdir[ti.face0 + f];
@@ -307,29 +406,26 @@ __global__ void __launch_bounds__(kTile) k_visc_prep(ViscArgs<R, D> a) {
         t[7 % ((D == 3) ? 8 : 4)] = pload<R, D>(a.W, v1.w);
       }
       R out[NV];
-      visc_flux<R, D>(L, Rr, t, dir, vp, out);
+      R mf = R(0);
+      if (a.turb)
+        visc_flux<R, D>(L, Rr, t, dir, vp, out, true, a.nu[st.y], a.nu[st.z], &mf);
+      else
+        visc_flux<R, D>(L, Rr, t, dir, vp, out);
       V o;
       o.x = out[0]; o.y = out[1]; o.z = out[2]; o.w = out[3];
       fl4[f] = o;
       if (D == 3) fl1[f] = out[4];
+      if (a.turb) mfl[f] = mf;
     }
   }
   __syncthreads();
   const int i = threadIdx.x;
   if (i >= ti.ncell) return;
   const int c = ti.cell0 + i;
-  // U_old = cons(W64): Get_Cons_Vars FlowVariables.H:43-48, e = Cv T + |u|^2/2
-  const Prim<double, D> w = pload<double, D>(a.W64, c);
-  const double T = w.p / (w.r * P.Rg);
-  const double ke = (D == 3) ? 0.5 * (w.u * w.u + w.v * w.v + w.w * w.w) : 0.5 * (w.u * w.u + w.v * w.v);
-  const double e = P.Cv * T + ke;
-  double U[NV];
-  U[0] = w.r; U[1] = w.r * w.u; U[2] = w.r * w.v;
-  if (D == 3) { U[3] = w.r * w.w; U[4] = w.r * e; } else U[3] = w.r * e;
-  vstore<double, D>(a.U0, c, U);
   double s[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) s[k] = 0.0;
+  double tot = 0.0;
   if (a.viscous) {
     int sl[2 * D];
     if (D == 2) {
@@ -348,55 +444,76 @@ __global__ void __launch_bounds__(kTile) k_visc_prep(ViscArgs<R, D> a) {
       s[2] += (double)fhi.z - (double)flo.z;
       s[3] += (double)fhi.w - (double)flo.w;
       if (D == 3) s[4] += (double)fl1[hi] - (double)fl1[lo];
+      if (a.turb) {
+        // total_face_flux += dirsign[f0] * mutflux, dirsign = {-1, +1}
+        tot += -(double)mfl[lo];
+        tot += (double)mfl[hi];
+      }
     }
   }
   R rv[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) rv[k] = (R)s[k];
   vstore<R, D>(a.Rv, c, rv);
+  if (a.turb) a.mutf[c] = (R)(tot / bi.vol / P.Re);
+}
+
+// Stages with a single substep write into buffer 1; this copies the marched
+// cells back into the home buffer afterwards.
+template <typename R, int D>
+__global__ void k_copy_back(int n, Pack<R, D> src, Pack<R, D> dst, const R *nusrc, R *nudst) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  dst.a[c] = src.a[c];
+  if (D == 3) dst.e[c] = src.e[c];
+  if (nudst) nudst[c] = nusrc[c];
 }
 
 // ---------------------------------------------------------- ghost fills
 // BCBoxTreat BCValues.H:51-188: -13 pressure inlet (:191-206), -12 pressure
 // outlet (:208-214), -50 outflow (:216-220); entries (cell, refcell, type,
-// case).
+// case).  SA working variable: copied with the state for -12/-50, the inlet
+// value for -13 (:203-205).
 template <int D>
-__global__ void k_bc(int n, const int4 *dmg, Pack<double, D> W64, Pack<float, D> W32, int mixed,
-                     const DevPhys *phys) {
+__global__ void k_bc(int n, const int4 *dmg, StateOut<D> o, const DevPhys *phys) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int4 e = dmg[i];
   const int g = e.x, ty = e.z;
   const DevPhys &P = phys[e.w];
-  const Prim<double, D> q = pload<double, D>(W64, e.y);
+  const Prim<double, D> q = pload<double, D>(o.m, e.y);
   if (ty == -50) {
-    wstore64<D>(W64, W32, mixed, g, q.r, q.u, q.v, q.w, q.p);
+    put_all<D>(o, g, q.r, q.u, q.v, q.w, q.p);
+    if (o.nm) put_nu<D>(o, g, o.nm[e.y]);
   } else if (ty == -12) {
     const double T = q.p / (q.r * P.Rg);
     const double p = P.outlet_p;
-    wstore64<D>(W64, W32, mixed, g, p / (P.Rg * T), q.u, q.v, q.w, p);
+    put_all<D>(o, g, p / (P.Rg * T), q.u, q.v, q.w, p);
+    if (o.nm) put_nu<D>(o, g, o.nm[e.y]);
   } else if (ty == -13) {
     const double speed0 = sqrt(q.u * q.u + q.v * q.v + q.w * q.w);
     const double u = speed0 * P.inlet_cos, v = speed0 * P.inlet_sin;
     const double T = P.inlet_T0 - (0.5 * (u * u + v * v)) / P.Cp;
     const double p = P.inlet_p0 * pow((T / P.inlet_T0), 3.5);
-    wstore64<D>(W64, W32, mixed, g, p / (P.Rg * T), u, v, 0.0, p);
+    put_all<D>(o, g, p / (P.Rg * T), u, v, 0.0, p);
+    put_nu<D>(o, g, P.nut_inlet);
   }
 }
 
 // PairRestriction[_2d] TwoLevelOp.H:217-231 with AverageCell[_2d] :45-87
+// (the SA variant :129-179 averages the working variable without weights)
 template <int D>
-__global__ void k_restrict(int n, const int *ci, const int *fi, Pack<double, D> W64,
-                           Pack<float, D> W32, int mixed, int twod, const DevPhys *phys,
-                           const int *cellcase) {
+__global__ void k_restrict(int n, const int *ci, const int *fi, StateOut<D> o, int twod,
+                           const DevPhys *phys, const int *cellcase) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int c = ci[i];
   const DevPhys &P = phys[cellcase[c]];
-  double r = 0, ru = 0, rv = 0, rw = 0, re = 0;
+  double r = 0, ru = 0, rv = 0, rw = 0, re = 0, nu = 0;
   for (int q = 0; q < 8; ++q) {
     if (twod && (q & 1)) continue;
-    const Prim<double, D> s = pload<double, D>(W64, fi[(size_t)i * 8 + q]);
+    const int a = fi[(size_t)i * 8 + q];
+    const Prim<double, D> s = pload<double, D>(o.m, a);
     const double T = s.p / (s.r * P.Rg);
     const double e = P.Cv * T + ((D == 3) ? 0.5 * (s.u * s.u + s.v * s.v + s.w * s.w) : 0.5 * (s.u * s.u + s.v * s.v));
     r += s.r;
@@ -404,42 +521,54 @@ __global__ void k_restrict(int n, const int *ci, const int *fi, Pack<double, D> 
     rv += s.v * s.r;
     rw += s.w * s.r;
     re += e * s.r;
+    if (o.nm) nu += o.nm[a];
   }
   const double u = ru / r, v = rv / r, w = rw / r, e = re / r;
   const double rho = r * (twod ? 0.25 : 0.125);
   const double ke = (D == 3) ? 0.5 * (u * u + v * v + w * w) : 0.5 * (u * u + v * v);
   const double p = (e - ke) * (P.gama - 1.0) * rho;
-  wstore64<D>(W64, W32, mixed, c, rho, u, v, w, p);
+  put_all<D>(o, c, rho, u, v, w, p);
+  if (o.nm) put_nu<D>(o, c, nu * (twod ? 0.25 : 0.125));
 }
 
 // PairProlongation[_2d] TwoLevelOp.H:183-215: central gradients of the
 // conserved variables over the 6 face neighbours (cneib, :352-358), sons at
-// +-1/4 (fincrem).
+// +-1/4 (fincrem); the SA variant (:91-127) does the same for the working
+// variable.
 template <int D>
-__global__ void k_prolong(int n, const int *ci, const int *fi, const int *nb27, Pack<double, D> W64,
-                          Pack<float, D> W32, int mixed, int twod, const DevPhys *phys,
-                          const int *cellcase) {
+__global__ void k_prolong(int n, const int *ci, const int *fi, const int *nb27, StateOut<D> o,
+                          int twod, const DevPhys *phys, const int *cellcase) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int c = ci[i];
   const DevPhys &P = phys[cellcase[c]];
   const int *nb = nb27 + (size_t)c * 27;
   auto cons = [&](int a, double *U) {
-    const Prim<double, D> q = pload<double, D>(W64, a);
+    const Prim<double, D> q = pload<double, D>(o.m, a);
     const double T = q.p / (q.r * P.Rg);
     const double e = P.Cv * T + ((D == 3) ? 0.5 * (q.u * q.u + q.v * q.v + q.w * q.w) : 0.5 * (q.u * q.u + q.v * q.v));
     U[0] = q.r; U[1] = q.r * q.u; U[2] = q.r * q.v; U[3] = q.r * q.w; U[4] = q.r * e;
   };
+  const int ax0 = nb[0 * 9 + 1 * 3 + 1], ax1 = nb[2 * 9 + 1 * 3 + 1];
+  const int ay0 = nb[1 * 9 + 0 * 3 + 1], ay1 = nb[1 * 9 + 2 * 3 + 1];
+  const int az0 = nb[1 * 9 + 1 * 3 + 0], az1 = nb[1 * 9 + 1 * 3 + 2];
   double Uc[5], Ux0[5], Ux1[5], Uy0[5], Uy1[5], Uz0[5], Uz1[5];
   cons(c, Uc);
-  cons(nb[0 * 9 + 1 * 3 + 1], Ux0); cons(nb[2 * 9 + 1 * 3 + 1], Ux1);
-  cons(nb[1 * 9 + 0 * 3 + 1], Uy0); cons(nb[1 * 9 + 2 * 3 + 1], Uy1);
-  cons(nb[1 * 9 + 1 * 3 + 0], Uz0); cons(nb[1 * 9 + 1 * 3 + 2], Uz1);
+  cons(ax0, Ux0); cons(ax1, Ux1);
+  cons(ay0, Uy0); cons(ay1, Uy1);
+  cons(az0, Uz0); cons(az1, Uz1);
   double gx[5], gy[5], gz[5];
   for (int t = 0; t < 5; ++t) {
     gx[t] = (Ux1[t] - Ux0[t]) * 0.5;
     gy[t] = (Uy1[t] - Uy0[t]) * 0.5;
     gz[t] = (Uz1[t] - Uz0[t]) * 0.5;
+  }
+  double nc = 0, ngx = 0, ngy = 0, ngz = 0;
+  if (o.nm) {
+    nc = o.nm[c];
+    ngx = (o.nm[ax1] - o.nm[ax0]) * 0.5;
+    ngy = (o.nm[ay1] - o.nm[ay0]) * 0.5;
+    ngz = (o.nm[az1] - o.nm[az0]) * 0.5;
   }
   const double fincrem[2] = {-0.25, 0.25};
   for (int si = 0; si < 2; ++si)
@@ -454,21 +583,29 @@ __global__ void k_prolong(int n, const int *ci, const int *fi, const int *nb27, 
         const double r = U[0], u = U[1] / U[0], v = U[2] / U[0], w = U[3] / U[0], e = U[4] / U[0];
         const double ke = (D == 3) ? 0.5 * (u * u + v * v + w * w) : 0.5 * (u * u + v * v);
         const double p = (e - ke) * (P.gama - 1.0) * r;
-        wstore64<D>(W64, W32, mixed, f, r, u, v, w, p);
+        put_all<D>(o, f, r, u, v, w, p);
+        if (o.nm) {
+          const double nf = twod ? nc + fincrem[si] * ngx + fincrem[sj] * ngy
+                                 : nc + fincrem[si] * ngx + fincrem[sj] * ngy + fincrem[sk] * ngz;
+          put_nu<D>(o, f, nf);
+        }
       }
 }
 
-// Body::IBConditions_Image Body.cpp:1366-1496 (laminar): image state from
-// the bracketing cells (Mesh::IntpHGgcellVar Mesh.cpp:1532-1547), then
-// LinearDist (Body.cpp:183-204) for signdis >= 0, the mirror otherwise.  The
-// image cells of an IB cell are never IB cells (checked at upload), so one
-// pass equals the reference's compute-all-then-write-all.
+// Body::IBConditions_Image Body.cpp:1366-1496: image state from the
+// bracketing cells (Mesh::IntpHGgcellVar Mesh.cpp:1532-1547).  Laminar:
+// LinearDist (Body.cpp:183-204) for signdis >= 0, the mirror otherwise.  SA
+// closure: ImageViseddy (Turbulence.cpp:57-138, a wall function on the
+// tangential velocity, Crocco-Busemann wall temperature, nu = kappa y+) for
+// signdis >= 0, and initvar with the wall velocity otherwise
+// (Body.cpp:1403-1407).  The image cells of an IB cell are never IB cells
+// (checked at upload), so one pass equals the reference's compute-all-then-
+// write-all.
 template <int D>
 __global__ void k_ib(int n, const int *cell, const int *intp, const double *coef,
-                     const double *hgdis, const double *attach, const int *body, const int *ibcase,
-                     const double *signdis, const Motion *mot, int have_motion,
-                     Pack<double, D> W64, Pack<float, D> W32, int mixed, const DevPhys *phys,
-                     unsigned long long *err) {
+                     const double *hgdis, const double *attach, const double *nvec, const int *body,
+                     const int *ibcase, const double *signdis, const Motion *mot, int have_motion,
+                     StateOut<D> o, const DevPhys *phys, unsigned long long *err) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const DevPhys &P = phys[ibcase[i]];
@@ -477,7 +614,7 @@ __global__ void k_ib(int n, const int *cell, const int *intp, const double *coef
     const int a = intp[(size_t)i * 8 + q];
     if (a < 0) continue;
     const double cq = coef[(size_t)i * 8 + q];
-    const Prim<double, D> s = pload<double, D>(W64, a);
+    const Prim<double, D> s = pload<double, D>(o.m, a);
     T += cq * (s.p / (s.r * P.Rg));
     p += cq * s.p;
     u += cq * s.u;
@@ -495,18 +632,57 @@ __global__ void k_ib(int n, const int *cell, const int *intp, const double *coef
   }
   const int c = cell[i];
   const double sd = signdis[c];
-  double ou, ov, ow;
-  if (sd < 0.0) {
-    ou = 2.0 * pv[0] - u; ov = 2.0 * pv[1] - v; ow = 2.0 * pv[2] - w;
-  } else {
-    const double rt = sd + hgdis[i];
-    const double r0 = sd / rt;
-    ou = pv[0] + (u - pv[0]) * r0;
-    ov = pv[1] + (v - pv[1]) * r0;
-    ow = pv[2] + (w - pv[2]) * r0;
-  }
   if (T < 0.0) report_error(err, c, 1050);
-  wstore64<D>(W64, W32, mixed, c, p / (P.Rg * T), ou, ov, (D == 3) ? ow : 0.0, p);
+  if (!o.nm) {
+    double ou, ov, ow;
+    if (sd < 0.0) {
+      ou = 2.0 * pv[0] - u; ov = 2.0 * pv[1] - v; ow = 2.0 * pv[2] - w;
+    } else {
+      const double rt = sd + hgdis[i];
+      const double r0 = sd / rt;
+      ou = pv[0] + (u - pv[0]) * r0;
+      ov = pv[1] + (v - pv[1]) * r0;
+      ow = pv[2] + (w - pv[2]) * r0;
+    }
+    put_all<D>(o, c, p / (P.Rg * T), ou, ov, (D == 3) ? ow : 0.0, p);
+    return;
+  }
+  if (sd < 0.0) {
+    put_all<D>(o, c, P.init[0], pv[0], pv[1], (D == 3) ? pv[2] : 0.0, P.init[4]);
+    put_nu<D>(o, c, P.nut_inf);
+    return;
+  }
+  // ImageViseddy
+  const double *nm = nvec + (size_t)i * 3;
+  const double rho_hg = p / (P.Rg * T);
+  const double rt0 = hgdis[i] + sd;
+  const double ib_rdis = sd / rt0;
+  const double hvn = u * nm[0] + v * nm[1] + w * nm[2];
+  const double hn[3] = {hvn * nm[0], hvn * nm[1], hvn * nm[2]};
+  const double ht[3] = {u - hn[0], v - hn[1], w - hn[2]};
+  const double wvn = pv[0] * nm[0] + pv[1] * nm[1] + pv[2] * nm[2];
+  const double wn[3] = {nm[0] * wvn, nm[1] * wvn, nm[2] * wvn};
+  const double wt[3] = {pv[0] - wn[0], pv[1] - wn[1], pv[2] - wn[2]};
+  const double df[3] = {ht[0] - wt[0], ht[1] - wt[1], ht[2] - wt[2]};
+  const double df_velt = sqrt(df[0] * df[0] + df[1] * df[1] + df[2] * df[2]);
+  const double hg_mu = sqrt(T * T * T) * ((1.0 + P.SoT) / (T + P.SoT));
+  const double hg_vis = hg_mu / rho_hg;
+  const double hg_re = df_velt * rt0 / hg_vis * P.Re;
+  const double hg_yplus = sa_yplus(hg_re);
+  const double ut = hg_yplus * hg_vis / rt0 / P.Re;
+  const double yplus = ib_rdis * hg_yplus;
+  const double uplus = sa_uplus(yplus);
+  const double ib_velt = uplus * ut;
+  const double ib_r0 = (fabs(df_velt) < 0.00000001) ? 0.0 : ib_velt / df_velt;
+  double ou[3];
+  for (int k = 0; k < 3; ++k) {
+    const double ibn = wn[k] + (hn[k] - wn[k]) * ib_rdis;
+    const double ibt = wt[k] + df[k] * ib_r0;
+    ou[k] = ibn + ibt;
+  }
+  const double Tib = T + 0.5 * P.pr_cuberoot / P.Cp * (df_velt * df_velt - ib_velt * ib_velt);
+  put_all<D>(o, c, p / (P.Rg * Tib), ou[0], ou[1], (D == 3) ? ou[2] : 0.0, p);
+  put_nu<D>(o, c, SA::kappa * yplus);
 }
 
 // Comptdt NS_Solver.cpp:229-262 per tile: min over the tile's cells of
@@ -571,7 +747,8 @@ template <int D>
 __global__ void k_forces(int npatch, const int *pcase, const int *intp, const double *coef,
                          const double *hgdis, const double *nv, const double *pc,
                          const double *area, const double *ccenter, const Motion *mot,
-                         int have_motion, Pack<double, D> W64, const DevPhys *phys, double *out) {
+                         int have_motion, Pack<double, D> W64, const DevPhys *phys, int turb,
+                         double *out) {
   const int c = blockIdx.x;
   const DevPhys &P = phys[c];
   __shared__ double acc[256][12];
@@ -580,12 +757,13 @@ __global__ void k_forces(int npatch, const int *pcase, const int *intp, const do
   const double *bc = ccenter + (size_t)c * 3;
   for (int i = threadIdx.x; i < npatch; i += blockDim.x) {
     if (pcase[i] != c) continue;
-    double p = 0, u = 0, v = 0, w = 0;
+    double T = 0, p = 0, u = 0, v = 0, w = 0;
     for (int q = 0; q < 8; ++q) {
       const int a = intp[(size_t)i * 8 + q];
       if (a < 0) continue;
       const double cq = coef[(size_t)i * 8 + q];
       const Prim<double, D> sp = pload<double, D>(W64, a);
+      T += cq * (sp.p / (sp.r * P.Rg));
       p += cq * sp.p;
       u += cq * sp.u;
       v += cq * sp.v;
@@ -601,13 +779,41 @@ __global__ void k_forces(int npatch, const int *pcase, const int *intp, const do
       pv[1] = M.tra[1] + (M.rot[2] * r1[0] - M.rot[0] * r1[2]);
       pv[2] = M.tra[2] + (M.rot[0] * r1[1] - M.rot[1] * r1[0]);
     }
-    const double du[3] = {u - pv[0], v - pv[1], w - pv[2]};
-    const double dn = du[0] * n[0] + du[1] * n[1] + du[2] * n[2];
     const double A = area[i];
     double df0[3], df1[3];
-    for (int k = 0; k < 3; ++k) {
-      df0[k] = ((du[k] - n[k] * dn) / hgdis[i]) * P.mu * A;
-      df1[k] = n[k] * (-p * A);
+    if (!turb) {
+      const double du[3] = {u - pv[0], v - pv[1], w - pv[2]};
+      const double dn = du[0] * n[0] + du[1] * n[1] + du[2] * n[2];
+      for (int k = 0; k < 3; ++k) {
+        df0[k] = ((du[k] - n[k] * dn) / hgdis[i]) * P.mu * A;
+        df1[k] = n[k] * (-p * A);
+      }
+    } else {
+      // Patchut (Turbulence.cpp:197-226): wall-function friction velocity
+      // and wall temperature; shear = roe_w ut^2 along the slip direction.
+      // FiltSinglePatchPressure (Filt.cpp:44-67) has coefficients b = {1,0..},
+      // a = {0..} (Filt.cpp:5-6): the identity, so p is used as is.
+      const double roe = p / (P.Rg * T);
+      const double hv = u * n[0] + v * n[1] + w * n[2];
+      const double pn = pv[0] * n[0] + pv[1] * n[1] + pv[2] * n[2];
+      double df[3];
+      df[0] = (u - n[0] * hv) - (pv[0] - n[0] * pn);
+      df[1] = (v - n[1] * hv) - (pv[1] - n[1] * pn);
+      df[2] = (w - n[2] * hv) - (pv[2] - n[2] * pn);
+      const double dv = sqrt(df[0] * df[0] + df[1] * df[1] + df[2] * df[2]);
+      double vt[3] = {0.0, 0.0, 0.0};
+      if (!(dv < 1e-8)) { vt[0] = df[0] / dv; vt[1] = df[1] / dv; vt[2] = df[2] / dv; }
+      const double hg_mu = sqrt(T * T * T) * ((1.0 + P.SoT) / (T + P.SoT));
+      const double hg_re = roe * dv * hgdis[i] / hg_mu * P.Re;
+      const double yplus = sa_yplus(hg_re);
+      const double ut = yplus * (hg_mu / roe) / hgdis[i] / P.Re;
+      const double wall_T = T + 0.5 * P.pr_cuberoot / P.Cp * (dv * dv);
+      const double wall_roe = p / (P.Rg * wall_T);
+      const double Cf = wall_roe * (ut * ut);
+      for (int k = 0; k < 3; ++k) {
+        df0[k] = vt[k] * Cf * A;
+        df1[k] = n[k] * (-p * A);
+      }
     }
     const double dx[3] = {x[0] - bc[0], x[1] - bc[1], x[2] - bc[2]};
     const double dm0[3] = {dx[1] * df0[2] - dx[2] * df0[1], dx[2] * df0[0] - dx[0] * df0[2], dx[0] * df0[1] - dx[1] * df0[0]};
@@ -622,26 +828,34 @@ __global__ void k_forces(int npatch, const int *pcase, const int *intp, const do
     for (int j = 0; j < (int)blockDim.x; ++j)
       for (int k = 0; k < 12; ++k) t[k] += acc[j][k];
     for (int k = 0; k < 3; ++k) {
-      out[c * 6 + k] = t[k] * P.p_ref_area + t[3 + k] * P.shear_ref_area;
-      out[c * 6 + 3 + k] = t[6 + k] * P.p_ref_area + t[9 + k] * P.shear_ref_area;
+      // the SA build scales the wall shear by p_ref*area_ref (Body.cpp:547-553)
+      const double sr = turb ? P.p_ref_area : P.shear_ref_area;
+      out[c * 6 + k] = t[k] * P.p_ref_area + t[3 + k] * sr;
+      out[c * 6 + 3 + k] = t[6 + k] * P.p_ref_area + t[9 + k] * sr;
     }
   }
 }
 
 // residual finalize: per case, over the tiles of the last march
+// out[case*12 + {0..4}] = L-inf, {5..9} = L2 of the flow residual, {10, 11}
+// = L-inf, L2 of the SA residual (res_vis_sum of NS_Solver.H:297, 321).
 __global__ void k_resid_final(int ntiles, const TileInfo *tiles, const BlkInfo *blk,
                               const double *rpart, double *out, int ncases) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncases) return;
-  double m[5] = {0, 0, 0, 0, 0}, q[5] = {0, 0, 0, 0, 0};
+  double m[6] = {0, 0, 0, 0, 0, 0}, q[6] = {0, 0, 0, 0, 0, 0};
   for (int t = 0; t < ntiles; ++t) {
     if (blk[tiles[t].blk].case_id != c) continue;
     for (int k = 0; k < 5; ++k) {
-      m[k] = fmax(m[k], rpart[(size_t)t * 10 + k]);
-      q[k] += rpart[(size_t)t * 10 + 5 + k];
+      m[k] = fmax(m[k], rpart[(size_t)t * kRpart + k]);
+      q[k] += rpart[(size_t)t * kRpart + 5 + k];
     }
+    m[5] = fmax(m[5], rpart[(size_t)t * kRpart + 10]);
+    q[5] += rpart[(size_t)t * kRpart + 11];
   }
-  for (int k = 0; k < 5; ++k) { out[c * 10 + k] = m[k]; out[c * 10 + 5 + k] = sqrt(q[k]); }
+  for (int k = 0; k < 5; ++k) { out[c * kRpart + k] = m[k]; out[c * kRpart + 5 + k] = sqrt(q[k]); }
+  out[c * kRpart + 10] = m[5];
+  out[c * kRpart + 11] = sqrt(q[5]);
 }
 
 // parity probes: inviscid / viscous flux of every face, [face][5]
@@ -668,7 +882,7 @@ __global__ void k_face_probe(int nface, const int4 *st, const int8_t *dir, const
 template <typename R, int D>
 __global__ void k_visc_probe(int nface, const int4 *st, const int4 *vst, const int8_t *dir,
                              const int16_t *fblk, Pack<R, D> W, const BlkInfo *blk,
-                             const DevPhys *phys, double *out) {
+                             const DevPhys *phys, const R *nu, double *out, double *mout) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nface) return;
   constexpr int NV = D + 2;
@@ -678,6 +892,7 @@ __global__ void k_visc_probe(int nface, const int4 *st, const int4 *vst, const i
   vp.Rg = (R)P.Rg; vp.Re = (R)P.Re; vp.qk = (R)P.qk; vp.SoT = (R)P.SoT;
   vp.ih0 = (R)bi.invh[0]; vp.ih1 = (R)bi.invh[1]; vp.ih2 = (R)bi.invh[2];
   vp.ar0 = (R)bi.area[0]; vp.ar1 = (R)bi.area[1]; vp.ar2 = (R)bi.area[2];
+  vp.qkt = (R)P.qkt_den;
   const int4 s = st[f];
   Prim<R, D> t[(D == 3) ? 8 : 4];
   const int4 v0 = vst[(size_t)f * (D - 1)];
@@ -690,7 +905,13 @@ __global__ void k_visc_probe(int nface, const int4 *st, const int4 *vst, const i
     t[7 % ((D == 3) ? 8 : 4)] = pload<R, D>(W, v1.w);
   }
   R o5[NV];
-  visc_flux<R, D>(pload<R, D>(W, s.y), pload<R, D>(W, s.z), t, dir[f], vp, o5);
+  R mf = R(0);
+  if (nu)
+    visc_flux<R, D>(pload<R, D>(W, s.y), pload<R, D>(W, s.z), t, dir[f], vp, o5, true, nu[s.y],
+                    nu[s.z], &mf);
+  else
+    visc_flux<R, D>(pload<R, D>(W, s.y), pload<R, D>(W, s.z), t, dir[f], vp, o5);
+  if (mout) mout[f] = mf;
   double *o = out + (size_t)f * 5;
   if (D == 3) { for (int k = 0; k < 5; ++k) o[k] = o5[k]; }
   else { o[0] = o5[0]; o[1] = o5[1]; o[2] = o5[2]; o[3] = 0.0; o[4] = o5[3]; }

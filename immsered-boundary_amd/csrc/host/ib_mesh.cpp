@@ -796,6 +796,11 @@ int32_t ibm_physics(const ibm_case *c, ibg_physics *o) {
   // F_ref = 0.5 u^2 roe (L^2 * span) (Userfunction_cy_naca15.cpp:439)
   o->force_ref = 0.5 * u_ref * u_ref * roe_ref * (L_ref * L_ref * c->span);
   o->moment_ref = o->force_ref * L_ref;
+  // SA closure: initvar.viseddy = far_vis_ratio * viseddy with viseddy = 1
+  // (Turbulence.cpp:14, FlowConstants.cpp:19), Pr_tur (FlowConstants.cpp:11)
+  o->nut_inf = 5.0 * 1.0;
+  o->nut_inlet = 0.0;
+  o->Pr_tur = 0.9;
   return 0;
 }
 

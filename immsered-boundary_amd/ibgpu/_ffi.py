@@ -57,6 +57,7 @@ class Physics(C.Structure):
         ("cfl", C.c_double), ("dt_ratio", C.c_double), ("dt_init", C.c_double),
         ("force_ref", C.c_double), ("moment_ref", C.c_double),
         ("p_ref_area", C.c_double), ("shear_ref_area", C.c_double),
+        ("nut_inf", C.c_double), ("nut_inlet", C.c_double), ("Pr_tur", C.c_double),
     ]
 
 
@@ -69,6 +70,7 @@ class Config(C.Structure):
         ("mode", C.c_int32), ("nstage", C.c_int32), ("alpha", C.c_double * 4),
         ("dt_var_step", C.c_int32), ("temporal_refine", C.c_int32),
         ("limiter", C.c_int32), ("viscous", C.c_int32), ("tile", C.c_int32),
+        ("turbulence", C.c_int32),
     ]
 
 
@@ -135,6 +137,8 @@ GPU_FUNCS = {
     "ibg_upload_mesh": (C.c_int, [C.c_void_p, C.POINTER(MeshSoA), C.POINTER(Physics)]),
     "ibg_set_state": (C.c_int, [C.c_void_p, f64p, C.c_int64]),
     "ibg_get_state": (C.c_int, [C.c_void_p, f64p, C.c_int64]),
+    "ibg_set_nut": (C.c_int, [C.c_void_p, f64p, C.c_int64]),
+    "ibg_get_nut": (C.c_int, [C.c_void_p, f64p, C.c_int64]),
     "ibg_bc_fill": (C.c_int, [C.c_void_p, C.c_int32]),
     "ibg_interface_exchange": (C.c_int, [C.c_void_p, C.c_int32]),
     "ibg_ib_fill": (C.c_int, [C.c_void_p, C.POINTER(BodyMotion)]),
@@ -144,10 +148,13 @@ GPU_FUNCS = {
     "ibg_stage": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "ibg_time_march": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "ibg_residual": (C.c_int, [C.c_void_p, f64p, f64p]),
+    "ibg_residual_nut": (C.c_int, [C.c_void_p, f64p, f64p]),
     "ibg_forces": (C.c_int, [C.c_void_p, C.POINTER(BodyMotion), f64p]),
     "ibg_get_faceflux": (C.c_int, [C.c_void_p, f64p, C.c_int64]),
     "ibg_get_vsflux": (C.c_int, [C.c_void_p, f64p, C.c_int64]),
     "ibg_get_source": (C.c_int, [C.c_void_p, f64p, C.c_int64]),
+    "ibg_get_mutflux": (C.c_int, [C.c_void_p, f64p, C.c_int64]),
+    "ibg_get_nut_source": (C.c_int, [C.c_void_p, f64p, C.c_int64]),
     "ibg_get_dt": (C.c_int, [C.c_void_p, f64p]),
     "ibg_set_dt": (C.c_int, [C.c_void_p, f64p]),
     "ibg_last_error": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),

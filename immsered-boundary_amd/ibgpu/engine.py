@@ -24,7 +24,7 @@ class IBGError(RuntimeError):
 
 
 def make_config(mode="fp64", nstage=3, dt_var_step=None, temporal_refine=True, limiter=0,
-                viscous=True, tile=0, nlevels=1):
+                viscous=True, tile=0, nlevels=1, turbulence=False):
     cfg = _ffi.Config()
     cfg.mode = MODES[mode]
     cfg.nstage = nstage
@@ -36,16 +36,18 @@ def make_config(mode="fp64", nstage=3, dt_var_step=None, temporal_refine=True, l
     cfg.limiter = limiter
     cfg.viscous = 1 if viscous else 0
     cfg.tile = tile
+    cfg.turbulence = 1 if turbulence else 0
     return cfg
 
 
 class Engine:
     def __init__(self, mesh, mode="fp64", nstage=3, temporal_refine=True, limiter=0,
-                 viscous=True, tile=0, dt_var_step=None):
+                 viscous=True, tile=0, dt_var_step=None, turbulence=False):
         self.lib = _ffi.gpu_lib()
         self.mesh = mesh
+        self.turbulence = bool(turbulence)
         self.cfg = make_config(mode, nstage, dt_var_step, temporal_refine, limiter, viscous, tile,
-                               mesh.nlevels)
+                               mesh.nlevels, turbulence)
         self.ctx = C.c_void_p()
         self._chk(self.lib.ibg_create(C.byref(self.cfg), C.byref(self.ctx)), "ibg_create")
         self._chk(self.lib.ibg_upload_mesh(self.ctx, mesh.soa_ptr, mesh.phys), "ibg_upload_mesh")
@@ -72,6 +74,17 @@ class Engine:
         self._chk(self.lib.ibg_get_state(self.ctx, fv.ctypes.data_as(_ffi.f64p), fv.shape[0]),
                   "ibg_get_state")
         return fv
+
+    def set_nut(self, nut):
+        nut = np.ascontiguousarray(nut, dtype=np.float64)
+        self._chk(self.lib.ibg_set_nut(self.ctx, nut.ctypes.data_as(_ffi.f64p), nut.shape[0]),
+                  "ibg_set_nut")
+
+    def get_nut(self):
+        nut = np.zeros(self.mesh.ncells)
+        self._chk(self.lib.ibg_get_nut(self.ctx, nut.ctypes.data_as(_ffi.f64p), nut.shape[0]),
+                  "ibg_get_nut")
+        return nut
 
     def set_dt(self, dt):
         dt = np.ascontiguousarray(dt, dtype=np.float64)
@@ -114,6 +127,13 @@ class Engine:
                                         l2.ctypes.data_as(_ffi.f64p)), "ibg_residual")
         return linf, l2
 
+    def ResinDomain_nut(self):
+        n = self.mesh.ncases
+        linf, l2 = np.zeros(n), np.zeros(n)
+        self._chk(self.lib.ibg_residual_nut(self.ctx, linf.ctypes.data_as(_ffi.f64p),
+                                            l2.ctypes.data_as(_ffi.f64p)), "ibg_residual_nut")
+        return linf, l2
+
     def ComptWallForce(self, motion=None):
         out = np.zeros(self.mesh.ncases * 6)
         self._chk(self.lib.ibg_forces(self.ctx, motion, out.ctypes.data_as(_ffi.f64p)), "ibg_forces")
@@ -136,6 +156,18 @@ class Engine:
         out = np.zeros((self.mesh.ncells, 5))
         self._chk(self.lib.ibg_get_source(self.ctx, out.ctypes.data_as(_ffi.f64p), out.shape[0]),
                   "ibg_get_source")
+        return out
+
+    def mutflux(self):
+        out = np.zeros(self.mesh.nfaces)
+        self._chk(self.lib.ibg_get_mutflux(self.ctx, out.ctypes.data_as(_ffi.f64p), out.shape[0]),
+                  "ibg_get_mutflux")
+        return out
+
+    def nut_source(self):
+        out = np.zeros(self.mesh.ncells)
+        self._chk(self.lib.ibg_get_nut_source(self.ctx, out.ctypes.data_as(_ffi.f64p),
+                                              out.shape[0]), "ibg_get_nut_source")
         return out
 
     def timing(self):

@@ -44,7 +44,7 @@ typedef enum ibg_status {
   IBG_ERR_NAN_RESIDUAL = 268,/* NS_Solver.H:355-375                          */
   IBG_ERR_LIMITER = 57,      /* Slopelimiter.H:54-57                         */
   IBG_ERR_NEG_DT = 424,      /* NS_Solver.cpp:253-258                        */
-  IBG_ERR_NEG_HG_T = 1050    /* Body.cpp:1442-1451                           */
+  IBG_ERR_NEG_HG_T = 1050    /* Body.cpp:1429-1437                           */
 } ibg_status;
 
 /* cell kinds on a level (Box.H:63-66 types plus the AMR pair role) */
@@ -136,6 +136,14 @@ typedef struct ibg_physics {
   double force_ref;            /* F_ref for C_L/C_D                           */
   double moment_ref;           /* M_ref                                       */
   double p_ref_area, shear_ref_area;  /* p_ref*area_ref, shear_ref*area_ref   */
+  /* SA closure (used when ibg_config.turbulence = 1)                        */
+  double nut_inf;              /* initvar.viseddy = far_vis_ratio*viseddy
+                                  (Userfunction_cy_naca15.cpp:368-370)         */
+  double nut_inlet;            /* facevar[0].viseddy of the -13 inlet
+                                  (BCValues.H:203-205); the reference never
+                                  sets it (BCValues.H:32), 0 is the value a
+                                  fresh allocation holds                      */
+  double Pr_tur;               /* FlowConstants.cpp:11                         */
 } ibg_physics;
 
 /* Rigid body motion of one body at the current step (Body.H:709-721).     */
@@ -153,6 +161,9 @@ typedef struct ibg_config {
   int32_t limiter;             /* 0 van Leer (Slopelimiter.H:17-20), 1 van Albada */
   int32_t viscous;             /* VISCOUSITY (NS_Solver.H:15)                 */
   int32_t tile;                /* cells per stage-kernel tile (0 = default)   */
+  int32_t turbulence;          /* 1: Spalart-Allmaras closure of the
+                                  -D TURBULENCE build (Turbulence.cpp,
+                                  NS_Solver.cpp:632-840); 0: laminar          */
 } ibg_config;
 
 typedef struct ibg_ctx ibg_ctx;
@@ -180,6 +191,11 @@ ibg_status ibg_upload_mesh(ibg_ctx *ctx, const ibg_mesh_soa *mesh,
  * Meshlevel::m_data Meshlevel.H:233 (and InitFlow NS_Solver.H:432-446).    */
 ibg_status ibg_set_state(ibg_ctx *ctx, const double *fv_aos, int64_t ncells);
 ibg_status ibg_get_state(ibg_ctx *ctx, double *fv_aos, int64_t ncells);
+/* SA working variable per cell, FlowVariables::viseddy (FlowVariables.H:
+ * 327-329, present with -D TURBULENCE); [ncells].  IBG_ERR_STATE when the
+ * context runs without the closure.                                        */
+ibg_status ibg_set_nut(ibg_ctx *ctx, const double *nut, int64_t ncells);
+ibg_status ibg_get_nut(ibg_ctx *ctx, double *nut, int64_t ncells);
 
 /* Per-step ghost fills.
  * BCValues::BCBoxTreat(Mesh&) BCValues.H:51-188                            */
@@ -213,6 +229,10 @@ ibg_status ibg_time_march(ibg_ctx *ctx, int32_t ts0, int32_t nsteps,
  * linf[case*5+v] = max |source[0][v]| over the marched cells (unnormalised),
  * l2[case*5+v]   = sqrt(sum source[0][v]^2) over the same cells.            */
 ibg_status ibg_residual(ibg_ctx *ctx, double *linf, double *l2);
+/* SA residual of the last stage: res_vis_sum of ResinDomain NS_Solver.H:
+ * 297, 321 (max |mutsource| over the same cells, unnormalised) and its L2
+ * norm; [ncases] each.                                                     */
+ibg_status ibg_residual_nut(ibg_ctx *ctx, double *linf, double *l2);
 /* Body::ComptWallForce(Mesh&) Body.cpp:439-558, summed over the bodies of
  * each case (Bodymove.cpp:49-55): out[case*6 + {Fx,Fy,Fz,Mx,My,Mz}] in the
  * reference's dimensional units (force, moment about body 0 center).       */
@@ -225,6 +245,13 @@ ibg_status ibg_forces(ibg_ctx *ctx, const ibg_body_motion *motion,
 ibg_status ibg_get_faceflux(ibg_ctx *ctx, double *out, int64_t nfaces);
 ibg_status ibg_get_vsflux(ibg_ctx *ctx, double *out, int64_t nfaces);
 ibg_status ibg_get_source(ibg_ctx *ctx, double *out, int64_t ncells);
+/* SA closure probes: mutflux [nfaces] of the current state
+ * (NS_Solver::mutflux NS_Solver.H:867, NS_Solver.cpp:541-542) and the
+ * mutsource [ncells] of the last stage (NS_Solver.H:868) on the cells the
+ * update consumes it (0 on IB cells, which the reference evaluates but
+ * never uses).                                                             */
+ibg_status ibg_get_mutflux(ibg_ctx *ctx, double *out, int64_t nfaces);
+ibg_status ibg_get_nut_source(ibg_ctx *ctx, double *out, int64_t ncells);
 ibg_status ibg_get_dt(ibg_ctx *ctx, double *dt /* [ncases] */);
 ibg_status ibg_set_dt(ibg_ctx *ctx, const double *dt /* [ncases] */);
 

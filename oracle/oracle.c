@@ -22,6 +22,96 @@ enum { RO = 0, UU = 1, VV = 2, WW = 3, PP = 4, TT = 5, EE = 6 };
 
 static inline int blk_of(const ibg_mesh_soa *m, int c, int l) { return c * m->nlevels + l; }
 
+/* Spalart-Allmaras constants, Turbulence.cpp:3-19 */
+#define ORC_SA_CB1 0.1355
+#define ORC_SA_SEGMA (2.0 / 3.0)
+#define ORC_SA_CB2 0.622
+#define ORC_SA_KAPPA 0.41
+#define ORC_SA_CW2 0.3
+#define ORC_SA_CW3 2.0
+#define ORC_SA_CV1 7.1
+#define ORC_SA_CT3 1.2
+#define ORC_SA_CT4 0.5
+static const double orc_sa_cw1 = ORC_SA_CB1 / (ORC_SA_KAPPA * ORC_SA_KAPPA) + (1.0 + ORC_SA_CB2) / ORC_SA_SEGMA;
+static const double orc_sa_cv1three = ORC_SA_CV1 * ORC_SA_CV1 * ORC_SA_CV1;
+
+/* Viseddy2mut Turbulence.cpp:23-30 */
+double orc_sa_mut(double nu, double roe, double mu) {
+  double bigx = nu / (mu / roe);
+  if (bigx < 0.0) bigx = 0.0;
+  double b3 = bigx * bigx * bigx;
+  double fv1 = b3 / (b3 + orc_sa_cv1three);
+  return roe * fv1 * nu;
+}
+
+/* yplus_as_function_of_re / uplus_as_function_of_yplus Turbulence.cpp:32-55 */
+double orc_sa_yplus(double re) {
+  double sq = sqrt(re);
+  if (re <= 9.0) return 1.002 * sq - 0.001223;
+  if (re <= 37.0) return 1.272 * sq - 0.9881 * sqrt(sq) + 0.9054;
+  if (re <= 98.0) return 0.2715 * sq + 1.144 * sqrt(sq) + 0.04718 * re;
+  if (re <= 250.0) return 0.04921 * sq + 1.576 * sqrt(sq) + 0.05582 * re;
+  if (re <= 790.0) return 0.5382 * sq + 0.2478 * sqrt(sq) + 0.04593 * re;
+  if (re <= 2800.0) return 1.105 * sq - 1.782 * sqrt(sq) + 0.03933 * re;
+  return 2.643 * sq - 10.52 * sqrt(sq) + 0.03207 * re;
+}
+
+double orc_sa_uplus(double y) {
+  if (y < 3.0) return y;
+  double lg = log(y), sq = sqrt(y), qu = sqrt(sq);
+  if (y < 8.5) return -6.645 * lg + 25.59 * sq - 21.6 * qu - 1.864 * y;
+  if (y < 50.0) return 12.04 * lg - 4.079 * sq - 4.504 * qu + 0.1715 * y;
+  if (y < 300.0) return 3.031 * lg - 0.6252 * sq + 2.639 * qu + 0.007553 * y;
+  if (y < 1000.0) return -3.535 * lg - 1.104 * sq + 13.89 * qu + 0.004011 * y;
+  return 16.66 * lg + 1.408 * sq - 23.82 * qu - 0.002909 * y;
+}
+
+/* ComptMutSource NS_Solver.cpp:648-808 for one non-solid cell.
+   g[k] = d(nu)/d(kesai_k), gu/gv/gw the velocity differences (central, /2),
+   keisa = diag(k0,k1,k2); mutf = the face-flux term sum(dirsign mutflux)/V/Re. */
+double orc_sa_cell_source(double u, double v, double w, double T, double roe, double nu,
+                          double sd, const double *g, const double *gu, const double *gv,
+                          const double *gw, double k0, double k1, double k2, double SoT,
+                          double Re, double mutf) {
+  double dvisx = g[0] * k0, dvisy = g[1] * k1, dvisz = g[2] * k2;
+  double duy = gu[1] * k1, duz = gu[2] * k2;
+  double dvx = gv[0] * k0, dvz = gv[2] * k2;
+  double dwx = gw[0] * k0, dwy = gw[1] * k1;
+  double ox = 0.5 * (dwy - dvz), oy = 0.5 * (duz - dwx), oz = 0.5 * (dvx - duy);
+  double mu0 = sqrt(pow(T, 3)) * ((1.0 + SoT) / (T + SoT));
+  double vis0 = mu0 / roe;
+  double bigx = nu / vis0;
+  double b3 = bigx * bigx * bigx;
+  double fv1 = b3 / (b3 + orc_sa_cv1three);
+  double fv2 = 1.0 - bigx / (1.0 + bigx * fv1);
+  double segdis, vis2d;
+  if (fabs(sd) < 0.00000001) {
+    segdis = 1.0;
+    vis2d = 1.0;
+  } else {
+    segdis = ORC_SA_KAPPA * sd;
+    vis2d = nu / sd;
+  }
+  segdis *= segdis;
+  double olen = sqrt(2.0) * sqrt(ox * ox + oy * oy + oz * oz);
+  double bigs = olen + nu * fv2 / segdis / Re;
+  bigs = (0.3 * olen < bigs) ? bigs : 0.3 * olen;            /* std::max */
+  double ft2 = ORC_SA_CT3 * exp(-ORC_SA_CT4 * bigx * bigx);
+  double r = nu / (bigs * segdis) / Re;
+  r = (10.0 < r) ? 10.0 : r;                                  /* std::min */
+  double rsix = pow(r, 6);
+  double gg = r + ORC_SA_CW2 * (rsix - r);
+  double gsix = pow(gg, 6);
+  double cw3six = pow(ORC_SA_CW3, 6);
+  double fw = gg * pow((1.0 + cw3six) / (gsix + cw3six), 1.0 / 6.0);
+  double src = -(u * dvisx + v * dvisy + w * dvisz);
+  src += ORC_SA_CB1 * (1.0 - ft2) * bigs * nu;
+  src -= (orc_sa_cw1 * fw - ORC_SA_CB1 * ft2 / (ORC_SA_KAPPA * ORC_SA_KAPPA)) * (vis2d * vis2d) / Re;
+  src += ORC_SA_CB2 * (dvisx * dvisx + dvisy * dvisy + dvisz * dvisz) / ORC_SA_SEGMA / Re;
+  src += mutf;
+  return src;
+}
+
 /* ---------------------------------------------------------------- schedule
  * mpiparams.cpp:186-212: marching_step[l] = 2^(L-1-l), marching_left = 0;
  * average/prolongation_step[l] = 2^(L-l), left = 2^(L-l-1); step_num =
@@ -214,8 +304,9 @@ void orc_face_flux(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *
  * pfpkesai0 / pfpkesai1 of NS_Solver.H:115-154.  The cell metric keisa is
  * diag(1/dh) (ComptCellParams MeshOp.H:10-12); its zero off-diagonal terms
  * contribute exact zeros and are dropped.                                 */
-void orc_viscous_flux(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *cfg,
-                      const double *fv, int32_t ts, double *vsflux) {
+static void visc_core(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *cfg,
+                      const double *fv, const double *nut, int32_t ts, double *vsflux,
+                      double *mutflux) {
   const int L = m->nlevels;
   for (int c = 0; c < m->ncases; ++c) {
     const ibg_physics *P = &ph[c];
@@ -259,6 +350,21 @@ void orc_viscous_flux(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cf
         double mu0 = sqrt(pow(ftemp, 3)) * ((1.0 + P->S_over_T_ref) / (ftemp + P->S_over_T_ref));
         double qk0 = P->qk * mu0;
         const double Re = P->Re;
+        if (nut) {
+          /* NS_Solver.cpp:534-542: mu_t of the face joins mu and the heat
+             conduction; mutflux = (nu_f + mu/roe_f) (d nu/d n)/sigma * A with
+             the face keisa the unit axis (dvis along d = (nu_R - nu_L)/dh) */
+          double fvis = (nut[abox] + nut[rb]) * 0.5;
+          double froe = (FL[RO] + FR[RO]) * 0.5;
+          double faceroexvis = fvis + mu0 / froe;
+          double mut0 = orc_sa_mut(fvis, froe, mu0);
+          double kd = d == 0 ? k0 : (d == 1 ? k1 : k2);
+          double dvis = (nut[rb] - nut[abox]) * kd;
+          mu0 += mut0;
+          qk0 += mut0 / (Re * P->Pr_tur * (P->gama - 1) * pow(P->Ma, 2));
+          mutflux[f] = faceroexvis * dvis / ORC_SA_SEGMA;
+          mutflux[f] *= face_area(h, d);
+        }
         double tauxx = 2.0 * mu0 * (dux - (dux + dvy + dwz) / 3.0) / Re;
         double tauyy = 2.0 * mu0 * (dvy - (dux + dvy + dwz) / 3.0) / Re;
         double tauzz = 2.0 * mu0 * (dwz - (dux + dvy + dwz) / 3.0) / Re;
@@ -283,6 +389,17 @@ void orc_viscous_flux(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cf
       }
     }
   }
+}
+
+void orc_viscous_flux(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *cfg,
+                      const double *fv, int32_t ts, double *vsflux) {
+  visc_core(m, ph, cfg, fv, NULL, ts, vsflux, NULL);
+}
+
+void orc_viscous_flux_sa(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *cfg,
+                         const double *fv, const double *nut, int32_t ts, double *vsflux,
+                         double *mutflux) {
+  visc_core(m, ph, cfg, fv, nut, ts, vsflux, mutflux);
 }
 
 /* ComptSource NS_Solver.cpp:282-363: (F_lo - F_hi) + (Fv_hi - Fv_lo) per
@@ -729,5 +846,349 @@ int32_t orc_step(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *cf
   if (!code) orc_residual(m, cfg, src, ts, linf, l2);
   if (source_out) memcpy(source_out, src, sizeof(double) * N * 5);
   free(ff); free(vs); free(src); free(cvo); free(cvn);
+  return code;
+}
+
+/* ===================================================== SA closure (TURBULENCE)
+ * The -D TURBULENCE build of the reference: FlowVariables carries the SA
+ * working variable viseddy (here the separate array nut[ncells]).          */
+
+/* ComptMutSource NS_Solver.cpp:632-840 over the non-solid normal cells of
+ * the marched levels; mutsource of every other cell is left at 0 (the
+ * reference never writes it).                                             */
+void orc_mut_source(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *cfg,
+                    const double *fv, const double *nut, const double *mutflux, int32_t ts,
+                    double *mutsource) {
+  const int L = m->nlevels;
+  const int D = m->dim;
+  for (int c = 0; c < m->ncases; ++c) {
+    const ibg_physics *P = &ph[c];
+    for (int l = 0; l < L; ++l) {
+      if (!orc_marched(L, l, ts, cfg->temporal_refine)) continue;
+      int b = blk_of(m, c, l);
+      const double *h = &m->blk_h[3 * b];
+      const double k0 = 1.0 / h[0], k1 = 1.0 / h[1], k2 = 1.0 / h[2];
+      const double vol = h[0] * h[1] * h[2];
+      int s = m->blk_start[b], e = s + m->blk_nnormal[b];
+#pragma omp parallel for schedule(static)
+      for (int g = s; g < e; ++g) {
+        mutsource[g] = 0.0;
+        if (m->signdis[g] < 0.0) continue;   /* solid */
+        int xl = nbr(m, g, -1, 0, 0), xr = nbr(m, g, 1, 0, 0);
+        int yl = nbr(m, g, 0, -1, 0), yr = nbr(m, g, 0, 1, 0);
+        int zl = nbr(m, g, 0, 0, -1), zr = nbr(m, g, 0, 0, 1);
+        double gn[3] = {(nut[xr] - nut[xl]) / 2.0, (nut[yr] - nut[yl]) / 2.0, (nut[zr] - nut[zl]) / 2.0};
+        double gu[3], gv[3], gw[3];
+        const double *FX0 = &fv[(size_t)xl * NF], *FX1 = &fv[(size_t)xr * NF];
+        const double *FY0 = &fv[(size_t)yl * NF], *FY1 = &fv[(size_t)yr * NF];
+        const double *FZ0 = &fv[(size_t)zl * NF], *FZ1 = &fv[(size_t)zr * NF];
+        gu[0] = (FX1[UU] - FX0[UU]) / 2.0; gu[1] = (FY1[UU] - FY0[UU]) / 2.0; gu[2] = (FZ1[UU] - FZ0[UU]) / 2.0;
+        gv[0] = (FX1[VV] - FX0[VV]) / 2.0; gv[1] = (FY1[VV] - FY0[VV]) / 2.0; gv[2] = (FZ1[VV] - FZ0[VV]) / 2.0;
+        gw[0] = (FX1[WW] - FX0[WW]) / 2.0; gw[1] = (FY1[WW] - FY0[WW]) / 2.0; gw[2] = (FZ1[WW] - FZ0[WW]) / 2.0;
+        /* total_face_flux over the cell's faces, dirsign = {-1, +1} */
+        double tot = 0.0;
+        for (int di = 0; di < D; ++di)
+          for (int f0 = 0; f0 < 2; ++f0) {
+            int fi = m->cell_faces[(size_t)g * 6 + di * 2 + f0];
+            tot += (f0 == 0 ? -1.0 : 1.0) * mutflux[fi];
+          }
+        const double *W = &fv[(size_t)g * NF];
+        mutsource[g] = orc_sa_cell_source(W[UU], W[VV], W[WW], W[TT], W[RO], nut[g], m->signdis[g],
+                                          gn, gu, gv, gw, k0, k1, k2, P->S_over_T_ref, P->Re,
+                                          tot / vol / P->Re);
+      }
+    }
+  }
+}
+
+/* The viseddy part of UpdateConsVars (NS_Solver.cpp:873-876, 954-957:
+ * max(c_vis_old + mutsource*level_ts, 0)) and of ComptNewPrimVar (m_data =
+ * initvar resets viseddy to initvar.viseddy, :32-54, :170-176), on the same
+ * cells as orc_update.                                                    */
+void orc_sa_update(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *cfg,
+                   const double *nut_old, const double *mutsource, const double *dt,
+                   int32_t ts, int32_t subt, double *nut) {
+  const int L = m->nlevels;
+  for (int c = 0; c < m->ncases; ++c) {
+    const ibg_physics *P = &ph[c];
+    const double dt0 = cfg->alpha[subt] * dt[c];
+    for (int l = 0; l < L; ++l) {
+      if (!orc_marched(L, l, ts, cfg->temporal_refine)) continue;
+      int b = blk_of(m, c, l);
+      const double level_ts = dt0 * orc_step_num(L, l, cfg->temporal_refine);
+      const int finest = (l == L - 1) && (m->nbody > 0);
+      int s = m->blk_start[b], e = s + m->blk_nnormal[b];
+#pragma omp parallel for schedule(static)
+      for (int g = s; g < e; ++g) {
+        const int solid = m->signdis[g] < 0.0;
+        const int ib = m->ibidx[g] >= 0;
+        if (!finest || (!solid && !ib)) {
+          double v = nut_old[g] + mutsource[g] * level_ts;
+          nut[g] = (v < 0.0) ? 0.0 : v;   /* max(v, min_viseddy[_wall] = 0) */
+        }
+        if (!finest) {
+          if (!(m->signdis[g] > 0.0)) nut[g] = P->nut_inf;
+        } else if (!ib && !(m->signdis[g] > 0.0)) {
+          nut[g] = P->nut_inf;
+        }
+      }
+    }
+  }
+}
+
+/* viseddy of the domain ghosts (BCValues.H:203-205 inlet = facevar, -12
+ * and -50 copy the reference cell with the rest of the state).            */
+int32_t orc_sa_bc_fill(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *cfg,
+                       double *nut, int32_t ts) {
+  const int L = m->nlevels;
+  for (int i = 0; i < m->ndmg; ++i) {
+    int g = m->dmg_cell[i], r = m->dmg_ref[i], ty = m->dmg_type[i];
+    int c = 0, l = 0;
+    for (int b = 0; b < m->ncases * L; ++b)
+      if (g >= m->blk_start[b] && g < m->blk_start[b + 1]) { c = b / L; l = b % L; break; }
+    if (!orc_marched(L, l, ts, cfg->temporal_refine)) continue;
+    if (ty == -99) {
+    } else if (ty == -50 || ty == -12) {
+      nut[g] = nut[r];
+    } else if (ty == -13) {
+      nut[g] = ph[c].nut_inlet;
+    } else {
+      return 52;
+    }
+  }
+  return 0;
+}
+
+/* viseddy of the AMR transfer: AverageCell[_2d] TwoLevelOp.H:129-179 (plain
+ * mean of the sons) and GradientInaCell / ComptFineCellData[_2d] :91-127
+ * (central gradient over the cneib neighbours, sons at +-1/4), on the same
+ * schedule as orc_interface_exchange.                                     */
+void orc_sa_exchange(const ibg_mesh_soa *m, const orc_cfg *cfg, double *nut, int32_t ts) {
+  const int L = m->nlevels;
+  for (int l = 0; l + 1 < L; ++l) {
+    if (!orc_transfer_due(L, l, ts, cfg->temporal_refine)) continue;
+    for (int i = 0; i < m->nres; ++i) {
+      int b = m->res_blk[i];
+      if (b % L != l) continue;
+      const int twod = m->blk_twod[b + 1];
+      const int *fi = &m->res_fi[(size_t)i * 8];
+      double v = 0.0;
+      for (int q = 0; q < 8; ++q) {
+        if (twod && (q & 1)) continue;
+        v += nut[fi[q]];
+      }
+      nut[m->res_ci[i]] = v * (twod ? 0.25 : 0.125);
+    }
+  }
+  const double fincrem[2] = {-0.25, 0.25};
+  for (int l = 0; l + 1 < L; ++l) {
+    if (!orc_transfer_due(L, l, ts, cfg->temporal_refine)) continue;
+    for (int i = 0; i < m->npro; ++i) {
+      int b = m->pro_blk[i];
+      if (b % L != l) continue;
+      const int twod = m->blk_twod[b + 1];
+      int ci = m->pro_ci[i];
+      double g0 = (nut[nbr(m, ci, 1, 0, 0)] - nut[nbr(m, ci, -1, 0, 0)]) * 0.5;
+      double g1 = (nut[nbr(m, ci, 0, 1, 0)] - nut[nbr(m, ci, 0, -1, 0)]) * 0.5;
+      double g2 = (nut[nbr(m, ci, 0, 0, 1)] - nut[nbr(m, ci, 0, 0, -1)]) * 0.5;
+      for (int si = 0; si < 2; ++si)
+        for (int sj = 0; sj < 2; ++sj)
+          for (int sk = 0; sk < (twod ? 1 : 2); ++sk) {
+            int f = m->pro_fi[(size_t)i * 8 + si * 4 + sj * 2 + sk];
+            if (twod)
+              nut[f] = nut[ci] + fincrem[si] * g0 + fincrem[sj] * g1;
+            else
+              nut[f] = nut[ci] + fincrem[si] * g0 + fincrem[sj] * g1 + fincrem[sk] * g2;
+          }
+    }
+  }
+}
+
+/* Body::IBConditions_Image with TURBULENCE (Body.cpp:1393-1426):
+ * signdis < 0 -> initvar with the wall velocity; otherwise ImageViseddy
+ * (Turbulence.cpp:57-138).  All IB cells are computed before any is written. */
+int32_t orc_ib_fill_sa(const ibg_mesh_soa *m, const ibg_physics *ph, double *fv, double *nut,
+                       const ibg_body_motion *motion) {
+  int32_t code = 0;
+  const size_t n = (size_t)(m->nib > 0 ? m->nib : 1);
+  double *tmp = (double *)malloc(sizeof(double) * (NF + 1) * n);
+  const int L = m->nlevels;
+  for (int i = 0; i < m->nib; ++i) {
+    int ci = m->ib_cell[i];
+    int c = 0;
+    for (int b = 0; b < m->ncases * L; ++b)
+      if (ci >= m->blk_start[b] && ci < m->blk_start[b + 1]) { c = b / L; break; }
+    const ibg_physics *P = &ph[c];
+    double hg[NF];
+    intp_hg(fv, &m->ib_intp[(size_t)i * 8], &m->ib_coef[(size_t)i * 8], P, hg);
+    double patv[3];
+    body_velocity(m, motion, m->ib_body[i], &m->ib_attach[3 * (size_t)i], patv);
+    const double *nmv = &m->ib_nv[3 * (size_t)i];
+    double *o = &tmp[(size_t)i * (NF + 1)];
+    double sd = m->signdis[ci];
+    if (sd < 0.0) {
+      memcpy(o, P->initvar, sizeof(double) * NF);
+      o[UU] = patv[0]; o[VV] = patv[1]; o[WW] = patv[2];
+      o[NF] = P->nut_inf;
+    } else {
+      double rt0 = m->ib_hgdis[i] + sd;
+      double ib_rdis = sd / rt0;
+      double hvn = hg[UU] * nmv[0] + hg[VV] * nmv[1] + hg[WW] * nmv[2];
+      double hn[3] = {hvn * nmv[0], hvn * nmv[1], hvn * nmv[2]};
+      double ht[3] = {hg[UU] - hn[0], hg[VV] - hn[1], hg[WW] - hn[2]};
+      double wvn = patv[0] * nmv[0] + patv[1] * nmv[1] + patv[2] * nmv[2];
+      double wn[3] = {nmv[0] * wvn, nmv[1] * wvn, nmv[2] * wvn};
+      double wt[3] = {patv[0] - wn[0], patv[1] - wn[1], patv[2] - wn[2]};
+      double df[3] = {ht[0] - wt[0], ht[1] - wt[1], ht[2] - wt[2]};
+      double df_velt = sqrt(df[0] * df[0] + df[1] * df[1] + df[2] * df[2]);
+      double hg_mu = sqrt(pow(hg[TT], 3)) * ((1.0 + P->S_over_T_ref) / (hg[TT] + P->S_over_T_ref));
+      double hg_vis = hg_mu / hg[RO];
+      double hg_re = df_velt * rt0 / hg_vis * P->Re;
+      double hg_yplus = orc_sa_yplus(hg_re);
+      double ut = hg_yplus * hg_vis / rt0 / P->Re;
+      double yplus = ib_rdis * hg_yplus;
+      double uplus = orc_sa_uplus(yplus);
+      double ib_velt = uplus * ut;
+      double ib_r0 = (fabs(df_velt) < 0.00000001) ? 0.0 : ib_velt / df_velt;
+      for (int k = 0; k < 3; ++k) {
+        double ibn = wn[k] + (hn[k] - wn[k]) * ib_rdis;
+        double ibt = wt[k] + df[k] * ib_r0;
+        o[UU + k] = ibn + ibt;
+      }
+      o[PP] = hg[PP];
+      o[TT] = hg[TT] + 0.5 * pow(P->Pr, 1.0 / 3.0) / P->Cp * (pow(df_velt, 2) - pow(ib_velt, 2));
+      o[RO] = o[PP] / (P->Rg * o[TT]);
+      o[EE] = P->Cv * o[TT] + (0.5 * (o[UU] * o[UU] + o[VV] * o[VV] + o[WW] * o[WW]));
+      o[NF] = ORC_SA_KAPPA * yplus;
+    }
+    if (hg[TT] < 0.0 && !code) code = 1050;
+  }
+  for (int i = 0; i < m->nib; ++i) {
+    memcpy(&fv[(size_t)m->ib_cell[i] * NF], &tmp[(size_t)i * (NF + 1)], sizeof(double) * NF);
+    nut[m->ib_cell[i]] = tmp[(size_t)i * (NF + 1) + NF];
+  }
+  free(tmp);
+  return code;
+}
+
+/* Body::ComptWallForce with TURBULENCE (Body.cpp:505-553): Patchut
+ * (Turbulence.cpp:197-226) wall shear roe_w ut^2 along the slip direction,
+ * pressure -p n A (FiltSinglePatchPressure is the identity: Filt.cpp:5-6),
+ * both scaled by p_ref*area_ref.                                           */
+void orc_forces_sa(const ibg_mesh_soa *m, const ibg_physics *ph, const double *fv,
+                   const ibg_body_motion *motion, double *out) {
+  for (int c = 0; c < m->ncases; ++c) {
+    double pf[3] = {0, 0, 0}, tf[3] = {0, 0, 0}, pm[3] = {0, 0, 0}, tm[3] = {0, 0, 0};
+    const ibg_physics *P = &ph[c];
+    int body0 = -1;
+    for (int bi = 0; bi < m->nbody; ++bi) if (m->body_case[bi] == c) { body0 = bi; break; }
+    for (int i = 0; i < m->npatch; ++i) {
+      if (m->pat_case[i] != c) continue;
+      double hg[NF];
+      intp_hg(fv, &m->pat_intp[(size_t)i * 8], &m->pat_coef[(size_t)i * 8], P, hg);
+      const double *pc = &m->pat_pc[3 * (size_t)i];
+      const double *nv = &m->pat_nv[3 * (size_t)i];
+      double area = m->pat_area[i], hgdis = m->pat_hgdis[i];
+      double patv[3];
+      body_velocity(m, motion, body0, pc, patv);
+      const double *bc = body0 >= 0 ? &m->body_center[3 * body0] : pc;
+      double dx[3] = {pc[0] - bc[0], pc[1] - bc[1], pc[2] - bc[2]};
+      double hv = hg[UU] * nv[0] + hg[VV] * nv[1] + hg[WW] * nv[2];
+      double pn = patv[0] * nv[0] + patv[1] * nv[1] + patv[2] * nv[2];
+      double df[3];
+      df[0] = (hg[UU] - nv[0] * hv) - (patv[0] - nv[0] * pn);
+      df[1] = (hg[VV] - nv[1] * hv) - (patv[1] - nv[1] * pn);
+      df[2] = (hg[WW] - nv[2] * hv) - (patv[2] - nv[2] * pn);
+      double dv = sqrt(df[0] * df[0] + df[1] * df[1] + df[2] * df[2]);
+      double vt[3] = {0.0, 0.0, 0.0};
+      if (!(dv < 1e-8)) { vt[0] = df[0] / dv; vt[1] = df[1] / dv; vt[2] = df[2] / dv; }
+      double hg_mu = sqrt(pow(hg[TT], 3)) * ((1.0 + P->S_over_T_ref) / (hg[TT] + P->S_over_T_ref));
+      double hg_re = hg[RO] * dv * hgdis / hg_mu * P->Re;
+      double yplus = orc_sa_yplus(hg_re);
+      double ut = yplus * (hg_mu / hg[RO]) / hgdis / P->Re;
+      double wall_T = hg[TT] + 0.5 * pow(P->Pr, 1.0 / 3.0) / P->Cp * pow(dv, 2);
+      double wall_roe = hg[PP] / (P->Rg * wall_T);
+      double Cf = wall_roe * pow(ut, 2);
+      double df0[3], df1[3];
+      for (int k = 0; k < 3; ++k) {
+        df0[k] = vt[k] * Cf * area;
+        df1[k] = nv[k] * (-hg[PP] * area);
+      }
+      double dm0[3] = {dx[1] * df0[2] - dx[2] * df0[1], dx[2] * df0[0] - dx[0] * df0[2], dx[0] * df0[1] - dx[1] * df0[0]};
+      double dm1[3] = {dx[1] * df1[2] - dx[2] * df1[1], dx[2] * df1[0] - dx[0] * df1[2], dx[0] * df1[1] - dx[1] * df1[0]};
+      for (int k = 0; k < 3; ++k) { pf[k] += df1[k]; tf[k] += df0[k]; pm[k] += dm1[k]; tm[k] += dm0[k]; }
+    }
+    for (int k = 0; k < 3; ++k) {
+      out[c * 6 + k] = pf[k] * P->p_ref_area + tf[k] * P->p_ref_area;
+      out[c * 6 + 3 + k] = pm[k] * P->p_ref_area + tm[k] * P->p_ref_area;
+    }
+  }
+}
+
+/* ResinDomain's res_vis_sum (NS_Solver.H:297, 321) and the L2 norm, over the
+ * cells of orc_residual.                                                  */
+void orc_residual_nut(const ibg_mesh_soa *m, const orc_cfg *cfg, const double *mutsource,
+                      int32_t ts, double *linf, double *l2) {
+  const int L = m->nlevels;
+  for (int c = 0; c < m->ncases; ++c) {
+    double mx = 0.0, sq = 0.0;
+    for (int l = 0; l < L; ++l) {
+      const int finest = (l == L - 1) && m->nbody > 0;
+      if (!finest && !orc_marched(L, l, ts, cfg->temporal_refine)) continue;
+      int b = blk_of(m, c, l);
+      int s = m->blk_start[b], e = s + m->blk_nnormal[b];
+      for (int g = s; g < e; ++g) {
+        if (finest && !(m->ibidx[g] == -1 && m->signdis[g] > 0.0)) continue;
+        double a = mutsource[g];
+        mx = fmax(fabs(a), mx);
+        sq += a * a;
+      }
+    }
+    if (linf) linf[c] = mx;
+    if (l2) l2[c] = sqrt(sq);
+  }
+}
+
+/* One step of the TURBULENCE build (main.cpp:84-99, TimeMarching
+ * NS_Solver.H:657-702 with ComptMutSource after ComptSource).              */
+int32_t orc_step_sa(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *cfg,
+                    double *fv, double *nut, double *dt, int32_t ts, double *source_out,
+                    double *mutsource_out, double *linf, double *l2, double *nlinf,
+                    double *nl2, int64_t *bad_cell) {
+  const size_t N = (size_t)m->ncells, F = (size_t)m->nfaces;
+  double *ff = (double *)calloc(F * 5 + 1, sizeof(double));
+  double *vs = (double *)calloc(F * 5 + 1, sizeof(double));
+  double *mf = (double *)calloc(F + 1, sizeof(double));
+  double *src = (double *)calloc(N * 5, sizeof(double));
+  double *msrc = (double *)calloc(N + 1, sizeof(double));
+  double *cvo = (double *)calloc(N * 5, sizeof(double));
+  double *cvn = (double *)calloc(N * 5, sizeof(double));
+  double *nuo = (double *)calloc(N + 1, sizeof(double));
+  int32_t code = 0;
+  orc_interface_exchange(m, ph, cfg, fv, ts);
+  orc_sa_exchange(m, cfg, nut, ts);
+  code = orc_ib_fill_sa(m, ph, fv, nut, NULL);
+  if (!code && cfg->dt_var_step > 0 && ts % cfg->dt_var_step == 0) code = orc_compute_dt(m, ph, fv, dt);
+  if (!code) {
+    orc_old_cons(m, cfg, fv, ts, cvo);
+    memcpy(nuo, nut, sizeof(double) * N);   /* c_vis_old */
+    code = orc_bc_fill(m, ph, cfg, fv, ts);
+    if (!code) code = orc_sa_bc_fill(m, ph, cfg, nut, ts);
+  }
+  if (!code) orc_viscous_flux_sa(m, ph, cfg, fv, nut, ts, vs, mf);
+  for (int st = 0; st < cfg->nstage && !code; ++st) {
+    orc_face_flux(m, ph, cfg, fv, ts, ff);
+    orc_source(m, cfg, ff, vs, ts, src);
+    orc_mut_source(m, ph, cfg, fv, nut, mf, ts, msrc);
+    code = orc_update(m, ph, cfg, cvo, src, dt, ts, st, cvn, fv, bad_cell);
+    orc_sa_update(m, ph, cfg, nuo, msrc, dt, ts, st, nut);
+  }
+  if (!code) {
+    orc_residual(m, cfg, src, ts, linf, l2);
+    orc_residual_nut(m, cfg, msrc, ts, nlinf, nl2);
+  }
+  if (source_out) memcpy(source_out, src, sizeof(double) * N * 5);
+  if (mutsource_out) memcpy(mutsource_out, msrc, sizeof(double) * N);
+  free(ff); free(vs); free(mf); free(src); free(msrc); free(cvo); free(cvn); free(nuo);
   return code;
 }

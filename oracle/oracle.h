@@ -80,6 +80,40 @@ int32_t orc_step(const ibg_mesh_soa *m, const ibg_physics *ph,
                  const orc_cfg *cfg, double *fv, double *dt, int32_t ts,
                  double *source, double *linf, double *l2, int64_t *bad_cell);
 
+/* ---------------------------------------------- SA closure (TURBULENCE)
+ * nut[ncells] is FlowVariables::viseddy of the -D TURBULENCE build.       */
+double orc_sa_mut(double nu, double roe, double mu);          /* Turbulence.cpp:23-30 */
+double orc_sa_yplus(double re);                                /* Turbulence.cpp:32-42 */
+double orc_sa_uplus(double yplus);                             /* Turbulence.cpp:44-55 */
+/* one cell of ComptMutSource (NS_Solver.cpp:648-808); g, gu, gv, gw are the
+   central differences /2 along kesai of nu, u, v, w; mutf the face term   */
+double orc_sa_cell_source(double u, double v, double w, double T, double roe, double nu,
+                          double sd, const double *g, const double *gu, const double *gv,
+                          const double *gw, double k0, double k1, double k2, double SoT,
+                          double Re, double mutf);
+void orc_viscous_flux_sa(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *cfg,
+                         const double *fv, const double *nut, int32_t ts, double *vsflux,
+                         double *mutflux /* [nfaces] */);
+void orc_mut_source(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *cfg,
+                    const double *fv, const double *nut, const double *mutflux, int32_t ts,
+                    double *mutsource /* [ncells] */);
+void orc_sa_update(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *cfg,
+                   const double *nut_old, const double *mutsource, const double *dt,
+                   int32_t ts, int32_t subt, double *nut);
+int32_t orc_sa_bc_fill(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *cfg,
+                       double *nut, int32_t ts);
+void orc_sa_exchange(const ibg_mesh_soa *m, const orc_cfg *cfg, double *nut, int32_t ts);
+int32_t orc_ib_fill_sa(const ibg_mesh_soa *m, const ibg_physics *ph, double *fv, double *nut,
+                       const ibg_body_motion *motion);
+void orc_forces_sa(const ibg_mesh_soa *m, const ibg_physics *ph, const double *fv,
+                   const ibg_body_motion *motion, double *out);
+void orc_residual_nut(const ibg_mesh_soa *m, const orc_cfg *cfg, const double *mutsource,
+                      int32_t ts, double *linf, double *l2);
+int32_t orc_step_sa(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *cfg,
+                    double *fv, double *nut, double *dt, int32_t ts, double *source_out,
+                    double *mutsource_out, double *linf, double *l2, double *nlinf,
+                    double *nl2, int64_t *bad_cell);
+
 #ifdef __cplusplus
 }
 #endif

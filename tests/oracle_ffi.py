@@ -61,6 +61,20 @@ def lib():
         "orc_forces": (None, [M, P, f64p, BM, f64p]),
         "orc_step": (C.c_int32, [M, P, G, f64p, f64p, C.c_int32, f64p, f64p, f64p,
                                  C.POINTER(C.c_int64)]),
+        "orc_sa_mut": (C.c_double, [C.c_double] * 3),
+        "orc_sa_yplus": (C.c_double, [C.c_double]),
+        "orc_sa_uplus": (C.c_double, [C.c_double]),
+        "orc_sa_cell_source": (C.c_double, [C.c_double] * 7 + [f64p] * 4 + [C.c_double] * 6),
+        "orc_viscous_flux_sa": (None, [M, P, G, f64p, f64p, C.c_int32, f64p, f64p]),
+        "orc_mut_source": (None, [M, P, G, f64p, f64p, f64p, C.c_int32, f64p]),
+        "orc_sa_update": (None, [M, P, G, f64p, f64p, f64p, C.c_int32, C.c_int32, f64p]),
+        "orc_sa_bc_fill": (C.c_int32, [M, P, G, f64p, C.c_int32]),
+        "orc_sa_exchange": (None, [M, G, f64p, C.c_int32]),
+        "orc_ib_fill_sa": (C.c_int32, [M, P, f64p, f64p, BM]),
+        "orc_forces_sa": (None, [M, P, f64p, BM, f64p]),
+        "orc_residual_nut": (None, [M, G, f64p, C.c_int32, f64p, f64p]),
+        "orc_step_sa": (C.c_int32, [M, P, G, f64p, f64p, f64p, C.c_int32, f64p, f64p, f64p,
+                                    f64p, f64p, f64p, C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -106,6 +120,45 @@ class Oracle:
         code = self.L.orc_step(self.m.soa_ptr, self.m.phys, C.byref(self.cfg), ptr(fv), ptr(dt),
                                ts, ptr(src), ptr(linf), ptr(l2), C.byref(bad))
         return code, src, linf, l2
+
+    def step_sa(self, fv, nut, dt, ts):
+        n = self.m.ncases
+        src = np.zeros((self.m.ncells, 5))
+        msrc = np.zeros(self.m.ncells)
+        linf, l2 = np.zeros(n * 5), np.zeros(n * 5)
+        nlinf, nl2 = np.zeros(n), np.zeros(n)
+        bad = C.c_int64(-1)
+        code = self.L.orc_step_sa(self.m.soa_ptr, self.m.phys, C.byref(self.cfg), ptr(fv),
+                                  ptr(nut), ptr(dt), ts, ptr(src), ptr(msrc), ptr(linf),
+                                  ptr(l2), ptr(nlinf), ptr(nl2), C.byref(bad))
+        return code, src, msrc, (linf, l2, nlinf, nl2)
+
+    def viscous_flux_sa(self, fv, nut, ts):
+        out = np.zeros((self.m.nfaces, 5))
+        mf = np.zeros(self.m.nfaces)
+        self.L.orc_viscous_flux_sa(self.m.soa_ptr, self.m.phys, C.byref(self.cfg), ptr(fv),
+                                   ptr(nut), ts, ptr(out), ptr(mf))
+        return out, mf
+
+    def mut_source(self, fv, nut, mutflux, ts):
+        out = np.zeros(self.m.ncells)
+        self.L.orc_mut_source(self.m.soa_ptr, self.m.phys, C.byref(self.cfg), ptr(fv), ptr(nut),
+                              ptr(mutflux), ts, ptr(out))
+        return out
+
+    def ib_fill_sa(self, fv, nut, motion=None):
+        return self.L.orc_ib_fill_sa(self.m.soa_ptr, self.m.phys, ptr(fv), ptr(nut), motion)
+
+    def sa_bc_fill(self, nut, ts):
+        return self.L.orc_sa_bc_fill(self.m.soa_ptr, self.m.phys, C.byref(self.cfg), ptr(nut), ts)
+
+    def sa_exchange(self, nut, ts):
+        self.L.orc_sa_exchange(self.m.soa_ptr, C.byref(self.cfg), ptr(nut), ts)
+
+    def forces_sa(self, fv, motion=None):
+        out = np.zeros(self.m.ncases * 6)
+        self.L.orc_forces_sa(self.m.soa_ptr, self.m.phys, ptr(fv), motion, ptr(out))
+        return out
 
     def face_flux(self, fv, ts):
         out = np.zeros((self.m.nfaces, 5))
