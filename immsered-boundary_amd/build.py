@@ -69,7 +69,8 @@ def build_oracle():
     os.makedirs(bdir, exist_ok=True)
     out = os.path.join(bdir, "liboracle.so")
     _run(["gcc", "-O2", "-std=c99", "-fPIC", "-shared", "-fopenmp", f"-I{INC}",
-          os.path.join(ROOT, "oracle", "oracle.c"), "-o", out + ".tmp", "-lm"])
+          os.path.join(ROOT, "oracle", "oracle.c"), os.path.join(ROOT, "oracle", "mlp.c"),
+          "-o", out + ".tmp", "-lm"])
     os.replace(out + ".tmp", out)
 
 

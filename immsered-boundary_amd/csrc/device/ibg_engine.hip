@@ -34,6 +34,7 @@
 #include "ibgpu.h"
 #include "ibg_common.h"
 #include "ibg_kernels.h"
+#include "ibg_closure.h"
 
 using namespace ibg;
 
@@ -66,6 +67,21 @@ struct ibg_ctx {
   void *NUs[3] = {nullptr, nullptr, nullptr};    // its stencil buffers (type R)
   void *mutf = nullptr;                          // [Nn] (type R)
   double *nuprobe = nullptr, *mprobe = nullptr;
+  // learned closure (separate allocations: they survive a mesh upload)
+  struct Closure {
+    bool ready = false, stats = false, on = false;
+    long long cap = 0;
+    float *X = nullptr, *Z = nullptr;
+    int *cell = nullptr;
+    unsigned long long *count = nullptr, *mlcount = nullptr;
+    float *p = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr, *lossp = nullptr, *gsum = nullptr;
+    __bf16 *W1t = nullptr, *W2t = nullptr, *W2 = nullptr;
+    float *w3 = nullptr, *b1 = nullptr, *b2 = nullptr, *b3 = nullptr;
+    float *stat = nullptr;    // fmean[8] fstd[8] flo[8] fhi[8] zst[2]
+    double *mom = nullptr;
+    int step = 0, max_wg = 0;
+    std::vector<void *> allocs;
+  } cl;
   uint8_t *flags = nullptr;
   int *cellcase = nullptr, *nb27 = nullptr;
   TileInfo *tiles = nullptr;
@@ -98,6 +114,14 @@ struct ibg_ctx {
   std::string errmsg;
 
   template <int D> Pack<double, D> w64() const { return Pack<double, D>{(D4 *)W64a, W64e}; }
+  MlpDev mlpdev() const {
+    MlpDev M;
+    M.W1t = cl.W1t; M.W2t = cl.W2t; M.W2 = cl.W2; M.w3 = cl.w3;
+    M.b1 = cl.b1; M.b2 = cl.b2; M.b3 = cl.b3;
+    M.fmean = cl.stat; M.fstd = cl.stat + 8; M.flo = cl.stat + 16; M.fhi = cl.stat + 24;
+    M.zst = cl.stat + 32;
+    return M;
+  }
   template <typename R, int D> Pack<R, D> rv() const {
     return Pack<R, D>{(typename V4<R>::T *)Rva, (R *)Rve};
   }
@@ -191,7 +215,9 @@ inline unsigned grid(int n, int b) { return (unsigned)((n + b - 1) / b); }
 size_t tile_smem(const ibg_ctx *c, size_t rsize, bool stage) {
   size_t per = 4 * rsize + (c->D == 3 ? rsize : 0);
   if (c->turb) per += stage ? 8 * rsize : rsize;
-  return (size_t)c->max_tile_faces * per;
+  const size_t b = (size_t)c->max_tile_faces * per;
+  if (stage && c->cl.on) return std::max(b, kStageMlSmem);
+  return b;
 }
 
 // Stencil buffers of stage k: S0 -> S1 -> S2 -> S1 ... with the last stage
@@ -205,7 +231,7 @@ void stage_bufs(int nstage, int subt, int *in, int *out) {
   }
 }
 
-template <typename R, int D>
+template <typename R, int D, bool ML>
 ibg_status launch_stage(ibg_ctx *c, int ntile, int ncell, int subt, bool last, double *src,
                         double *nusrc) {
   if (ntile <= 0) return IBG_OK;
@@ -224,6 +250,8 @@ ibg_status launch_stage(ibg_ctx *c, int ntile, int ncell, int subt, bool last, d
   a.nu64 = c->nu64;
   a.mutf = (const R *)c->mutf;
   a.signdis = c->signdis;
+  a.mlp = c->mlpdev();
+  a.mlcount = c->cl.mlcount;
   a.alpha = c->cfg.alpha[subt];
   a.limiter = c->cfg.limiter;
   a.last = last ? 1 : 0;
@@ -231,7 +259,7 @@ ibg_status launch_stage(ibg_ctx *c, int ntile, int ncell, int subt, bool last, d
   a.src = src;
   a.nusrc = nusrc;
   a.err = c->err;
-  hipLaunchKernelGGL((k_stage<R, D>), dim3(ntile), dim3(c->tile), tile_smem(c, sizeof(R), true), c->s, a);
+  hipLaunchKernelGGL((k_stage<R, D, ML>), dim3(ntile), dim3(c->tile), tile_smem(c, sizeof(R), true), c->s, a);
   HIPCHK(hipGetLastError());
   if (c->cfg.nstage == 1) {
     // single-stage scheme: the marched cells go back into the home buffer
@@ -248,11 +276,14 @@ ibg_status run_stage(ibg_ctx *c, int ts, int subt, double *src, double *nusrc) {
   const int nt = c->tile_end[lm], ncell = c->cell_end[lm];
   const bool last = (subt == c->cfg.nstage - 1);
   if (last) c->last_ntiles = nt;
+  if (c->cl.on)   // learned closure: mixed mode with the SA closure only
+    return c->D == 2 ? launch_stage<float, 2, true>(c, nt, ncell, subt, last, src, nusrc)
+                     : launch_stage<float, 3, true>(c, nt, ncell, subt, last, src, nusrc);
   if (c->D == 2)
-    return c->mixed ? launch_stage<float, 2>(c, nt, ncell, subt, last, src, nusrc)
-                    : launch_stage<double, 2>(c, nt, ncell, subt, last, src, nusrc);
-  return c->mixed ? launch_stage<float, 3>(c, nt, ncell, subt, last, src, nusrc)
-                  : launch_stage<double, 3>(c, nt, ncell, subt, last, src, nusrc);
+    return c->mixed ? launch_stage<float, 2, false>(c, nt, ncell, subt, last, src, nusrc)
+                    : launch_stage<double, 2, false>(c, nt, ncell, subt, last, src, nusrc);
+  return c->mixed ? launch_stage<float, 3, false>(c, nt, ncell, subt, last, src, nusrc)
+                  : launch_stage<double, 3, false>(c, nt, ncell, subt, last, src, nusrc);
 }
 
 template <typename R, int D>
@@ -534,6 +565,8 @@ ibg_status ibg_destroy(ibg_ctx *c) {
   if (!c) return IBG_ERR_ARG;
   (void)hipStreamSynchronize(c->s);
   free_all(c);
+  for (void *p : c->cl.allocs) (void)hipFree(p);
+  c->cl.allocs.clear();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->s) (void)hipStreamDestroy(c->s);
@@ -1364,6 +1397,305 @@ ibg_status ibg_bench_copy(ibg_ctx *c, int64_t nbytes, int32_t reps, double *avg_
 ibg_status ibg_cell_stage_updates(ibg_ctx *c, int64_t *count) {
   if (!c || !count) return IBG_ERR_ARG;
   *count = c->last_updates;
+  return IBG_OK;
+}
+
+}  // extern "C"
+
+// ======================================================= learned closure
+namespace {
+
+template <typename T>
+ibg_status cl_alloc(ibg_ctx *c, T **p, size_t n) {
+  void *q = nullptr;
+  HIPCHK(hipMalloc(&q, std::max<size_t>(n * sizeof(T), 64)));
+  HIPCHK(hipMemset(q, 0, std::max<size_t>(n * sizeof(T), 64)));
+  c->cl.allocs.push_back(q);
+  *p = reinterpret_cast<T *>(q);
+  return IBG_OK;
+}
+
+uint64_t host_mix64(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+ibg_status cl_pack(ibg_ctx *c) {
+  hipLaunchKernelGGL(k_closure_pack, dim3(grid(64 * kH, 256)), dim3(256), 0, c->s, c->cl.p, c->cl.W1t,
+                     c->cl.W2t, c->cl.W2, c->cl.w3, c->cl.b1, c->cl.b2, c->cl.b3);
+  HIPCHK(hipGetLastError());
+  return IBG_OK;
+}
+
+constexpr size_t kTrainSmem = (size_t)64 * kX * 2 * 2 + (size_t)64 * kH * 2 * 5 + (5 * 64 + 256) * 4;
+
+ibg_status cl_train_launch(ibg_ctx *c, const float *X, const float *Z, long long n, int nwg,
+                           int batch, int explicit_rows, uint64_t seed) {
+  TrainArgs a;
+  a.M = c->mlpdev();
+  a.X = X; a.Z = Z; a.n = n; a.batch = batch; a.explicit_rows = explicit_rows; a.seed = seed;
+  a.grad = c->cl.grad; a.lossp = c->cl.lossp;
+  hipLaunchKernelGGL(k_closure_train, dim3(nwg), dim3(256), kTrainSmem, c->s, a);
+  HIPCHK(hipGetLastError());
+  return IBG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ibg_status ibg_closure_init(ibg_ctx *c, int64_t capacity, uint64_t seed) {
+  if (!c || capacity < 64) return IBG_ERR_ARG;
+  for (void *p : c->cl.allocs) (void)hipFree(p);
+  c->cl = ibg_ctx::Closure();
+  ibg_ctx::Closure &L = c->cl;
+  L.cap = capacity;
+  L.max_wg = 1024;   // batch up to 65536 rows
+  ibg_status st;
+  if ((st = cl_alloc(c, &L.X, (size_t)capacity * kMlpIn))) return st;
+  if ((st = cl_alloc(c, &L.Z, (size_t)capacity))) return st;
+  if ((st = cl_alloc(c, &L.cell, (size_t)capacity))) return st;
+  if ((st = cl_alloc(c, &L.count, 4))) return st;
+  L.mlcount = L.count + 2;
+  if ((st = cl_alloc(c, &L.p, kNParam))) return st;
+  if ((st = cl_alloc(c, &L.m, kNParam))) return st;
+  if ((st = cl_alloc(c, &L.v, kNParam))) return st;
+  if ((st = cl_alloc(c, &L.gsum, kNParam))) return st;
+  if ((st = cl_alloc(c, &L.grad, (size_t)L.max_wg * kNParam))) return st;
+  if ((st = cl_alloc(c, &L.lossp, (size_t)L.max_wg))) return st;
+  if ((st = cl_alloc(c, &L.W1t, (size_t)64 * kX))) return st;
+  if ((st = cl_alloc(c, &L.W2t, (size_t)64 * kH))) return st;
+  if ((st = cl_alloc(c, &L.W2, (size_t)64 * kH))) return st;
+  if ((st = cl_alloc(c, &L.w3, 64))) return st;
+  if ((st = cl_alloc(c, &L.b1, 64))) return st;
+  if ((st = cl_alloc(c, &L.b2, 64))) return st;
+  if ((st = cl_alloc(c, &L.b3, 1))) return st;
+  if ((st = cl_alloc(c, &L.stat, 34))) return st;
+  if ((st = cl_alloc(c, &L.mom, (size_t)256 * (kMlpIn + 1) * 4))) return st;
+  // weights U(-1/sqrt(fan_in), 1/sqrt(fan_in)), biases 0 (seeded)
+  std::vector<float> p(kNParam, 0.0f);
+  uint64_t s = seed;
+  auto uni = [&](float a) {
+    s = host_mix64(s);
+    return (float)(((double)(s >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0) * a);
+  };
+  for (int i = 0; i < 8 * 64; ++i) p[kOffW1 + i] = uni(1.0f / std::sqrt(8.0f));
+  for (int i = 0; i < 64 * 64; ++i) p[kOffW2 + i] = uni(1.0f / 8.0f);
+  for (int i = 0; i < 64; ++i) p[kOffW3 + i] = uni(1.0f / 8.0f);
+  HIPCHK(hipMemcpy(L.p, p.data(), sizeof(float) * kNParam, hipMemcpyHostToDevice));
+  std::vector<float> st0(34, 0.0f);
+  for (int k = 0; k < 8; ++k) { st0[8 + k] = 1.0f; st0[16 + k] = -1e30f; st0[24 + k] = 1e30f; }
+  st0[33] = 1.0f;
+  HIPCHK(hipMemcpy(L.stat, st0.data(), sizeof(float) * 34, hipMemcpyHostToDevice));
+  if ((st = cl_pack(c))) return st;
+  HIPCHK(hipStreamSynchronize(c->s));
+  L.ready = true;
+  return IBG_OK;
+}
+
+static int *cl_cells(ibg_ctx *c) { return c->cl.cell; }
+
+ibg_status ibg_closure_sample(ibg_ctx *c, double frac, uint64_t seed, int64_t *total) {
+  if (!c || !c->have_mesh || !c->cl.ready) return IBG_ERR_STATE;
+  if (!c->turb || frac < 0.0 || frac > 1.0) return IBG_ERR_ARG;
+  if (c->ntiles > 0) {
+    if (c->D == 2)
+      hipLaunchKernelGGL((k_closure_sample<2>), dim3(c->ntiles), dim3(c->tile), 0, c->s, c->tiles,
+                         c->blk, c->phys, c->flags, c->w64<2>(), c->nu64, c->signdis, c->nb27, frac,
+                         seed, c->cl.X, c->cl.Z, cl_cells(c), c->cl.count, (long long)c->cl.cap);
+    else
+      hipLaunchKernelGGL((k_closure_sample<3>), dim3(c->ntiles), dim3(c->tile), 0, c->s, c->tiles,
+                         c->blk, c->phys, c->flags, c->w64<3>(), c->nu64, c->signdis, c->nb27, frac,
+                         seed, c->cl.X, c->cl.Z, cl_cells(c), c->cl.count, (long long)c->cl.cap);
+    HIPCHK(hipGetLastError());
+  }
+  unsigned long long n = 0;
+  HIPCHK(hipMemcpyAsync(&n, c->cl.count, sizeof(n), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (total) *total = (int64_t)std::min<unsigned long long>(n, (unsigned long long)c->cl.cap);
+  return IBG_OK;
+}
+
+static long long cl_n(ibg_ctx *c) {
+  unsigned long long n = 0;
+  if (hipMemcpy(&n, c->cl.count, sizeof(n), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  return (long long)std::min<unsigned long long>(n, (unsigned long long)c->cl.cap);
+}
+
+ibg_status ibg_closure_fit_stats(ibg_ctx *c, double margin) {
+  if (!c || !c->cl.ready) return IBG_ERR_STATE;
+  HIPCHK(hipStreamSynchronize(c->s));
+  const long long n = cl_n(c);
+  if (n < 2) return IBG_ERR_STATE;
+  const int nb = 256;
+  hipLaunchKernelGGL(k_closure_moments, dim3(nb), dim3(256), 0, c->s, c->cl.X, c->cl.Z, n, c->cl.mom);
+  HIPCHK(hipGetLastError());
+  std::vector<double> h((size_t)nb * (kMlpIn + 1) * 4);
+  HIPCHK(hipMemcpyAsync(h.data(), c->cl.mom, h.size() * 8, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  std::vector<float> st(34);
+  for (int j = 0; j <= kMlpIn; ++j) {
+    double s = 0, s2 = 0, mn = 1e300, mx = -1e300;
+    for (int b = 0; b < nb; ++b) {
+      const double *o = &h[((size_t)b * (kMlpIn + 1) + j) * 4];
+      s += o[0]; s2 += o[1]; mn = std::min(mn, o[2]); mx = std::max(mx, o[3]);
+    }
+    const double mean = s / n;
+    const double sd = std::sqrt(std::max(s2 / n - mean * mean, 1e-12));
+    if (j < kMlpIn) {
+      st[j] = (float)mean;
+      st[8 + j] = (float)sd;
+      const double pad = margin * (mx - mn);
+      st[16 + j] = (float)(mn - pad);
+      st[24 + j] = (float)(mx + pad);
+    } else {
+      st[32] = (float)mean;
+      st[33] = (float)sd;
+    }
+  }
+  HIPCHK(hipMemcpy(c->cl.stat, st.data(), sizeof(float) * 34, hipMemcpyHostToDevice));
+  c->cl.stats = true;
+  return IBG_OK;
+}
+
+ibg_status ibg_closure_train(ibg_ctx *c, int32_t steps, int32_t batch, double lr, uint64_t seed,
+                             double *loss) {
+  if (!c || !c->cl.ready || !c->cl.stats) return IBG_ERR_STATE;
+  if (steps < 0 || batch < 64 || batch % 64 != 0 || batch / 64 > c->cl.max_wg) return IBG_ERR_ARG;
+  const long long n = cl_n(c);
+  if (n < 1) return IBG_ERR_STATE;
+  const int nwg = batch / 64;
+  ibg_status st;
+  for (int k = 0; k < steps; ++k) {
+    const uint64_t sk = host_mix64(seed) + (uint64_t)(c->cl.step) * (uint64_t)batch;
+    if ((st = cl_train_launch(c, c->cl.X, c->cl.Z, n, nwg, batch, 0, sk))) return st;
+    ++c->cl.step;
+    hipLaunchKernelGGL(k_closure_adam, dim3(grid(kNParam, 256)), dim3(256), 0, c->s, c->cl.p, c->cl.m,
+                       c->cl.v, c->cl.grad, nwg, (float)lr, c->cl.step, (float *)nullptr);
+    HIPCHK(hipGetLastError());
+    if ((st = cl_pack(c))) return st;
+  }
+  if (loss) {
+    std::vector<float> h(nwg);
+    HIPCHK(hipMemcpyAsync(h.data(), c->cl.lossp, sizeof(float) * nwg, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    double s = 0;
+    for (float v : h) s += v;
+    *loss = s / batch;
+  }
+  return IBG_OK;
+}
+
+ibg_status ibg_closure_enable(ibg_ctx *c, int32_t on) {
+  if (!c) return IBG_ERR_ARG;
+  if (on) {
+    if (!c->cl.ready || !c->cl.stats) return IBG_ERR_STATE;
+    if (!c->mixed || !c->turb) return IBG_ERR_ARG;
+  }
+  if (c->cl.on != (on != 0)) {
+    for (int k = 0; k < 2; ++k)
+      if (c->gexec[k]) { (void)hipGraphExecDestroy(c->gexec[k]); c->gexec[k] = nullptr; }
+  }
+  c->cl.on = (on != 0);
+  return IBG_OK;
+}
+
+ibg_status ibg_closure_params(ibg_ctx *c, float *out, int64_t n) {
+  if (!c || !out || !c->cl.ready) return IBG_ERR_STATE;
+  if (n != kNParam + 34) return IBG_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(c->s));
+  HIPCHK(hipMemcpy(out, c->cl.p, sizeof(float) * kNParam, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(out + kNParam, c->cl.stat, sizeof(float) * 34, hipMemcpyDeviceToHost));
+  return IBG_OK;
+}
+
+ibg_status ibg_closure_set_params(ibg_ctx *c, const float *in, int64_t n) {
+  if (!c || !in || !c->cl.ready) return IBG_ERR_STATE;
+  if (n != kNParam + 34) return IBG_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(c->s));
+  HIPCHK(hipMemcpy(c->cl.p, in, sizeof(float) * kNParam, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->cl.stat, in + kNParam, sizeof(float) * 34, hipMemcpyHostToDevice));
+  c->cl.stats = true;
+  return cl_pack(c);
+}
+
+ibg_status ibg_closure_infer(ibg_ctx *c, const float *x, float *z, int64_t n) {
+  if (!c || !x || !z || !c->cl.ready || n < 1) return IBG_ERR_ARG;
+  float *dx = nullptr, *dz = nullptr;
+  HIPCHK(hipMalloc(&dx, sizeof(float) * n * kMlpIn));
+  if (hipMalloc(&dz, sizeof(float) * n) != hipSuccess) { (void)hipFree(dx); return IBG_ERR_HIP; }
+  hipError_t e = hipMemcpy(dx, x, sizeof(float) * n * kMlpIn, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_closure_infer, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, c->s, c->mlpdev(),
+                       dx, dz, (long long)n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->s);
+  if (e == hipSuccess) e = hipMemcpy(z, dz, sizeof(float) * n, hipMemcpyDeviceToHost);
+  (void)hipFree(dx);
+  (void)hipFree(dz);
+  return e == hipSuccess ? IBG_OK : IBG_ERR_HIP;
+}
+
+ibg_status ibg_closure_grad(ibg_ctx *c, const float *x, const float *z, int64_t n, float *grad,
+                            double *loss) {
+  if (!c || !x || !z || !grad || !c->cl.ready || n < 1) return IBG_ERR_ARG;
+  const int nwg = (int)((n + 63) / 64);
+  if (nwg > c->cl.max_wg) return IBG_ERR_ARG;
+  float *dx = nullptr, *dz = nullptr;
+  HIPCHK(hipMalloc(&dx, sizeof(float) * n * kMlpIn));
+  if (hipMalloc(&dz, sizeof(float) * n) != hipSuccess) { (void)hipFree(dx); return IBG_ERR_HIP; }
+  hipError_t e = hipMemcpy(dx, x, sizeof(float) * n * kMlpIn, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dz, z, sizeof(float) * n, hipMemcpyHostToDevice);
+  ibg_status st = IBG_OK;
+  if (e == hipSuccess) st = cl_train_launch(c, dx, dz, n, nwg, (int)n, 1, 0);
+  if (e == hipSuccess && !st) {
+    // the Adam kernel in its summing mode: the gradient, no update
+    hipLaunchKernelGGL(k_closure_adam, dim3(grid(kNParam, 256)), dim3(256), 0, c->s, c->cl.p, c->cl.m,
+                       c->cl.v, c->cl.grad, nwg, 0.0f, 1, c->cl.gsum);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->s);
+    if (e == hipSuccess) e = hipMemcpy(grad, c->cl.gsum, sizeof(float) * kNParam, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && loss) {
+      std::vector<float> h(nwg);
+      e = hipMemcpy(h.data(), c->cl.lossp, sizeof(float) * nwg, hipMemcpyDeviceToHost);
+      double s = 0;
+      for (float q : h) s += q;
+      *loss = s / (double)n;
+    }
+  }
+  (void)hipFree(dx);
+  (void)hipFree(dz);
+  if (st) return st;
+  return e == hipSuccess ? IBG_OK : IBG_ERR_HIP;
+}
+
+ibg_status ibg_closure_dataset(ibg_ctx *c, float *x, float *z, int64_t *cell, int64_t n) {
+  if (!c || !c->cl.ready || n < 0) return IBG_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(c->s));
+  const long long have = cl_n(c);
+  if (n > have) return IBG_ERR_ARG;
+  if (n == 0) return IBG_OK;
+  if (x) HIPCHK(hipMemcpy(x, c->cl.X, sizeof(float) * n * kMlpIn, hipMemcpyDeviceToHost));
+  if (z) HIPCHK(hipMemcpy(z, c->cl.Z, sizeof(float) * n, hipMemcpyDeviceToHost));
+  if (cell) {
+    std::vector<int> h(n);
+    HIPCHK(hipMemcpy(h.data(), cl_cells(c), sizeof(int) * n, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; ++i) cell[i] = (h[i] >= 0 && h[i] < c->N) ? c->d2h[h[i]] : -1;
+  }
+  return IBG_OK;
+}
+
+ibg_status ibg_closure_counts(ibg_ctx *c, int64_t *on_net, int64_t *fallback) {
+  if (!c || !c->cl.ready) return IBG_ERR_STATE;
+  unsigned long long h[2] = {0, 0};
+  HIPCHK(hipStreamSynchronize(c->s));
+  HIPCHK(hipMemcpy(h, c->cl.mlcount, sizeof(h), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemset(c->cl.mlcount, 0, sizeof(h)));
+  if (on_net) *on_net = (int64_t)h[0];
+  if (fallback) *fallback = (int64_t)h[1];
   return IBG_OK;
 }
 

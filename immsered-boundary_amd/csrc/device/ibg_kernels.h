@@ -9,6 +9,7 @@
 
 #include "ibg_common.h"
 #include "ibg_flux.h"
+#include "ibg_mlp.h"
 
 #include <type_traits>
 
@@ -104,6 +105,8 @@ struct StageArgs {
   const R *mutf;             // [Nn] frozen face-flux term of the step
   const double *signdis;
   int turb;
+  MlpDev mlp;                // learned closure (the ML instantiation only)
+  unsigned long long *mlcount;   // [2] cells on the network / on the fallback
   double alpha;
   int limiter;
   int last;                  // last stage: residual partials (+ W64 in mixed)
@@ -115,18 +118,27 @@ struct StageArgs {
 
 constexpr int kRpart = 12;   // per tile: max[5], sumsq[5], SA max, SA sumsq
 
+// LDS of the learned-closure part of k_stage (aliases the face arrays,
+// which are dead by then): weights, one 16-row input block and one hidden
+// block per wave, the outputs of the 256 rows.
+constexpr size_t kStageMlSmem = (size_t)64 * kX * 2 + (size_t)64 * kH * 2 + 3 * 64 * 4 +
+                                4 * ((size_t)16 * kX * 2 + (size_t)16 * kH * 2) + kTile * 4;
+
 // One workgroup per tile of Morton-consecutive normal cells of one
 // (case, level).  Phase 1: every face touching the tile — MUSCL + all-speed
 // Roe (NS_Solver.H:704-832) — into LDS (with the SA closure, also the
 // velocity and working variable of both face cells).  Phase 2: ComptSource
 // (NS_Solver.cpp:282-363), ComptMutSource (:632-840), UpdateConsVars
-// (:844-985) and ComptNewPrimVar (:3-194) for the tile's cells.
+// (:844-985) and ComptNewPrimVar (:3-194) for the tile's cells.  The ML
+// instantiation (mixed mode) evaluates the point-wise SA term with the
+// network for the cells inside its training range, between the two halves
+// of phase 2, on MFMA.
 //
 // Win and Wout are different buffers, so no workgroup reads a state another
 // one has already advanced within the launch.  U_old is cons(W64): the
 // master holds the step-start state until the last stage rewrites it, and
 // each cell's master entry is read and written by the same thread.
-template <typename R, int D>
+template <typename R, int D, bool ML>
 __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
   constexpr int NV = D + 2;
   constexpr bool kMixed = std::is_same<R, float>::value;
@@ -178,9 +190,19 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
   for (int k = 0; k < NV; ++k) { mx[k] = 0.0; sq[k] = 0.0; }
   double nmx = 0.0, nsq = 0.0;
   const int i = threadIdx.x;
-  if (i < ti.ncell) {
-    const int c = ti.cell0 + i;
-    const uint8_t fg = a.flags[c];
+  const bool live = i < ti.ncell;
+  const int c = ti.cell0 + (live ? i : 0);
+  const uint8_t fg = live ? a.flags[c] : 0;
+  // ---- phase 2a: residual of the cell, the SA terms
+  double s[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) s[k] = 0.0;
+  double snu = 0.0;
+  bool sa_on = false;
+  bool use_ml = false;
+  R ml_x[kMlpIn];
+  R ml_scale = R(0), sa_rest = R(0), sa_nu = R(0);
+  if (live) {
     int sl[2 * D];
     if (D == 2) {
       const ushort4 s4 = *reinterpret_cast<const ushort4 *>(a.slots + (size_t)c * 4);
@@ -191,9 +213,6 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
     }
     // ComptSource: per direction (F_lo - F_hi), plus the frozen viscous
     // residual sum_d (Fv_hi - Fv_lo)
-    double s[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) s[k] = 0.0;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const V lo = fl4[sl[2 * d]], hi = fl4[sl[2 * d + 1]];
@@ -211,8 +230,8 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
     }
     // ComptMutSource for a non-solid updated cell (zero otherwise: the
     // reference never writes mutsource of solid cells)
-    double snu = 0.0;
     if (a.turb && (fg & F_UPDATE) && a.signdis[c] >= 0.0) {
+      sa_on = true;
       const Prim<R, D> me = pload<R, D>(a.Win, c);
       R nl[3], nr[3], ul[3], ur[3], vl[3], vr[3], wl[3], wr[3];
 #pragma unroll
@@ -226,10 +245,89 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
         }
       }
       const R T = v_div(me.p, me.r * (R)P.Rg);
-      snu = (double)sa_source<R, D>(me.u, me.v, me.w, T, me.r, a.nuin[c], (R)a.signdis[c], nl, nr,
-                                    ul, ur, vl, vr, wl, wr, (R)bi.invh[0], (R)bi.invh[1],
-                                    (R)bi.invh[2], (R)P.SoT, (R)P.Re, a.mutf[c]);
+      const R nu = a.nuin[c];
+      const R sd = (R)a.signdis[c];
+      if constexpr (!ML) {
+        snu = (double)sa_source<R, D>(me.u, me.v, me.w, T, me.r, nu, sd, nl, nr, ul, ur, vl, vr,
+                                      wl, wr, (R)bi.invh[0], (R)bi.invh[1], (R)bi.invh[2],
+                                      (R)P.SoT, (R)P.Re, a.mutf[c]);
+      } else {
+        const R h = R(0.5);
+        const R i0 = (R)bi.invh[0], i1 = (R)bi.invh[1], i2 = (R)bi.invh[2];
+        VelGrad<R> g;
+        g.dux = ((ur[0] - ul[0]) * h) * i0; g.dvx = ((vr[0] - vl[0]) * h) * i0; g.dwx = ((wr[0] - wl[0]) * h) * i0;
+        g.duy = ((ur[1] - ul[1]) * h) * i1; g.dvy = ((vr[1] - vl[1]) * h) * i1; g.dwy = ((wr[1] - wl[1]) * h) * i1;
+        if (D == 3) {
+          g.duz = ((ur[2] - ul[2]) * h) * i2; g.dvz = ((vr[2] - vl[2]) * h) * i2; g.dwz = ((wr[2] - wl[2]) * h) * i2;
+        } else {
+          g.duz = g.dvz = g.dwz = R(0);
+        }
+        const R dvisx = ((nr[0] - nl[0]) * h) * i0, dvisy = ((nr[1] - nl[1]) * h) * i1;
+        const R dvisz = (D == 3) ? ((nr[2] - nl[2]) * h) * i2 : R(0);
+        const R Re = (R)P.Re;
+        const R olen = sa_olen<R>(g);
+        const R mu0 = v_sqrt(T * T * T) * v_div(R(1.0) + (R)P.SoT, T + (R)P.SoT);
+        // derivative terms in their discrete form
+        sa_rest = -(me.u * dvisx + me.v * dvisy + me.w * dvisz) +
+                  v_div(v_div(R(SA::cb2) * (dvisx * dvisx + dvisy * dvisy + dvisz * dvisz), R(SA::segma)), Re) +
+                  a.mutf[c];
+        sa_nu = nu;
+        bool ok = sa_features<R>(me.u, me.v, me.w, me.r, me.p, mu0, nu, sd, g, olen, Re,
+                                 (R)P.init[4], ml_x, ml_scale);
+#pragma unroll
+        for (int k = 0; k < kMlpIn; ++k) ok = ok && ml_x[k] >= a.mlp.flo[k] && ml_x[k] <= a.mlp.fhi[k];
+        use_ml = ok;
+        if (!ok) snu = (double)(sa_rest + sa_pointwise<R>(nu, mu0, me.r, sd, olen, Re));
+      }
     }
+  }
+  // ---- learned closure: the network for the cells in its range
+  if constexpr (ML) {
+    __syncthreads();   // every phase-2a read of the face arrays is done
+    __bf16 *mW1t = reinterpret_cast<__bf16 *>(smem_raw);
+    __bf16 *mW2t = mW1t + 64 * kX;
+    float *mw3 = reinterpret_cast<float *>(mW2t + 64 * kH);
+    float *mb1 = mw3 + 64, *mb2 = mb1 + 64;
+    __bf16 *mX = reinterpret_cast<__bf16 *>(mb2 + 64);
+    __bf16 *mH = mX + 4 * 16 * kX;
+    float *mY = reinterpret_cast<float *>(mH + 4 * 16 * kH);
+    MlpLds Wl{mW1t, mW2t, mw3, mb1, mb2};
+    mlp_load_lds(a.mlp, Wl);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __bf16 *xs = mX + w * 16 * kX;
+    for (int sb = 0; sb < 4; ++sb) {
+      // lanes 16sb..16sb+15 of the wave own the 16 rows of this block
+      if ((lane >> 4) == sb) {
+        const int row = lane & 15;
+#pragma unroll
+        for (int k = 0; k < kX; ++k) {
+          float v = 0.0f;
+          if (k < kMlpIn && use_ml) v = ((float)ml_x[k] - a.mlp.fmean[k]) / a.mlp.fstd[k];
+          xs[row * kX + k] = (__bf16)v;
+        }
+      }
+      __syncthreads();
+      float yv[4];
+      mlp_fwd16(xs, mH + w * 16 * kH, nullptr, nullptr, nullptr, Wl, yv);
+      if ((lane & 15) == 0)
+        for (int v = 0; v < 4; ++v) mY[64 * w + 16 * sb + 4 * (lane >> 4) + v] = yv[v];
+    }
+    __syncthreads();
+    if (use_ml) {
+      const float z = a.mlp.zst[0] + a.mlp.zst[1] * (mY[threadIdx.x] + a.mlp.b3[0]);
+      snu = (double)(sa_rest + (R)z * ml_scale);
+    }
+    if (a.last && a.mlcount) {
+      // cells on the network / on the fallback in this tile
+      unsigned long long nm = __ballot(use_ml) , nf = __ballot(sa_on && !use_ml);
+      if (lane == 0) {
+        atomicAdd(&a.mlcount[0], (unsigned long long)__popcll(nm));
+        atomicAdd(&a.mlcount[1], (unsigned long long)__popcll(nf));
+      }
+    }
+  }
+  // ---- phase 2b: update
+  if (live) {
     bool write = false;
     double wr = 0, wu = 0, wv = 0, ww = 0, wp = 0;
     double nu_new = 0.0;

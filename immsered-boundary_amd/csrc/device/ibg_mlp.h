@@ -1,0 +1,224 @@
+// ibg_mlp.h — learned SA closure: features, BF16 MFMA network (gfx950).
+//
+// The network stands in for the point-wise part of the SA source that
+// ComptMutSource evaluates per cell (NS_Solver.cpp:700-767): the production
+// minus destruction Q = cb1 (1-ft2) S~ nu - (cw1 fw - cb1 ft2/kappa^2)
+// (nu/d)^2 / Re.  It predicts the bounded ratio z = Q / (nu S^) with the
+// production scale S^ = |Omega| + nu/(kappa^2 d^2 Re), from 8 per-cell
+// features (raw, then standardised with the training statistics):
+//   0 ln(|Omega| + eps)      vorticity magnitude (the reference's sqrt2|curl/2|)
+//   1 ln(|S| + eps)          strain-rate magnitude sqrt(2 S_ij S_ij)
+//   2 ln(d)                  wall distance
+//   3 ln(chi + eps)          chi = nu / (mu/roe), eddy-viscosity ratio
+//   4 ln(|u| d Re/nu_l + eps) local Reynolds number
+//   5 roe
+//   6 p / p_inf
+//   7 ln(mu/roe)             laminar kinematic viscosity
+// The derivative terms of the source (advection, the cb2 gradient term and
+// the frozen face flux) stay in their discrete form.  A cell whose raw
+// features leave the training range [lo, hi] falls back to the plain
+// closure.
+//
+// Network: 8 -> 64 -> 64 -> 1, SiLU, BF16 weights and activations, FP32
+// accumulation:  X = bf16(x),  H1 = bf16(silu(X W1 + b1)),
+// H2 = bf16(silu(H1 W2 + b2)),  y = H2 w3 + b3.  The two inner layers run
+// on v_mfma_f32_16x16x32_bf16 (16x16 output tiles, K = 32 per instruction):
+//   A fragment  lane l: row l%16, k = 8*(l/16) .. +7
+//   B fragment  lane l: col l%16, k = 8*(l/16) .. +7
+//   D           lane l: col l%16, rows 4*(l/16) .. +3
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ibg {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kMlpIn = 8;
+constexpr int kMlpHid = 64;
+constexpr int kX = 40;     // LDS row stride (bf16) of the padded 32-wide input rows
+constexpr int kH = 72;     // LDS row stride (bf16) of the 64-wide hidden rows
+// fp32 parameter vector: W1[8][64] b1[64] W2[64][64] b2[64] w3[64] b3
+constexpr int kOffW1 = 0, kOffB1 = 512, kOffW2 = 576, kOffB2 = 4672, kOffW3 = 4736, kOffB3 = 4800;
+constexpr int kNParam = 4801;
+
+// Device copy of the network in the layouts the kernels read.
+struct MlpDev {
+  const __bf16 *W1t;   // [64][kX]  W1 transposed, inputs padded to 32
+  const __bf16 *W2t;   // [64][kH]  W2 transposed (forward B operand)
+  const __bf16 *W2;    // [64][kH]  W2 (backward B operand)
+  const float *w3;     // [64] bf16-rounded values
+  const float *b1, *b2, *b3;
+  const float *fmean, *fstd;   // [8] feature standardisation
+  const float *flo, *fhi;      // [8] raw feature range of the training data
+  const float *zst;            // [2] target mean, std
+};
+
+__device__ __forceinline__ float mlp_silu(float z) { return z / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float mlp_dsilu(float z) {
+  const float s = 1.0f / (1.0f + __expf(-z));
+  return s + z * s * (1.0f - s);
+}
+__device__ __forceinline__ float bf16r(float x) { return (float)(__bf16)x; }
+
+__device__ __forceinline__ floatx4_t mfma16(bf16x8_t a, bf16x8_t b, floatx4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Weights into LDS (cooperative over the workgroup).
+struct MlpLds {
+  __bf16 *W1t;   // [64][kX]
+  __bf16 *W2t;   // [64][kH]
+  float *w3, *b1, *b2;   // [64] each
+};
+
+__device__ __forceinline__ void mlp_load_lds(const MlpDev &M, const MlpLds &S) {
+  for (int i = threadIdx.x; i < 64 * kX; i += blockDim.x) S.W1t[i] = M.W1t[i];
+  for (int i = threadIdx.x; i < 64 * kH; i += blockDim.x) S.W2t[i] = M.W2t[i];
+  for (int i = threadIdx.x; i < 64; i += blockDim.x) {
+    S.w3[i] = M.w3[i];
+    S.b1[i] = M.b1[i];
+    S.b2[i] = M.b2[i];
+  }
+}
+
+// Forward of 16 rows by one wave.  Xs: [16][kX] bf16 rows (inputs in columns
+// 0..7, zeros in 8..31).  Hs: [16][kH] bf16 scratch of the wave.  The
+// pre-activations are optionally kept (Z1s/Z2s [16][kH] as silu' in bf16)
+// and the activations H1/H2 (H1s, H2s) for the backward pass.  Returns
+// y for rows 4*(l/16)+v in yv[v] of the lanes with l%16 == 0 (summed over
+// the 16 lanes of each group), b3 not included.  All 64 lanes must call it
+// together; the barriers are workgroup barriers, so every wave of the
+// workgroup calls it the same number of times.
+__device__ __forceinline__ void mlp_fwd16(const __bf16 *Xs, __bf16 *H1s, __bf16 *H2s,
+                                          __bf16 *S1s, __bf16 *S2s, const MlpLds &W,
+                                          float *yv) {
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15, q = lane >> 4;
+  // layer 1: [16 x 32] x [32 x 64]
+  floatx4_t acc[4];
+  const bf16x8_t a1 = *reinterpret_cast<const bf16x8_t *>(Xs + r * kX + 8 * q);
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    const float bb = W.b1[16 * nt + r];
+    acc[nt] = floatx4_t{bb, bb, bb, bb};
+    const bf16x8_t b = *reinterpret_cast<const bf16x8_t *>(W.W1t + (16 * nt + r) * kX + 8 * q);
+    acc[nt] = mfma16(a1, b, acc[nt]);
+  }
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = 4 * q + v, col = 16 * nt + r;
+      H1s[row * kH + col] = (__bf16)mlp_silu(acc[nt][v]);
+      if (S1s) S1s[row * kH + col] = (__bf16)mlp_dsilu(acc[nt][v]);
+    }
+  __syncthreads();
+  // layer 2: [16 x 64] x [64 x 64]
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    const float bb = W.b2[16 * nt + r];
+    acc[nt] = floatx4_t{bb, bb, bb, bb};
+  }
+#pragma unroll
+  for (int kt = 0; kt < 2; ++kt) {
+    const bf16x8_t a = *reinterpret_cast<const bf16x8_t *>(H1s + r * kH + 32 * kt + 8 * q);
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+      const bf16x8_t b = *reinterpret_cast<const bf16x8_t *>(W.W2t + (16 * nt + r) * kH + 32 * kt + 8 * q);
+      acc[nt] = mfma16(a, b, acc[nt]);
+    }
+  }
+  // layer 3: y = bf16(silu(Z2)) . w3, reduced over the 16 lanes of a group
+#pragma unroll
+  for (int v = 0; v < 4; ++v) yv[v] = 0.0f;
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = 4 * q + v, col = 16 * nt + r;
+      const __bf16 h = (__bf16)mlp_silu(acc[nt][v]);
+      if (H2s) H2s[row * kH + col] = h;
+      if (S2s) S2s[row * kH + col] = (__bf16)mlp_dsilu(acc[nt][v]);
+      yv[v] += (float)h * W.w3[col];
+    }
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    for (int off = 1; off < 16; off <<= 1) yv[v] += __shfl_xor(yv[v], off, 64);
+  __syncthreads();
+}
+
+// Velocity gradient of one cell from central differences (physical units).
+template <typename R>
+struct VelGrad {
+  R dux, duy, duz, dvx, dvy, dvz, dwx, dwy, dwz;
+};
+
+// |Omega| as the reference forms it: sqrt(2) |curl(u)/2| (NS_Solver.cpp:688-731)
+template <typename R>
+__device__ __forceinline__ R sa_olen(const VelGrad<R> &g) {
+  const R ox = R(0.5) * (g.dwy - g.dvz), oy = R(0.5) * (g.duz - g.dwx), oz = R(0.5) * (g.dvx - g.duy);
+  return v_sqrt(R(2.0)) * v_sqrt(ox * ox + oy * oy + oz * oz);
+}
+
+// Point-wise production - destruction of the SA source for one cell, the
+// terms of NS_Solver.cpp:700-767 (production cb1(1-ft2) S~ nu, destruction
+// (cw1 fw - cb1 ft2/kappa^2)(nu/d)^2/Re), from the working variable, the
+// laminar viscosity mu, roe, the wall distance and |Omega|.
+template <typename R>
+__device__ __forceinline__ R sa_pointwise(R nu, R mu0, R roe, R sd, R olen, R Re) {
+  const R vis0 = v_div(mu0, roe);
+  const R bigx = v_div(nu, vis0);
+  const R b3 = bigx * bigx * bigx;
+  const R fv1 = v_div(b3, b3 + R(SA::cv1three));
+  const R fv2 = R(1.0) - v_div(bigx, R(1.0) + bigx * fv1);
+  R segdis, vis2d;
+  if (r_abs(sd) < R(0.00000001)) {
+    segdis = R(1.0);
+    vis2d = R(1.0);
+  } else {
+    segdis = R(SA::kappa) * sd;
+    vis2d = v_div(nu, sd);
+  }
+  segdis *= segdis;
+  R bigs = olen + v_div(v_div(nu * fv2, segdis), Re);
+  bigs = (R(0.3) * olen < bigs) ? bigs : R(0.3) * olen;
+  const R ft2 = R(SA::ct3) * r_exp(-R(SA::ct4) * bigx * bigx);
+  R rr = v_div(v_div(nu, bigs * segdis), Re);
+  rr = (R(10.0) < rr) ? R(10.0) : rr;
+  const R rsix = r_pow(rr, R(6));
+  const R g = rr + R(SA::cw2) * (rsix - rr);
+  const R gsix = r_pow(g, R(6));
+  const R fw = g * r_pow(v_div(R(65.0), gsix + R(64.0)), R(1.0 / 6.0));
+  const R prod = R(SA::cb1) * (R(1.0) - ft2) * bigs * nu;
+  const R dest = v_div((R(SA::cw1) * fw - v_div(R(SA::cb1) * ft2, R(SA::kappa * SA::kappa))) * (vis2d * vis2d), Re);
+  return prod - dest;
+}
+
+// Raw features (see the header) and the production scale nu S^.
+// Returns false when the cell is outside the network's domain by
+// construction (no wall distance or no working variable).
+template <typename R>
+__device__ __forceinline__ bool sa_features(R u, R v, R w, R roe, R p, R mu0, R nu, R d,
+                                            const VelGrad<R> &g, R olen, R Re, R pinf, R *x,
+                                            R &scale) {
+  const R eps = R(1e-8);
+  const R smag = v_sqrt(R(2.0) * (g.dux * g.dux + g.dvy * g.dvy + g.dwz * g.dwz) +
+                        (g.duy + g.dvx) * (g.duy + g.dvx) + (g.duz + g.dwx) * (g.duz + g.dwx) +
+                        (g.dvz + g.dwy) * (g.dvz + g.dwy));
+  const R nul = v_div(mu0, roe);
+  const R speed = v_sqrt(u * u + v * v + w * w);
+  x[0] = log(olen + eps);
+  x[1] = log(smag + eps);
+  x[2] = log(d > eps ? d : eps);
+  x[3] = log(v_div(nu, nul) + eps);
+  x[4] = log(v_div(speed * d * Re, nul) + eps);
+  x[5] = roe;
+  x[6] = v_div(p, pinf);
+  x[7] = log(nul);
+  scale = nu * (olen + v_div(nu, R(SA::kappa * SA::kappa) * d * d * Re));
+  return d > eps && nu > eps && scale > R(0);
+}
+
+}  // namespace ibg

@@ -19,6 +19,10 @@ IBM_MAX_BODIES = 8
 i32p = C.POINTER(C.c_int32)
 i8p = C.POINTER(C.c_int8)
 f64p = C.POINTER(C.c_double)
+f32p = C.POINTER(C.c_float)
+i64p = C.POINTER(C.c_int64)
+IBG_CLOSURE_NPARAM = 4801
+IBG_CLOSURE_NFEAT = 8
 
 
 class MeshSoA(C.Structure):
@@ -164,6 +168,17 @@ GPU_FUNCS = {
                                   C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "ibg_bench_copy": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, f64p]),
     "ibg_cell_stage_updates": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "ibg_closure_init": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint64]),
+    "ibg_closure_sample": (C.c_int, [C.c_void_p, C.c_double, C.c_uint64, C.POINTER(C.c_int64)]),
+    "ibg_closure_fit_stats": (C.c_int, [C.c_void_p, C.c_double]),
+    "ibg_closure_train": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_uint64, f64p]),
+    "ibg_closure_enable": (C.c_int, [C.c_void_p, C.c_int32]),
+    "ibg_closure_params": (C.c_int, [C.c_void_p, f32p, C.c_int64]),
+    "ibg_closure_set_params": (C.c_int, [C.c_void_p, f32p, C.c_int64]),
+    "ibg_closure_infer": (C.c_int, [C.c_void_p, f32p, f32p, C.c_int64]),
+    "ibg_closure_grad": (C.c_int, [C.c_void_p, f32p, f32p, C.c_int64, f32p, f64p]),
+    "ibg_closure_dataset": (C.c_int, [C.c_void_p, f32p, f32p, i64p, C.c_int64]),
+    "ibg_closure_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
 }
 
 

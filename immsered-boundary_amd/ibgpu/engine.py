@@ -170,6 +170,73 @@ class Engine:
                                               out.shape[0]), "ibg_get_nut_source")
         return out
 
+    # learned closure ----------------------------------------------------------
+    def closure_init(self, capacity, seed=20261202):
+        self._chk(self.lib.ibg_closure_init(self.ctx, int(capacity), seed), "ibg_closure_init")
+
+    def closure_sample(self, frac, seed):
+        n = C.c_int64()
+        self._chk(self.lib.ibg_closure_sample(self.ctx, float(frac), seed, C.byref(n)),
+                  "ibg_closure_sample")
+        return n.value
+
+    def closure_fit_stats(self, margin=0.0):
+        self._chk(self.lib.ibg_closure_fit_stats(self.ctx, float(margin)), "ibg_closure_fit_stats")
+
+    def closure_train(self, steps, batch=8192, lr=1e-3, seed=1):
+        loss = C.c_double()
+        self._chk(self.lib.ibg_closure_train(self.ctx, steps, batch, float(lr), seed, C.byref(loss)),
+                  "ibg_closure_train")
+        return loss.value
+
+    def closure_enable(self, on=True):
+        self._chk(self.lib.ibg_closure_enable(self.ctx, 1 if on else 0), "ibg_closure_enable")
+
+    def closure_params(self):
+        out = np.zeros(_ffi.IBG_CLOSURE_NPARAM + 34, dtype=np.float32)
+        self._chk(self.lib.ibg_closure_params(self.ctx, out.ctypes.data_as(_ffi.f32p), out.size),
+                  "ibg_closure_params")
+        return out
+
+    def closure_set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._chk(self.lib.ibg_closure_set_params(self.ctx, p.ctypes.data_as(_ffi.f32p), p.size),
+                  "ibg_closure_set_params")
+
+    def closure_infer(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        z = np.zeros(x.shape[0], dtype=np.float32)
+        self._chk(self.lib.ibg_closure_infer(self.ctx, x.ctypes.data_as(_ffi.f32p),
+                                             z.ctypes.data_as(_ffi.f32p), x.shape[0]),
+                  "ibg_closure_infer")
+        return z
+
+    def closure_grad(self, x, z):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        z = np.ascontiguousarray(z, dtype=np.float32)
+        g = np.zeros(_ffi.IBG_CLOSURE_NPARAM, dtype=np.float32)
+        loss = C.c_double()
+        self._chk(self.lib.ibg_closure_grad(self.ctx, x.ctypes.data_as(_ffi.f32p),
+                                            z.ctypes.data_as(_ffi.f32p), x.shape[0],
+                                            g.ctypes.data_as(_ffi.f32p), C.byref(loss)),
+                  "ibg_closure_grad")
+        return g, loss.value
+
+    def closure_dataset(self, n):
+        x = np.zeros((n, 8), dtype=np.float32)
+        z = np.zeros(n, dtype=np.float32)
+        cell = np.zeros(n, dtype=np.int64)
+        self._chk(self.lib.ibg_closure_dataset(self.ctx, x.ctypes.data_as(_ffi.f32p),
+                                               z.ctypes.data_as(_ffi.f32p),
+                                               cell.ctypes.data_as(_ffi.i64p), n),
+                  "ibg_closure_dataset")
+        return x, z, cell
+
+    def closure_counts(self):
+        a, b = C.c_int64(), C.c_int64()
+        self._chk(self.lib.ibg_closure_counts(self.ctx, C.byref(a), C.byref(b)), "ibg_closure_counts")
+        return a.value, b.value
+
     def timing(self):
         tot, st = C.c_double(), C.c_double()
         n = C.c_int64()

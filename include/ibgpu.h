@@ -286,6 +286,45 @@ ibg_status ibg_bench_copy(ibg_ctx *ctx, int64_t nbytes, int32_t reps, double *av
  * update count of NS_Solver.cpp:868-872, 948-953).                        */
 ibg_status ibg_cell_stage_updates(ibg_ctx *ctx, int64_t *count);
 
+/* ------------------------------------------------------- learned closure
+ * A small network stands in for the point-wise part of the SA source that
+ * ComptMutSource evaluates per cell (NS_Solver.cpp:700-767); the reference
+ * has no learned component, so these calls replace nothing there and are
+ * the training / inference surface of the closure (the reference's
+ * per-cell evaluation stays the fallback and the judge of its error).
+ * Network 8 -> 64 -> 64 -> 1 (SiLU), BF16 weights and activations, FP32
+ * accumulation, trained on cells sampled from this context's own state.
+ * Parameter vector (floats): W1[8][64] b1[64] W2[64][64] b2[64] w3[64] b3,
+ * then the statistics fmean[8] fstd[8] flo[8] fhi[8] zmean zstd
+ * (IBG_CLOSURE_NPARAM + 34 values).                                        */
+#define IBG_CLOSURE_NPARAM 4801
+#define IBG_CLOSURE_NFEAT 8
+/* allocate the sample store (capacity rows) and initialise the weights */
+ibg_status ibg_closure_init(ibg_ctx *ctx, int64_t capacity, uint64_t seed);
+/* append the features and plain-closure targets of the current state's
+ * fluid cells, each kept with probability frac (seeded); total = rows held */
+ibg_status ibg_closure_sample(ibg_ctx *ctx, double frac, uint64_t seed, int64_t *total);
+/* standardisation and the input range (min/max widened by margin * span) */
+ibg_status ibg_closure_fit_stats(ibg_ctx *ctx, double margin);
+/* Adam steps on random batches of the sample store; loss of the last batch */
+ibg_status ibg_closure_train(ibg_ctx *ctx, int32_t steps, int32_t batch, double lr,
+                             uint64_t seed, double *loss);
+/* use the network in the stage (mixed mode with the SA closure only)       */
+ibg_status ibg_closure_enable(ibg_ctx *ctx, int32_t on);
+ibg_status ibg_closure_params(ibg_ctx *ctx, float *out, int64_t n);
+ibg_status ibg_closure_set_params(ibg_ctx *ctx, const float *in, int64_t n);
+/* network output z (destandardised) for n raw feature rows x[n][8]          */
+ibg_status ibg_closure_infer(ibg_ctx *ctx, const float *x, float *z, int64_t n);
+/* gradient of the batch loss (1/n) sum (y - (z - zmean)/zstd)^2 w.r.t. the
+ * parameter vector for n explicit rows; loss = that mean                    */
+ibg_status ibg_closure_grad(ibg_ctx *ctx, const float *x, const float *z, int64_t n,
+                            float *grad /* [IBG_CLOSURE_NPARAM] */, double *loss);
+/* first n rows of the sample store: features, targets, global cell index   */
+ibg_status ibg_closure_dataset(ibg_ctx *ctx, float *x, float *z, int64_t *cell, int64_t n);
+/* cells that used the network / the plain closure in the last stages since
+ * the previous call (then reset)                                           */
+ibg_status ibg_closure_counts(ibg_ctx *ctx, int64_t *on_net, int64_t *fallback);
+
 #ifdef __cplusplus
 }
 #endif

@@ -114,6 +114,19 @@ int32_t orc_step_sa(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg 
                     double *mutsource_out, double *linf, double *l2, double *nlinf,
                     double *nl2, int64_t *bad_cell);
 
+/* ------------------------------------------- learned closure (mlp.c)
+ * prm = the 4801 network parameters then fmean[8] fstd[8] flo[8] fhi[8]
+ * zmean zstd, as the engine's parameter vector.                           */
+float orc_bf16(float x);
+void orc_mlp_forward(const float *prm, const float *x, int64_t n, float *z);
+void orc_mlp_grad(const float *prm, const float *x, const float *z, int64_t n, double *grad,
+                  double *loss);
+double orc_mlp_loss_f64(const double *p, const double *xs, const double *t, int64_t n);
+void orc_mlp_grad_f64(const double *p, const double *xs, const double *t, int64_t n,
+                      double *grad, double *loss);
+int32_t orc_closure_features(const ibg_mesh_soa *m, const ibg_physics *ph, const double *fv,
+                             const double *nut, int32_t cell, double *x, double *z);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,7 +16,7 @@ if PKG not in sys.path:
 
 from ibgpu import _ffi  # noqa: E402
 
-f64p, i32p = _ffi.f64p, _ffi.i32p
+f64p, i32p, f32p = _ffi.f64p, _ffi.i32p, _ffi.f32p
 
 
 class OrcCfg(C.Structure):
@@ -75,6 +75,12 @@ def lib():
         "orc_residual_nut": (None, [M, G, f64p, C.c_int32, f64p, f64p]),
         "orc_step_sa": (C.c_int32, [M, P, G, f64p, f64p, f64p, C.c_int32, f64p, f64p, f64p,
                                     f64p, f64p, f64p, C.POINTER(C.c_int64)]),
+        "orc_bf16": (C.c_float, [C.c_float]),
+        "orc_mlp_forward": (None, [f32p, f32p, C.c_int64, f32p]),
+        "orc_mlp_grad": (None, [f32p, f32p, f32p, C.c_int64, f64p, f64p]),
+        "orc_mlp_loss_f64": (C.c_double, [f64p, f64p, f64p, C.c_int64]),
+        "orc_mlp_grad_f64": (None, [f64p, f64p, f64p, C.c_int64, f64p, f64p]),
+        "orc_closure_features": (C.c_int32, [M, P, f64p, f64p, C.c_int32, f64p, f64p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -86,6 +92,56 @@ def lib():
 
 def ptr(a):
     return a.ctypes.data_as(f64p)
+
+
+def fptr(a):
+    return a.ctypes.data_as(f32p)
+
+
+NPARAM = 4801
+
+
+def mlp_forward(prm, x):
+    prm = np.ascontiguousarray(prm, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    z = np.zeros(x.shape[0], dtype=np.float32)
+    lib().orc_mlp_forward(fptr(prm), fptr(x), x.shape[0], fptr(z))
+    return z
+
+
+def mlp_grad(prm, x, z):
+    prm = np.ascontiguousarray(prm, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    z = np.ascontiguousarray(z, dtype=np.float32)
+    g = np.zeros(NPARAM)
+    loss = C.c_double()
+    lib().orc_mlp_grad(fptr(prm), fptr(x), fptr(z), x.shape[0], ptr(g), C.byref(loss))
+    return g, loss.value
+
+
+def mlp_grad_f64(p, xs, t):
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    xs = np.ascontiguousarray(xs, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    g = np.zeros(NPARAM)
+    loss = C.c_double()
+    lib().orc_mlp_grad_f64(ptr(p), ptr(xs), ptr(t), xs.shape[0], ptr(g), C.byref(loss))
+    return g, loss.value
+
+
+def mlp_loss_f64(p, xs, t):
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    xs = np.ascontiguousarray(xs, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    return lib().orc_mlp_loss_f64(ptr(p), ptr(xs), ptr(t), xs.shape[0])
+
+
+def closure_features(mesh, fv, nut, cell):
+    x = np.zeros(8)
+    z = C.c_double()
+    ok = lib().orc_closure_features(mesh.soa_ptr, mesh.phys, ptr(fv), ptr(nut), int(cell), ptr(x),
+                                    C.byref(z))
+    return bool(ok), x, z.value
 
 
 def cfg(nstage=3, alpha=None, dt_var_step=1, temporal_refine=1, limiter=0, viscous=1):
