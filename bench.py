@@ -1,18 +1,26 @@
 """Throughput of the per-stage residual path on one MI355X.
 
-Workload: the mid-size 2D case — NACA0015 at Mach 0.3, Re 10000, in
-[-10,10]^2 with a 256x256 base grid and 5 AMR levels (6 levels, ~2.6M leaf
-cells), laminar, 4-stage low-storage scheme (NS_Solver.H:204-208) with
-temporal sub-cycling, body held at its initial pose.  A "step" is one
-InterfaceExchange + IBConditions_Image + TimeMarching (main.cpp:87-97).
+Workload: the mid-size 2D case with the SA closure — NACA0015 at Mach 0.3,
+Re 10000, in [-10,10]^2 with a 256x256 base grid and 5 AMR levels (6
+levels, ~2.6M leaf cells), 4-stage low-storage scheme (NS_Solver.H:204-208)
+with temporal sub-cycling, body held at its initial pose, Spalart-Allmaras
+working variable from a freestream of 5 (the reference's TURBULENCE build).
+A "step" is one InterfaceExchange + IBConditions_Image + TimeMarching
+(main.cpp:87-97).
 
+Three forms run in the same process on the same mesh:
+  plain     every operation FP64, plain SA closure (the reference's form);
+  mixed     FP32 limiter/Roe/viscous/SA arithmetic, FP64 state and
+            accumulation, plain closure;
+  mixed+ML  as mixed, with the point-wise SA term from the learned closure
+            (BF16 MFMA), trained in this run on cells sampled from the plain
+            run (the plain closure is the fallback outside its range).
 The metric is cell-stage updates/s (one UpdateConsVars cell body per stage,
-NS_Solver.cpp:868-872 / 948-953, only marched levels).  `value` is the mixed
-precision form (FP32 limiter/Roe/viscous arithmetic, FP64 state and
-accumulation); the plain FP64 form is run in the same process and reported
-beside it with the C_L/C_D difference.  The CPU baseline is the FP64 oracle
-(this project's C restatement of the reference's path, OpenMP) on the host
-cores, timed on a bounded sample of the same workload.
+NS_Solver.cpp:868-872 / 948-953, only marched levels); `value` is mixed+ML.
+Accuracy: C_L/C_D of each form against the plain form after the same
+steps.  The CPU baseline is the FP64 oracle (this project's C restatement
+of the reference's path, OpenMP) on the host cores, timed on a bounded
+sample of the same workload.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -32,17 +40,21 @@ import ibgpu  # noqa: E402
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 
 
-def stage_bytes(mode, dim, cells, faces):
-    """Algorithmic HBM bytes of one stage-kernel launch.
+def stage_bytes(mode, dim, turb, cells, faces):
+    """Algorithmic HBM bytes of one (non-last) stage-kernel launch.
 
-    Per normal cell: the cell's W read once (its neighbours' reads hit in
-    cache), U_old read (FP64), frozen viscous residual read, W written, the
-    face slots (2 B per cell face) and the flag byte; per tile face the 4
-    stencil indices and the direction byte.
+    Per normal cell: its stencil state read once (neighbour reads hit in
+    cache) and written once, the frozen viscous residual, the FP64 master
+    read for U_old, the face slots (2 B per cell face) and the flag byte;
+    with the SA closure the working variable read and written, its FP64
+    master (the old value), the frozen face term and the wall distance.
+    Per tile face the 4 stencil indices and the direction byte.
     """
     nv = dim + 2
     sw = 8 if mode == "fp64" else 4
-    per_cell = nv * sw + nv * 8 + nv * sw + nv * sw + 2 * dim * 2 + 1
+    per_cell = 3 * nv * sw + nv * 8 + 2 * dim * 2 + 1
+    if turb:
+        per_cell += 3 * sw + 8 + 8
     return cells * per_cell + faces * 17
 
 
@@ -65,9 +77,14 @@ def updates_per_step(mesh, nstage, ts):
     return n * nstage
 
 
-def run_mode(mesh, mode, nstage, warmup, steps, reps):
-    eng = ibgpu.Engine(mesh, mode=mode, nstage=nstage)
+def new_engine(mesh, mode, nstage):
+    eng = ibgpu.Engine(mesh, mode=mode, nstage=nstage, turbulence=True)
     eng.set_state(mesh.init_state())
+    eng.set_nut(np.full(mesh.ncells, mesh.phys[0].nut_inf))
+    return eng
+
+
+def timed_run(eng, mesh, warmup, steps, reps):
     eng.TimeMarching(0, warmup, use_graph=True)
     eng.synchronize()
     t0 = time.perf_counter()
@@ -78,14 +95,38 @@ def run_mode(mesh, mode, nstage, warmup, steps, reps):
     P = mesh.phys[0]
     F = eng.ComptWallForce()
     cd, cl = F[0] / P.force_ref, F[1] / P.force_ref
-    state = eng.get_state()
+    state, nut = eng.get_state(), eng.get_nut()
     # stage kernel alone, all levels marched (ts multiple of the period)
     period = 1 << (mesh.nlevels - 1)
     ts_full = ((warmup + steps) // period) * period
     ms, nc, nf = eng.bench_stage(ts_full, 0, reps)
-    eng.close()
     return {"seconds": t1 - t0, "updates": upd, "rate": upd / (t1 - t0), "cd": cd, "cl": cl,
-            "stage_ms": ms, "stage_cells": nc, "stage_faces": nf, "state": state}
+            "stage_ms": ms, "stage_cells": nc, "stage_faces": nf, "state": state, "nut": nut}
+
+
+def train_closure(mesh, nstage, args):
+    """Network trained on cells sampled from the plain FP64 run."""
+    t0 = time.perf_counter()
+    src = new_engine(mesh, "fp64", nstage)
+    src.closure_init(args.samples, 20261202)
+    period = 1 << (mesh.nlevels - 1)
+    total, ts, k = 0, 0, 0
+    frac = min(1.0, args.samples / (args.sample_rounds * 2.5e6))
+    while total < args.samples and k < 4 * args.sample_rounds:
+        src.TimeMarching(ts, period, use_graph=True)
+        ts += period
+        total = src.closure_sample(frac, 1000 + k)
+        k += 1
+    src.closure_fit_stats(0.0)
+    t1 = time.perf_counter()
+    steps = max(1, int(args.epochs * total / args.batch))
+    loss = src.closure_train(steps, batch=args.batch, lr=1e-3, seed=7)
+    src.synchronize()
+    t2 = time.perf_counter()
+    prm = src.closure_params()
+    src.close()
+    return prm, {"samples": int(total), "sampled_steps": ts, "train_steps": steps, "batch": args.batch,
+                 "epochs": args.epochs, "final_loss": loss, "sample_s": t1 - t0, "train_s": t2 - t1}
 
 
 def cpu_baseline(mesh, nstage, budget_s):
@@ -93,10 +134,11 @@ def cpu_baseline(mesh, nstage, budget_s):
     import oracle_ffi
     orc = oracle_ffi.Oracle(mesh, oracle_ffi.cfg(nstage=nstage, dt_var_step=1 << (mesh.nlevels - 1)))
     fv = mesh.init_state()
+    nut = np.full(mesh.ncells, mesh.phys[0].nut_inf)
     dt = np.array([mesh.phys[c].dt_init for c in range(mesh.ncases)])
     upd, ts, t0 = 0, 0, time.perf_counter()
     while True:
-        code, _, _, _ = orc.step(fv, dt, ts)
+        code, _, _, _ = orc.step_sa(fv, nut, dt, ts)
         if code:
             raise RuntimeError(f"oracle abort {code}")
         upd += updates_per_step(mesh, nstage, ts)
@@ -106,8 +148,9 @@ def cpu_baseline(mesh, nstage, budget_s):
     el = time.perf_counter() - t0
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": upd / el, "unit": "cell-stage updates/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle (C restatement of the reference path, OpenMP), same mesh, "
-                      f"steps 0..{ts - 1} from freestream ({upd} cell-stage updates in {el:.1f} s)"}
+            "sample": f"FP64 oracle with the SA closure (C restatement of the reference path, "
+                      f"OpenMP), same mesh, steps 0..{ts - 1} from freestream "
+                      f"({upd} cell-stage updates in {el:.1f} s)"}
 
 
 def main():
@@ -117,6 +160,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=32)
     ap.add_argument("--stages", type=int, default=4)
     ap.add_argument("--reps", type=int, default=50)
+    ap.add_argument("--samples", type=int, default=5_000_000)
+    ap.add_argument("--sample-rounds", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--epochs", type=float, default=20.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -124,63 +171,95 @@ def main():
         print(f"bench.py: this build runs on one GPU (the whole mesh lives in one device's HBM); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
-    spec = ibgpu.config2()
-    mesh = ibgpu.Mesh(spec)
+    mesh = ibgpu.Mesh(ibgpu.config2())
+    leaf = int(sum(mesh.counts()[0]))
     print(f"[bench] mesh: {mesh.ncells} cells, normal per level {mesh.counts()[0]}", file=sys.stderr, flush=True)
     res = {}
     for mode in ("fp64", "mixed"):
-        res[mode] = run_mode(mesh, mode, args.stages, args.warmup, args.steps, args.reps)
+        eng = new_engine(mesh, mode, args.stages)
+        res[mode] = timed_run(eng, mesh, args.warmup, args.steps, args.reps)
+        eng.close()
         r = res[mode]
-        print(f"[bench] {mode}: {r['rate']:.4e} cell-stage/s, {1e3 * r['seconds'] / args.steps:.3f} ms/step, "
+        print(f"[bench] {mode}+SA: {r['rate']:.4e} cell-stage/s, {1e3 * r['seconds'] / args.steps:.3f} ms/step, "
               f"CL {r['cl']:.6f} CD {r['cd']:.6f}, stage kernel {r['stage_ms'] * 1e3:.1f} us",
               file=sys.stderr, flush=True)
+    prm, tinfo = train_closure(mesh, args.stages, args)
+    print(f"[bench] closure: {tinfo['samples']} samples, {tinfo['train_steps']} Adam steps "
+          f"(batch {args.batch}), loss {tinfo['final_loss']:.3e}, sampling {tinfo['sample_s']:.1f} s, "
+          f"training {tinfo['train_s']:.2f} s", file=sys.stderr, flush=True)
+    eng = new_engine(mesh, "mixed", args.stages)
+    eng.closure_init(1 << 10)
+    eng.closure_set_params(prm)
+    eng.closure_enable(True)
+    res["ml"] = timed_run(eng, mesh, args.warmup, args.steps, args.reps)
+    on_net, fallback = eng.closure_counts()
+    eng.close()
+    r = res["ml"]
+    print(f"[bench] mixed+ML: {r['rate']:.4e} cell-stage/s, {1e3 * r['seconds'] / args.steps:.3f} ms/step, "
+          f"CL {r['cl']:.6f} CD {r['cd']:.6f}, stage kernel {r['stage_ms'] * 1e3:.1f} us, "
+          f"network cells {on_net}, fallback cells {fallback}", file=sys.stderr, flush=True)
     # plain copy of 1 GiB: the achievable HBM bandwidth on this box
     probe = ibgpu.Engine(mesh, mode="fp64", nstage=args.stages)
     copy_ms = probe.bench_copy(1 << 30, 20)
     probe.close()
     copy_gbs = 2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9
     print(f"[bench] copy probe: 1 GiB in {copy_ms:.3f} ms = {copy_gbs:.0f} GB/s (read+write)", file=sys.stderr, flush=True)
-    mx, fp = res["mixed"], res["fp64"]
-    nbytes = stage_bytes("mixed", mesh.dim, mx["stage_cells"], mx["stage_faces"])
-    achieved = nbytes / (mx["stage_ms"] * 1e-3) / 1e9
-    fp_bytes = stage_bytes("fp64", mesh.dim, fp["stage_cells"], fp["stage_faces"])
-    fp_achieved = fp_bytes / (fp["stage_ms"] * 1e-3) / 1e9
-    w_rel = float(np.linalg.norm(mx["state"][:, :5] - fp["state"][:, :5]) / np.linalg.norm(fp["state"][:, :5]))
+    fp, mx, ml = res["fp64"], res["mixed"], res["ml"]
+
+    def kernel_gbs(r, mode):
+        b = stage_bytes(mode, mesh.dim, True, r["stage_cells"], r["stage_faces"])
+        return b, b / (r["stage_ms"] * 1e-3) / 1e9
+
+    ml_bytes, ml_gbs = kernel_gbs(ml, "mixed")
+    _, mx_gbs = kernel_gbs(mx, "mixed")
+    _, fp_gbs = kernel_gbs(fp, "fp64")
+
+    def acc(r):
+        return {"CL": r["cl"], "CD": r["cd"],
+                "dCL_rel": abs(r["cl"] - fp["cl"]) / max(abs(fp["cl"]), 1e-30),
+                "dCD_rel": abs(r["cd"] - fp["cd"]) / max(abs(fp["cd"]), 1e-30),
+                "state_rel_L2": float(np.linalg.norm(r["state"][:, :5] - fp["state"][:, :5]) /
+                                      np.linalg.norm(fp["state"][:, :5])),
+                "nut_rel_L2": float(np.linalg.norm(r["nut"] - fp["nut"]) / np.linalg.norm(fp["nut"]))}
+
     out = {
         "metric": "cell-stage updates/s, mixed+ML vs plain FP64 at equal C_L/C_D error",
-        "value": mx["rate"],
+        "value": ml["rate"],
         "unit": "cell-stage updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * mx["seconds"] / args.steps,
+        "ms_per_step": 1e3 * ml["seconds"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 flux + f64 state",
-        "data": "synthetic (freestream start, analytic NACA0015 geometry)",
+        "dtype": "f32 flux + f64 state, bf16 closure network",
+        "data": "synthetic (freestream start, analytic NACA0015 geometry); closure trained in-run "
+                "on cells of the plain FP64 run",
         "config": {
             "workload": "NACA0015 2D, M0.3 Re1e4, [-10,10]^2, 256^2 base + 5 AMR levels, "
-                        f"{int(sum(mesh.counts()[0]))} leaf cells, 4-stage, temporal sub-cycling, "
-                        "body at initial pose, laminar (no closure)",
+                        f"{leaf} leaf cells, 4-stage, temporal sub-cycling, body at initial pose, "
+                        "Spalart-Allmaras closure (plain FP64 vs mixed + learned)",
             "cells": int(mesh.ncells),
-            "leaf_cells": int(sum(mesh.counts()[0])),
+            "leaf_cells": leaf,
             "levels": mesh.nlevels,
             "stages": args.stages,
             "parallelism": "1 GPU",
         },
         "plain_fp64": {"value": fp["rate"], "ms_per_step": 1e3 * fp["seconds"] / args.steps,
-                       "stage_kernel_GBs": fp_achieved},
+                       "stage_kernel_ms": fp["stage_ms"], "stage_kernel_GBs": fp_gbs},
+        "mixed_plain_closure": {"value": mx["rate"], "ms_per_step": 1e3 * mx["seconds"] / args.steps,
+                                "stage_kernel_ms": mx["stage_ms"], "stage_kernel_GBs": mx_gbs},
+        "ml_over_fp64": ml["rate"] / fp["rate"],
         "mixed_over_fp64": mx["rate"] / fp["rate"],
-        "accuracy": {"CL_fp64": fp["cl"], "CD_fp64": fp["cd"], "CL_mixed": mx["cl"], "CD_mixed": mx["cd"],
-                     "dCL_rel": abs(mx["cl"] - fp["cl"]) / max(abs(fp["cl"]), 1e-30),
-                     "dCD_rel": abs(mx["cd"] - fp["cd"]) / max(abs(fp["cd"]), 1e-30),
-                     "state_rel_L2": w_rel,
-                     "after_steps": args.warmup + args.steps},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_stage<float,2,mixed>", "bytes_per_launch": nbytes,
-                     "avg_launch_ms": mx["stage_ms"], "cells_per_launch": mx["stage_cells"],
+        "accuracy": {"vs": "plain FP64 after the same steps", "after_steps": args.warmup + args.steps,
+                     "CL_fp64": fp["cl"], "CD_fp64": fp["cd"],
+                     "mixed": acc(mx), "mixed_ml": acc(ml)},
+        "closure": dict(tinfo, network_cells=on_net, fallback_cells=fallback),
+        "roofline": {"bound": "hbm", "achieved": ml_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ml_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_stage<float,2,ML>", "bytes_per_launch": ml_bytes,
+                     "avg_launch_ms": ml["stage_ms"], "cells_per_launch": ml["stage_cells"],
                      "copy_probe_GBs": copy_gbs},
         "cpu_baseline": None,
     }

@@ -1394,6 +1394,43 @@ ibg_status ibg_bench_copy(ibg_ctx *c, int64_t nbytes, int32_t reps, double *avg_
   return IBG_OK;
 }
 
+ibg_status ibg_autotune_tile(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics *ph,
+                             const int32_t *cand, int32_t ncand, int32_t reps, int32_t *best,
+                             double *ms) {
+  if (!c || !m || !ph || !cand || ncand < 1 || reps < 1) return IBG_ERR_ARG;
+  for (int k = 0; k < ncand; ++k)
+    if (cand[k] < 64 || cand[k] > kTile || cand[k] % 64 != 0) return IBG_ERR_ARG;
+  // freestream state of every case (InitFlow NS_Solver.H:432-446)
+  std::vector<double> fv((size_t)m->ncells * IBG_NFV), nu(m->ncells);
+  const int L = m->nlevels;
+  for (int cs = 0; cs < m->ncases; ++cs)
+    for (int g = m->blk_start[cs * L]; g < m->blk_start[(cs + 1) * L]; ++g) {
+      for (int v = 0; v < IBG_NFV; ++v) fv[(size_t)g * IBG_NFV + v] = ph[cs].initvar[v];
+      nu[g] = ph[cs].nut_inf;
+    }
+  double tbest = 1e300;
+  int kbest = 0;
+  ibg_status st;
+  for (int k = 0; k < ncand; ++k) {
+    c->cfg.tile = cand[k];
+    c->tile = cand[k];
+    if ((st = ibg_upload_mesh(c, m, ph))) return st;
+    if ((st = ibg_set_state(c, fv.data(), m->ncells))) return st;
+    if (c->turb && (st = ibg_set_nut(c, nu.data(), m->ncells))) return st;
+    if ((st = run_dt(c))) return st;
+    double t = 0.0;
+    if ((st = ibg_bench_stage(c, 0, 0, 2, &t, nullptr, nullptr))) return st;   // warm
+    if ((st = ibg_bench_stage(c, 0, 0, reps, &t, nullptr, nullptr))) return st;
+    if (ms) ms[k] = t;
+    if (t < tbest) { tbest = t; kbest = k; }
+  }
+  c->cfg.tile = cand[kbest];
+  c->tile = cand[kbest];
+  if ((st = ibg_upload_mesh(c, m, ph))) return st;
+  if (best) *best = cand[kbest];
+  return IBG_OK;
+}
+
 ibg_status ibg_cell_stage_updates(ibg_ctx *c, int64_t *count) {
   if (!c || !count) return IBG_ERR_ARG;
   *count = c->last_updates;

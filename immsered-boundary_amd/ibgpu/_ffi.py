@@ -168,6 +168,8 @@ GPU_FUNCS = {
                                   C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "ibg_bench_copy": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, f64p]),
     "ibg_cell_stage_updates": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "ibg_autotune_tile": (C.c_int, [C.c_void_p, C.POINTER(MeshSoA), C.POINTER(Physics), i32p,
+                                    C.c_int32, C.c_int32, i32p, f64p]),
     "ibg_closure_init": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint64]),
     "ibg_closure_sample": (C.c_int, [C.c_void_p, C.c_double, C.c_uint64, C.POINTER(C.c_int64)]),
     "ibg_closure_fit_stats": (C.c_int, [C.c_void_p, C.c_double]),

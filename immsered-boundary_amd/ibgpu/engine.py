@@ -170,6 +170,17 @@ class Engine:
                                               out.shape[0]), "ibg_get_nut_source")
         return out
 
+    def autotune_tile(self, candidates=(64, 128, 192, 256), reps=20):
+        """Pick the stage-kernel tile size by timing (resets the state)."""
+        cand = np.ascontiguousarray(candidates, dtype=np.int32)
+        ms = np.zeros(cand.size)
+        best = C.c_int32()
+        self._chk(self.lib.ibg_autotune_tile(self.ctx, self.mesh.soa_ptr, self.mesh.phys,
+                                             cand.ctypes.data_as(_ffi.i32p), cand.size, reps,
+                                             C.byref(best), ms.ctypes.data_as(_ffi.f64p)),
+                  "ibg_autotune_tile")
+        return best.value, dict(zip(cand.tolist(), ms.tolist()))
+
     # learned closure ----------------------------------------------------------
     def closure_init(self, capacity, seed=20261202):
         self._chk(self.lib.ibg_closure_init(self.ctx, int(capacity), seed), "ibg_closure_init")

@@ -107,24 +107,29 @@ def tiny3d_case(levels=3, base=24, nz=4, aoa=4.0, Ma=0.5, Re=2000.0):
 
 def config3(levels=5, base=128, nz=8):
     """3D quasi-span: NACA0012 extruded over a periodic span of 0.1 chord,
-    128x128x8 base, 4 AMR levels (x,y refinement), Mach 0.6, AoA 4, Re 5e4."""
-    bands = [(-0.3, 3.0), (-0.15, 1.5), (-0.08, 0.8), (-0.04, 0.4)]
+    128x128x8 base, 4 AMR levels, Mach 0.6, AoA 4, Re 5e4 (~20M leaf cells).
+    The first two refinements also halve the span spacing (0.0125 -> 0.003);
+    the finer ones refine x,y only, where the cells have become near
+    isotropic."""
+    bands = [(-0.3, 6.0), (-0.15, 4.0), (-0.08, 3.2), (-0.04, 2.8)]
     return CaseSpec(name="naca0012_3d_m06_re5e4", dim=3, nx=base, ny=base, nz=nz, span=0.1,
-                    nlevels=levels, twod_from=1, bands=bands[:levels - 1], thickness=0.12,
+                    nlevels=levels, twod_from=3, bands=bands[:levels - 1], thickness=0.12,
                     npts=100, aoa=4.0, Ma=0.6, Re=50000.0, span_patches=nz)
 
 
-def ensemble_cases(n=32, seed=20261202, base=128, levels=5):
+def ensemble_cases(n=32, seed=20261202, base=256, levels=6):
     """Batched ensemble: NACA 4-digit sections, thickness U[0.09,0.18], AoA
-    U[-4,8] deg, Mach U[0.3,0.7], Re 1e4 (seeded)."""
+    U[-4,8] deg, Mach U[0.3,0.7], Re 1e4 (seeded); the refinement extent of
+    each case is drawn too (uneven AMR, ~0.6M to ~3M leaf cells per case on
+    the 256^2 base with 5 AMR levels)."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n):
         t = float(rng.uniform(0.09, 0.18))
         a = float(rng.uniform(-4.0, 8.0))
         m = float(rng.uniform(0.3, 0.7))
-        w = float(rng.uniform(0.6, 1.4))   # uneven refinement extent
-        bands = [(-0.3, 4.0 * w), (-0.15, 2.5 * w), (-0.08, 1.5 * w), (-0.04, 0.8 * w)]
+        w = float(rng.uniform(0.3, 1.15))   # uneven refinement extent
+        bands = [(-0.3, 6.0 * w), (-0.15, 4.0 * w), (-0.08, 2.5 * w), (-0.04, 1.6 * w), (-0.02, 1.0 * w)]
         out.append(CaseSpec(name=f"ens{i:02d}", dim=2, nx=base, ny=base, nz=1, nlevels=levels,
                             bands=bands[:levels - 1], thickness=t, npts=100, aoa=a, Ma=m,
                             Re=10000.0))

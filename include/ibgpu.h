@@ -281,6 +281,15 @@ ibg_status ibg_bench_stage(ibg_ctx *ctx, int32_t ts, int32_t subt, int32_t reps,
  * context stream; avg_ms per copy.  Calibrates HBM counters and bandwidth. */
 ibg_status ibg_bench_copy(ibg_ctx *ctx, int64_t nbytes, int32_t reps, double *avg_ms);
 
+/* Tile-size autotuner with no counterpart in the reference: for each
+ * candidate (a multiple of 64 up to 256 cells per tile) rebuild the tiles,
+ * time the stage kernel of a full-march step on the freestream, and keep
+ * the fastest (ms[k]: the time of candidate k).  The mesh is re-uploaded
+ * and the state reset: call it before ibg_set_state.                      */
+ibg_status ibg_autotune_tile(ibg_ctx *ctx, const ibg_mesh_soa *mesh, const ibg_physics *phys,
+                             const int32_t *candidates, int32_t ncand, int32_t reps,
+                             int32_t *best, double *ms);
+
 /* Counters for the metric: normal cells updated per step, summed over the
  * steps of the last ibg_time_march, times the stage count (the cell-stage
  * update count of NS_Solver.cpp:868-872, 948-953).                        */
