@@ -111,8 +111,8 @@ __global__ void k_closure_moments(const float *X, const float *Z, long long n, d
 }
 
 // fp32 master parameters -> the bf16 layouts of the kernels.
-__global__ void k_closure_pack(const float *p, __bf16 *W1t, __bf16 *W2t, __bf16 *W2, float *w3,
-                               float *b1, float *b2, float *b3) {
+__global__ void k_closure_pack(const float *p, __bf16 *W1t, __bf16 *W2t, __bf16 *W2, __bf16 *W2p,
+                               float *w3, float *b1, float *b2, float *b3) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < 64 * kX) {
     const int j = t / kX, k = t % kX;
@@ -122,6 +122,13 @@ __global__ void k_closure_pack(const float *p, __bf16 *W1t, __bf16 *W2t, __bf16 
     const int a = t / kH, b = t % kH;
     W2t[t] = (__bf16)(b < 64 ? p[kOffW2 + b * 64 + a] : 0.0f);   // W2t[j][i] = W2[i][j]
     W2[t] = (__bf16)(b < 64 ? p[kOffW2 + a * 64 + b] : 0.0f);    // W2[i][j]
+    // W2p[j][k] = W2[pi(k)][j], pi(32 kt + 8 q + e) = 16 (2 kt + e/4) + 4 q + e%4
+    int pk = 0;
+    if (b < 64) {
+      const int kt = b / 32, q = (b % 32) / 8, e = b % 8;
+      pk = 16 * (2 * kt + e / 4) + 4 * q + e % 4;
+    }
+    W2p[t] = (__bf16)(b < 64 ? p[kOffW2 + pk * 64 + a] : 0.0f);
   }
   if (t < 64) {
     w3[t] = (float)(__bf16)p[kOffW3 + t];
@@ -131,33 +138,30 @@ __global__ void k_closure_pack(const float *p, __bf16 *W1t, __bf16 *W2t, __bf16 
   if (t == 0) b3[0] = p[kOffB3];
 }
 
-// Stand-alone inference over raw feature rows: z = zmean + zstd * y.
-// 64 rows per workgroup of 256 threads (4 waves x 16 rows).
+// Stand-alone inference over raw feature rows: z = zmean + zstd * y, with
+// the register forward of the stage kernel (mlp_fwd_reg): 64 rows per wave,
+// one row per lane.
 __global__ void __launch_bounds__(256) k_closure_infer(MlpDev M, const float *X, float *Zout, long long n) {
   __shared__ __align__(16) __bf16 sW1t[64 * kX];
-  __shared__ __align__(16) __bf16 sW2t[64 * kH];
-  __shared__ float sw3[64], sb1[64], sb2[64];
-  __shared__ __align__(16) __bf16 sX[64 * kX];
-  __shared__ __align__(16) __bf16 sH[64 * kH];
-  MlpLds Wl{sW1t, sW2t, sw3, sb1, sb2};
-  mlp_load_lds(M, Wl);
-  const long long r0 = (long long)blockIdx.x * 64;
-  for (int i = threadIdx.x; i < 64 * kX; i += blockDim.x) {
-    const int row = i / kX, k = i % kX;
-    float v = 0.0f;
-    if (k < kMlpIn && r0 + row < n) v = (X[(r0 + row) * kMlpIn + k] - M.fmean[k]) / M.fstd[k];
-    sX[i] = (__bf16)v;
-  }
+  __shared__ __align__(16) __bf16 sW2p[64 * kH];
+  __shared__ __align__(16) float sw3[64], sb1[64], sb2[64];
+  for (int i = threadIdx.x; i < 64 * kX; i += blockDim.x) sW1t[i] = M.W1t[i];
+  for (int i = threadIdx.x; i < 64 * kH; i += blockDim.x) sW2p[i] = M.W2p[i];
+  for (int i = threadIdx.x; i < 64; i += blockDim.x) { sw3[i] = M.w3[i]; sb1[i] = M.b1[i]; sb2[i] = M.b2[i]; }
   __syncthreads();
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  float yv[4];
-  mlp_fwd16(sX + 16 * w * kX, sH + 16 * w * kH, nullptr, nullptr, nullptr, Wl, yv);
-  if ((lane & 15) == 0) {
-    for (int v = 0; v < 4; ++v) {
-      const long long row = r0 + 16 * w + 4 * (lane >> 4) + v;
-      if (row < n) Zout[row] = M.zst[0] + M.zst[1] * (yv[v] + M.b3[0]);
-    }
+  const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  bf16x8_t xf;
+#pragma unroll
+  for (int k = 0; k < kMlpIn; ++k)
+    xf[k] = (__bf16)(row < n ? (X[row * kMlpIn + k] - M.fmean[k]) / M.fstd[k] : 0.0f);
+  const int lane = threadIdx.x & 63;
+  float yown = 0.0f;
+#pragma unroll 1
+  for (int sb = 0; sb < 4; ++sb) {
+    const float y = mlp_fwd_reg(xf, sb, sW1t, sW2p, sb1, sb2, sw3);
+    if ((lane >> 4) == sb) yown = y;
   }
+  if (row < n) Zout[row] = M.zst[0] + M.zst[1] * (yown + M.b3[0]);
 }
 
 // Forward + backward of 64 rows per workgroup (4 waves x 16 rows), loss

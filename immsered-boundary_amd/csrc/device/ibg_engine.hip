@@ -75,7 +75,7 @@ struct ibg_ctx {
     int *cell = nullptr;
     unsigned long long *count = nullptr, *mlcount = nullptr;
     float *p = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr, *lossp = nullptr, *gsum = nullptr;
-    __bf16 *W1t = nullptr, *W2t = nullptr, *W2 = nullptr;
+    __bf16 *W1t = nullptr, *W2t = nullptr, *W2 = nullptr, *W2p = nullptr;
     float *w3 = nullptr, *b1 = nullptr, *b2 = nullptr, *b3 = nullptr;
     float *stat = nullptr;    // fmean[8] fstd[8] flo[8] fhi[8] zst[2]
     double *mom = nullptr;
@@ -116,7 +116,7 @@ struct ibg_ctx {
   template <int D> Pack<double, D> w64() const { return Pack<double, D>{(D4 *)W64a, W64e}; }
   MlpDev mlpdev() const {
     MlpDev M;
-    M.W1t = cl.W1t; M.W2t = cl.W2t; M.W2 = cl.W2; M.w3 = cl.w3;
+    M.W1t = cl.W1t; M.W2t = cl.W2t; M.W2 = cl.W2; M.W2p = cl.W2p; M.w3 = cl.w3;
     M.b1 = cl.b1; M.b2 = cl.b2; M.b3 = cl.b3;
     M.fmean = cl.stat; M.fstd = cl.stat + 8; M.flo = cl.stat + 16; M.fhi = cl.stat + 24;
     M.zst = cl.stat + 32;
@@ -1461,7 +1461,7 @@ uint64_t host_mix64(uint64_t x) {
 
 ibg_status cl_pack(ibg_ctx *c) {
   hipLaunchKernelGGL(k_closure_pack, dim3(grid(64 * kH, 256)), dim3(256), 0, c->s, c->cl.p, c->cl.W1t,
-                     c->cl.W2t, c->cl.W2, c->cl.w3, c->cl.b1, c->cl.b2, c->cl.b3);
+                     c->cl.W2t, c->cl.W2, c->cl.W2p, c->cl.w3, c->cl.b1, c->cl.b2, c->cl.b3);
   HIPCHK(hipGetLastError());
   return IBG_OK;
 }
@@ -1505,6 +1505,7 @@ ibg_status ibg_closure_init(ibg_ctx *c, int64_t capacity, uint64_t seed) {
   if ((st = cl_alloc(c, &L.W1t, (size_t)64 * kX))) return st;
   if ((st = cl_alloc(c, &L.W2t, (size_t)64 * kH))) return st;
   if ((st = cl_alloc(c, &L.W2, (size_t)64 * kH))) return st;
+  if ((st = cl_alloc(c, &L.W2p, (size_t)64 * kH))) return st;
   if ((st = cl_alloc(c, &L.w3, 64))) return st;
   if ((st = cl_alloc(c, &L.b1, 64))) return st;
   if ((st = cl_alloc(c, &L.b2, 64))) return st;
@@ -1609,8 +1610,10 @@ ibg_status ibg_closure_train(ibg_ctx *c, int32_t steps, int32_t batch, double lr
     const uint64_t sk = host_mix64(seed) + (uint64_t)(c->cl.step) * (uint64_t)batch;
     if ((st = cl_train_launch(c, c->cl.X, c->cl.Z, n, nwg, batch, 0, sk))) return st;
     ++c->cl.step;
+    // cosine decay of the learning rate over the call, down to lr/100
+    const double lrk = lr * (0.01 + 0.99 * 0.5 * (1.0 + std::cos(3.141592653589793 * k / std::max(steps, 1))));
     hipLaunchKernelGGL(k_closure_adam, dim3(grid(kNParam, 256)), dim3(256), 0, c->s, c->cl.p, c->cl.m,
-                       c->cl.v, c->cl.grad, nwg, (float)lr, c->cl.step, (float *)nullptr);
+                       c->cl.v, c->cl.grad, nwg, (float)lrk, c->cl.step, (float *)nullptr);
     HIPCHK(hipGetLastError());
     if ((st = cl_pack(c))) return st;
   }
@@ -1665,7 +1668,7 @@ ibg_status ibg_closure_infer(ibg_ctx *c, const float *x, float *z, int64_t n) {
   if (hipMalloc(&dz, sizeof(float) * n) != hipSuccess) { (void)hipFree(dx); return IBG_ERR_HIP; }
   hipError_t e = hipMemcpy(dx, x, sizeof(float) * n * kMlpIn, hipMemcpyHostToDevice);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_closure_infer, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, c->s, c->mlpdev(),
+    hipLaunchKernelGGL(k_closure_infer, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->s, c->mlpdev(),
                        dx, dz, (long long)n);
     e = hipGetLastError();
   }

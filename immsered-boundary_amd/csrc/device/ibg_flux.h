@@ -431,12 +431,25 @@ __device__ __forceinline__ void visc_flux(const Prim<R, D> &L, const Prim<R, D> 
   }
 }
 
+// exp, x^6 and x^(1/6) of the SA functions: the libm forms in FP64 (the
+// reference's pow/exp); in the mixed mode x^6 = (x^2)^3 and the hardware
+// exp2/log2 (v_exp_f32, v_log_f32, ~1 ulp).
 template <typename R> __device__ __forceinline__ R r_exp(R x);
 template <> __device__ __forceinline__ double r_exp<double>(double x) { return exp(x); }
-template <> __device__ __forceinline__ float r_exp<float>(float x) { return expf(x); }
-template <typename R> __device__ __forceinline__ R r_pow(R x, R y);
-template <> __device__ __forceinline__ double r_pow<double>(double x, double y) { return pow(x, y); }
-template <> __device__ __forceinline__ float r_pow<float>(float x, float y) { return powf(x, y); }
+template <> __device__ __forceinline__ float r_exp<float>(float x) {
+  return __builtin_amdgcn_exp2f(x * 1.4426950408889634f);
+}
+template <typename R> __device__ __forceinline__ R r_pow6(R x);
+template <> __device__ __forceinline__ double r_pow6<double>(double x) { return pow(x, 6.0); }
+template <> __device__ __forceinline__ float r_pow6<float>(float x) {
+  const float x2 = x * x;
+  return x2 * x2 * x2;
+}
+template <typename R> __device__ __forceinline__ R r_root6(R x);
+template <> __device__ __forceinline__ double r_root6<double>(double x) { return pow(x, 1.0 / 6.0); }
+template <> __device__ __forceinline__ float r_root6<float>(float x) {
+  return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.0f / 6.0f));
+}
 
 // ComptMutSource (NS_Solver.cpp:648-808) for one non-solid cell: central
 // differences over the face neighbours (nl/nr: working variable, ul..wr:
@@ -485,11 +498,11 @@ __device__ __forceinline__ R sa_source(R u, R v, R w, R T, R roe, R nu, R sd, co
   const R ft2 = R(SA::ct3) * r_exp(-R(SA::ct4) * bigx * bigx);
   R rr = v_div(v_div(nu, bigs * segdis), Re);
   rr = (R(10.0) < rr) ? R(10.0) : rr;
-  const R rsix = r_pow(rr, R(6));
+  const R rsix = r_pow6(rr);
   const R g = rr + R(SA::cw2) * (rsix - rr);
-  const R gsix = r_pow(g, R(6));
+  const R gsix = r_pow6(g);
   const R cw3six = R(64.0);
-  const R fw = g * r_pow(v_div(R(1.0) + cw3six, gsix + cw3six), R(1.0 / 6.0));
+  const R fw = g * r_root6(v_div(R(1.0) + cw3six, gsix + cw3six));
   R src = -(u * dvisx + v * dvisy + w * dvisz);
   src += R(SA::cb1) * (R(1.0) - ft2) * bigs * nu;
   src -= v_div((R(SA::cw1) * fw - v_div(R(SA::cb1) * ft2, R(SA::kappa * SA::kappa))) * (vis2d * vis2d), Re);

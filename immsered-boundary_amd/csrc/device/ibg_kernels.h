@@ -119,10 +119,8 @@ struct StageArgs {
 constexpr int kRpart = 12;   // per tile: max[5], sumsq[5], SA max, SA sumsq
 
 // LDS of the learned-closure part of k_stage (aliases the face arrays,
-// which are dead by then): weights, one 16-row input block and one hidden
-// block per wave, the outputs of the 256 rows.
-constexpr size_t kStageMlSmem = (size_t)64 * kX * 2 + (size_t)64 * kH * 2 + 3 * 64 * 4 +
-                                4 * ((size_t)16 * kX * 2 + (size_t)16 * kH * 2) + kTile * 4;
+// which are dead by then): the weights (W1t, W2p, b1, b2, w3).
+constexpr size_t kStageMlSmem = (size_t)64 * kX * 2 + (size_t)64 * kH * 2 + 3 * 64 * 4;
 
 // One workgroup per tile of Morton-consecutive normal cells of one
 // (case, level).  Phase 1: every face touching the tile — MUSCL + all-speed
@@ -285,41 +283,63 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
   if constexpr (ML) {
     __syncthreads();   // every phase-2a read of the face arrays is done
     __bf16 *mW1t = reinterpret_cast<__bf16 *>(smem_raw);
-    __bf16 *mW2t = mW1t + 64 * kX;
-    float *mw3 = reinterpret_cast<float *>(mW2t + 64 * kH);
+    __bf16 *mW2p = mW1t + 64 * kX;
+    float *mw3 = reinterpret_cast<float *>(mW2p + 64 * kH);
     float *mb1 = mw3 + 64, *mb2 = mb1 + 64;
-    __bf16 *mX = reinterpret_cast<__bf16 *>(mb2 + 64);
-    __bf16 *mH = mX + 4 * 16 * kX;
-    float *mY = reinterpret_cast<float *>(mH + 4 * 16 * kH);
-    MlpLds Wl{mW1t, mW2t, mw3, mb1, mb2};
-    mlp_load_lds(a.mlp, Wl);
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    __bf16 *xs = mX + w * 16 * kX;
-    for (int sb = 0; sb < 4; ++sb) {
-      // lanes 16sb..16sb+15 of the wave own the 16 rows of this block
-      if ((lane >> 4) == sb) {
-        const int row = lane & 15;
-#pragma unroll
-        for (int k = 0; k < kX; ++k) {
-          float v = 0.0f;
-          if (k < kMlpIn && use_ml) v = ((float)ml_x[k] - a.mlp.fmean[k]) / a.mlp.fstd[k];
-          xs[row * kX + k] = (__bf16)v;
-        }
+    {
+      // weights to LDS in 16-byte chunks (W1t then W2p), all loads issued
+      // before the stores
+      constexpr int kW1Chunks = 64 * kX * 2 / 16, kWChunks = kW1Chunks + 64 * kH * 2 / 16;
+      static_assert(kW1Chunks == 320 && kWChunks == 896, "weight chunk counts");
+      uint4 *dst = reinterpret_cast<uint4 *>(mW1t);   // W2p follows W1t in LDS
+      const uint4 *s1 = reinterpret_cast<const uint4 *>(a.mlp.W1t);   // chunks [0, 320)
+      const uint4 *s2 = reinterpret_cast<const uint4 *>(a.mlp.W2p);   // chunks [320, 896)
+      // chunk id of W1t or W2p; both indices stay in range whichever is read
+      auto chunk = [&](int id) {
+        const uint4 x1 = s1[id < kW1Chunks ? id : 0];
+        const uint4 x2 = s2[id < kW1Chunks ? 0 : id - kW1Chunks];
+        return id < kW1Chunks ? x1 : x2;
+      };
+      if (blockDim.x == 256) {
+        // chunks t, t+256, t+512 always exist; t+768 for t < 128
+        const int t = threadIdx.x;
+        const uint4 c0 = chunk(t);
+        const uint4 c1 = chunk(t + 256);
+        const uint4 c2 = s2[t + 512 - kW1Chunks];
+        const uint4 c3 = s2[(t < kWChunks - 768 ? t + 768 : 768) - kW1Chunks];
+        dst[t] = c0;
+        dst[t + 256] = c1;
+        dst[t + 512] = c2;
+        if (t < kWChunks - 768) dst[t + 768] = c3;
+      } else {
+        for (int id = threadIdx.x; id < kWChunks; id += blockDim.x) dst[id] = chunk(id);
       }
-      __syncthreads();
-      float yv[4];
-      mlp_fwd16(xs, mH + w * 16 * kH, nullptr, nullptr, nullptr, Wl, yv);
-      if ((lane & 15) == 0)
-        for (int v = 0; v < 4; ++v) mY[64 * w + 16 * sb + 4 * (lane >> 4) + v] = yv[v];
+      for (int i = threadIdx.x; i < 64; i += blockDim.x) {
+        mw3[i] = a.mlp.w3[i]; mb1[i] = a.mlp.b1[i]; mb2[i] = a.mlp.b2[i];
+      }
     }
     __syncthreads();
+    const int lane = threadIdx.x & 63;
+    bf16x8_t xf;
+#pragma unroll
+    for (int k = 0; k < kMlpIn; ++k)
+      xf[k] = (__bf16)(use_ml ? ((float)ml_x[k] - a.mlp.fmean[k]) / a.mlp.fstd[k] : 0.0f);
+    float yown = 0.0f;
+    // one 16-cell block per pass; the compiler barrier keeps the weight
+    // fragments from being hoisted out of the loop (96 registers if they were)
+#pragma unroll 1
+    for (int sb = 0; sb < 4; ++sb) {
+      asm volatile("" ::: "memory");
+      const float y = mlp_fwd_reg(xf, sb, mW1t, mW2p, mb1, mb2, mw3);
+      if ((lane >> 4) == sb) yown = y;
+    }
     if (use_ml) {
-      const float z = a.mlp.zst[0] + a.mlp.zst[1] * (mY[threadIdx.x] + a.mlp.b3[0]);
+      const float z = a.mlp.zst[0] + a.mlp.zst[1] * (yown + a.mlp.b3[0]);
       snu = (double)(sa_rest + (R)z * ml_scale);
     }
     if (a.last && a.mlcount) {
       // cells on the network / on the fallback in this tile
-      unsigned long long nm = __ballot(use_ml) , nf = __ballot(sa_on && !use_ml);
+      unsigned long long nm = __ballot(use_ml), nf = __ballot(sa_on && !use_ml);
       if (lane == 0) {
         atomicAdd(&a.mlcount[0], (unsigned long long)__popcll(nm));
         atomicAdd(&a.mlcount[1], (unsigned long long)__popcll(nf));

@@ -7,7 +7,7 @@
 // production scale S^ = |Omega| + nu/(kappa^2 d^2 Re), from 8 per-cell
 // features (raw, then standardised with the training statistics):
 //   0 ln(|Omega| + eps)      vorticity magnitude (the reference's sqrt2|curl/2|)
-//   1 ln(|S| + eps)          strain-rate magnitude sqrt(2 S_ij S_ij)
+//   1 a = |Omega| / S^       vorticity share of the production scale, in [0, 1]
 //   2 ln(d)                  wall distance
 //   3 ln(chi + eps)          chi = nu / (mu/roe), eddy-viscosity ratio
 //   4 ln(|u| d Re/nu_l + eps) local Reynolds number
@@ -48,6 +48,7 @@ struct MlpDev {
   const __bf16 *W1t;   // [64][kX]  W1 transposed, inputs padded to 32
   const __bf16 *W2t;   // [64][kH]  W2 transposed (forward B operand)
   const __bf16 *W2;    // [64][kH]  W2 (backward B operand)
+  const __bf16 *W2p;   // [64][kH]  W2p[j][k] = W2[pi(k)][j] (transposed forward, see mlp_fwd_reg)
   const float *w3;     // [64] bf16-rounded values
   const float *b1, *b2, *b3;
   const float *fmean, *fstd;   // [8] feature standardisation
@@ -55,9 +56,13 @@ struct MlpDev {
   const float *zst;            // [2] target mean, std
 };
 
-__device__ __forceinline__ float mlp_silu(float z) { return z / (1.0f + __expf(-z)); }
+// sigmoid with the hardware exp2 and reciprocal (~1 ulp each)
+__device__ __forceinline__ float mlp_sigm(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-z * 1.4426950408889634f));
+}
+__device__ __forceinline__ float mlp_silu(float z) { return z * mlp_sigm(z); }
 __device__ __forceinline__ float mlp_dsilu(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = mlp_sigm(z);
   return s + z * s * (1.0f - s);
 }
 __device__ __forceinline__ float bf16r(float x) { return (float)(__bf16)x; }
@@ -149,6 +154,65 @@ __device__ __forceinline__ void mlp_fwd16(const __bf16 *Xs, __bf16 *H1s, __bf16 
   __syncthreads();
 }
 
+// Transposed forward with the activations in registers (the stage kernel).
+// The wave handles 16 cells per call: the cells of lanes 16*sb .. 16*sb+15,
+// each lane holding its own cell's 8 standardised features (bf16x8 xf).
+// Layer 1 computes Z1^T = W1^T X^T: only the K group sb of the A operand is
+// nonzero (lane (r, q) = lane r + 16 q contributes W1 when q == sb), so the
+// B operand is every lane's own feature vector.  The output of lane (r, q)
+// is hidden units 16 nt + 4 q + v of cell r of the block.  Layer 2 takes its
+// B operand straight from those registers: the K order of layer 2 is the
+// permutation pi(32 kt + 8 q + e) = 16 (2 kt + e/4) + 4 q + e%4 of the
+// hidden units, applied to W2 once (W2p).  Layer 3 sums the 16 products of
+// a lane and reduces over the 4 lanes of a cell.  Returns y (b3 not
+// included) of cell r in every lane (r, q).  No LDS for activations, no
+// barriers.  W1t/W2p/b1/b2/w3 come from LDS.
+__device__ __forceinline__ float mlp_fwd_reg(bf16x8_t xf, int sb, const __bf16 *W1t,
+                                             const __bf16 *W2p, const float *b1, const float *b2,
+                                             const float *w3) {
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15, q = lane >> 4;
+  const bf16x8_t zero8 = {};
+  floatx4_t h1[4];
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    const float4 bb = *reinterpret_cast<const float4 *>(b1 + 16 * nt + 4 * q);
+    h1[nt] = floatx4_t{bb.x, bb.y, bb.z, bb.w};
+    const bf16x8_t aw = *reinterpret_cast<const bf16x8_t *>(W1t + (16 * nt + r) * kX);
+    h1[nt] = mfma16(q == sb ? aw : zero8, xf, h1[nt]);
+  }
+  // silu -> bf16, packed as the layer-2 B fragments (kt = 0: tiles 0,1; kt = 1: tiles 2,3)
+  bf16x8_t b2f[2];
+#pragma unroll
+  for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+    for (int e = 0; e < 8; ++e) b2f[kt][e] = (__bf16)mlp_silu(h1[2 * kt + e / 4][e % 4]);
+  float y = 0.0f;
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    const float4 bb = *reinterpret_cast<const float4 *>(b2 + 16 * nt + 4 * q);
+    floatx4_t acc = floatx4_t{bb.x, bb.y, bb.z, bb.w};
+#pragma unroll
+    for (int kt = 0; kt < 2; ++kt) {
+      const bf16x8_t aw = *reinterpret_cast<const bf16x8_t *>(W2p + (16 * nt + r) * kH + 32 * kt + 8 * q);
+      acc = mfma16(aw, b2f[kt], acc);
+    }
+    const float4 ww = *reinterpret_cast<const float4 *>(w3 + 16 * nt + 4 * q);
+    y += (float)(__bf16)mlp_silu(acc[0]) * ww.x + (float)(__bf16)mlp_silu(acc[1]) * ww.y +
+         (float)(__bf16)mlp_silu(acc[2]) * ww.z + (float)(__bf16)mlp_silu(acc[3]) * ww.w;
+  }
+  y += __shfl_xor(y, 16, 64);
+  y += __shfl_xor(y, 32, 64);
+  return y;
+}
+
+// natural log: libm in FP64, the hardware log2 (v_log_f32) in FP32
+template <typename R> __device__ __forceinline__ R r_log(R x);
+template <> __device__ __forceinline__ double r_log<double>(double x) { return log(x); }
+template <> __device__ __forceinline__ float r_log<float>(float x) {
+  return __builtin_amdgcn_logf(x) * 0.6931471805599453f;
+}
+
 // Velocity gradient of one cell from central differences (physical units).
 template <typename R>
 struct VelGrad {
@@ -187,10 +251,10 @@ __device__ __forceinline__ R sa_pointwise(R nu, R mu0, R roe, R sd, R olen, R Re
   const R ft2 = R(SA::ct3) * r_exp(-R(SA::ct4) * bigx * bigx);
   R rr = v_div(v_div(nu, bigs * segdis), Re);
   rr = (R(10.0) < rr) ? R(10.0) : rr;
-  const R rsix = r_pow(rr, R(6));
+  const R rsix = r_pow6(rr);
   const R g = rr + R(SA::cw2) * (rsix - rr);
-  const R gsix = r_pow(g, R(6));
-  const R fw = g * r_pow(v_div(R(65.0), gsix + R(64.0)), R(1.0 / 6.0));
+  const R gsix = r_pow6(g);
+  const R fw = g * r_root6(v_div(R(65.0), gsix + R(64.0)));
   const R prod = R(SA::cb1) * (R(1.0) - ft2) * bigs * nu;
   const R dest = v_div((R(SA::cw1) * fw - v_div(R(SA::cb1) * ft2, R(SA::kappa * SA::kappa))) * (vis2d * vis2d), Re);
   return prod - dest;
@@ -204,20 +268,19 @@ __device__ __forceinline__ bool sa_features(R u, R v, R w, R roe, R p, R mu0, R 
                                             const VelGrad<R> &g, R olen, R Re, R pinf, R *x,
                                             R &scale) {
   const R eps = R(1e-8);
-  const R smag = v_sqrt(R(2.0) * (g.dux * g.dux + g.dvy * g.dvy + g.dwz * g.dwz) +
-                        (g.duy + g.dvx) * (g.duy + g.dvx) + (g.duz + g.dwx) * (g.duz + g.dwx) +
-                        (g.dvz + g.dwy) * (g.dvz + g.dwy));
   const R nul = v_div(mu0, roe);
   const R speed = v_sqrt(u * u + v * v + w * w);
-  x[0] = log(olen + eps);
-  x[1] = log(smag + eps);
-  x[2] = log(d > eps ? d : eps);
-  x[3] = log(v_div(nu, nul) + eps);
-  x[4] = log(v_div(speed * d * Re, nul) + eps);
+  const R shat = olen + v_div(nu, R(SA::kappa * SA::kappa) * d * d * Re);
+  x[0] = r_log(olen + eps);
+  x[1] = shat > R(0) ? v_div(olen, shat) : R(0);
+  x[2] = r_log(d > eps ? d : eps);
+  x[3] = r_log(v_div(nu, nul) + eps);
+  x[4] = r_log(v_div(speed * d * Re, nul) + eps);
   x[5] = roe;
   x[6] = v_div(p, pinf);
-  x[7] = log(nul);
-  scale = nu * (olen + v_div(nu, R(SA::kappa * SA::kappa) * d * d * Re));
+  x[7] = r_log(nul);
+  scale = nu * shat;
+  (void)g;
   return d > eps && nu > eps && scale > R(0);
 }
 

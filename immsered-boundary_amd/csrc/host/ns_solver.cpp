@@ -1,7 +1,10 @@
 // ns_solver.cpp — C++ driver in the shape of the reference's main.cpp.
 //
-//   ibg_run [steps] [fp64|mixed] [stages]
+//   ibg_run [steps] [fp64|mixed] [stages] [laminar|sa|ml]
 //
+// sa runs the TURBULENCE build (SA closure, working variable from its
+// freestream, Userfunction_cy_naca15.cpp:368-370); ml trains the learned
+// closure on cells of an FP64 SA run of the same case first (mixed mode).
 // Builds the small 2D case (NACA0012, Mach 0.5, AoA 2 deg, Re 5000, 64x64
 // base grid + 3 AMR levels), initialises the freestream (InitFlow
 // NS_Solver.H:432-446) and runs the time loop of main.cpp:84-99 through the
@@ -22,6 +25,8 @@ int main(int argc, char **argv) {
   const int steps = argc > 1 ? std::atoi(argv[1]) : 64;
   const int mode = (argc > 2 && std::string(argv[2]) == "mixed") ? IBG_MODE_MIXED : IBG_MODE_FP64;
   const int stages = argc > 3 ? std::atoi(argv[3]) : 3;
+  const std::string closure = argc > 4 ? std::string(argv[4]) : std::string("laminar");
+  const bool turb = closure != "laminar";
 
   const int npts = 100;
   std::vector<double> xy(4 * npts);
@@ -56,10 +61,34 @@ int main(int argc, char **argv) {
   std::printf("mesh: %d cells, %d faces, %d IB cells, %d patches\n", m->ncells, m->nfaces, m->nib, m->npatch);
 
   try {
-    ibgpu::NS_Solver solver(*m, phys, stages, mode, true);
+    std::vector<float> net;
+    if (closure == "ml") {
+      // training data from the plain closure (FP64), as the reference runs it
+      ibgpu::NS_Solver teacher(*m, phys, stages, IBG_MODE_FP64, true, true);
+      std::vector<double> fv((size_t)m->ncells * IBG_NFV);
+      ibm_init_state(h, phys.data(), fv.data());
+      teacher.SetState(fv);
+      teacher.SetViseddy(std::vector<double>(m->ncells, phys[0].nut_inf));
+      teacher.ClosureInit(1 << 20, 20261202);
+      long long n = 0;
+      for (int k = 0; k < 16; ++k) {
+        teacher.TimeMarching(8 * k, 8, true);
+        n = teacher.ClosureSample(0.5, 100 + k);
+      }
+      const double loss = teacher.ClosureTrain(2000, 1024, 1e-3, 7);
+      std::printf("closure: %lld samples, training loss %.3e\n", n, loss);
+      net = teacher.ClosureParams();
+    }
+    ibgpu::NS_Solver solver(*m, phys, stages, mode, true, turb);
     std::vector<double> fv((size_t)m->ncells * IBG_NFV);
     ibm_init_state(h, phys.data(), fv.data());
     solver.SetState(fv);
+    if (turb) solver.SetViseddy(std::vector<double>(m->ncells, phys[0].nut_inf));
+    if (!net.empty()) {
+      solver.ClosureInit(64, 1);
+      solver.ClosureSetParams(net);
+      solver.ClosureEnable(true);
+    }
     std::vector<double> linf, l2, init_res;
     for (int ts = 0; ts < steps; ++ts) {
       solver.TimeMarching(ts, 1, false);

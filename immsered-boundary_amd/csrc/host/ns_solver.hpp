@@ -29,9 +29,10 @@ class NS_Solver {
  public:
   // NS_Solver(Mesh&, Domain&, int) NS_Solver.H:195-237: the stage table
   // (tp[]) is chosen by the stage count, as sub_time_step selects it.
+  // turbulence selects the -D TURBULENCE build (the SA closure).
   NS_Solver(const ibg_mesh_soa &mesh, const std::vector<ibg_physics> &phys, int sub_time_step = 3,
-            int mode = IBG_MODE_FP64, bool temporal_refine = true)
-      : ncells_(mesh.ncells), ncases_(mesh.ncases) {
+            int mode = IBG_MODE_FP64, bool temporal_refine = true, bool turbulence = false)
+      : ncells_(mesh.ncells), ncases_(mesh.ncases), mesh_(&mesh), phys_(phys) {
     ibg_config cfg{};
     cfg.mode = mode;
     cfg.nstage = sub_time_step;
@@ -42,6 +43,7 @@ class NS_Solver {
     cfg.temporal_refine = temporal_refine ? 1 : 0;
     cfg.dt_var_step = temporal_refine ? (1 << (mesh.nlevels - 1)) : 1;
     cfg.viscous = 1;
+    cfg.turbulence = turbulence ? 1 : 0;
     check(ibg_create(&cfg, &ctx_), "ibg_create");
     check(ibg_upload_mesh(ctx_, &mesh, phys.data()), "ibg_upload_mesh");
   }
@@ -55,6 +57,13 @@ class NS_Solver {
     std::vector<double> fv((size_t)ncells_ * IBG_NFV);
     check(ibg_get_state(ctx_, fv.data(), ncells_), "ibg_get_state");
     return fv;
+  }
+  // FlowVariables::viseddy of the TURBULENCE build (FlowVariables.H:327-329)
+  void SetViseddy(const std::vector<double> &nut) { check(ibg_set_nut(ctx_, nut.data(), ncells_), "ibg_set_nut"); }
+  std::vector<double> GetViseddy() {
+    std::vector<double> nut(ncells_);
+    check(ibg_get_nut(ctx_, nut.data(), ncells_), "ibg_get_nut");
+    return nut;
   }
 
   // the per-step sequence of main.cpp:87-97
@@ -86,6 +95,47 @@ class NS_Solver {
     check(ibg_forces(ctx_, motion, f.data()), "ComptWallForce");
     return f;
   }
+  // SA residual (res_vis_sum, NS_Solver.H:297, 321)
+  void ResinDomainViseddy(std::vector<double> &linf, std::vector<double> &l2) {
+    linf.assign(ncases_, 0.0);
+    l2.assign(ncases_, 0.0);
+    check(ibg_residual_nut(ctx_, linf.data(), l2.data()), "ibg_residual_nut");
+  }
+
+  // Learned closure (no reference counterpart): sample the current state,
+  // fit the statistics, train, and switch the stage to the network.
+  void ClosureInit(long long capacity, unsigned long long seed) {
+    check(ibg_closure_init(ctx_, capacity, seed), "ibg_closure_init");
+  }
+  long long ClosureSample(double frac, unsigned long long seed) {
+    int64_t n = 0;
+    check(ibg_closure_sample(ctx_, frac, seed, &n), "ibg_closure_sample");
+    return (long long)n;
+  }
+  double ClosureTrain(int steps, int batch, double lr, unsigned long long seed) {
+    double loss = 0.0;
+    check(ibg_closure_fit_stats(ctx_, 0.0), "ibg_closure_fit_stats");
+    check(ibg_closure_train(ctx_, steps, batch, lr, seed, &loss), "ibg_closure_train");
+    return loss;
+  }
+  std::vector<float> ClosureParams() {
+    std::vector<float> p(IBG_CLOSURE_NPARAM + 34);
+    check(ibg_closure_params(ctx_, p.data(), (int64_t)p.size()), "ibg_closure_params");
+    return p;
+  }
+  void ClosureSetParams(const std::vector<float> &p) {
+    check(ibg_closure_set_params(ctx_, p.data(), (int64_t)p.size()), "ibg_closure_set_params");
+  }
+  void ClosureEnable(bool on) { check(ibg_closure_enable(ctx_, on ? 1 : 0), "ibg_closure_enable"); }
+  // Stage-kernel tile size by timing (resets the state: call before SetState)
+  int AutotuneTile(const std::vector<int> &cand, int reps = 20) {
+    int32_t best = 0;
+    std::vector<int32_t> c(cand.begin(), cand.end());
+    check(ibg_autotune_tile(ctx_, mesh_, phys_.data(), c.data(), (int32_t)c.size(), reps, &best, nullptr),
+          "ibg_autotune_tile");
+    return best;
+  }
+
   long long CellStageUpdates() {
     int64_t n = 0;
     ibg_cell_stage_updates(ctx_, &n);
@@ -103,6 +153,8 @@ class NS_Solver {
   }
   ibg_ctx *ctx_ = nullptr;
   int ncells_, ncases_;
+  const ibg_mesh_soa *mesh_;
+  std::vector<ibg_physics> phys_;
 };
 
 }  // namespace ibgpu

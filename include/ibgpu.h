@@ -315,7 +315,8 @@ ibg_status ibg_closure_init(ibg_ctx *ctx, int64_t capacity, uint64_t seed);
 ibg_status ibg_closure_sample(ibg_ctx *ctx, double frac, uint64_t seed, int64_t *total);
 /* standardisation and the input range (min/max widened by margin * span) */
 ibg_status ibg_closure_fit_stats(ibg_ctx *ctx, double margin);
-/* Adam steps on random batches of the sample store; loss of the last batch */
+/* Adam steps on random batches of the sample store, the learning rate
+ * decayed from lr to lr/100 (cosine) over the call; loss of the last batch */
 ibg_status ibg_closure_train(ibg_ctx *ctx, int32_t steps, int32_t batch, double lr,
                              uint64_t seed, double *loss);
 /* use the network in the stage (mixed mode with the SA closure only)       */

@@ -209,24 +209,23 @@ int32_t orc_closure_features(const ibg_mesh_soa *m, const ibg_physics *ph, const
   }
   double ox = 0.5 * (dwy - dvz), oy = 0.5 * (duz - dwx), oz = 0.5 * (dvx - duy);
   double olen = sqrt(2.0) * sqrt(ox * ox + oy * oy + oz * oz);
-  double smag = sqrt(2.0 * (dux * dux + dvy * dvy + dwz * dwz) + (duy + dvx) * (duy + dvx) +
-                     (duz + dwx) * (duz + dwx) + (dvz + dwy) * (dvz + dwy));
   double roe = W[0], p = W[4];
   double T = p / (roe * P->Rg);
   double mu0 = sqrt(T * T * T) * ((1.0 + P->S_over_T_ref) / (T + P->S_over_T_ref));
   double nul = mu0 / roe, nu = nut[cell], d = m->signdis[cell];
   double eps = 1e-8;
   double speed = sqrt(W[1] * W[1] + W[2] * W[2] + W[3] * W[3]);
+  const double kap2 = 0.41 * 0.41;
+  double shat = olen + nu / (kap2 * d * d * P->Re);
   x[0] = log(olen + eps);
-  x[1] = log(smag + eps);
+  x[1] = shat > 0.0 ? olen / shat : 0.0;
   x[2] = log(d > eps ? d : eps);
   x[3] = log(nu / nul + eps);
   x[4] = log(speed * d * P->Re / nul + eps);
   x[5] = roe;
   x[6] = p / P->initvar[4];
   x[7] = log(nul);
-  const double kap2 = 0.41 * 0.41;
-  double scale = nu * (olen + nu / (kap2 * d * d * P->Re));
+  double scale = nu * shat;
   if (!(d > eps && nu > eps && scale > 0.0)) return 0;
   /* point-wise SA term: the full cell source minus its derivative terms,
      evaluated with zero gradients of nu and zero face term               */

@@ -8,7 +8,8 @@
                 BF16 rounding of inputs, weights and activations
   features      the target z times the production scale is the point-wise SA
                 term: the mesh-level SA source with a uniform working
-                variable and no face term; a uniform flow has |Omega| = |S| = 0
+                variable and no face term; a uniform flow has |Omega| = 0 and
+                a vorticity share of 0
 """
 import numpy as np
 import pytest
@@ -99,6 +100,7 @@ def test_features_target_is_pointwise_sa_term(tiny):
         olen = np.exp(xf[0]) - 1e-8
         d = np.exp(xf[2])
         scale = nut[g] * (olen + nut[g] / (kappa2 * d * d * P.Re))
+        assert xf[1] == pytest.approx(olen / (olen + nut[g] / (kappa2 * d * d * P.Re)), rel=1e-6, abs=1e-12)
         assert z * scale == pytest.approx(ms[g], rel=1e-9, abs=1e-12)
         assert d == pytest.approx(tiny.signdis[g], rel=1e-12)
         checked += 1
@@ -112,5 +114,5 @@ def test_features_uniform_flow(tiny):
     ok, xf, z = O.closure_features(tiny, fv, nut, g)
     assert ok
     assert xf[0] == pytest.approx(np.log(1e-8))
-    assert xf[1] == pytest.approx(np.log(1e-8))
+    assert xf[1] == 0.0
     assert xf[5] == 1.0 and xf[6] == 1.0
