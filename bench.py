@@ -39,15 +39,24 @@ import ibgpu  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 
+# Measured HBM traffic of the learned-closure stage kernel, bytes per cell of
+# a launch: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch) over
+# every dispatch of k_stage<float,2,true> in this script's profiled run,
+# scaled by what the counters read for a plain 1 GiB copy (FETCH_SIZE x2.0,
+# WRITE_SIZE x1.0), divided by the cells of the launch.  Recorded in
+# profiles/k_stage_ml_naca2d_traffic.md; None until profiled.
+TRAFFIC_BYTES_PER_CELL = None
 
-def stage_bytes(mode, dim, turb, cells, faces):
-    """Algorithmic HBM bytes of one (non-last) stage-kernel launch.
+
+def stage_bytes(mode, dim, turb, cells, faces, last=False):
+    """Algorithmic HBM bytes of one stage-kernel launch.
 
     Per normal cell: its stencil state read once (neighbour reads hit in
     cache) and written once, the frozen viscous residual, the FP64 master
     read for U_old, the face slots (2 B per cell face) and the flag byte;
     with the SA closure the working variable read and written, its FP64
     master (the old value), the frozen face term and the wall distance.
+    The last stage of a step in mixed mode also writes the FP64 masters.
     Per tile face the 4 stencil indices and the direction byte.
     """
     nv = dim + 2
@@ -55,7 +64,26 @@ def stage_bytes(mode, dim, turb, cells, faces):
     per_cell = 3 * nv * sw + nv * 8 + 2 * dim * 2 + 1
     if turb:
         per_cell += 3 * sw + 8 + 8
+    if last and mode != "fp64":
+        per_cell += nv * 8 + (8 if turb else 0)
     return cells * per_cell + faces * 17
+
+
+def stage_mix(eng, mesh, mode, ts0, nstage, reps):
+    """Stage-kernel launches of one sub-cycling period (every stage of the
+    steps ts0 .. ts0 + 2^(L-1) - 1, the launch mix of the timed steps),
+    each timed with HIP events on the engine's stream: average launch
+    duration, algorithmic bytes and cells per launch."""
+    period = 1 << (mesh.nlevels - 1)
+    ms_sum, by_sum, c_sum, n = 0.0, 0.0, 0, 0
+    for ts in range(ts0, ts0 + period):
+        for k in range(nstage):
+            ms, nc, nf = eng.bench_stage(ts, k, reps)
+            ms_sum += ms
+            by_sum += stage_bytes(mode, mesh.dim, True, nc, nf, last=(k == nstage - 1))
+            c_sum += nc
+            n += 1
+    return ms_sum / n, by_sum / n, c_sum / n
 
 
 def updates_per_step(mesh, nstage, ts):
@@ -84,7 +112,7 @@ def new_engine(mesh, mode, nstage):
     return eng
 
 
-def timed_run(eng, mesh, warmup, steps, reps):
+def timed_run(eng, mesh, mode, warmup, steps, reps, ml=False):
     eng.TimeMarching(0, warmup, use_graph=True)
     eng.synchronize()
     t0 = time.perf_counter()
@@ -96,12 +124,12 @@ def timed_run(eng, mesh, warmup, steps, reps):
     F = eng.ComptWallForce()
     cd, cl = F[0] / P.force_ref, F[1] / P.force_ref
     state, nut = eng.get_state(), eng.get_nut()
-    # stage kernel alone, all levels marched (ts multiple of the period)
-    period = 1 << (mesh.nlevels - 1)
-    ts_full = ((warmup + steps) // period) * period
-    ms, nc, nf = eng.bench_stage(ts_full, 0, reps)
+    counts = eng.closure_counts() if ml else (0, 0)   # cells of the warmup + timed steps
+    # the stage kernel alone, launch by launch over the timed steps' mix
+    ms, nbytes, ncell = stage_mix(eng, mesh, mode, warmup + steps, eng.cfg.nstage, reps)
     return {"seconds": t1 - t0, "updates": upd, "rate": upd / (t1 - t0), "cd": cd, "cl": cl,
-            "stage_ms": ms, "stage_cells": nc, "stage_faces": nf, "state": state, "nut": nut}
+            "stage_ms": ms, "stage_bytes": nbytes, "stage_cells": ncell, "state": state, "nut": nut,
+            "counts": counts}
 
 
 def train_closure(mesh, nstage, args):
@@ -159,7 +187,7 @@ def main():
     ap.add_argument("--steps", type=int, default=128)
     ap.add_argument("--warmup", type=int, default=32)
     ap.add_argument("--stages", type=int, default=4)
-    ap.add_argument("--reps", type=int, default=50)
+    ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--samples", type=int, default=5_000_000)
     ap.add_argument("--sample-rounds", type=int, default=8)
     ap.add_argument("--batch", type=int, default=8192)
@@ -177,7 +205,7 @@ def main():
     res = {}
     for mode in ("fp64", "mixed"):
         eng = new_engine(mesh, mode, args.stages)
-        res[mode] = timed_run(eng, mesh, args.warmup, args.steps, args.reps)
+        res[mode] = timed_run(eng, mesh, mode, args.warmup, args.steps, args.reps)
         eng.close()
         r = res[mode]
         print(f"[bench] {mode}+SA: {r['rate']:.4e} cell-stage/s, {1e3 * r['seconds'] / args.steps:.3f} ms/step, "
@@ -191,8 +219,8 @@ def main():
     eng.closure_init(1 << 10)
     eng.closure_set_params(prm)
     eng.closure_enable(True)
-    res["ml"] = timed_run(eng, mesh, args.warmup, args.steps, args.reps)
-    on_net, fallback = eng.closure_counts()
+    res["ml"] = timed_run(eng, mesh, "mixed", args.warmup, args.steps, args.reps, ml=True)
+    on_net, fallback = res["ml"]["counts"]
     eng.close()
     r = res["ml"]
     print(f"[bench] mixed+ML: {r['rate']:.4e} cell-stage/s, {1e3 * r['seconds'] / args.steps:.3f} ms/step, "
@@ -206,13 +234,11 @@ def main():
     print(f"[bench] copy probe: 1 GiB in {copy_ms:.3f} ms = {copy_gbs:.0f} GB/s (read+write)", file=sys.stderr, flush=True)
     fp, mx, ml = res["fp64"], res["mixed"], res["ml"]
 
-    def kernel_gbs(r, mode):
-        b = stage_bytes(mode, mesh.dim, True, r["stage_cells"], r["stage_faces"])
-        return b, b / (r["stage_ms"] * 1e-3) / 1e9
+    def kernel_gbs(r):
+        return r["stage_bytes"] / (r["stage_ms"] * 1e-3) / 1e9
 
-    ml_bytes, ml_gbs = kernel_gbs(ml, "mixed")
-    _, mx_gbs = kernel_gbs(mx, "mixed")
-    _, fp_gbs = kernel_gbs(fp, "fp64")
+    ml_bytes, ml_gbs = ml["stage_bytes"], kernel_gbs(ml)
+    mx_gbs, fp_gbs = kernel_gbs(mx), kernel_gbs(fp)
 
     def acc(r):
         return {"CL": r["cl"], "CD": r["cd"],
@@ -257,7 +283,9 @@ def main():
                      "mixed": acc(mx), "mixed_ml": acc(ml)},
         "closure": dict(tinfo, network_cells=on_net, fallback_cells=fallback),
         "roofline": {"bound": "hbm", "achieved": ml_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ml_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": ml_gbs / HBM_PEAK_GBS,
+                     "traffic": (TRAFFIC_BYTES_PER_CELL * ml["stage_cells"]
+                                 if TRAFFIC_BYTES_PER_CELL else None),
                      "kernel": "k_stage<float,2,ML>", "bytes_per_launch": ml_bytes,
                      "avg_launch_ms": ml["stage_ms"], "cells_per_launch": ml["stage_cells"],
                      "copy_probe_GBs": copy_gbs},

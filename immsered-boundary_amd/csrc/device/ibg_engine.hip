@@ -1494,8 +1494,8 @@ ibg_status ibg_closure_init(ibg_ctx *c, int64_t capacity, uint64_t seed) {
   if ((st = cl_alloc(c, &L.X, (size_t)capacity * kMlpIn))) return st;
   if ((st = cl_alloc(c, &L.Z, (size_t)capacity))) return st;
   if ((st = cl_alloc(c, &L.cell, (size_t)capacity))) return st;
-  if ((st = cl_alloc(c, &L.count, 4))) return st;
-  L.mlcount = L.count + 2;
+  if ((st = cl_alloc(c, &L.count, 2))) return st;
+  if ((st = cl_alloc(c, &L.mlcount, (size_t)kMlSlots * kMlSlotStride))) return st;
   if ((st = cl_alloc(c, &L.p, kNParam))) return st;
   if ((st = cl_alloc(c, &L.m, kNParam))) return st;
   if ((st = cl_alloc(c, &L.v, kNParam))) return st;
@@ -1730,10 +1730,16 @@ ibg_status ibg_closure_dataset(ibg_ctx *c, float *x, float *z, int64_t *cell, in
 
 ibg_status ibg_closure_counts(ibg_ctx *c, int64_t *on_net, int64_t *fallback) {
   if (!c || !c->cl.ready) return IBG_ERR_STATE;
-  unsigned long long h[2] = {0, 0};
+  std::vector<unsigned long long> slots((size_t)kMlSlots * kMlSlotStride);
+  const size_t bytes = slots.size() * sizeof(unsigned long long);
   HIPCHK(hipStreamSynchronize(c->s));
-  HIPCHK(hipMemcpy(h, c->cl.mlcount, sizeof(h), hipMemcpyDeviceToHost));
-  HIPCHK(hipMemset(c->cl.mlcount, 0, sizeof(h)));
+  HIPCHK(hipMemcpy(slots.data(), c->cl.mlcount, bytes, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemset(c->cl.mlcount, 0, bytes));
+  unsigned long long h[2] = {0, 0};
+  for (int k = 0; k < kMlSlots; ++k) {
+    h[0] += slots[(size_t)k * kMlSlotStride];
+    h[1] += slots[(size_t)k * kMlSlotStride + 1];
+  }
   if (on_net) *on_net = (int64_t)h[0];
   if (fallback) *fallback = (int64_t)h[1];
   return IBG_OK;

@@ -117,6 +117,10 @@ struct StageArgs {
 };
 
 constexpr int kRpart = 12;   // per tile: max[5], sumsq[5], SA max, SA sumsq
+// learned-closure cell counters: kMlSlots pairs (network, fallback), one
+// 128-byte line apart, each workgroup adding into slot blockIdx % kMlSlots;
+// the reader sums the slots
+constexpr int kMlSlots = 64, kMlSlotStride = 16;
 
 // LDS of the learned-closure part of k_stage (aliases the face arrays,
 // which are dead by then): the weights (W1t, W2p, b1, b2, w3).
@@ -148,6 +152,7 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
   V *sL = reinterpret_cast<V *>(smem_raw + (sizeof(V) + (D == 3 ? sizeof(R) : 0)) * (size_t)ti.nface);
   V *sR = sL + ti.nface;   // (u,v,w,nu) of the low / high cell of each face (SA)
   __shared__ double red[kRpart][kTile / 64];
+  __shared__ unsigned mlc[2];   // learned closure: network / fallback cells of this tile
 
   const BlkInfo &bi = a.blk[ti.blk];
   const DevPhys &P = a.phys[bi.case_id];
@@ -281,6 +286,7 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
   }
   // ---- learned closure: the network for the cells in its range
   if constexpr (ML) {
+    if (threadIdx.x == 0) { mlc[0] = 0u; mlc[1] = 0u; }
     __syncthreads();   // every phase-2a read of the face arrays is done
     __bf16 *mW1t = reinterpret_cast<__bf16 *>(smem_raw);
     __bf16 *mW2p = mW1t + 64 * kX;
@@ -338,11 +344,12 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
       snu = (double)(sa_rest + (R)z * ml_scale);
     }
     if (a.last && a.mlcount) {
-      // cells on the network / on the fallback in this tile
+      // cells on the network / on the fallback in this tile (LDS sums,
+      // added to the device counters once per workgroup below)
       unsigned long long nm = __ballot(use_ml), nf = __ballot(sa_on && !use_ml);
       if (lane == 0) {
-        atomicAdd(&a.mlcount[0], (unsigned long long)__popcll(nm));
-        atomicAdd(&a.mlcount[1], (unsigned long long)__popcll(nf));
+        atomicAdd(&mlc[0], (unsigned)__popcll(nm));
+        atomicAdd(&mlc[1], (unsigned)__popcll(nf));
       }
     }
   }
@@ -464,6 +471,13 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
       for (int w = 0; w < nw; ++w) { m = fmax(m, red[10][w]); q += red[11][w]; }
       o[10] = m;
       o[11] = q;
+      if constexpr (ML) {
+        if (a.mlcount) {
+          unsigned long long *slot = a.mlcount + (size_t)(blockIdx.x % kMlSlots) * kMlSlotStride;
+          if (mlc[0]) atomicAdd(&slot[0], (unsigned long long)mlc[0]);
+          if (mlc[1]) atomicAdd(&slot[1], (unsigned long long)mlc[1]);
+        }
+      }
     }
   }
 }
