@@ -18,9 +18,12 @@ LIB = os.path.join(HERE, "lib")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("IBG_ARCH", "gfx950")
 
-# packed FP32 instructions are disabled for every HIP build of this project
+# packed FP32 instructions are disabled for every HIP build of this project;
+# MFMA accumulators are kept in ordinary VGPRs (no accvgpr moves between the
+# network layers and the activation arithmetic)
 HIP_FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-shared",
-             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-mllvm", "-amdgpu-mfma-vgpr-form"]
 
 
 def _run(cmd):

@@ -144,10 +144,11 @@ __global__ void k_closure_pack(const float *p, __bf16 *W1t, __bf16 *W2t, __bf16 
 __global__ void __launch_bounds__(256) k_closure_infer(MlpDev M, const float *X, float *Zout, long long n) {
   __shared__ __align__(16) __bf16 sW1t[64 * kX];
   __shared__ __align__(16) __bf16 sW2p[64 * kH];
-  __shared__ __align__(16) float sw3[64], sb1[64], sb2[64];
+  __shared__ __align__(16) float sb1[64], sb2[64];
+  __shared__ __align__(16) __bf16 sw3[64];
   for (int i = threadIdx.x; i < 64 * kX; i += blockDim.x) sW1t[i] = M.W1t[i];
   for (int i = threadIdx.x; i < 64 * kH; i += blockDim.x) sW2p[i] = M.W2p[i];
-  for (int i = threadIdx.x; i < 64; i += blockDim.x) { sw3[i] = M.w3[i]; sb1[i] = M.b1[i]; sb2[i] = M.b2[i]; }
+  for (int i = threadIdx.x; i < 64; i += blockDim.x) { sw3[i] = (__bf16)M.w3[i]; sb1[i] = M.b1[i]; sb2[i] = M.b2[i]; }
   __syncthreads();
   const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   bf16x8_t xf;

@@ -290,8 +290,8 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
     __syncthreads();   // every phase-2a read of the face arrays is done
     __bf16 *mW1t = reinterpret_cast<__bf16 *>(smem_raw);
     __bf16 *mW2p = mW1t + 64 * kX;
-    float *mw3 = reinterpret_cast<float *>(mW2p + 64 * kH);
-    float *mb1 = mw3 + 64, *mb2 = mb1 + 64;
+    float *mb1 = reinterpret_cast<float *>(mW2p + 64 * kH), *mb2 = mb1 + 64;
+    __bf16 *mw3 = reinterpret_cast<__bf16 *>(mb2 + 64);
     {
       // weights to LDS in 16-byte chunks (W1t then W2p), all loads issued
       // before the stores
@@ -321,7 +321,7 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
         for (int id = threadIdx.x; id < kWChunks; id += blockDim.x) dst[id] = chunk(id);
       }
       for (int i = threadIdx.x; i < 64; i += blockDim.x) {
-        mw3[i] = a.mlp.w3[i]; mb1[i] = a.mlp.b1[i]; mb2[i] = a.mlp.b2[i];
+        mw3[i] = (__bf16)a.mlp.w3[i]; mb1[i] = a.mlp.b1[i]; mb2[i] = a.mlp.b2[i];
       }
     }
     __syncthreads();

@@ -33,6 +33,7 @@
 namespace ibg {
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kMlpIn = 8;
@@ -157,29 +158,31 @@ __device__ __forceinline__ void mlp_fwd16(const __bf16 *Xs, __bf16 *H1s, __bf16 
 // Transposed forward with the activations in registers (the stage kernel).
 // The wave handles 16 cells per call: the cells of lanes 16*sb .. 16*sb+15,
 // each lane holding its own cell's 8 standardised features (bf16x8 xf).
-// Layer 1 computes Z1^T = W1^T X^T: only the K group sb of the A operand is
-// nonzero (lane (r, q) = lane r + 16 q contributes W1 when q == sb), so the
-// B operand is every lane's own feature vector.  The output of lane (r, q)
+// Layer 1 computes Z1^T = W1^T X^T: every K group of the A operand holds W1
+// and only the K group sb of the B operand is nonzero (lane (r, q) = lane
+// r + 16 q contributes its own feature vector when q == sb).  The output of lane (r, q)
 // is hidden units 16 nt + 4 q + v of cell r of the block.  Layer 2 takes its
 // B operand straight from those registers: the K order of layer 2 is the
 // permutation pi(32 kt + 8 q + e) = 16 (2 kt + e/4) + 4 q + e%4 of the
-// hidden units, applied to W2 once (W2p).  Layer 3 sums the 16 products of
-// a lane and reduces over the 4 lanes of a cell.  Returns y (b3 not
-// included) of cell r in every lane (r, q).  No LDS for activations, no
-// barriers.  W1t/W2p/b1/b2/w3 come from LDS.
+// hidden units, applied to W2 once (W2p).  Layer 3 takes the 16 products of
+// a lane in bf16 pairs (v_dot2_f32_bf16, FP32 accumulate) and reduces over
+// the 4 lanes of a cell.  Returns y (b3 not included) of cell r in every
+// lane (r, q).  No LDS for activations, no barriers.  W1t/W2p/b1/b2 and the
+// bf16 w3 come from LDS.
 __device__ __forceinline__ float mlp_fwd_reg(bf16x8_t xf, int sb, const __bf16 *W1t,
                                              const __bf16 *W2p, const float *b1, const float *b2,
-                                             const float *w3) {
+                                             const __bf16 *w3b) {
   const int lane = threadIdx.x & 63;
   const int r = lane & 15, q = lane >> 4;
   const bf16x8_t zero8 = {};
+  const bf16x8_t xm = (q == sb) ? xf : zero8;
   floatx4_t h1[4];
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) {
     const float4 bb = *reinterpret_cast<const float4 *>(b1 + 16 * nt + 4 * q);
     h1[nt] = floatx4_t{bb.x, bb.y, bb.z, bb.w};
     const bf16x8_t aw = *reinterpret_cast<const bf16x8_t *>(W1t + (16 * nt + r) * kX);
-    h1[nt] = mfma16(q == sb ? aw : zero8, xf, h1[nt]);
+    h1[nt] = mfma16(aw, xm, h1[nt]);
   }
   // silu -> bf16, packed as the layer-2 B fragments (kt = 0: tiles 0,1; kt = 1: tiles 2,3)
   bf16x8_t b2f[2];
@@ -197,9 +200,11 @@ __device__ __forceinline__ float mlp_fwd_reg(bf16x8_t xf, int sb, const __bf16 *
       const bf16x8_t aw = *reinterpret_cast<const bf16x8_t *>(W2p + (16 * nt + r) * kH + 32 * kt + 8 * q);
       acc = mfma16(aw, b2f[kt], acc);
     }
-    const float4 ww = *reinterpret_cast<const float4 *>(w3 + 16 * nt + 4 * q);
-    y += (float)(__bf16)mlp_silu(acc[0]) * ww.x + (float)(__bf16)mlp_silu(acc[1]) * ww.y +
-         (float)(__bf16)mlp_silu(acc[2]) * ww.z + (float)(__bf16)mlp_silu(acc[3]) * ww.w;
+    const bf16x2_t *ww = reinterpret_cast<const bf16x2_t *>(w3b + 16 * nt + 4 * q);
+    const bf16x2_t h01 = {(__bf16)mlp_silu(acc[0]), (__bf16)mlp_silu(acc[1])};
+    const bf16x2_t h23 = {(__bf16)mlp_silu(acc[2]), (__bf16)mlp_silu(acc[3])};
+    y = __builtin_amdgcn_fdot2_f32_bf16(h01, ww[0], y, false);
+    y = __builtin_amdgcn_fdot2_f32_bf16(h23, ww[1], y, false);
   }
   y += __shfl_xor(y, 16, 64);
   y += __shfl_xor(y, 32, 64);
