@@ -43,9 +43,10 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 # a launch: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch) over
 # every dispatch of k_stage<float,2,true> in this script's profiled run,
 # scaled by what the counters read for a plain 1 GiB copy (FETCH_SIZE x2.0,
-# WRITE_SIZE x1.0), divided by the cells of the launch.  Recorded in
-# profiles/k_stage_ml_naca2d_traffic.md; None until profiled.
-TRAFFIC_BYTES_PER_CELL = None
+# WRITE_SIZE x1.0), divided by the cells of the launch (the launch mix of the
+# timed steps).  Recorded in profiles/k_stage_naca2d_sa_traffic.md; update it
+# with the kernel.
+TRAFFIC_BYTES_PER_CELL = 192.3
 
 
 def stage_bytes(mode, dim, turb, cells, faces, last=False):
