@@ -49,25 +49,30 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 TRAFFIC_BYTES_PER_CELL = 192.3
 
 
-def stage_bytes(mode, dim, turb, cells, faces, last=False):
+def stage_bytes(mode, dim, turb, cells, faces, last=False, first=False):
     """Algorithmic HBM bytes of one stage-kernel launch.
 
     Per normal cell: its stencil state read once (neighbour reads hit in
     cache) and written once, the frozen viscous residual, the FP64 master
     read for U_old, the face slots (2 B per cell face) and the flag byte;
     with the SA closure the working variable read and written, its FP64
-    master (the old value), the frozen face term and the wall distance.
-    The last stage of a step in mixed mode also writes the FP64 masters.
-    Per tile face the 4 stencil indices and the direction byte.
+    master (the old value), the frozen face term and the wall distance
+    (FP32 in the mixed mode).  The last stage of a step in mixed mode also
+    writes the FP64 masters.  Per tile face the 4 stencil indices and the
+    direction byte.  The first stage of a 2D FP64 step also computes the
+    frozen viscous residual and face term: it writes them instead of reading
+    them (same bytes) and reads the tangential viscous stencil (16 B per
+    face).
     """
     nv = dim + 2
     sw = 8 if mode == "fp64" else 4
     per_cell = 3 * nv * sw + nv * 8 + 2 * dim * 2 + 1
     if turb:
-        per_cell += 3 * sw + 8 + 8
+        per_cell += 3 * sw + 8 + (8 if mode == "fp64" else 4)
     if last and mode != "fp64":
         per_cell += nv * 8 + (8 if turb else 0)
-    return cells * per_cell + faces * 17
+    per_face = 17 + (16 if first and dim == 2 and mode == "fp64" else 0)
+    return cells * per_cell + faces * per_face
 
 
 def stage_mix(eng, mesh, mode, ts0, nstage, reps):
@@ -81,7 +86,7 @@ def stage_mix(eng, mesh, mode, ts0, nstage, reps):
         for k in range(nstage):
             ms, nc, nf = eng.bench_stage(ts, k, reps)
             ms_sum += ms
-            by_sum += stage_bytes(mode, mesh.dim, True, nc, nf, last=(k == nstage - 1))
+            by_sum += stage_bytes(mode, mesh.dim, True, nc, nf, last=(k == nstage - 1), first=(k == 0))
             c_sum += nc
             n += 1
     return ms_sum / n, by_sum / n, c_sum / n

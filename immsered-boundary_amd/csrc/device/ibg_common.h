@@ -28,8 +28,21 @@ enum CellFlag : uint8_t {
   F_PRIM = 2,     // ComptNewPrimVar recovers W from U
   F_FREE = 4,     // ComptNewPrimVar resets W to initvar
   F_RESID = 8,    // counted by ResinDomain
-  F_FINEST = 16   // finest level with bodies (abort code 611 vs 523)
+  F_FINEST = 16,  // finest level with bodies (abort code 611 vs 523)
+  F_SA = 32       // signdis >= 0: ComptMutSource evaluates the cell
 };
+
+// Tile of this workgroup for the tiled stage and viscous kernels.  The
+// workgroups of a launch are dispatched round-robin over the 8 XCDs
+// (blockIdx % 8), each with its own L2; XCD x takes one contiguous run of the
+// (Morton-ordered) tiles, so a tile and the neighbour tiles its stencils read
+// share an L2.  A bijection of [0, gridDim.x).
+__device__ __forceinline__ int xcd_tile() {
+  constexpr int kXcd = 8;
+  const int n = (int)gridDim.x, b = (int)blockIdx.x;
+  const int per = n / kXcd, rem = n % kXcd, x = b % kXcd;
+  return x * per + (x < rem ? x : rem) + b / kXcd;
+}
 
 struct DevPhys {
   double gama, Ma, Re, Rg, Cv, Cp, mu, qk, SoT;

@@ -98,6 +98,7 @@ struct ibg_ctx {
   int *ib_cell = nullptr, *ib_intp = nullptr, *ib_body = nullptr, *ib_case = nullptr;
   double *ib_coef = nullptr, *ib_hgdis = nullptr, *ib_attach = nullptr, *ib_nv = nullptr,
          *signdis = nullptr;
+  float *sd32 = nullptr;   // signdis in FP32 (the mixed SA source)
   int *pat_intp = nullptr, *pat_case = nullptr;
   double *pat_coef = nullptr, *pat_hgdis = nullptr, *pat_nv = nullptr, *pat_pc = nullptr,
          *pat_area = nullptr, *case_center = nullptr;
@@ -231,9 +232,23 @@ void stage_bufs(int nstage, int subt, int *in, int *out) {
   }
 }
 
+template <typename R, int D>
+ViscArgs<R, D> visc_args(ibg_ctx *c) {
+  ViscArgs<R, D> a;
+  a.tiles = c->tiles; a.tst = c->tst; a.tvs = c->tvs; a.tdir = c->tdir; a.slots = c->slots;
+  a.blk = c->blk; a.phys = c->phys;
+  a.W = c->sbuf<R, D>(0);
+  a.Rv = c->rv<R, D>();
+  a.nu = c->nubuf<R>(0);
+  a.mutf = (R *)c->mutf;
+  a.viscous = c->cfg.viscous;
+  a.turb = c->turb ? 1 : 0;
+  return a;
+}
+
 template <typename R, int D, bool ML>
 ibg_status launch_stage(ibg_ctx *c, int ntile, int ncell, int subt, bool last, double *src,
-                        double *nusrc) {
+                        double *nusrc, bool fuse) {
   if (ntile <= 0) return IBG_OK;
   int bin = 0, bout = 0;
   stage_bufs(c->cfg.nstage, subt, &bin, &bout);
@@ -250,6 +265,7 @@ ibg_status launch_stage(ibg_ctx *c, int ntile, int ncell, int subt, bool last, d
   a.nu64 = c->nu64;
   a.mutf = (const R *)c->mutf;
   a.signdis = c->signdis;
+  a.sd32 = c->sd32;
   a.mlp = c->mlpdev();
   a.mlcount = c->cl.mlcount;
   a.alpha = c->cfg.alpha[subt];
@@ -259,6 +275,9 @@ ibg_status launch_stage(ibg_ctx *c, int ntile, int ncell, int subt, bool last, d
   a.src = src;
   a.nusrc = nusrc;
   a.err = c->err;
+  // the step's first stage reads the home buffer S0, the state k_visc_prep reads
+  a.fuse_visc = (fuse && subt == 0 && kFuseVisc<R, D>) ? 1 : 0;
+  a.va = visc_args<R, D>(c);
   hipLaunchKernelGGL((k_stage<R, D, ML>), dim3(ntile), dim3(c->tile), tile_smem(c, sizeof(R), true), c->s, a);
   HIPCHK(hipGetLastError());
   if (c->cfg.nstage == 1) {
@@ -271,33 +290,30 @@ ibg_status launch_stage(ibg_ctx *c, int ntile, int ncell, int subt, bool last, d
   return IBG_OK;
 }
 
-ibg_status run_stage(ibg_ctx *c, int ts, int subt, double *src, double *nusrc) {
+// fuse: the stage-0 launch also computes the frozen viscous residual (the
+// time march does this; ibg_stage keeps the separate ibg_viscous_flux call)
+ibg_status run_stage(ibg_ctx *c, int ts, int subt, double *src, double *nusrc, bool fuse = false) {
   const int lm = lmin_of(c, ts);
   const int nt = c->tile_end[lm], ncell = c->cell_end[lm];
   const bool last = (subt == c->cfg.nstage - 1);
   if (last) c->last_ntiles = nt;
   if (c->cl.on)   // learned closure: mixed mode with the SA closure only
-    return c->D == 2 ? launch_stage<float, 2, true>(c, nt, ncell, subt, last, src, nusrc)
-                     : launch_stage<float, 3, true>(c, nt, ncell, subt, last, src, nusrc);
+    return c->D == 2 ? launch_stage<float, 2, true>(c, nt, ncell, subt, last, src, nusrc, fuse)
+                     : launch_stage<float, 3, true>(c, nt, ncell, subt, last, src, nusrc, fuse);
   if (c->D == 2)
-    return c->mixed ? launch_stage<float, 2, false>(c, nt, ncell, subt, last, src, nusrc)
-                    : launch_stage<double, 2, false>(c, nt, ncell, subt, last, src, nusrc);
-  return c->mixed ? launch_stage<float, 3, false>(c, nt, ncell, subt, last, src, nusrc)
-                  : launch_stage<double, 3, false>(c, nt, ncell, subt, last, src, nusrc);
+    return c->mixed ? launch_stage<float, 2, false>(c, nt, ncell, subt, last, src, nusrc, fuse)
+                    : launch_stage<double, 2, false>(c, nt, ncell, subt, last, src, nusrc, fuse);
+  return c->mixed ? launch_stage<float, 3, false>(c, nt, ncell, subt, last, src, nusrc, fuse)
+                  : launch_stage<double, 3, false>(c, nt, ncell, subt, last, src, nusrc, fuse);
 }
+
+// whether the time march folds k_visc_prep into the first stage
+inline bool fused_visc(const ibg_ctx *c) { return c->D == 2 && !c->mixed; }
 
 template <typename R, int D>
 ibg_status launch_visc_prep(ibg_ctx *c, int ntile) {
   if (ntile <= 0) return IBG_OK;
-  ViscArgs<R, D> a;
-  a.tiles = c->tiles; a.tst = c->tst; a.tvs = c->tvs; a.tdir = c->tdir; a.slots = c->slots;
-  a.blk = c->blk; a.phys = c->phys;
-  a.W = c->sbuf<R, D>(0);
-  a.Rv = c->rv<R, D>();
-  a.nu = c->nubuf<R>(0);
-  a.mutf = (R *)c->mutf;
-  a.viscous = c->cfg.viscous;
-  a.turb = c->turb ? 1 : 0;
+  const ViscArgs<R, D> a = visc_args<R, D>(c);
   hipLaunchKernelGGL((k_visc_prep<R, D>), dim3(ntile), dim3(c->tile), tile_smem(c, sizeof(R), false), c->s, a);
   HIPCHK(hipGetLastError());
   return IBG_OK;
@@ -398,9 +414,11 @@ ibg_status run_step(ibg_ctx *c, int ts) {
   if (c->cfg.dt_var_step > 0 && ts % c->cfg.dt_var_step == 0)
     if ((st = run_dt(c))) return st;
   if ((st = run_bc(c, ts))) return st;
-  if ((st = run_visc_prep(c, ts))) return st;
+  const bool fuse = fused_visc(c);
+  if (!fuse)
+    if ((st = run_visc_prep(c, ts))) return st;
   for (int k = 0; k < c->cfg.nstage; ++k)
-    if ((st = run_stage(c, ts, k, nullptr, nullptr))) return st;
+    if ((st = run_stage(c, ts, k, nullptr, nullptr, fuse))) return st;
   return IBG_OK;
 }
 
@@ -712,6 +730,7 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
       if (m->signdis[g] > 0.0 && !ib) f |= F_PRIM | F_RESID;
       else if (!ib) f |= F_FREE;
     }
+    if (m->signdis[g] >= 0.0) f |= F_SA;
     flags[i] = f;
   }
   std::vector<int> cfaces((size_t)Nn * 2 * D);
@@ -946,6 +965,10 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
   UP(c->ib_cell, ib_cell); UP(c->ib_intp, ib_intp); UP(c->ib_body, ib_body); UP(c->ib_case, ib_case);
   UP(c->ib_coef, ib_coef); UP(c->ib_hgdis, ib_hgdis); UP(c->ib_attach, ib_att); UP(c->ib_nv, ib_nvh);
   UP(c->signdis, sd);
+  {
+    std::vector<float> sd32(sd.begin(), sd.end());
+    UP(c->sd32, sd32);
+  }
   UP(c->pat_intp, pat_intp); UP(c->pat_case, pat_case); UP(c->pat_coef, pat_coef);
   UP(c->pat_hgdis, pat_hgdis); UP(c->pat_nv, pat_nv); UP(c->pat_pc, pat_pc); UP(c->pat_area, pat_area);
   UP(c->case_center, case_center);
@@ -1358,7 +1381,7 @@ ibg_status ibg_bench_stage(ibg_ctx *c, int32_t ts, int32_t subt, int32_t reps, d
   ibg_status st;
   HIPCHK(hipEventRecord(c->ev0, c->s));
   for (int r = 0; r < reps; ++r)
-    if ((st = run_stage(c, ts, subt, nullptr, nullptr))) return st;
+    if ((st = run_stage(c, ts, subt, nullptr, nullptr, fused_visc(c)))) return st;
   HIPCHK(hipEventRecord(c->ev1, c->s));
   HIPCHK(hipEventSynchronize(c->ev1));
   float ms = 0.f;

@@ -83,6 +83,120 @@ __device__ __forceinline__ void report_error(unsigned long long *err, int cell, 
 }
 
 // ------------------------------------------------------------ stage kernel
+// ----------------------------------------- viscous flux + step preparation
+template <typename R, int D>
+struct ViscArgs {
+  const TileInfo *tiles;
+  const int4 *tst;           // L, R from .y, .z
+  const int4 *tvs;           // [nface][D-1] tangential stencil
+  const int8_t *tdir;
+  const uint16_t *slots;
+  const BlkInfo *blk;
+  const DevPhys *phys;
+  Pack<R, D> W;              // step-start stencil state
+  Pack<R, D> Rv;
+  const R *nu;               // SA working variable (turb)
+  R *mutf;                   // [Nn] frozen face-flux term (turb)
+  int viscous, turb;
+};
+
+// Once per step (ComptViscousStress NS_Solver.cpp:365-629 is frozen across
+// the stages, NS_Solver.H:669-699): viscous fluxes of the tile faces into
+// LDS, then per cell Rv = sum_d (Fv_hi - Fv_lo).  With the SA closure the
+// face flux of the working variable (mutflux, :534-542) is reduced the same
+// way into sum_d (mutflux_hi - mutflux_lo) / V / Re (:791-800).
+template <typename R, int D>
+__device__ __forceinline__ void visc_prep_tile(const ViscArgs<R, D> &a, const TileInfo &ti,
+                                               unsigned char *smem_raw) {
+  constexpr int NV = D + 2;
+  using V = typename V4<R>::T;
+  V *fl4 = reinterpret_cast<V *>(smem_raw);
+  R *fl1 = reinterpret_cast<R *>(smem_raw + sizeof(V) * (size_t)ti.nface);
+  R *mfl = reinterpret_cast<R *>(smem_raw + (sizeof(V) + (D == 3 ? sizeof(R) : 0)) * (size_t)ti.nface);
+  const BlkInfo &bi = a.blk[ti.blk];
+  const DevPhys &P = a.phys[bi.case_id];
+  if (a.viscous) {
+    ViscPhys<R, D> vp;
+    vp.Rg = (R)P.Rg; vp.Re = (R)P.Re; vp.qk = (R)P.qk; vp.SoT = (R)P.SoT;
+    vp.ih0 = (R)bi.invh[0]; vp.ih1 = (R)bi.invh[1]; vp.ih2 = (R)bi.invh[2];
+    vp.ar0 = (R)bi.area[0]; vp.ar1 = (R)bi.area[1]; vp.ar2 = (R)bi.area[2];
+    vp.qkt = (R)P.qkt_den;
+    for (int f = threadIdx.x; f < ti.nface; f += blockDim.x) {
+      const int4 st = a.tst[ti.face0 + f];
+      const int dir = a.tdir[ti.face0 + f];
+      const Prim<R, D> L = pload<R, D>(a.W, st.y);
+      const Prim<R, D> Rr = pload<R, D>(a.W, st.z);
+      Prim<R, D> t[(D == 3) ? 8 : 4];
+      const int4 v0 = a.tvs[(size_t)(ti.face0 + f) * (D - 1)];
+      t[0] = pload<R, D>(a.W, v0.x); t[1] = pload<R, D>(a.W, v0.y);
+      t[2] = pload<R, D>(a.W, v0.z); t[3] = pload<R, D>(a.W, v0.w);
+      if (D == 3) {
+        const int4 v1 = a.tvs[(size_t)(ti.face0 + f) * (D - 1) + 1];
+        t[4 % ((D == 3) ? 8 : 4)] = pload<R, D>(a.W, v1.x);
+        t[5 % ((D == 3) ? 8 : 4)] = pload<R, D>(a.W, v1.y);
+        t[6 % ((D == 3) ? 8 : 4)] = pload<R, D>(a.W, v1.z);
+        t[7 % ((D == 3) ? 8 : 4)] = pload<R, D>(a.W, v1.w);
+      }
+      R out[NV];
+      R mf = R(0);
+      if (a.turb)
+        visc_flux<R, D>(L, Rr, t, dir, vp, out, true, a.nu[st.y], a.nu[st.z], &mf);
+      else
+        visc_flux<R, D>(L, Rr, t, dir, vp, out);
+      V o;
+      o.x = out[0]; o.y = out[1]; o.z = out[2]; o.w = out[3];
+      fl4[f] = o;
+      if (D == 3) fl1[f] = out[4];
+      if (a.turb) mfl[f] = mf;
+    }
+  }
+  __syncthreads();
+  const int i = threadIdx.x;
+  if (i >= ti.ncell) return;   // no barrier follows in this function
+  const int c = ti.cell0 + i;
+  double s[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) s[k] = 0.0;
+  double tot = 0.0;
+  if (a.viscous) {
+    int sl[2 * D];
+    if (D == 2) {
+      const ushort4 s4 = *reinterpret_cast<const ushort4 *>(a.slots + (size_t)c * 4);
+      sl[0] = s4.x; sl[1] = s4.y; sl[2] = s4.z; sl[3] = s4.w;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 2 * D; ++q) sl[q] = a.slots[(size_t)c * 2 * D + q];
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int lo = sl[2 * d], hi = sl[2 * d + 1];
+      const V flo = fl4[lo], fhi = fl4[hi];
+      s[0] += (double)fhi.x - (double)flo.x;
+      s[1] += (double)fhi.y - (double)flo.y;
+      s[2] += (double)fhi.z - (double)flo.z;
+      s[3] += (double)fhi.w - (double)flo.w;
+      if (D == 3) s[4] += (double)fl1[hi] - (double)fl1[lo];
+      if (a.turb) {
+        // total_face_flux += dirsign[f0] * mutflux, dirsign = {-1, +1}
+        tot += -(double)mfl[lo];
+        tot += (double)mfl[hi];
+      }
+    }
+  }
+  R rv[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) rv[k] = (R)s[k];
+  vstore<R, D>(a.Rv, c, rv);
+  if (a.turb) a.mutf[c] = (R)(tot / bi.vol / P.Re);
+}
+
+template <typename R, int D>
+__global__ void __launch_bounds__(kTile) k_visc_prep(ViscArgs<R, D> a) {
+  extern __shared__ __align__(32) unsigned char smem_raw[];
+  const TileInfo ti = a.tiles[xcd_tile()];
+  visc_prep_tile<R, D>(a, ti, smem_raw);
+}
+
 template <typename R, int D>
 struct StageArgs {
   const TileInfo *tiles;
@@ -104,6 +218,7 @@ struct StageArgs {
   double *nu64;
   const R *mutf;             // [Nn] frozen face-flux term of the step
   const double *signdis;
+  const float *sd32;         // signdis in FP32 (mixed mode)
   int turb;
   MlpDev mlp;                // learned closure (the ML instantiation only)
   unsigned long long *mlcount;   // [2] cells on the network / on the fallback
@@ -114,7 +229,21 @@ struct StageArgs {
   double *src;               // optional probe [N][5] written at the last stage
   double *nusrc;             // optional probe [N] (SA source, last stage)
   unsigned long long *err;
+  // first stage of a step: the frozen viscous residual of the tile first
+  // (the work of k_visc_prep, into the same Rv / mutf arrays)
+  int fuse_visc;
+  ViscArgs<R, D> va;
 };
+
+// The fused first stage for the 2D FP64 variant only.  There the viscous
+// part needs fewer registers than the stage and the fused launch saves most
+// of the separate pass (1.14 -> 1.10 ms per step on the 2.6M-cell NACA0015
+// SA case).  In the FP32 variants the separate k_visc_prep runs at a higher
+// occupancy than the stage kernel, and fusing measured even (plain closure)
+// or slower (learned closure: +100 us on the first stage); in 3D it would
+// lower the occupancy of every stage launch.
+template <typename R, int D>
+constexpr bool kFuseVisc = (D == 2 && std::is_same<R, double>::value);
 
 constexpr int kRpart = 12;   // per tile: max[5], sumsq[5], SA max, SA sumsq
 // learned-closure cell counters: kMlSlots pairs (network, fallback), one
@@ -146,7 +275,17 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
   constexpr bool kMixed = std::is_same<R, float>::value;
   using V = typename V4<R>::T;
   extern __shared__ __align__(32) unsigned char smem_raw[];
-  const TileInfo ti = a.tiles[blockIdx.x];
+  const int tile = xcd_tile();
+  const TileInfo ti = a.tiles[tile];
+  if constexpr (kFuseVisc<R, D>) {
+    if (a.fuse_visc) {
+      // Rv and mutf of this tile's cells from the step-start state (the
+      // buffer this stage reads); each thread reads back only its own
+      // cell's entries in phase 2, so no other workgroup is involved
+      visc_prep_tile<R, D>(a.va, ti, smem_raw);
+      __syncthreads();   // the face arrays are reused by phase 1
+    }
+  }
   V *fl4 = reinterpret_cast<V *>(smem_raw);
   R *fl1 = reinterpret_cast<R *>(smem_raw + sizeof(V) * (size_t)ti.nface);   // 3D 5th value
   V *sL = reinterpret_cast<V *>(smem_raw + (sizeof(V) + (D == 3 ? sizeof(R) : 0)) * (size_t)ti.nface);
@@ -233,7 +372,7 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
     }
     // ComptMutSource for a non-solid updated cell (zero otherwise: the
     // reference never writes mutsource of solid cells)
-    if (a.turb && (fg & F_UPDATE) && a.signdis[c] >= 0.0) {
+    if (a.turb && (fg & F_UPDATE) && (fg & F_SA)) {
       sa_on = true;
       const Prim<R, D> me = pload<R, D>(a.Win, c);
       R nl[3], nr[3], ul[3], ur[3], vl[3], vr[3], wl[3], wr[3];
@@ -249,7 +388,7 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
       }
       const R T = v_div(me.p, me.r * (R)P.Rg);
       const R nu = a.nuin[c];
-      const R sd = (R)a.signdis[c];
+      const R sd = kMixed ? (R)a.sd32[c] : (R)a.signdis[c];
       if constexpr (!ML) {
         snu = (double)sa_source<R, D>(me.u, me.v, me.w, T, me.r, nu, sd, nl, nr, ul, ur, vl, vr,
                                       wl, wr, (R)bi.invh[0], (R)bi.invh[1], (R)bi.invh[2],
@@ -331,11 +470,14 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
     for (int k = 0; k < kMlpIn; ++k)
       xf[k] = (__bf16)(use_ml ? ((float)ml_x[k] - a.mlp.fmean[k]) / a.mlp.fstd[k] : 0.0f);
     float yown = 0.0f;
-    // one 16-cell block per pass; the compiler barrier keeps the weight
+    // one 16-cell block per pass, skipped (wave-uniformly) when none of its
+    // cells is on the network; the compiler barrier keeps the weight
     // fragments from being hoisted out of the loop (96 registers if they were)
+    const unsigned long long on = __ballot(use_ml);
 #pragma unroll 1
     for (int sb = 0; sb < 4; ++sb) {
       asm volatile("" ::: "memory");
+      if (((on >> (16 * sb)) & 0xffffull) == 0ull) continue;
       const float y = mlp_fwd_reg(xf, sb, mW1t, mW2p, mb1, mb2, mw3);
       if ((lane >> 4) == sb) yown = y;
     }
@@ -458,7 +600,7 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
     __syncthreads();
     if (threadIdx.x == 0) {
       const int nw = (blockDim.x + 63) >> 6;
-      double *o = a.rpart + (size_t)blockIdx.x * kRpart;
+      double *o = a.rpart + (size_t)tile * kRpart;
       for (int k = 0; k < kRpart; ++k) o[k] = 0.0;
       for (int k = 0; k < NV; ++k) {
         double m = 0.0, q = 0.0;
@@ -480,114 +622,6 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
       }
     }
   }
-}
-
-// ----------------------------------------- viscous flux + step preparation
-template <typename R, int D>
-struct ViscArgs {
-  const TileInfo *tiles;
-  const int4 *tst;           // L, R from .y, .z
-  const int4 *tvs;           // [nface][D-1] tangential stencil
-  const int8_t *tdir;
-  const uint16_t *slots;
-  const BlkInfo *blk;
-  const DevPhys *phys;
-  Pack<R, D> W;              // step-start stencil state
-  Pack<R, D> Rv;
-  const R *nu;               // SA working variable (turb)
-  R *mutf;                   // [Nn] frozen face-flux term (turb)
-  int viscous, turb;
-};
-
-// Once per step (ComptViscousStress NS_Solver.cpp:365-629 is frozen across
-// the stages, NS_Solver.H:669-699): viscous fluxes of the tile faces into
-// LDS, then per cell Rv = sum_d (Fv_hi - Fv_lo).  With the SA closure the
-// face flux of the working variable (mutflux, :534-542) is reduced the same
-// way into sum_d (mutflux_hi - mutflux_lo) / V / Re (:791-800).
-template <typename R, int D>
-__global__ void __launch_bounds__(kTile) k_visc_prep(ViscArgs<R, D> a) {
-  constexpr int NV = D + 2;
-  using V = typename V4<R>::T;
-  extern __shared__ __align__(32) unsigned char smem_raw[];
-  const TileInfo ti = a.tiles[blockIdx.x];
-  V *fl4 = reinterpret_cast<V *>(smem_raw);
-  R *fl1 = reinterpret_cast<R *>(smem_raw + sizeof(V) * (size_t)ti.nface);
-  R *mfl = reinterpret_cast<R *>(smem_raw + (sizeof(V) + (D == 3 ? sizeof(R) : 0)) * (size_t)ti.nface);
-  const BlkInfo &bi = a.blk[ti.blk];
-  const DevPhys &P = a.phys[bi.case_id];
-  if (a.viscous) {
-    ViscPhys<R, D> vp;
-    vp.Rg = (R)P.Rg; vp.Re = (R)P.Re; vp.qk = (R)P.qk; vp.SoT = (R)P.SoT;
-    vp.ih0 = (R)bi.invh[0]; vp.ih1 = (R)bi.invh[1]; vp.ih2 = (R)bi.invh[2];
-    vp.ar0 = (R)bi.area[0]; vp.ar1 = (R)bi.area[1]; vp.ar2 = (R)bi.area[2];
-    vp.qkt = (R)P.qkt_den;
-    for (int f = threadIdx.x; f < ti.nface; f += blockDim.x) {
-      const int4 st = a.tst[ti.face0 + f];
-      const int dir = a.tdir[ti.face0 + f];
-      const Prim<R, D> L = pload<R, D>(a.W, st.y);
-      const Prim<R, D> Rr = pload<R, D>(a.W, st.z);
-      Prim<R, D> t[(D == 3) ? 8 : 4];
-      const int4 v0 = a.tvs[(size_t)(ti.face0 + f) * (D - 1)];
-      t[0] = pload<R, D>(a.W, v0.x); t[1] = pload<R, D>(a.W, v0.y);
-      t[2] = pload<R, D>(a.W, v0.z); t[3] = pload<R, D>(a.W, v0.w);
-      if (D == 3) {
-        const int4 v1 = a.tvs[(size_t)(ti.face0 + f) * (D - 1) + 1];
-        t[4 % ((D == 3) ? 8 : 4)] = pload<R, D>(a.W, v1.x);
-        t[5 % ((D == 3) ? 8 : 4)] = pload<R, D>(a.W, v1.y);
-        t[6 % ((D == 3) ? 8 : 4)] = pload<R, D>(a.W, v1.z);
-        t[7 % ((D == 3) ? 8 : 4)] = pload<R, D>(a.W, v1.w);
-      }
-      R out[NV];
-      R mf = R(0);
-      if (a.turb)
-        visc_flux<R, D>(L, Rr, t, dir, vp, out, true, a.nu[st.y], a.nu[st.z], &mf);
-      else
-        visc_flux<R, D>(L, Rr, t, dir, vp, out);
-      V o;
-      o.x = out[0]; o.y = out[1]; o.z = out[2]; o.w = out[3];
-      fl4[f] = o;
-      if (D == 3) fl1[f] = out[4];
-      if (a.turb) mfl[f] = mf;
-    }
-  }
-  __syncthreads();
-  const int i = threadIdx.x;
-  if (i >= ti.ncell) return;
-  const int c = ti.cell0 + i;
-  double s[NV];
-#pragma unroll
-  for (int k = 0; k < NV; ++k) s[k] = 0.0;
-  double tot = 0.0;
-  if (a.viscous) {
-    int sl[2 * D];
-    if (D == 2) {
-      const ushort4 s4 = *reinterpret_cast<const ushort4 *>(a.slots + (size_t)c * 4);
-      sl[0] = s4.x; sl[1] = s4.y; sl[2] = s4.z; sl[3] = s4.w;
-    } else {
-#pragma unroll
-      for (int q = 0; q < 2 * D; ++q) sl[q] = a.slots[(size_t)c * 2 * D + q];
-    }
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      const int lo = sl[2 * d], hi = sl[2 * d + 1];
-      const V flo = fl4[lo], fhi = fl4[hi];
-      s[0] += (double)fhi.x - (double)flo.x;
-      s[1] += (double)fhi.y - (double)flo.y;
-      s[2] += (double)fhi.z - (double)flo.z;
-      s[3] += (double)fhi.w - (double)flo.w;
-      if (D == 3) s[4] += (double)fl1[hi] - (double)fl1[lo];
-      if (a.turb) {
-        // total_face_flux += dirsign[f0] * mutflux, dirsign = {-1, +1}
-        tot += -(double)mfl[lo];
-        tot += (double)mfl[hi];
-      }
-    }
-  }
-  R rv[NV];
-#pragma unroll
-  for (int k = 0; k < NV; ++k) rv[k] = (R)s[k];
-  vstore<R, D>(a.Rv, c, rv);
-  if (a.turb) a.mutf[c] = (R)(tot / bi.vol / P.Re);
 }
 
 // Stages with a single substep write into buffer 1; this copies the marched

@@ -149,6 +149,40 @@ def test_graph_replay_equals_eager(tiny):
     assert np.array_equal(a.get_state(), b.get_state())
 
 
+@pytest.mark.parametrize("mode,turb", [("fp64", False), ("mixed", False), ("fp64", True), ("mixed", True)])
+def test_fused_first_stage_equals_phase_calls(tiny, mode, turb):
+    """The time march folds ComptViscousStress (the frozen viscous residual
+    and the SA face term) into the first stage launch in 2D FP64; the
+    phase-by-phase ABI calls keep it a launch of its own.  Same bits (the
+    mixed cases check that the unfused march does too)."""
+    fv0 = perturbed_state(tiny)
+    nstage = 4
+    P = 1 << (tiny.nlevels - 1)
+
+    def make():
+        e = ibgpu.Engine(tiny, mode=mode, nstage=nstage, turbulence=turb)
+        e.set_state(fv0)
+        if turb:
+            e.set_nut(np.full(tiny.ncells, tiny.phys[0].nut_inf))
+        return e
+
+    a = make()
+    a.TimeMarching(0, P + 3, use_graph=False)
+    b = make()
+    for ts in range(P + 3):
+        b.InterfaceExchange(ts)
+        b.IBConditions_Image()
+        if ts % P == 0:
+            b.Comptdt(ts)
+        b.BCBoxTreat(ts)
+        b.ComptViscousStress(ts)
+        for k in range(nstage):
+            b.Stage(ts, k)
+    assert np.array_equal(a.get_state(), b.get_state())
+    if turb:
+        assert np.array_equal(a.get_nut(), b.get_nut())
+
+
 def test_mixed_mode_error_bounded(tiny):
     fv0 = tiny.init_state()
     nsteps = 40
