@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(256) k_closure_infer(MlpDev M, const float *X,
   bf16x8_t xf;
 #pragma unroll
   for (int k = 0; k < kMlpIn; ++k)
-    xf[k] = (__bf16)(row < n ? (X[row * kMlpIn + k] - M.fmean[k]) / M.fstd[k] : 0.0f);
+    xf[k] = (__bf16)(row < n ? (X[row * kMlpIn + k] - M.fmean[k]) * M.finv[k] : 0.0f);
   const int lane = threadIdx.x & 63;
   float yown = 0.0f;
 #pragma unroll 1
@@ -205,7 +205,7 @@ __global__ void __launch_bounds__(256) k_closure_train(TrainArgs a) {
     st[threadIdx.x] = t;
     for (int k = 0; k < kX; ++k) {
       float v = 0.0f;
-      if (k < kMlpIn && row < a.n) v = (a.X[row * kMlpIn + k] - M.fmean[k]) / M.fstd[k];
+      if (k < kMlpIn && row < a.n) v = (a.X[row * kMlpIn + k] - M.fmean[k]) * M.finv[k];
       sX[threadIdx.x * kX + k] = (__bf16)v;
     }
   }

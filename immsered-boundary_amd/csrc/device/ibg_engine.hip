@@ -77,7 +77,7 @@ struct ibg_ctx {
     float *p = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr, *lossp = nullptr, *gsum = nullptr;
     __bf16 *W1t = nullptr, *W2t = nullptr, *W2 = nullptr, *W2p = nullptr;
     float *w3 = nullptr, *b1 = nullptr, *b2 = nullptr, *b3 = nullptr;
-    float *stat = nullptr;    // fmean[8] fstd[8] flo[8] fhi[8] zst[2]
+    float *stat = nullptr;    // fmean[8] fstd[8] flo[8] fhi[8] zst[2], then 1/fstd[8]
     double *mom = nullptr;
     int step = 0, max_wg = 0;
     std::vector<void *> allocs;
@@ -121,6 +121,7 @@ struct ibg_ctx {
     M.b1 = cl.b1; M.b2 = cl.b2; M.b3 = cl.b3;
     M.fmean = cl.stat; M.fstd = cl.stat + 8; M.flo = cl.stat + 16; M.fhi = cl.stat + 24;
     M.zst = cl.stat + 32;
+    M.finv = cl.stat + 34;
     return M;
   }
   template <typename R, int D> Pack<R, D> rv() const {
@@ -1506,6 +1507,16 @@ ibg_status cl_train_launch(ibg_ctx *c, const float *X, const float *Z, long long
 
 extern "C" {
 
+// the 34 statistics plus the reciprocal standard deviations (IEEE FP32
+// division, the value the standardisation multiplies by)
+static ibg_status cl_put_stats(ibg_ctx *c, const float *st34) {
+  float h[42];
+  for (int k = 0; k < 34; ++k) h[k] = st34[k];
+  for (int k = 0; k < 8; ++k) h[34 + k] = 1.0f / st34[8 + k];
+  HIPCHK(hipMemcpy(c->cl.stat, h, sizeof(h), hipMemcpyHostToDevice));
+  return IBG_OK;
+}
+
 ibg_status ibg_closure_init(ibg_ctx *c, int64_t capacity, uint64_t seed) {
   if (!c || capacity < 64) return IBG_ERR_ARG;
   for (void *p : c->cl.allocs) (void)hipFree(p);
@@ -1533,7 +1544,7 @@ ibg_status ibg_closure_init(ibg_ctx *c, int64_t capacity, uint64_t seed) {
   if ((st = cl_alloc(c, &L.b1, 64))) return st;
   if ((st = cl_alloc(c, &L.b2, 64))) return st;
   if ((st = cl_alloc(c, &L.b3, 1))) return st;
-  if ((st = cl_alloc(c, &L.stat, 34))) return st;
+  if ((st = cl_alloc(c, &L.stat, 42))) return st;
   if ((st = cl_alloc(c, &L.mom, (size_t)256 * (kMlpIn + 1) * 4))) return st;
   // weights U(-1/sqrt(fan_in), 1/sqrt(fan_in)), biases 0 (seeded)
   std::vector<float> p(kNParam, 0.0f);
@@ -1549,7 +1560,7 @@ ibg_status ibg_closure_init(ibg_ctx *c, int64_t capacity, uint64_t seed) {
   std::vector<float> st0(34, 0.0f);
   for (int k = 0; k < 8; ++k) { st0[8 + k] = 1.0f; st0[16 + k] = -1e30f; st0[24 + k] = 1e30f; }
   st0[33] = 1.0f;
-  HIPCHK(hipMemcpy(L.stat, st0.data(), sizeof(float) * 34, hipMemcpyHostToDevice));
+  if ((st = cl_put_stats(c, st0.data()))) return st;
   if ((st = cl_pack(c))) return st;
   HIPCHK(hipStreamSynchronize(c->s));
   L.ready = true;
@@ -1616,7 +1627,8 @@ ibg_status ibg_closure_fit_stats(ibg_ctx *c, double margin) {
       st[33] = (float)sd;
     }
   }
-  HIPCHK(hipMemcpy(c->cl.stat, st.data(), sizeof(float) * 34, hipMemcpyHostToDevice));
+  ibg_status rs = cl_put_stats(c, st.data());
+  if (rs) return rs;
   c->cl.stats = true;
   return IBG_OK;
 }
@@ -1679,7 +1691,8 @@ ibg_status ibg_closure_set_params(ibg_ctx *c, const float *in, int64_t n) {
   if (n != kNParam + 34) return IBG_ERR_ARG;
   HIPCHK(hipStreamSynchronize(c->s));
   HIPCHK(hipMemcpy(c->cl.p, in, sizeof(float) * kNParam, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->cl.stat, in + kNParam, sizeof(float) * 34, hipMemcpyHostToDevice));
+  ibg_status rs = cl_put_stats(c, in + kNParam);
+  if (rs) return rs;
   c->cl.stats = true;
   return cl_pack(c);
 }

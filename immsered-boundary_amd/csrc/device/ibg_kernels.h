@@ -468,7 +468,7 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
     bf16x8_t xf;
 #pragma unroll
     for (int k = 0; k < kMlpIn; ++k)
-      xf[k] = (__bf16)(use_ml ? ((float)ml_x[k] - a.mlp.fmean[k]) / a.mlp.fstd[k] : 0.0f);
+      xf[k] = (__bf16)(use_ml ? ((float)ml_x[k] - a.mlp.fmean[k]) * a.mlp.finv[k] : 0.0f);
     float yown = 0.0f;
     // one 16-cell block per pass, skipped (wave-uniformly) when none of its
     // cells is on the network; the compiler barrier keeps the weight

@@ -53,18 +53,22 @@ struct MlpDev {
   const float *w3;     // [64] bf16-rounded values
   const float *b1, *b2, *b3;
   const float *fmean, *fstd;   // [8] feature standardisation
+  const float *finv;           // [8] 1 / fstd (x_s = (x - fmean) * finv)
   const float *flo, *fhi;      // [8] raw feature range of the training data
   const float *zst;            // [2] target mean, std
 };
 
-// sigmoid with the hardware exp2 and reciprocal (~1 ulp each)
+// Hidden units: SiLU in base 2, g(z) = z / (1 + 2^-z) = SiLU(z ln2) / ln2
+// (the ln2 scalings fold into the adjacent layers' weights, so the network
+// is a SiLU network).  The gate is one v_exp_f32 (negated operand) and one
+// v_rcp_f32, ~1 ulp each, with no multiply in front of the exp.
 __device__ __forceinline__ float mlp_sigm(float z) {
-  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-z * 1.4426950408889634f));
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-z));
 }
 __device__ __forceinline__ float mlp_silu(float z) { return z * mlp_sigm(z); }
 __device__ __forceinline__ float mlp_dsilu(float z) {
   const float s = mlp_sigm(z);
-  return s + z * s * (1.0f - s);
+  return s + z * 0.69314718055994530942f * s * (1.0f - s);
 }
 __device__ __forceinline__ float bf16r(float x) { return (float)(__bf16)x; }
 

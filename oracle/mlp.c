@@ -40,17 +40,28 @@ float orc_bf16(float x) {
   return y;
 }
 
-static float silu_f(float z) { return z / (1.0f + expf(-z)); }
+/* The hidden units are SiLU units in base 2: g(z) = z / (1 + 2^-z)
+ * = SiLU(z ln2) / ln2.  The scalings ln2 and 1/ln2 fold into the adjacent
+ * layers (W1, b1, b2 times ln2 and w3 over ln2 give the network with
+ * g replaced by SiLU), so the function class is that of a SiLU network;
+ * the GPU evaluates 2^-z with one v_exp_f32 and no multiply.
+ * g'(z) = s + z ln2 s (1 - s), s = 1 / (1 + 2^-z). */
+#define LN2 0.69314718055994530942
+static float silu_f(float z) { return z / (1.0f + exp2f(-z)); }
 static float dsilu_f(float z) {
-  float s = 1.0f / (1.0f + expf(-z));
-  return s + z * s * (1.0f - s);
+  float s = 1.0f / (1.0f + exp2f(-z));
+  return s + z * (float)LN2 * s * (1.0f - s);
 }
 
 /* prm: parameters then fmean[8] fstd[8] flo[8] fhi[8] zmean zstd */
 static void fwd_row(const float *prm, const float *xraw, float *xb, float *h1, float *s1,
                     float *h2, float *s2, float *y) {
   const float *fm = prm + NP, *fs = prm + NP + 8;
-  for (int k = 0; k < 8; ++k) xb[k] = orc_bf16((xraw[k] - fm[k]) / fs[k]);
+  /* standardised with the FP32 reciprocal of the deviation, as the GPU */
+  for (int k = 0; k < 8; ++k) {
+    const float inv = 1.0f / fs[k];
+    xb[k] = orc_bf16((xraw[k] - fm[k]) * inv);
+  }
   for (int j = 0; j < 64; ++j) {
     double a = 0.0;
     for (int k = 0; k < 8; ++k) a += (double)xb[k] * (double)orc_bf16(prm[OW1 + k * 64 + j]);
@@ -115,10 +126,10 @@ void orc_mlp_grad(const float *prm, const float *x, const float *z, int64_t n, d
 }
 
 /* plain FP64 network on standardised inputs xs[n][8] and targets t[n] */
-static double silu_d(double z) { return z / (1.0 + exp(-z)); }
+static double silu_d(double z) { return z / (1.0 + exp2(-z)); }
 static double dsilu_d(double z) {
-  double s = 1.0 / (1.0 + exp(-z));
-  return s + z * s * (1.0 - s);
+  double s = 1.0 / (1.0 + exp2(-z));
+  return s + z * LN2 * s * (1.0 - s);
 }
 
 double orc_mlp_loss_f64(const double *p, const double *xs, const double *t, int64_t n) {
