@@ -47,38 +47,90 @@ __device__ __forceinline__ VelGrad<double> cell_grad(const Pack<double, D> &W, c
 // Training data from the solver's own state: every normal fluid cell that
 // the update advances (F_UPDATE, signdis > 0) is kept with probability frac
 // (seeded hash of the cell), with its raw features and the target
-// z = Q / (nu S^) of the plain closure in FP64.  One workgroup per tile.
-template <int D>
+// z = Q / (nu S^) of the plain closure in FP64.  One workgroup per tile, in
+// two passes so that the rows land in a fixed order (tile, then cell):
+// kWrite = false stores the tile's count of kept cells, k_closure_offsets
+// turns the counts into row offsets after the rows already stored, and
+// kWrite = true writes each kept cell at its offset plus its rank in the
+// tile.  Rows past the capacity are dropped.
+template <int D, bool kWrite>
 __global__ void __launch_bounds__(kTile) k_closure_sample(const TileInfo *tiles, const BlkInfo *blk,
                                                          const DevPhys *phys, const uint8_t *flags,
                                                          Pack<double, D> W, const double *nu,
                                                          const double *signdis, const int *nb27,
                                                          double frac, uint64_t seed, float *X,
-                                                         float *Z, int *Cell,
-                                                         unsigned long long *count, long long cap) {
+                                                         float *Z, int *Cell, int *tcount,
+                                                         const long long *toff, long long cap) {
+  __shared__ int wsum[kTile / 64];
   const TileInfo ti = tiles[blockIdx.x];
-  if ((int)threadIdx.x >= ti.ncell) return;
   const int c = ti.cell0 + threadIdx.x;
-  if (!(flags[c] & F_UPDATE) || !(signdis[c] > 0.0)) return;
-  const double u01 = (double)(mix64(seed ^ ((uint64_t)c * 0x632be59bd9b4e019ull)) >> 11) * (1.0 / 9007199254740992.0);
-  if (u01 >= frac) return;
-  const BlkInfo &bi = blk[ti.blk];
-  const DevPhys &P = phys[bi.case_id];
-  const Prim<double, D> me = pload<double, D>(W, c);
-  const double T = me.p / (me.r * P.Rg);
-  const double mu0 = sqrt(T * T * T) * ((1.0 + P.SoT) / (T + P.SoT));
-  const VelGrad<double> g = cell_grad<D>(W, nb27, c, bi);
-  const double olen = sa_olen<double>(g);
-  double x[kMlpIn], scale;
-  if (!sa_features<double>(me.u, me.v, me.w, me.r, me.p, mu0, nu[c], signdis[c], g, olen, P.Re,
-                           P.init[4], x, scale))
-    return;
-  const double q = sa_pointwise<double>(nu[c], mu0, me.r, signdis[c], olen, P.Re);
-  const unsigned long long k = atomicAdd(count, 1ull);
-  if ((long long)k >= cap) return;
-  for (int j = 0; j < kMlpIn; ++j) X[(size_t)k * kMlpIn + j] = (float)x[j];
-  Z[k] = (float)(q / scale);
-  Cell[k] = c;
+  bool keep = (int)threadIdx.x < ti.ncell && (flags[c] & F_UPDATE) && signdis[c] > 0.0;
+  if (keep) {
+    const double u01 = (double)(mix64(seed ^ ((uint64_t)c * 0x632be59bd9b4e019ull)) >> 11) *
+                       (1.0 / 9007199254740992.0);
+    keep = u01 < frac;
+  }
+  double x[kMlpIn], z = 0.0;
+  if (keep) {
+    const BlkInfo &bi = blk[ti.blk];
+    const DevPhys &P = phys[bi.case_id];
+    const Prim<double, D> me = pload<double, D>(W, c);
+    const double T = me.p / (me.r * P.Rg);
+    const double mu0 = sqrt(T * T * T) * ((1.0 + P.SoT) / (T + P.SoT));
+    const VelGrad<double> g = cell_grad<D>(W, nb27, c, bi);
+    const double olen = sa_olen<double>(g);
+    double scale;
+    keep = sa_features<double>(me.u, me.v, me.w, me.r, me.p, mu0, nu[c], signdis[c], g, olen,
+                               P.Re, P.init[4], x, scale);
+    if (keep) z = sa_pointwise<double>(nu[c], mu0, me.r, signdis[c], olen, P.Re) / scale;
+  }
+  // rank of the kept cell within the tile: wave prefix + earlier waves
+  const unsigned long long bal = __ballot(keep);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int below = __popcll(bal & ((1ull << lane) - 1ull));
+  if (lane == 0) wsum[wid] = __popcll(bal);
+  __syncthreads();
+  int before = 0, total = 0;
+  for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) {
+    if (w < wid) before += wsum[w];
+    total += wsum[w];
+  }
+  if constexpr (!kWrite) {
+    if (threadIdx.x == 0) tcount[blockIdx.x] = total;
+  } else {
+    if (keep) {
+      const long long k = toff[blockIdx.x] + before + below;
+      if (k < cap) {
+        for (int j = 0; j < kMlpIn; ++j) X[(size_t)k * kMlpIn + j] = (float)x[j];
+        Z[k] = (float)z;
+        Cell[k] = c;
+      }
+    }
+  }
+}
+
+// Exclusive prefix sum of the per-tile counts after the count of rows
+// already stored (count[0]), then count[0] += the total.  One workgroup.
+__global__ void __launch_bounds__(1024) k_closure_offsets(const int *tcount, int ntiles,
+                                                          long long *toff,
+                                                          unsigned long long *count) {
+  __shared__ long long part[1024];
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int per = (ntiles + nt - 1) / nt, a = t * per, b = min(ntiles, a + per);
+  long long s = 0;
+  for (int i = a; i < b; ++i) s += tcount[i];
+  part[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    long long run = (long long)count[0];
+    for (int k = 0; k < nt; ++k) { const long long v = part[k]; part[k] = run; run += v; }
+    count[1] = (unsigned long long)run;   // new total, published below
+  }
+  __syncthreads();
+  long long run = part[t];
+  for (int i = a; i < b; ++i) { toff[i] = run; run += tcount[i]; }
+  __syncthreads();
+  if (t == 0) count[0] = count[1];
 }
 
 // Per-block partial moments of the dataset: for each of the 8 features and

@@ -73,7 +73,10 @@ struct ibg_ctx {
     long long cap = 0;
     float *X = nullptr, *Z = nullptr;
     int *cell = nullptr;
-    unsigned long long *count = nullptr, *mlcount = nullptr;
+    unsigned long long *count = nullptr, *mlcount = nullptr;   // count[0]: rows stored
+    int *tcount = nullptr;      // sampler: kept cells per tile
+    long long *toff = nullptr;  // sampler: first row of each tile
+    int ntc = 0;                // tiles the two arrays hold
     float *p = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr, *lossp = nullptr, *gsum = nullptr;
     __bf16 *W1t = nullptr, *W2t = nullptr, *W2 = nullptr, *W2p = nullptr;
     float *w3 = nullptr, *b1 = nullptr, *b2 = nullptr, *b3 = nullptr;
@@ -1573,15 +1576,46 @@ ibg_status ibg_closure_sample(ibg_ctx *c, double frac, uint64_t seed, int64_t *t
   if (!c || !c->have_mesh || !c->cl.ready) return IBG_ERR_STATE;
   if (!c->turb || frac < 0.0 || frac > 1.0) return IBG_ERR_ARG;
   if (c->ntiles > 0) {
-    if (c->D == 2)
-      hipLaunchKernelGGL((k_closure_sample<2>), dim3(c->ntiles), dim3(c->tile), 0, c->s, c->tiles,
-                         c->blk, c->phys, c->flags, c->w64<2>(), c->nu64, c->signdis, c->nb27, frac,
-                         seed, c->cl.X, c->cl.Z, cl_cells(c), c->cl.count, (long long)c->cl.cap);
-    else
-      hipLaunchKernelGGL((k_closure_sample<3>), dim3(c->ntiles), dim3(c->tile), 0, c->s, c->tiles,
-                         c->blk, c->phys, c->flags, c->w64<3>(), c->nu64, c->signdis, c->nb27, frac,
-                         seed, c->cl.X, c->cl.Z, cl_cells(c), c->cl.count, (long long)c->cl.cap);
-    HIPCHK(hipGetLastError());
+    if (c->cl.ntc < c->ntiles) {
+      // per-tile counts and row offsets of the two-pass sampler
+      ibg_status st;
+      if ((st = cl_alloc(c, &c->cl.tcount, (size_t)c->ntiles))) return st;
+      if ((st = cl_alloc(c, &c->cl.toff, (size_t)c->ntiles))) return st;
+      c->cl.ntc = c->ntiles;
+    }
+    const long long cap = (long long)c->cl.cap;
+    int *cells = cl_cells(c);
+    for (int pass = 0; pass < 2; ++pass) {
+      if (c->D == 2) {
+        if (pass == 0)
+          hipLaunchKernelGGL((k_closure_sample<2, false>), dim3(c->ntiles), dim3(c->tile), 0, c->s,
+                             c->tiles, c->blk, c->phys, c->flags, c->w64<2>(), c->nu64, c->signdis,
+                             c->nb27, frac, seed, c->cl.X, c->cl.Z, cells, c->cl.tcount,
+                             (const long long *)c->cl.toff, cap);
+        else
+          hipLaunchKernelGGL((k_closure_sample<2, true>), dim3(c->ntiles), dim3(c->tile), 0, c->s,
+                             c->tiles, c->blk, c->phys, c->flags, c->w64<2>(), c->nu64, c->signdis,
+                             c->nb27, frac, seed, c->cl.X, c->cl.Z, cells, c->cl.tcount,
+                             (const long long *)c->cl.toff, cap);
+      } else {
+        if (pass == 0)
+          hipLaunchKernelGGL((k_closure_sample<3, false>), dim3(c->ntiles), dim3(c->tile), 0, c->s,
+                             c->tiles, c->blk, c->phys, c->flags, c->w64<3>(), c->nu64, c->signdis,
+                             c->nb27, frac, seed, c->cl.X, c->cl.Z, cells, c->cl.tcount,
+                             (const long long *)c->cl.toff, cap);
+        else
+          hipLaunchKernelGGL((k_closure_sample<3, true>), dim3(c->ntiles), dim3(c->tile), 0, c->s,
+                             c->tiles, c->blk, c->phys, c->flags, c->w64<3>(), c->nu64, c->signdis,
+                             c->nb27, frac, seed, c->cl.X, c->cl.Z, cells, c->cl.tcount,
+                             (const long long *)c->cl.toff, cap);
+      }
+      HIPCHK(hipGetLastError());
+      if (pass == 0) {
+        hipLaunchKernelGGL(k_closure_offsets, dim3(1), dim3(1024), 0, c->s, c->cl.tcount, c->ntiles,
+                           c->cl.toff, c->cl.count);
+        HIPCHK(hipGetLastError());
+      }
+    }
   }
   unsigned long long n = 0;
   HIPCHK(hipMemcpyAsync(&n, c->cl.count, sizeof(n), hipMemcpyDeviceToHost, c->s));
