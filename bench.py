@@ -187,6 +187,86 @@ def cpu_baseline(mesh, nstage, budget_s):
                       f"({upd} cell-stage updates in {el:.1f} s)"}
 
 
+def secondary(args):
+    """The 3D quasi-span and the batched-ensemble configurations: plain FP64
+    and mixed (FP32 flux, FP64 state) with the SA closure, one line each.
+    The ensemble also picks its tile size with the autotuner and serves as
+    the closure's training-data generator (one sampling pass, timed)."""
+    if args.workload == "naca3d":
+        mesh = ibgpu.Mesh(ibgpu.config3())
+        desc = ("NACA0012 3D quasi-span, M0.6 Re5e4 AoA 4, periodic span 0.1 chord, 128x128x8 base "
+                "+ 4 AMR levels, 4-stage, temporal sub-cycling, Spalart-Allmaras closure")
+    else:
+        mesh = ibgpu.Mesh(ibgpu.ensemble_cases())
+        desc = ("32 NACA 4-digit sections (t U[0.09,0.18], AoA U[-4,8] deg, M U[0.3,0.7], seed "
+                "20261202), Re1e4, 256^2 base + 5 AMR levels each with uneven refinement, one launch "
+                "per stage for all cases, Spalart-Allmaras closure")
+    per_case = [int(sum(r)) for r in mesh.counts()]
+    print(f"[bench] {args.workload}: {mesh.ncases} case(s), {mesh.ncells} cells, normal cells per case "
+          f"{min(per_case)}..{max(per_case)} (total {sum(per_case)})", file=sys.stderr, flush=True)
+    res, extra = {}, {}
+    for mode in ("fp64", "mixed"):
+        eng = new_engine(mesh, mode, args.stages)
+        if args.workload == "ensemble" and mode == "mixed":
+            best, ms = eng.autotune_tile()
+            eng.set_state(mesh.init_state())
+            eng.set_nut(np.full(mesh.ncells, mesh.phys[0].nut_inf))
+            extra[f"tile_{mode}"] = {"chosen": best, "stage_ms_by_tile": ms}
+        r = timed_run(eng, mesh, mode, args.warmup, args.steps, args.reps)
+        r["forces"] = eng.ComptWallForce().reshape(mesh.ncases, 6)[:, :2].copy()
+        if args.workload == "ensemble" and mode == "fp64":
+            eng.closure_init(1 << 23)
+            t0 = time.perf_counter()
+            n = eng.closure_sample(0.1, 1)
+            eng.synchronize()
+            extra["training_data"] = {"rows": n, "fraction": 0.1, "seconds": time.perf_counter() - t0}
+        res[mode] = r
+        eng.close()
+        print(f"[bench] {mode}+SA: {r['rate']:.4e} cell-stage/s, {1e3 * r['seconds'] / args.steps:.3f} "
+              f"ms/step, stage kernel {r['stage_ms'] * 1e3:.1f} us "
+              f"({r['stage_bytes'] / (r['stage_ms'] * 1e-3) / 1e9:.0f} GB/s algorithmic)",
+              file=sys.stderr, flush=True)
+    # plain copy of 1 GiB (the achievable HBM bandwidth; the context needs no large mesh)
+    probe = ibgpu.Engine(ibgpu.Mesh(ibgpu.tiny_case()), mode="fp64")
+    copy_ms = probe.bench_copy(1 << 30, 20)
+    probe.close()
+    copy_gbs = 2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9
+    fp, mx = res["fp64"], res["mixed"]
+    cf_fp = fp["forces"] / np.array([[mesh.phys[c].force_ref] for c in range(mesh.ncases)])
+    cf_mx = mx["forces"] / np.array([[mesh.phys[c].force_ref] for c in range(mesh.ncases)])
+    dF = np.abs(cf_mx - cf_fp) / np.maximum(np.abs(cf_fp), 1e-30)
+    out = {
+        "metric": "cell-stage updates/s, mixed vs plain FP64 (SA closure)",
+        "value": mx["rate"], "unit": "cell-stage updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * mx["seconds"] / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 flux + f64 state", "data": "synthetic (freestream start, analytic NACA geometry)",
+        "config": {"workload": desc, "cases": mesh.ncases, "cells": int(mesh.ncells),
+                   "normal_cells": int(sum(per_case)), "levels": mesh.nlevels, "stages": args.stages},
+        "plain_fp64": {"value": fp["rate"], "ms_per_step": 1e3 * fp["seconds"] / args.steps,
+                       "stage_kernel_ms": fp["stage_ms"],
+                       "stage_kernel_GBs": fp["stage_bytes"] / (fp["stage_ms"] * 1e-3) / 1e9},
+        "mixed_over_fp64": mx["rate"] / fp["rate"],
+        "accuracy": {"vs": "plain FP64 after the same steps",
+                     "max_dCD_rel": float(dF[:, 0].max()), "max_dCL_rel": float(dF[:, 1].max()),
+                     "max_dCL_abs": float(np.abs(cf_mx[:, 1] - cf_fp[:, 1]).max()),
+                     "max_dCD_abs": float(np.abs(cf_mx[:, 0] - cf_fp[:, 0]).max()),
+                     "CD_fp64": cf_fp[:, 0].tolist(), "CL_fp64": cf_fp[:, 1].tolist(),
+                     "CD_mixed": cf_mx[:, 0].tolist(), "CL_mixed": cf_mx[:, 1].tolist(),
+                     "state_rel_L2": float(np.linalg.norm(mx["state"][:, :5] - fp["state"][:, :5]) /
+                                           np.linalg.norm(fp["state"][:, :5]))},
+        "roofline": {"bound": "hbm", "achieved": mx["stage_bytes"] / (mx["stage_ms"] * 1e-3) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": mx["stage_bytes"] / (mx["stage_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": f"k_stage<float,{mesh.dim},false>",
+                     "bytes_per_launch": mx["stage_bytes"], "avg_launch_ms": mx["stage_ms"],
+                     "cells_per_launch": mx["stage_cells"], "copy_probe_GBs": copy_gbs},
+        "cpu_baseline": None,
+    }
+    out.update(extra)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,11 +280,17 @@ def main():
     ap.add_argument("--epochs", type=float, default=20.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("naca2d", "naca3d", "ensemble"), default="naca2d",
+                    help="naca2d (default): the measured configuration with the learned closure; "
+                         "naca3d, ensemble: plain FP64 vs mixed on the 3D and batched cases")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU (the whole mesh lives in one device's HBM); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "naca2d":
+        secondary(args)
+        return
     mesh = ibgpu.Mesh(ibgpu.config2())
     leaf = int(sum(mesh.counts()[0]))
     print(f"[bench] mesh: {mesh.ncells} cells, normal per level {mesh.counts()[0]}", file=sys.stderr, flush=True)

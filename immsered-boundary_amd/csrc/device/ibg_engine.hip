@@ -215,11 +215,11 @@ inline int transfer_due(const ibg_ctx *c, int cl, int ts) {
 inline unsigned grid(int n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // LDS of the tiled kernels: face fluxes (4 values + the 3D fifth), then for
-// the SA closure either the two (u,v,w,nu) face-cell records (stage) or the
-// face flux of the working variable (viscous pass).
+// the SA closure either the two face-cell indices (stage) or the face flux
+// of the working variable (viscous pass; a fused first stage holds either).
 size_t tile_smem(const ibg_ctx *c, size_t rsize, bool stage) {
   size_t per = 4 * rsize + (c->D == 3 ? rsize : 0);
-  if (c->turb) per += stage ? 8 * rsize : rsize;
+  if (c->turb) per += stage ? std::max<size_t>(2 * sizeof(int), rsize) : rsize;
   const size_t b = (size_t)c->max_tile_faces * per;
   if (stage && c->cl.on) return std::max(b, kStageMlSmem);
   return b;

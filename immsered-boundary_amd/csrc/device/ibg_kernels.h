@@ -28,6 +28,14 @@ __device__ __forceinline__ Prim<R, D> pload(const Pack<R, D> &W, int c) {
   return o;
 }
 
+// velocity of cell c (the 4-vector only; w = 0 in 2D)
+template <typename R, int D>
+__device__ __forceinline__ void uvw_load(const Pack<R, D> &W, int c, R &u, R &v, R &w) {
+  const typename V4<R>::T q = W.a[c];
+  u = q.y; v = q.z;
+  w = (D == 3) ? q.w : R(0);
+}
+
 template <typename R, int D>
 __device__ __forceinline__ void pstore(const Pack<R, D> &W, int c, R r, R u, R v, R w, R p) {
   typename V4<R>::T q;
@@ -288,8 +296,10 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
   }
   V *fl4 = reinterpret_cast<V *>(smem_raw);
   R *fl1 = reinterpret_cast<R *>(smem_raw + sizeof(V) * (size_t)ti.nface);   // 3D 5th value
-  V *sL = reinterpret_cast<V *>(smem_raw + (sizeof(V) + (D == 3 ? sizeof(R) : 0)) * (size_t)ti.nface);
-  V *sR = sL + ti.nface;   // (u,v,w,nu) of the low / high cell of each face (SA)
+  // SA: the low / high cell of each face; the cell phase gathers their
+  // velocity and working variable (cache hits: phase 1 has just read them)
+  int *fL = reinterpret_cast<int *>(smem_raw + (sizeof(V) + (D == 3 ? sizeof(R) : 0)) * (size_t)ti.nface);
+  int *fR = fL + ti.nface;
   __shared__ double red[kRpart][kTile / 64];
   __shared__ unsigned mlc[2];   // learned closure: network / fallback cells of this tile
 
@@ -316,11 +326,8 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
     fl4[f] = o;
     if (D == 3) fl1[f] = flux[4];
     if (a.turb) {
-      V l, r;
-      l.x = q1.u; l.y = q1.v; l.z = q1.w; l.w = a.nuin[st.y];
-      r.x = q2.u; r.y = q2.v; r.z = q2.w; r.w = a.nuin[st.z];
-      sL[f] = l;
-      sR[f] = r;
+      fL[f] = st.y;
+      fR[f] = st.z;
     }
   }
   __syncthreads();
@@ -379,9 +386,11 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         if (d < D) {
-          const V lo = sL[sl[2 * d]], hi = sR[sl[2 * d + 1]];
-          ul[d] = lo.x; vl[d] = lo.y; wl[d] = lo.z; nl[d] = lo.w;
-          ur[d] = hi.x; vr[d] = hi.y; wr[d] = hi.z; nr[d] = hi.w;
+          const int lo = fL[sl[2 * d]], hi = fR[sl[2 * d + 1]];
+          uvw_load<R, D>(a.Win, lo, ul[d], vl[d], wl[d]);
+          uvw_load<R, D>(a.Win, hi, ur[d], vr[d], wr[d]);
+          nl[d] = a.nuin[lo];
+          nr[d] = a.nuin[hi];
         } else {
           ul[d] = ur[d] = vl[d] = vr[d] = wl[d] = wr[d] = nl[d] = nr[d] = R(0);
         }
