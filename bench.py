@@ -46,7 +46,7 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 # WRITE_SIZE x1.0), divided by the cells of the launch (the launch mix of the
 # timed steps).  Recorded in profiles/k_stage_naca2d_sa_traffic.md; update it
 # with the kernel.
-TRAFFIC_BYTES_PER_CELL = 192.3
+TRAFFIC_BYTES_PER_CELL = 162.0
 
 
 def stage_bytes(mode, dim, turb, cells, faces, last=False, first=False):
