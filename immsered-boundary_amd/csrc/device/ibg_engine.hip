@@ -88,6 +88,7 @@ struct ibg_ctx {
   uint8_t *flags = nullptr;
   int *cellcase = nullptr, *nb27 = nullptr;
   TileInfo *tiles = nullptr;
+  int *case_tiles = nullptr;   // [case][level] = {first tile, end tile}
   int4 *tst = nullptr, *tvs = nullptr;
   int8_t *tdir = nullptr;
   uint16_t *slots = nullptr;
@@ -404,7 +405,7 @@ ibg_status run_dt(ibg_ctx *c) {
                          c->blk, c->phys, c->cflpart, c->err);
     HIPCHK(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_dt_final, dim3(c->ncases), dim3(256), 0, c->s, c->ntiles, c->tiles, c->blk,
+  hipLaunchKernelGGL(k_dt_final, dim3(c->ncases), dim3(256), 0, c->s, c->L, c->case_tiles,
                      c->cflpart, c->dt, c->phys, c->ncases);
   HIPCHK(hipGetLastError());
   return IBG_OK;
@@ -784,6 +785,7 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
   std::vector<int8_t> tdir;
   std::vector<uint16_t> slots((size_t)Nn * 2 * D, 0);
   c->tile_end.assign(L, 0);
+  std::vector<int> case_tiles((size_t)nc * L * 2, 0);   // [case][level] = {first, end}
   std::vector<int> slot_of(F, -1);
   int maxf = 1;
   {
@@ -792,6 +794,7 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
       for (int cs = 0; cs < nc; ++cs) {
         const int b = cs * L + l;
         const int n = m->blk_nnormal[b];
+        case_tiles[(size_t)b * 2] = (int)tiles.size();
         for (int t0 = 0; t0 < n; t0 += c->tile) {
           TileInfo ti{};
           ti.cell0 = pos + t0;
@@ -819,6 +822,7 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
           maxf = std::max(maxf, ti.nface);
           tiles.push_back(ti);
         }
+        case_tiles[(size_t)b * 2 + 1] = (int)tiles.size();
         pos += n;
       }
       c->tile_end[l] = (int)tiles.size();
@@ -981,6 +985,7 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
   for (int cs = 0; cs < nc; ++cs) dt0[cs] = ph[cs].dt_init;
   UP(c->dt, dt0);
   AL(c->cflpart, (size_t)std::max(c->ntiles, 1));
+  UP(c->case_tiles, case_tiles);
   AL(c->err, 1);
   AL(c->rpart, (size_t)std::max(c->ntiles, 1) * kRpart);
   AL(c->resid, (size_t)nc * kRpart);
@@ -1200,8 +1205,8 @@ ibg_status ibg_time_march(ibg_ctx *c, int32_t ts0, int32_t nsteps, int32_t use_g
 
 ibg_status ibg_residual(ibg_ctx *c, double *linf, double *l2) {
   if (!c || !c->have_mesh) return IBG_ERR_STATE;
-  hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(64), 0, c->s, c->last_ntiles, c->tiles, c->blk,
-                     c->rpart, c->resid, c->ncases);
+  hipLaunchKernelGGL(k_resid_final, dim3(c->ncases), dim3(256), 0, c->s, c->last_ntiles, c->L,
+                     c->case_tiles, c->rpart, c->resid, c->ncases);
   HIPCHK(hipGetLastError());
   std::vector<double> h((size_t)c->ncases * kRpart);
   HIPCHK(hipMemcpyAsync(h.data(), c->resid, h.size() * 8, hipMemcpyDeviceToHost, c->s));
@@ -1217,8 +1222,8 @@ ibg_status ibg_residual(ibg_ctx *c, double *linf, double *l2) {
 ibg_status ibg_residual_nut(ibg_ctx *c, double *linf, double *l2) {
   if (!c || !c->have_mesh) return IBG_ERR_STATE;
   if (!c->turb) return IBG_ERR_STATE;
-  hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(64), 0, c->s, c->last_ntiles, c->tiles, c->blk,
-                     c->rpart, c->resid, c->ncases);
+  hipLaunchKernelGGL(k_resid_final, dim3(c->ncases), dim3(256), 0, c->s, c->last_ntiles, c->L,
+                     c->case_tiles, c->rpart, c->resid, c->ncases);
   HIPCHK(hipGetLastError());
   std::vector<double> h((size_t)c->ncases * kRpart);
   HIPCHK(hipMemcpyAsync(h.data(), c->resid, h.size() * 8, hipMemcpyDeviceToHost, c->s));

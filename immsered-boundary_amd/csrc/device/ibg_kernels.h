@@ -892,15 +892,18 @@ __global__ void __launch_bounds__(kTile) k_cfl(const TileInfo *tiles, Pack<doubl
 }
 
 // dt = min(CFL_dt, dt*dt_ratio) per case, CFL_dt capped at 10000
-// (NS_Solver.cpp:198, 263)
-__global__ void k_dt_final(int ntiles, const TileInfo *tiles, const BlkInfo *blk,
-                           const double *cflpart, double *dt, const DevPhys *phys, int ncases) {
+// (NS_Solver.cpp:198, 263).  One workgroup per case over the case's tile
+// range of each level (ct[case][level] = {first, end}).
+__global__ void k_dt_final(int L, const int *ct, const double *cflpart, double *dt,
+                           const DevPhys *phys, int ncases) {
   const int cs = blockIdx.x;
   if (cs >= ncases) return;
   __shared__ double red[256];
   double m = 10000.0;
-  for (int t = threadIdx.x; t < ntiles; t += blockDim.x)
-    if (blk[tiles[t].blk].case_id == cs) m = fmin(m, cflpart[t]);
+  for (int l = 0; l < L; ++l) {
+    const int a = ct[(cs * L + l) * 2], b = ct[(cs * L + l) * 2 + 1];
+    for (int t = a + (int)threadIdx.x; t < b; t += blockDim.x) m = fmin(m, cflpart[t]);
+  }
   red[threadIdx.x] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1014,23 +1017,37 @@ __global__ void k_forces(int npatch, const int *pcase, const int *intp, const do
 // residual finalize: per case, over the tiles of the last march
 // out[case*12 + {0..4}] = L-inf, {5..9} = L2 of the flow residual, {10, 11}
 // = L-inf, L2 of the SA residual (res_vis_sum of NS_Solver.H:297, 321).
-__global__ void k_resid_final(int ntiles, const TileInfo *tiles, const BlkInfo *blk,
-                              const double *rpart, double *out, int ncases) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncases) return;
+__global__ void k_resid_final(int ntiles, int L, const int *ct, const double *rpart, double *out,
+                              int ncases) {
+  // one workgroup per case over its tiles of the last march (tiles < ntiles)
+  const int cs = blockIdx.x;
+  if (cs >= ncases) return;
+  __shared__ double red[kRpart][256];
   double m[6] = {0, 0, 0, 0, 0, 0}, q[6] = {0, 0, 0, 0, 0, 0};
-  for (int t = 0; t < ntiles; ++t) {
-    if (blk[tiles[t].blk].case_id != c) continue;
-    for (int k = 0; k < 5; ++k) {
-      m[k] = fmax(m[k], rpart[(size_t)t * kRpart + k]);
-      q[k] += rpart[(size_t)t * kRpart + 5 + k];
+  for (int l = 0; l < L; ++l) {
+    const int a = ct[(cs * L + l) * 2], b = min(ct[(cs * L + l) * 2 + 1], ntiles);
+    for (int t = a + (int)threadIdx.x; t < b; t += blockDim.x) {
+      for (int k = 0; k < 5; ++k) {
+        m[k] = fmax(m[k], rpart[(size_t)t * kRpart + k]);
+        q[k] += rpart[(size_t)t * kRpart + 5 + k];
+      }
+      m[5] = fmax(m[5], rpart[(size_t)t * kRpart + 10]);
+      q[5] += rpart[(size_t)t * kRpart + 11];
     }
-    m[5] = fmax(m[5], rpart[(size_t)t * kRpart + 10]);
-    q[5] += rpart[(size_t)t * kRpart + 11];
   }
-  for (int k = 0; k < 5; ++k) { out[c * kRpart + k] = m[k]; out[c * kRpart + 5 + k] = sqrt(q[k]); }
-  out[c * kRpart + 10] = m[5];
-  out[c * kRpart + 11] = sqrt(q[5]);
+  for (int k = 0; k < 6; ++k) { red[k][threadIdx.x] = m[k]; red[6 + k][threadIdx.x] = q[k]; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 6; ++k) {
+      double mm = 0.0, qq = 0.0;
+      for (int w = 0; w < (int)blockDim.x; ++w) { mm = fmax(mm, red[k][w]); qq += red[6 + k][w]; }
+      m[k] = mm;
+      q[k] = qq;
+    }
+    for (int k = 0; k < 5; ++k) { out[cs * kRpart + k] = m[k]; out[cs * kRpart + 5 + k] = sqrt(q[k]); }
+    out[cs * kRpart + 10] = m[5];
+    out[cs * kRpart + 11] = sqrt(q[5]);
+  }
 }
 
 // parity probes: inviscid / viscous flux of every face, [face][5]
