@@ -129,13 +129,14 @@ def timed_run(eng, mesh, mode, warmup, steps, reps, ml=False):
     P = mesh.phys[0]
     F = eng.ComptWallForce()
     cd, cl = F[0] / P.force_ref, F[1] / P.force_ref
+    forces = F.reshape(mesh.ncases, 6)[:, :2].copy()   # every case, before the timing launches
     state, nut = eng.get_state(), eng.get_nut()
     counts = eng.closure_counts() if ml else (0, 0)   # cells of the warmup + timed steps
     # the stage kernel alone, launch by launch over the timed steps' mix
     ms, nbytes, ncell = stage_mix(eng, mesh, mode, warmup + steps, eng.cfg.nstage, reps)
     return {"seconds": t1 - t0, "updates": upd, "rate": upd / (t1 - t0), "cd": cd, "cl": cl,
             "stage_ms": ms, "stage_bytes": nbytes, "stage_cells": ncell, "state": state, "nut": nut,
-            "counts": counts}
+            "counts": counts, "forces": forces}
 
 
 def train_closure(mesh, nstage, args):
@@ -213,7 +214,6 @@ def secondary(args):
             eng.set_nut(np.full(mesh.ncells, mesh.phys[0].nut_inf))
             extra[f"tile_{mode}"] = {"chosen": best, "stage_ms_by_tile": ms}
         r = timed_run(eng, mesh, mode, args.warmup, args.steps, args.reps)
-        r["forces"] = eng.ComptWallForce().reshape(mesh.ncases, 6)[:, :2].copy()
         if args.workload == "ensemble" and mode == "fp64":
             eng.closure_init(1 << 23)
             t0 = time.perf_counter()
