@@ -300,7 +300,7 @@ ibg_status ibg_cell_stage_updates(ibg_ctx *ctx, int64_t *count);
  * ComptMutSource evaluates per cell (NS_Solver.cpp:700-767); the reference
  * has no learned component, so these calls replace nothing there and are
  * the training / inference surface of the closure (the reference's
- * per-cell evaluation stays the fallback and the judge of its error).
+ * per-cell evaluation stays the fallback and the measure of its error).
  * Network 8 -> 64 -> 64 -> 1 with SiLU units in base 2, g(z) = z/(1+2^-z)
  * = SiLU(z ln2)/ln2 (the scalings fold into the adjacent layers), BF16
  * weights and activations, FP32 accumulation, trained on cells sampled
