@@ -313,6 +313,16 @@ def main():
     eng.closure_enable(True)
     res["ml"] = timed_run(eng, mesh, "mixed", args.warmup, args.steps, args.reps, ml=True)
     on_net, fallback = res["ml"]["counts"]
+    # host boundary: the AoS state in and out through the C ABI (PCIe + layout
+    # conversion), which a caller pays once around a run of steps
+    fv_host = mesh.init_state()
+    eng.synchronize()
+    t0 = time.perf_counter()
+    eng.set_state(fv_host)
+    eng.set_nut(np.full(mesh.ncells, mesh.phys[0].nut_inf))
+    _ = eng.get_state()
+    _ = eng.get_nut()
+    host_io_s = time.perf_counter() - t0
     eng.close()
     r = res["ml"]
     print(f"[bench] mixed+ML: {r['rate']:.4e} cell-stage/s, {1e3 * r['seconds'] / args.steps:.3f} ms/step, "
@@ -374,6 +384,8 @@ def main():
                      "CL_fp64": fp["cl"], "CD_fp64": fp["cd"],
                      "mixed": acc(mx), "mixed_ml": acc(ml)},
         "closure": dict(tinfo, network_cells=on_net, fallback_cells=fallback),
+        "host_boundary": {"state_in_out_ms": 1e3 * host_io_s,
+                          "value_incl_state_in_out": ml["updates"] / (ml["seconds"] + host_io_s)},
         "roofline": {"bound": "hbm", "achieved": ml_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ml_gbs / HBM_PEAK_GBS,
                      "traffic": (TRAFFIC_BYTES_PER_CELL * ml["stage_cells"]
