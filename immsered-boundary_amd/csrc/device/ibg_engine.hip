@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -117,6 +118,8 @@ struct ibg_ctx {
   int64_t last_stage_launches = 0, last_updates = 0;
   std::vector<void *> allocs;
   std::string errmsg;
+  void *hstage = nullptr;      // pinned host staging of the state boundary
+  size_t hstage_bytes = 0;
 
   template <int D> Pack<double, D> w64() const { return Pack<double, D>{(D4 *)W64a, W64e}; }
   MlpDev mlpdev() const {
@@ -593,6 +596,7 @@ ibg_status ibg_destroy(ibg_ctx *c) {
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->s) (void)hipStreamDestroy(c->s);
+  if (c->hstage) (void)hipHostFree(c->hstage);
   delete c;
   return IBG_OK;
 }
@@ -1011,35 +1015,72 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
   return IBG_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// host loops of the state boundary: split over up to 16 threads
+template <typename F>
+void host_parallel(int n, F f) {
+  const int nt = std::max(1, std::min<int>(16, (int)std::thread::hardware_concurrency()));
+  if (n < (1 << 16) || nt == 1) { f(0, n); return; }
+  std::vector<std::thread> th;
+  const int chunk = (n + nt - 1) / nt;
+  for (int t = 0; t < nt; ++t) {
+    const int a = t * chunk, b = std::min(n, a + chunk);
+    if (a < b) th.emplace_back([=] { f(a, b); });
+  }
+  for (auto &x : th) x.join();
+}
+
+// pinned staging for the state boundary (grown on demand, kept until the
+// context is destroyed): page-locked copies run at the link rate
+void *host_stage(ibg_ctx *c, size_t bytes) {
+  if (c->hstage_bytes < bytes) {
+    if (c->hstage) (void)hipHostFree(c->hstage);
+    c->hstage = nullptr;
+    c->hstage_bytes = 0;
+    if (hipHostMalloc(&c->hstage, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+    c->hstage_bytes = bytes;
+  }
+  return c->hstage;
+}
+
+// the stencil copies from the FP64 master on the device
+template <typename R, int D>
+ibg_status fill_stencil_from_master(ibg_ctx *c) {
+  for (int k = 0; k < c->nbuf; ++k) {
+    if (c->Sa[k] == c->W64a) continue;
+    hipLaunchKernelGGL((k_fill_from_master<R, D>), dim3(grid(c->N, 256)), dim3(256), 0, c->s, c->N,
+                       c->w64<D>(), c->sbuf<R, D>(k));
+    HIPCHK(hipGetLastError());
+  }
+  return IBG_OK;
+}
+}  // namespace
+
+extern "C" {
+
 ibg_status ibg_set_state(ibg_ctx *c, const double *fv, int64_t n) {
   if (!c || !fv || !c->have_mesh || n != c->N) return IBG_ERR_ARG;
   const int N = c->N;
-  std::vector<D4> a(N);
-  std::vector<double> e(c->D == 3 ? N : 0);
-  std::vector<F4> a32(c->mixed ? N : 0);
-  std::vector<float> e32(c->mixed && c->D == 3 ? N : 0);
-  for (int i = 0; i < N; ++i) {
-    const double *q = fv + (size_t)c->d2h[i] * IBG_NFV;   // roe u v w p T e
-    if (c->D == 2) a[i] = D4{q[0], q[1], q[2], q[4]};
-    else { a[i] = D4{q[0], q[1], q[2], q[3]}; e[i] = q[4]; }
-    if (c->mixed) {
-      a32[i] = F4{(float)a[i].x, (float)a[i].y, (float)a[i].z, (float)a[i].w};
-      if (c->D == 3) e32[i] = (float)q[4];
-    }
-  }
   HIPCHK(hipStreamSynchronize(c->s));
-  HIPCHK(hipMemcpy(c->W64a, a.data(), sizeof(D4) * N, hipMemcpyHostToDevice));
-  if (c->D == 3) HIPCHK(hipMemcpy(c->W64e, e.data(), sizeof(double) * N, hipMemcpyHostToDevice));
-  for (int k = 0; k < c->nbuf; ++k) {
-    if (c->Sa[k] == c->W64a) continue;
-    if (c->mixed) {
-      HIPCHK(hipMemcpy(c->Sa[k], a32.data(), sizeof(F4) * N, hipMemcpyHostToDevice));
-      if (c->D == 3) HIPCHK(hipMemcpy(c->Se[k], e32.data(), sizeof(float) * N, hipMemcpyHostToDevice));
-    } else {
-      HIPCHK(hipMemcpy(c->Sa[k], a.data(), sizeof(D4) * N, hipMemcpyHostToDevice));
-      if (c->D == 3) HIPCHK(hipMemcpy(c->Se[k], e.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+  D4 *a = (D4 *)host_stage(c, (sizeof(D4) + (c->D == 3 ? sizeof(double) : 0)) * (size_t)N);
+  if (!a) return IBG_ERR_HIP;
+  double *e = (double *)(a + N);
+  host_parallel(N, [&](int i0, int i1) {
+    for (int i = i0; i < i1; ++i) {
+      const double *q = fv + (size_t)c->d2h[i] * IBG_NFV;   // roe u v w p T e
+      if (c->D == 2) a[i] = D4{q[0], q[1], q[2], q[4]};
+      else { a[i] = D4{q[0], q[1], q[2], q[3]}; e[i] = q[4]; }
     }
-  }
+  });
+  HIPCHK(hipMemcpy(c->W64a, a, sizeof(D4) * N, hipMemcpyHostToDevice));
+  if (c->D == 3) HIPCHK(hipMemcpy(c->W64e, e, sizeof(double) * N, hipMemcpyHostToDevice));
+  ibg_status st;
+  if (c->D == 2) st = c->mixed ? fill_stencil_from_master<float, 2>(c) : fill_stencil_from_master<double, 2>(c);
+  else st = c->mixed ? fill_stencil_from_master<float, 3>(c) : fill_stencil_from_master<double, 3>(c);
+  if (st) return st;
+  HIPCHK(hipStreamSynchronize(c->s));
   return IBG_OK;
 }
 
@@ -1047,16 +1088,24 @@ ibg_status ibg_set_nut(ibg_ctx *c, const double *nut, int64_t n) {
   if (!c || !nut || !c->have_mesh || n != c->N) return IBG_ERR_ARG;
   if (!c->turb) return IBG_ERR_STATE;
   const int N = c->N;
-  std::vector<double> a(N);
-  std::vector<float> a32(N);
-  for (int i = 0; i < N; ++i) { a[i] = nut[c->d2h[i]]; a32[i] = (float)a[i]; }
   HIPCHK(hipStreamSynchronize(c->s));
-  HIPCHK(hipMemcpy(c->nu64, a.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+  double *a = (double *)host_stage(c, sizeof(double) * (size_t)N);
+  if (!a) return IBG_ERR_HIP;
+  host_parallel(N, [&](int i0, int i1) {
+    for (int i = i0; i < i1; ++i) a[i] = nut[c->d2h[i]];
+  });
+  HIPCHK(hipMemcpy(c->nu64, a, sizeof(double) * N, hipMemcpyHostToDevice));
   for (int k = 0; k < c->nbuf; ++k) {
     if (c->NUs[k] == (void *)c->nu64) continue;
-    if (c->mixed) HIPCHK(hipMemcpy(c->NUs[k], a32.data(), sizeof(float) * N, hipMemcpyHostToDevice));
-    else HIPCHK(hipMemcpy(c->NUs[k], a.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    if (c->mixed)
+      hipLaunchKernelGGL((k_fill_nu_from_master<float>), dim3(grid(N, 256)), dim3(256), 0, c->s, N,
+                         (const double *)c->nu64, (float *)c->NUs[k]);
+    else
+      hipLaunchKernelGGL((k_fill_nu_from_master<double>), dim3(grid(N, 256)), dim3(256), 0, c->s, N,
+                         (const double *)c->nu64, (double *)c->NUs[k]);
+    HIPCHK(hipGetLastError());
   }
+  HIPCHK(hipStreamSynchronize(c->s));
   return IBG_OK;
 }
 
@@ -1064,10 +1113,13 @@ ibg_status ibg_get_nut(ibg_ctx *c, double *nut, int64_t n) {
   if (!c || !nut || !c->have_mesh || n != c->N) return IBG_ERR_ARG;
   if (!c->turb) return IBG_ERR_STATE;
   const int N = c->N;
-  std::vector<double> a(N);
   HIPCHK(hipStreamSynchronize(c->s));
-  HIPCHK(hipMemcpy(a.data(), c->nu64, sizeof(double) * N, hipMemcpyDeviceToHost));
-  for (int i = 0; i < N; ++i) nut[c->d2h[i]] = a[i];
+  double *a = (double *)host_stage(c, sizeof(double) * (size_t)N);
+  if (!a) return IBG_ERR_HIP;
+  HIPCHK(hipMemcpy(a, c->nu64, sizeof(double) * N, hipMemcpyDeviceToHost));
+  host_parallel(N, [&](int i0, int i1) {
+    for (int i = i0; i < i1; ++i) nut[c->d2h[i]] = a[i];
+  });
   return IBG_OK;
 }
 
@@ -1075,21 +1127,24 @@ ibg_status ibg_get_state(ibg_ctx *c, double *fv, int64_t n) {
   if (!c || !fv || !c->have_mesh || n != c->N) return IBG_ERR_ARG;
   const int N = c->N;
   HIPCHK(hipStreamSynchronize(c->s));
-  std::vector<D4> a(N);
-  std::vector<double> e(c->D == 3 ? N : 0);
-  HIPCHK(hipMemcpy(a.data(), c->W64a, sizeof(D4) * N, hipMemcpyDeviceToHost));
-  if (c->D == 3) HIPCHK(hipMemcpy(e.data(), c->W64e, sizeof(double) * N, hipMemcpyDeviceToHost));
-  for (int i = 0; i < N; ++i) {
-    double *q = fv + (size_t)c->d2h[i] * IBG_NFV;
-    const DevPhys &P = c->hphys[c->hcellblk[i] / c->L];
-    const double r = a[i].x, u = a[i].y, v = a[i].z;
-    const double w = (c->D == 3) ? a[i].w : 0.0;
-    const double p = (c->D == 3) ? e[i] : a[i].w;
-    q[0] = r; q[1] = u; q[2] = v; q[3] = w; q[4] = p;
-    // derived: Get_T_ideal_gas and Get_E (FlowVariables.H:23,33)
-    q[5] = p / (r * P.Rg);
-    q[6] = P.Cv * q[5] + 0.5 * (u * u + v * v + w * w);
-  }
+  D4 *a = (D4 *)host_stage(c, (sizeof(D4) + (c->D == 3 ? sizeof(double) : 0)) * (size_t)N);
+  if (!a) return IBG_ERR_HIP;
+  double *e = (double *)(a + N);
+  HIPCHK(hipMemcpy(a, c->W64a, sizeof(D4) * N, hipMemcpyDeviceToHost));
+  if (c->D == 3) HIPCHK(hipMemcpy(e, c->W64e, sizeof(double) * N, hipMemcpyDeviceToHost));
+  host_parallel(N, [&](int i0, int i1) {
+    for (int i = i0; i < i1; ++i) {
+      double *q = fv + (size_t)c->d2h[i] * IBG_NFV;
+      const DevPhys &P = c->hphys[c->hcellblk[i] / c->L];
+      const double r = a[i].x, u = a[i].y, v = a[i].z;
+      const double w = (c->D == 3) ? a[i].w : 0.0;
+      const double p = (c->D == 3) ? e[i] : a[i].w;
+      q[0] = r; q[1] = u; q[2] = v; q[3] = w; q[4] = p;
+      // derived: Get_T_ideal_gas and Get_E (FlowVariables.H:23,33)
+      q[5] = p / (r * P.Rg);
+      q[6] = P.Cv * q[5] + 0.5 * (u * u + v * v + w * w);
+    }
+  });
   return IBG_OK;
 }
 

@@ -914,6 +914,24 @@ __global__ void k_dt_final(int L, const int *ct, const double *cflpart, double *
   }
 }
 
+// Stencil copies of the FP64 master after ibg_set_state / ibg_set_nut (one
+// upload, the conversions on the device).
+template <typename R, int D>
+__global__ void k_fill_from_master(int n, Pack<double, D> src, Pack<R, D> dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const D4 q = src.a[i];
+  typename V4<R>::T o;
+  o.x = (R)q.x; o.y = (R)q.y; o.z = (R)q.z; o.w = (R)q.w;
+  dst.a[i] = o;
+  if (D == 3) dst.e[i] = (R)src.e[i];
+}
+template <typename R>
+__global__ void k_fill_nu_from_master(int n, const double *src, R *dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (R)src[i];
+}
+
 __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
