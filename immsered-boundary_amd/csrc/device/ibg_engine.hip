@@ -286,7 +286,10 @@ ibg_status launch_stage(ibg_ctx *c, int ntile, int ncell, int subt, bool last, d
   // the step's first stage reads the home buffer S0, the state k_visc_prep reads
   a.fuse_visc = (fuse && subt == 0 && kFuseVisc<R, D>) ? 1 : 0;
   a.va = visc_args<R, D>(c);
-  hipLaunchKernelGGL((k_stage<R, D, ML>), dim3(ntile), dim3(c->tile), tile_smem(c, sizeof(R), true), c->s, a);
+  if constexpr (ML)
+    hipLaunchKernelGGL((k_stage_ml<D>), dim3(ntile), dim3(c->tile), tile_smem(c, sizeof(R), true), c->s, a);
+  else
+    hipLaunchKernelGGL((k_stage<R, D, false>), dim3(ntile), dim3(c->tile), tile_smem(c, sizeof(R), true), c->s, a);
   HIPCHK(hipGetLastError());
   if (c->cfg.nstage == 1) {
     // single-stage scheme: the marched cells go back into the home buffer

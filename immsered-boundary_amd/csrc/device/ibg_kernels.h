@@ -278,7 +278,7 @@ constexpr size_t kStageMlSmem = (size_t)64 * kX * 2 + (size_t)64 * kH * 2 + 3 * 
 // master holds the step-start state until the last stage rewrites it, and
 // each cell's master entry is read and written by the same thread.
 template <typename R, int D, bool ML>
-__global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
+__device__ __forceinline__ void stage_tile(const StageArgs<R, D> &a) {
   constexpr int NV = D + 2;
   constexpr bool kMixed = std::is_same<R, float>::value;
   using V = typename V4<R>::T;
@@ -631,6 +631,20 @@ __global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
       }
     }
   }
+}
+
+template <typename R, int D, bool ML>
+__global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
+  stage_tile<R, D, ML>(a);
+}
+
+// The learned-closure stage with the scalar registers held to what eight
+// waves per SIMD allow (its many parameter pointers otherwise take 102
+// SGPRs, which caps the occupancy at seven).
+template <int D>
+__global__ void __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_stage_ml(StageArgs<float, D> a) {
+  stage_tile<float, D, true>(a);
 }
 
 // Stages with a single substep write into buffer 1; this copies the marched
