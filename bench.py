@@ -41,12 +41,12 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 
 # Measured HBM traffic of the learned-closure stage kernel, bytes per cell of
 # a launch: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch) over
-# every dispatch of k_stage<float,2,true> in this script's profiled run,
+# every dispatch of k_stage_ml<2> in this script's profiled run,
 # scaled by what the counters read for a plain 1 GiB copy (FETCH_SIZE x2.0,
 # WRITE_SIZE x1.0), divided by the cells of the launch (the launch mix of the
 # timed steps).  Recorded in profiles/k_stage_naca2d_sa_traffic.md; update it
 # with the kernel.
-TRAFFIC_BYTES_PER_CELL = 162.0
+TRAFFIC_BYTES_PER_CELL = 162.6
 
 
 def stage_bytes(mode, dim, turb, cells, faces, last=False, first=False):
@@ -390,7 +390,7 @@ def main():
                      "frac": ml_gbs / HBM_PEAK_GBS,
                      "traffic": (TRAFFIC_BYTES_PER_CELL * ml["stage_cells"]
                                  if TRAFFIC_BYTES_PER_CELL else None),
-                     "kernel": "k_stage<float,2,ML>", "bytes_per_launch": ml_bytes,
+                     "kernel": "k_stage_ml<2>", "bytes_per_launch": ml_bytes,
                      "avg_launch_ms": ml["stage_ms"], "cells_per_launch": ml["stage_cells"],
                      "copy_probe_GBs": copy_gbs},
         "cpu_baseline": None,
