@@ -19,8 +19,9 @@
 // 23,33).  The SA working variable, when the closure is on, has the same
 // master + stencil buffer structure.
 //
-// Hot kernel: k_stage (ibg_kernels.h), one workgroup per tile of TILE
-// Morton-consecutive normal cells; face fluxes live only in LDS.
+// Hot kernel: k_stage / k_stage_ml (ibg_kernels.h), one workgroup per tile
+// of TILE Morton-consecutive normal cells (tiles given to the XCDs in
+// contiguous runs); face fluxes live only in LDS.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -265,13 +265,16 @@ constexpr size_t kStageMlSmem = (size_t)64 * kX * 2 + (size_t)64 * kH * 2 + 3 * 
 
 // One workgroup per tile of Morton-consecutive normal cells of one
 // (case, level).  Phase 1: every face touching the tile — MUSCL + all-speed
-// Roe (NS_Solver.H:704-832) — into LDS (with the SA closure, also the
-// velocity and working variable of both face cells).  Phase 2: ComptSource
+// Roe (NS_Solver.H:704-832) — into LDS (with the SA closure, also the two
+// face cells' indices; phase 2 gathers their velocity and working variable
+// from cache, phase 1 having just read them).  Phase 2: ComptSource
 // (NS_Solver.cpp:282-363), ComptMutSource (:632-840), UpdateConsVars
 // (:844-985) and ComptNewPrimVar (:3-194) for the tile's cells.  The ML
-// instantiation (mixed mode) evaluates the point-wise SA term with the
-// network for the cells inside its training range, between the two halves
-// of phase 2, on MFMA.
+// instantiation (mixed mode, launched as k_stage_ml) evaluates the
+// point-wise SA term with the network for the cells inside its training
+// range, between the two halves of phase 2, on MFMA.  The 2D FP64 first
+// stage of a time-march step also does the viscous pass of the tile first
+// (fuse_visc).
 //
 // Win and Wout are different buffers, so no workgroup reads a state another
 // one has already advanced within the launch.  U_old is cons(W64): the
