@@ -114,6 +114,35 @@ def test_steps_fp64_match_oracle(which, nstage, request):
     assert np.allclose(gl2, l2, rtol=1e-10, atol=1e-300)
 
 
+@pytest.mark.parametrize("opt", [dict(limiter=1), dict(viscous=False), dict(temporal_refine=False)])
+def test_steps_fp64_scheme_options(tiny, opt):
+    """The van Albada limiter, the inviscid path and the march without
+    temporal sub-cycling against the oracle with the same options."""
+    fv0 = perturbed_state(tiny)
+    nsteps, nstage = 6, 4
+    tr = opt.get("temporal_refine", True)
+    P = (1 << (tiny.nlevels - 1)) if tr else 1
+    orc = oracle_ffi.Oracle(tiny, oracle_ffi.cfg(nstage=nstage, dt_var_step=P,
+                                                 temporal_refine=1 if tr else 0,
+                                                 limiter=opt.get("limiter", 0),
+                                                 viscous=0 if opt.get("viscous", True) is False else 1))
+    cpu = fv0.copy()
+    dt = np.array([tiny.phys[0].dt_init])
+    for ts in range(nsteps):
+        code, _, _, _ = orc.step(cpu, dt, ts)
+        assert code == 0, code
+    eng = ibgpu.Engine(tiny, mode="fp64", nstage=nstage, **opt)
+    eng.set_state(fv0)
+    eng.TimeMarching(0, nsteps, use_graph=False)
+    assert rel(eng.get_state()[:, :5], cpu[:, :5]) < TOL
+    assert np.allclose(eng.get_dt(), dt, rtol=1e-14, atol=0)
+    # the mixed mode with the same options stays within its bound
+    mx = ibgpu.Engine(tiny, mode="mixed", nstage=nstage, **opt)
+    mx.set_state(fv0)
+    mx.TimeMarching(0, nsteps, use_graph=True)
+    assert rel(mx.get_state()[:, :5], cpu[:, :5]) < 1e-5
+
+
 def test_stagewise_source_matches_oracle(tiny):
     """Phase by phase through the C ABI; source[0] of the last stage."""
     mesh = tiny

@@ -80,6 +80,16 @@ def test_vanalbada_known_value():
     assert L.orc_muscl_left(0.0, 1.0, 3.0, 1) == pytest.approx(1.0 + 0.5 * 1.2, rel=1e-15)
 
 
+@pytest.mark.parametrize("r,phi", [(0.5, 0.6), (1.0, 1.0), (3.0, 1.2), (0.25, 5.0 / 17.0)])
+def test_vanalbada_values_and_midpoint(r, phi):
+    # (r^2 + r)/(r^2 + 1) at r = 1/2, 1, 3, 1/4; the right state mirrors the left one
+    assert L.orc_muscl_left(0.0, 1.0, 1.0 + r, 1) == pytest.approx(1.0 + 0.5 * phi, rel=1e-15)
+    assert L.orc_muscl_right(1.0 + r, 1.0, 0.0, 1) == pytest.approx(1.0 + 0.5 * phi, rel=1e-15)
+    for a, sl in [(0.3, 0.1), (-2.0, 0.7)]:
+        q = [a - sl, a, a + sl]
+        assert L.orc_muscl_left(q[0], q[1], q[2], 1) == pytest.approx(a + 0.5 * sl, rel=1e-15)
+
+
 # ----------------------------------------------------------------- Roe flux
 def fv_of(P, r, u, v, w, p):
     T = p / (r * P.Rg)
