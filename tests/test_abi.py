@@ -6,6 +6,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from ibgpu import _ffi
@@ -54,3 +55,73 @@ def test_bad_arguments_rejected():
     assert lib.ibg_create(C.byref(cfg), C.byref(ctx)) == 1
     assert lib.ibg_destroy(None) == 1
     assert lib.ibg_check_mesh(None, None) == 1
+
+
+# ---------------------------------------------------------- mesh validation
+# ibg_upload_mesh runs ibg_check_mesh before any launch, so a malformed
+# mesh is refused on the host; each corruption below must be caught with
+# its message, and the clean mesh must pass.
+@pytest.fixture(scope="module")
+def tiny_mesh():
+    import ibgpu
+    return ibgpu.Mesh(ibgpu.tiny_case())
+
+
+def check(lib, mesh):
+    msg = C.c_char_p()
+    st = lib.ibg_check_mesh(mesh.soa_ptr, C.byref(msg))
+    return st, (msg.value.decode() if msg.value else "")
+
+
+def test_clean_mesh_passes(tiny_mesh):
+    lib = _ffi.gpu_lib()
+    assert check(lib, tiny_mesh)[0] == 0
+
+
+@pytest.mark.parametrize("what,expect", [
+    ("face_cells", "face"),
+    ("nb_face_stencil", "face stencil without LL/RR"),
+    ("res_fi", "restriction son missing"),
+    ("pro_ci", "prolongation coarse cell"),
+    ("ib_intp", "IB interpolation cell"),
+    ("ib_into_ib", "image point interpolates from an IB cell"),
+    ("pat_intp", "patch interpolation cell missing"),
+    ("blk_start", "blk_start"),
+])
+def test_corrupted_mesh_rejected(tiny_mesh, what, expect):
+    lib = _ffi.gpu_lib()
+    m = tiny_mesh
+    v = np.ctypeslib.as_array
+    if what == "face_cells":
+        arr, idx, bad = m.face_cells, (3, 1), m.ncells + 7
+    elif what == "nb_face_stencil":
+        # both the LL neighbour of an x face's low cell and the RR neighbour
+        # of its high cell missing (one missing is the reference's fallback)
+        f = int(np.nonzero(m.face_dir == 0)[0][0])
+        lo, hi = int(m.face_cells[f, 0]), int(m.face_cells[f, 1])
+        keep = m.nb[hi, 2 * 9 + 1 * 3 + 1]
+        m.nb[hi, 2 * 9 + 1 * 3 + 1] = -1
+        arr, idx, bad = m.nb, (lo, 0 * 9 + 1 * 3 + 1), -1
+    elif what == "res_fi":
+        arr, idx, bad = m.res_fi, (0, 0), -1
+    elif what == "pro_ci":
+        arr, idx, bad = m.pro_ci, (0,), m.ncells
+    elif what == "ib_intp":
+        arr, idx, bad = m.ib_intp, (0, 1), -3
+    elif what == "ib_into_ib":
+        arr, idx, bad = m.ib_intp, (0, 1), int(m.ib_cell[1])
+    elif what == "pat_intp":
+        arr, idx, bad = m.pat_intp, (0, 1), m.ncells + 1
+    else:
+        arr = v(m.soa.blk_start, (m.ncases * m.nlevels + 1,))
+        idx, bad = (1,), -5
+    old = arr[idx]
+    try:
+        arr[idx] = bad
+        st, msg = check(lib, m)
+        assert st == 1 and expect in msg, (st, msg)
+    finally:
+        arr[idx] = old
+        if what == "nb_face_stencil":
+            m.nb[hi, 2 * 9 + 1 * 3 + 1] = keep
+    assert check(lib, m)[0] == 0
