@@ -1591,6 +1591,11 @@ static ibg_status cl_put_stats(ibg_ctx *c, const float *st34) {
 
 ibg_status ibg_closure_init(ibg_ctx *c, int64_t capacity, uint64_t seed) {
   if (!c || capacity < 64) return IBG_ERR_ARG;
+  // captured time-march graphs may hold the closure buffers about to be
+  // freed (the network variant of the stage kernel): drop them
+  HIPCHK(hipStreamSynchronize(c->s));
+  for (int k = 0; k < 2; ++k)
+    if (c->gexec[k]) { (void)hipGraphExecDestroy(c->gexec[k]); c->gexec[k] = nullptr; }
   for (void *p : c->cl.allocs) (void)hipFree(p);
   c->cl = ibg_ctx::Closure();
   ibg_ctx::Closure &L = c->cl;

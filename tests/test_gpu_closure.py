@@ -108,6 +108,34 @@ def test_sampler_and_training_deterministic(tiny):
     assert len(np.unique(a[4][:a[0]])) == a[0]   # one row per kept cell in a round
 
 
+def test_reinit_drops_captured_graphs(tiny):
+    """A time-march graph captured with the network on holds the closure's
+    buffers; re-initialising the closure (which frees them and turns the
+    network off) must not replay it: the march continues as with the
+    closure switched off."""
+    P = 1 << (tiny.nlevels - 1)
+    prm = rand_prm(seed=5, scale=0.05)
+    prm[O.NPARAM + 16:O.NPARAM + 24] = -1e30     # training range: everything
+    prm[O.NPARAM + 24:O.NPARAM + 32] = 1e30
+
+    def start():
+        e = eng_sa(tiny)
+        e.closure_init(1 << 10)
+        e.closure_set_params(prm)
+        e.closure_enable(True)
+        e.TimeMarching(0, P, use_graph=True)
+        return e
+
+    a = start()
+    a.closure_init(1 << 10)
+    a.TimeMarching(P, P, use_graph=True)
+    b = start()
+    b.closure_enable(False)
+    b.TimeMarching(P, P, use_graph=True)
+    assert np.array_equal(a.get_state(), b.get_state())
+    assert np.array_equal(a.get_nut(), b.get_nut())
+
+
 def trained(mesh, steps_run=64, train_steps=1500, seed=20261202):
     """Network trained on cells of the plain-closure run (FP64 SA)."""
     src = eng_sa(mesh, mode="fp64")
