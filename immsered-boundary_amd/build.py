@@ -41,8 +41,8 @@ def build_host():
     os.makedirs(LIB, exist_ok=True)
     src = os.path.join(HERE, "csrc", "host", "ib_mesh.cpp")
     out = os.path.join(LIB, "libibhost.so")
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", f"-I{INC}",
-          src, "-o", out + ".tmp"])
+    _run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-fopenmp",
+          f"-I{INC}", src, "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
 
 
@@ -71,8 +71,9 @@ def build_oracle():
     bdir = os.path.join(ROOT, "oracle", "_build")
     os.makedirs(bdir, exist_ok=True)
     out = os.path.join(bdir, "liboracle.so")
-    _run(["gcc", "-O2", "-std=c99", "-fPIC", "-shared", "-fopenmp", f"-I{INC}",
-          os.path.join(ROOT, "oracle", "oracle.c"), os.path.join(ROOT, "oracle", "mlp.c"),
+    _run(["gcc", "-O2", "-std=c99", "-ffp-contract=off", "-fPIC", "-shared", "-fopenmp",
+          f"-I{INC}", os.path.join(ROOT, "oracle", "oracle.c"),
+          os.path.join(ROOT, "oracle", "mlp.c"), os.path.join(ROOT, "oracle", "retag.c"),
           "-o", out + ".tmp", "-lm"])
     os.replace(out + ".tmp", out)
 

@@ -37,6 +37,7 @@
 #include "ibg_common.h"
 #include "ibg_kernels.h"
 #include "ibg_closure.h"
+#include "ibg_move.h"
 
 using namespace ibg;
 
@@ -110,6 +111,27 @@ struct ibg_ctx {
          *pat_area = nullptr, *case_center = nullptr;
   Motion *motion = nullptr;
   int have_motion = 0;
+  // moving bodies (2D sections on a fixed level set); device arrays live in
+  // allocs with the mesh
+  struct MoveState {
+    bool on = false, law = false;
+    int npoint = 0, cap = 0, nblk256 = 0;
+    int *key = nullptr, *cpatch = nullptr, *cp1 = nullptr;
+    int8_t *kind = nullptr, *fin = nullptr, *l1 = nullptr, *isib = nullptr;
+    int16_t *cblk = nullptr;
+    double *origin = nullptr, *dist1 = nullptr, *sd1 = nullptr;
+    double *pt_off = nullptr, *pat_dz = nullptr;
+    int *pat_pt0 = nullptr, *pat_body = nullptr, *body_p0 = nullptr, *pat_cc = nullptr;
+    int8_t *pat_flip = nullptr;
+    Pose *pose = nullptr;
+    int *ib_n = nullptr, *bcount = nullptr, *boff = nullptr;
+    unsigned long long *upd = nullptr, *lvlcnt = nullptr;
+    PitchHeave *lawd = nullptr;
+    Motion *motc = nullptr;       // per case, for the wall force
+    int *bcase_d = nullptr;       // case of each body
+    double *t = nullptr;
+    std::vector<int> body_case;   // host: case of each body
+  } mv;
   double *dt = nullptr, *cflpart = nullptr, *rpart = nullptr, *resid = nullptr, *probe = nullptr;
   unsigned long long *err = nullptr;
   int last_ntiles = 0;
@@ -201,6 +223,7 @@ void free_all(ibg_ctx *c) {
   for (void *p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
   c->have_mesh = false;
+  c->mv = ibg_ctx::MoveState{};
 }
 
 inline int marched(const ibg_ctx *c, int level, int ts) {
@@ -390,8 +413,10 @@ ibg_status run_exchange(ibg_ctx *c, int ts) {
 
 template <int D>
 ibg_status launch_ib(ibg_ctx *c) {
-  if (c->nib > 0) {
-    hipLaunchKernelGGL((k_ib<D>), dim3(grid(c->nib, 256)), dim3(256), 0, c->s, c->nib, c->ib_cell,
+  const int n = c->mv.on ? c->mv.cap : c->nib;
+  if (n > 0) {
+    hipLaunchKernelGGL((k_ib<D>), dim3(grid(n, 256)), dim3(256), 0, c->s, n,
+                       c->mv.on ? c->mv.ib_n : (const int *)nullptr, c->ib_cell,
                        c->ib_intp, c->ib_coef, c->ib_hgdis, c->ib_attach, c->ib_nv, c->ib_body,
                        c->ib_case, c->signdis, c->motion, c->have_motion, c->sout<D>(), c->phys,
                        c->err);
@@ -401,6 +426,64 @@ ibg_status launch_ib(ibg_ctx *c) {
 }
 
 ibg_status run_ib(ibg_ctx *c) { return c->D == 2 ? launch_ib<2>(c) : launch_ib<3>(c); }
+
+MoveArgs move_args(ibg_ctx *c, unsigned long long *upd, int lmin) {
+  MoveArgs a{};
+  ibg_ctx::MoveState &v = c->mv;
+  a.N = c->N; a.Nn = c->Nn; a.npatch = c->npatch;
+  a.key = v.key; a.kind = v.kind; a.fin = v.fin; a.cblk = v.cblk; a.blk = c->blk;
+  a.origin = v.origin; a.nb27 = c->nb27;
+  a.signdis = c->signdis; a.sd32 = c->sd32;
+  a.cpatch = v.cpatch; a.cp1 = v.cp1; a.dist1 = v.dist1; a.sd1 = v.sd1;
+  a.l1 = v.l1; a.isib = v.isib; a.flags = c->flags;
+  a.pt_off = v.pt_off; a.pat_pt0 = v.pat_pt0; a.pat_body = v.pat_body; a.pat_case = c->pat_case;
+  a.pat_flip = v.pat_flip; a.pat_dz = v.pat_dz; a.body_p0 = v.body_p0; a.pose = v.pose;
+  a.pat_pc = c->pat_pc; a.pat_nv = c->pat_nv; a.pat_area = c->pat_area;
+  a.pat_intp = c->pat_intp; a.pat_cc = v.pat_cc; a.pat_coef = c->pat_coef; a.pat_hgdis = c->pat_hgdis;
+  a.ib_cap = v.cap; a.ib_n = v.ib_n; a.bcount = v.bcount; a.boff = v.boff;
+  a.ib_cell = c->ib_cell; a.ib_intp = c->ib_intp; a.ib_body = c->ib_body; a.ib_case = c->ib_case;
+  a.ib_coef = c->ib_coef; a.ib_hgdis = c->ib_hgdis; a.ib_attach = c->ib_attach; a.ib_nv = c->ib_nv;
+  a.err = c->err; a.upd = upd; a.lvlcnt = v.lvlcnt; a.upd_lmin = lmin; a.nstage = c->cfg.nstage;
+  return a;
+}
+
+// One move of the bodies to the poses in mv.pose: surface, distances, solid
+// and IB sets, image points, stage masks.  Stream-ordered, capturable.
+template <int D>
+ibg_status launch_move(ibg_ctx *c, unsigned long long *upd, int lmin) {
+  const MoveArgs a = move_args(c, upd, lmin);
+  const int nb = c->mv.nblk256;
+  hipLaunchKernelGGL(k_motion_center, dim3(grid(c->ncases, 64)), dim3(64), 0, c->s, c->ncases,
+                     (const Pose *)c->mv.pose, c->case_center);
+  hipLaunchKernelGGL(k_move_patches, dim3(grid(c->npatch, 256)), dim3(256), 0, c->s, a);
+  hipLaunchKernelGGL(k_move_walk, dim3(grid(c->N, 256)), dim3(256), 0, c->s, a);
+  hipLaunchKernelGGL((k_move_cycle<D>), dim3(grid(c->N, 256)), dim3(256), 0, c->s, a);
+  hipLaunchKernelGGL((k_move_l1<D>), dim3(grid(c->N, 256)), dim3(256), 0, c->s, a);
+  hipLaunchKernelGGL((k_move_ib<D>), dim3(nb), dim3(256), 0, c->s, a);
+  hipLaunchKernelGGL(k_move_scan, dim3(1), dim3(1024), 0, c->s, a, nb);
+  hipLaunchKernelGGL(k_move_list, dim3(nb), dim3(256), 0, c->s, a);
+  hipLaunchKernelGGL((k_move_image<D>), dim3(grid(c->mv.cap, 256)), dim3(256), 0, c->s, a);
+  hipLaunchKernelGGL((k_move_patch_image<D>), dim3(grid(c->npatch, 64)), dim3(64), 0, c->s, a);
+  hipLaunchKernelGGL(k_move_flags, dim3(grid(c->Nn, 256)), dim3(256), 0, c->s, a);
+  HIPCHK(hipGetLastError());
+  return IBG_OK;
+}
+
+ibg_status run_move(ibg_ctx *c, unsigned long long *upd, int lmin) {
+  if (c->D != 2) return IBG_ERR_STATE;
+  return launch_move<2>(c, upd, lmin);
+}
+
+// Pose and wall motion of the prescribed law at each case's time.
+ibg_status run_motion_law(ibg_ctx *c) {
+  ibg_ctx::MoveState &v = c->mv;
+  const int n = std::max(c->ncases, c->nbody);
+  hipLaunchKernelGGL(k_motion_pose, dim3(grid(n, 64)), dim3(64), 0, c->s, c->ncases,
+                     (const PitchHeave *)v.lawd, (const double *)v.t, v.pose, c->nbody,
+                     (const int *)v.bcase_d, c->motion, v.motc, (const double *)c->case_center);
+  HIPCHK(hipGetLastError());
+  return IBG_OK;
+}
 
 ibg_status run_dt(ibg_ctx *c) {
   if (c->ntiles > 0) {
@@ -422,6 +505,11 @@ ibg_status run_dt(ibg_ctx *c) {
 ibg_status run_step(ibg_ctx *c, int ts) {
   ibg_status st;
   if ((st = run_exchange(c, ts))) return st;
+  if (c->mv.law) {
+    // MovingBody_Image at the start of IBConditions_Image (Body.cpp:1353-1357)
+    if ((st = run_motion_law(c))) return st;
+    if ((st = run_move(c, c->mv.upd, lmin_of(c, ts)))) return st;
+  }
   if ((st = run_ib(c))) return st;
   if (c->cfg.dt_var_step > 0 && ts % c->cfg.dt_var_step == 0)
     if ((st = run_dt(c))) return st;
@@ -431,6 +519,11 @@ ibg_status run_step(ibg_ctx *c, int ts) {
     if ((st = run_visc_prep(c, ts))) return st;
   for (int k = 0; k < c->cfg.nstage; ++k)
     if ((st = run_stage(c, ts, k, nullptr, nullptr, fuse))) return st;
+  if (c->mv.law) {
+    hipLaunchKernelGGL(k_motion_advance, dim3(grid(c->ncases, 64)), dim3(64), 0, c->s, c->ncases,
+                       c->mv.t, (const double *)c->dt);
+    HIPCHK(hipGetLastError());
+  }
   return IBG_OK;
 }
 
@@ -1215,11 +1308,16 @@ ibg_status ibg_time_march(ibg_ctx *c, int32_t ts0, int32_t nsteps, int32_t use_g
   if (nsteps < 0) return IBG_ERR_ARG;
   ibg_status st;
   if ((st = reset_err(c))) return st;
-  c->have_motion = 0;
+  c->have_motion = c->mv.law ? 1 : 0;
   const int P = c->cfg.temporal_refine ? (1 << (c->L - 1)) : 1;
   int64_t upd = 0;
   for (int k = 0; k < nsteps; ++k) upd += c->upd_per_lmin[lmin_of(c, ts0 + k)] * c->cfg.nstage;
   c->last_updates = upd;
+  if (c->mv.law) {
+    // the moving-body masks change every step: counted on the device
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->s, c->mv.upd, 1, 0ull);
+    HIPCHK(hipGetLastError());
+  }
   HIPCHK(hipEventRecord(c->ev0, c->s));
   int ts = ts0;
   const int end = ts0 + nsteps;
@@ -1255,6 +1353,12 @@ ibg_status ibg_time_march(ibg_ctx *c, int32_t ts0, int32_t nsteps, int32_t use_g
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_total_ms = ms;
+  if (c->mv.law) {
+    unsigned long long u = 0;
+    HIPCHK(hipMemcpyAsync(&u, c->mv.upd, sizeof(u), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->last_updates = (int64_t)u;
+  }
   int64_t cell;
   int32_t code;
   if ((st = read_err(c, &cell, &code))) return st;
@@ -1314,17 +1418,294 @@ ibg_status ibg_forces(ibg_ctx *c, const ibg_body_motion *mo, double *out) {
     HIPCHK(hipMemcpyAsync(c->motion, h.data(), sizeof(Motion) * h.size(), hipMemcpyHostToDevice, c->s));
     hm = 1;
   }
+  const Motion *mp = c->motion;
+  if (!mo && c->mv.law) { mp = c->mv.motc; hm = 1; }   // the prescribed law's motion
   if (c->D == 2)
     hipLaunchKernelGGL((k_forces<2>), dim3(c->ncases), dim3(256), 0, c->s, c->npatch, c->pat_case,
                        c->pat_intp, c->pat_coef, c->pat_hgdis, c->pat_nv, c->pat_pc, c->pat_area,
-                       c->case_center, c->motion, hm, c->w64<2>(), c->phys, (int)c->turb, c->resid);
+                       c->case_center, mp, hm, c->w64<2>(), c->phys, (int)c->turb, c->resid);
   else
     hipLaunchKernelGGL((k_forces<3>), dim3(c->ncases), dim3(256), 0, c->s, c->npatch, c->pat_case,
                        c->pat_intp, c->pat_coef, c->pat_hgdis, c->pat_nv, c->pat_pc, c->pat_area,
-                       c->case_center, c->motion, hm, c->w64<3>(), c->phys, (int)c->turb, c->resid);
+                       c->case_center, mp, hm, c->w64<3>(), c->phys, (int)c->turb, c->resid);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, c->resid, sizeof(double) * 6 * c->ncases, hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
+  return IBG_OK;
+}
+
+// ------------------------------------------------------------ moving bodies
+ibg_status ibg_upload_surface(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_surface *sf) {
+  if (!c || !m || !sf) return IBG_ERR_ARG;
+  if (!c->have_mesh) return IBG_ERR_STATE;
+  auto fail = [&](const char *msg) {
+    c->errmsg = msg;
+    fprintf(stderr, "ibgpu: surface rejected: %s\n", msg);
+    return IBG_ERR_ARG;
+  };
+  if (m->ncells != c->N || m->npatch != c->npatch || m->nbody != c->nbody || m->dim != c->D)
+    return fail("surface and uploaded mesh differ");
+  if (c->D != 2) return fail("moving bodies: 2D cases only");
+  if (!sf->pt_off || !sf->pat_pt0 || !sf->pat_body || !sf->pat_flip || !sf->pat_dz ||
+      !sf->body_p0 || !sf->case_origin || !sf->cell_patch || !sf->pat_close || sf->npoint < 2)
+    return fail("missing surface array");
+  const int N = c->N, np = c->npatch, nbody = c->nbody, L = c->L;
+  if (sf->body_p0[0] != 0 || sf->body_p0[nbody] != np) return fail("body patch ranges");
+  for (int b = 0; b < nbody; ++b)
+    if (sf->body_p0[b + 1] < sf->body_p0[b]) return fail("body patch ranges");
+  std::vector<int> pbody(np), pcase(np);
+  HIPCHK(hipMemcpy(pcase.data(), c->pat_case, sizeof(int) * (size_t)np, hipMemcpyDeviceToHost));
+  for (int p = 0; p < np; ++p) {
+    const int b = sf->pat_body[p];
+    if (b < 0 || b >= nbody || p < sf->body_p0[b] || p >= sf->body_p0[b + 1])
+      return fail("patch body out of range");
+    if (c->hbody_case[b] != pcase[p]) return fail("patch and body cases differ");
+    if (sf->pat_pt0[p] < 0 || sf->pat_pt0[p] + 1 >= sf->npoint) return fail("patch point out of range");
+  }
+  auto finest_cell = [&](int g) {
+    const int b = c->hcellblk[c->h2d[g]];
+    return (b % L) == L - 1;
+  };
+  for (int g = 0; g < N; ++g) {
+    const int p = sf->cell_patch[g];
+    if (p < -1 || p >= np) return fail("cell patch out of range");
+    if (p >= 0 && pcase[p] != (int)c->hcellblk[c->h2d[g]] / L) return fail("cell patch of another case");
+  }
+  for (int p = 0; p < np; ++p) {
+    const int g = sf->pat_close[p];
+    if (g < 0 || g >= N || !finest_cell(g)) return fail("patch close cell not on the finest level");
+  }
+  (void)hipStreamSynchronize(c->s);
+  // device-order per-cell arrays
+  std::vector<int> key((size_t)N * 3), cp(N);
+  std::vector<int8_t> kind(N), fin(N);
+  for (int i = 0; i < N; ++i) {
+    const int g = c->d2h[i];
+    for (int d = 0; d < 3; ++d) key[(size_t)i * 3 + d] = m->key[(size_t)g * 3 + d];
+    kind[i] = m->kind[g];
+    const int b = c->hcellblk[i];
+    fin[i] = ((b % L) == L - 1 && nbody > 0) ? 1 : 0;
+    cp[i] = sf->cell_patch[g];
+  }
+  std::vector<int> pt0(sf->pat_pt0, sf->pat_pt0 + np), pb(sf->pat_body, sf->pat_body + np);
+  std::vector<int8_t> flip(sf->pat_flip, sf->pat_flip + np);
+  std::vector<double> dz(sf->pat_dz, sf->pat_dz + np);
+  std::vector<int> bp0(sf->body_p0, sf->body_p0 + nbody + 1);
+  std::vector<double> off(sf->pt_off, sf->pt_off + 2 * (size_t)sf->npoint);
+  std::vector<double> org(sf->case_origin, sf->case_origin + 3 * (size_t)c->ncases);
+  std::vector<int> cc(np), bcase(c->hbody_case);
+  for (int p = 0; p < np; ++p) cc[p] = c->h2d[sf->pat_close[p]];
+  std::vector<int16_t> cblk(c->hcellblk);
+  ibg_ctx::MoveState &v = c->mv;
+  v = ibg_ctx::MoveState{};
+  ibg_status st;
+#define UP(dst, vec) if ((st = dupload(c, &dst, vec))) return st
+#define AL(dst, n) if ((st = dalloc(c, &dst, n))) return st
+  UP(v.key, key); UP(v.kind, kind); UP(v.fin, fin); UP(v.cblk, cblk); UP(v.cpatch, cp);
+  UP(v.origin, org); UP(v.pt_off, off); UP(v.pat_pt0, pt0); UP(v.pat_body, pb);
+  UP(v.pat_flip, flip); UP(v.pat_dz, dz); UP(v.body_p0, bp0); UP(v.pat_cc, cc);
+  UP(v.bcase_d, bcase);
+  {
+    std::vector<int8_t> isib0(N, 0);   // the current IB set
+    for (int j = 0; j < m->nib; ++j) isib0[c->h2d[m->ib_cell[j]]] = 1;
+    UP(v.isib, isib0);
+  }
+  AL(v.cp1, N); AL(v.dist1, N); AL(v.sd1, N); AL(v.l1, N);
+  AL(v.pose, c->ncases); AL(v.motc, c->ncases); AL(v.lawd, c->ncases); AL(v.t, c->ncases);
+  v.nblk256 = (int)grid(N, 256);
+  AL(v.bcount, v.nblk256); AL(v.boff, v.nblk256);
+  AL(v.ib_n, 1); AL(v.upd, 1); AL(v.lvlcnt, IBG_MAX_LEVELS);
+  // IB arrays at capacity, holding the current list
+  const int cap = sf->ib_capacity > 0 ? sf->ib_capacity : 2 * c->nib + 4096;
+  if (cap < c->nib) return fail("IB capacity below the current list");
+  int *nc_ = nullptr, *ni = nullptr, *nbd = nullptr, *ncs = nullptr;
+  double *nco = nullptr, *nh = nullptr, *nat = nullptr, *nnv = nullptr;
+  AL(nc_, cap); AL(ni, (size_t)cap * 8); AL(nbd, cap); AL(ncs, cap);
+  AL(nco, (size_t)cap * 8); AL(nh, cap); AL(nat, (size_t)cap * 3); AL(nnv, (size_t)cap * 3);
+#undef UP
+#undef AL
+  const size_t n0 = (size_t)c->nib;
+  if (n0) {
+    HIPCHK(hipMemcpy(nc_, c->ib_cell, n0 * sizeof(int), hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(ni, c->ib_intp, n0 * 8 * sizeof(int), hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(nbd, c->ib_body, n0 * sizeof(int), hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(ncs, c->ib_case, n0 * sizeof(int), hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(nco, c->ib_coef, n0 * 8 * sizeof(double), hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(nh, c->ib_hgdis, n0 * sizeof(double), hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(nat, c->ib_attach, n0 * 3 * sizeof(double), hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(nnv, c->ib_nv, n0 * 3 * sizeof(double), hipMemcpyDeviceToDevice));
+  }
+  c->ib_cell = nc_; c->ib_intp = ni; c->ib_body = nbd; c->ib_case = ncs;
+  c->ib_coef = nco; c->ib_hgdis = nh; c->ib_attach = nat; c->ib_nv = nnv;
+  {
+    const int n = c->nib;
+    HIPCHK(hipMemcpy(v.ib_n, &n, sizeof(int), hipMemcpyHostToDevice));
+  }
+  v.cap = cap;
+  v.npoint = sf->npoint;
+  v.body_case = c->hbody_case;
+  v.on = true;
+  // captured graphs hold the old IB launch
+  for (int k = 0; k < 2; ++k)
+    if (c->gexec[k]) { (void)hipGraphExecDestroy(c->gexec[k]); c->gexec[k] = nullptr; }
+  return IBG_OK;
+}
+
+namespace {
+// per-level update counts after a move: the cell-stage updates of a step
+ibg_status recount_updates(ibg_ctx *c) {
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->s, c->mv.lvlcnt, IBG_MAX_LEVELS, 0ull);
+  const MoveArgs a = move_args(c, nullptr, 0);
+  hipLaunchKernelGGL(k_move_levelcount, dim3(grid(c->Nn, 256)), dim3(256), 0, c->s, a, c->L);
+  HIPCHK(hipGetLastError());
+  std::vector<unsigned long long> h(IBG_MAX_LEVELS);
+  HIPCHK(hipMemcpyAsync(h.data(), c->mv.lvlcnt, sizeof(unsigned long long) * IBG_MAX_LEVELS,
+                        hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  for (int lm = 0; lm < c->L; ++lm) {
+    int64_t s = 0;
+    for (int l = lm; l < c->L; ++l) s += (int64_t)h[l];
+    c->upd_per_lmin[lm] = s;
+  }
+  return IBG_OK;
+}
+}  // namespace
+
+ibg_status ibg_move_bodies(ibg_ctx *c, const ibg_body_pose *pose) {
+  if (!c || !pose) return IBG_ERR_ARG;
+  if (!c->have_mesh || !c->mv.on) return IBG_ERR_STATE;
+  std::vector<Pose> hp(c->ncases);
+  for (int k = 0; k < c->ncases; ++k) {
+    hp[k].px = pose[k].pivot[0]; hp[k].py = pose[k].pivot[1];
+    hp[k].c = pose[k].cosa; hp[k].s = pose[k].sina;
+  }
+  for (int b = 0; b < c->nbody; ++b) {
+    c->body_center[(size_t)b * 3] = hp[c->hbody_case[b]].px;
+    c->body_center[(size_t)b * 3 + 1] = hp[c->hbody_case[b]].py;
+  }
+  ibg_status st;
+  if ((st = reset_err(c))) return st;
+  HIPCHK(hipMemcpyAsync(c->mv.pose, hp.data(), sizeof(Pose) * hp.size(), hipMemcpyHostToDevice, c->s));
+  if ((st = run_move(c, nullptr, 0))) return st;
+  if ((st = recount_updates(c))) return st;
+  int64_t cell;
+  int32_t code;
+  if ((st = read_err(c, &cell, &code))) return st;
+  return code ? (ibg_status)code : IBG_OK;
+}
+
+ibg_status ibg_set_motion(ibg_ctx *c, const ibg_pitch_heave *law, double t0) {
+  if (!c) return IBG_ERR_ARG;
+  if (!c->have_mesh || !c->mv.on) return IBG_ERR_STATE;
+  for (int k = 0; k < 2; ++k)
+    if (c->gexec[k]) { (void)hipGraphExecDestroy(c->gexec[k]); c->gexec[k] = nullptr; }
+  if (!law) { c->mv.law = false; return IBG_OK; }
+  const double d2r = 3.14159265358979323846 / 180.0;
+  std::vector<PitchHeave> h(c->ncases);
+  std::vector<double> t(c->ncases, t0);
+  for (int k = 0; k < c->ncases; ++k) {
+    h[k].x0 = law[k].x0; h[k].y0 = law[k].y0;
+    h[k].aoa0 = law[k].aoa0_deg * d2r; h[k].amp = law[k].amp_deg * d2r;
+    h[k].omega = law[k].omega; h[k].heave = law[k].heave;
+  }
+  HIPCHK(hipMemcpyAsync(c->mv.lawd, h.data(), sizeof(PitchHeave) * h.size(), hipMemcpyHostToDevice, c->s));
+  HIPCHK(hipMemcpyAsync(c->mv.t, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  for (int b = 0; b < c->nbody; ++b) {
+    c->body_center[(size_t)b * 3] = h[c->hbody_case[b]].x0;
+    c->body_center[(size_t)b * 3 + 1] = h[c->hbody_case[b]].y0;
+  }
+  c->mv.law = true;
+  return IBG_OK;
+}
+
+ibg_status ibg_get_time(ibg_ctx *c, double *t) {
+  if (!c || !t) return IBG_ERR_ARG;
+  if (!c->have_mesh || !c->mv.on) return IBG_ERR_STATE;
+  HIPCHK(hipMemcpyAsync(t, c->mv.t, sizeof(double) * c->ncases, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return IBG_OK;
+}
+
+ibg_status ibg_get_geometry(ibg_ctx *c, double *signdis, int32_t *cell_patch, int8_t *ib,
+                            int32_t *nib) {
+  if (!c) return IBG_ERR_ARG;
+  if (!c->have_mesh || !c->mv.on) return IBG_ERR_STATE;
+  const int N = c->N;
+  std::vector<double> sd(N);
+  std::vector<int> cp(N);
+  std::vector<int8_t> isib(N);
+  HIPCHK(hipMemcpyAsync(sd.data(), c->signdis, sizeof(double) * N, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemcpyAsync(cp.data(), c->mv.cpatch, sizeof(int) * N, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemcpyAsync(isib.data(), c->mv.isib, N, hipMemcpyDeviceToHost, c->s));
+  int n = 0;
+  HIPCHK(hipMemcpyAsync(&n, c->mv.ib_n, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  for (int i = 0; i < N; ++i) {
+    const int g = c->d2h[i];
+    if (signdis) signdis[g] = sd[i];
+    if (cell_patch) cell_patch[g] = cp[i];
+    if (ib) ib[g] = isib[i] == 1 ? 1 : 0;
+  }
+  if (nib) *nib = n;
+  return IBG_OK;
+}
+
+ibg_status ibg_get_ib_list(ibg_ctx *c, int32_t cap, int32_t *cell, int32_t *intp, double *coef,
+                           double *hgdis, double *nv, double *attach, int32_t *body) {
+  if (!c || cap < 0) return IBG_ERR_ARG;
+  if (!c->have_mesh) return IBG_ERR_STATE;
+  int n = c->nib;
+  if (c->mv.on) {
+    HIPCHK(hipMemcpyAsync(&n, c->mv.ib_n, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+  }
+  if (n > cap) return IBG_ERR_ARG;
+  std::vector<int> hc(n), hi((size_t)n * 8), hb(n);
+  std::vector<double> hco((size_t)n * 8), hh(n), hn((size_t)n * 3), ha((size_t)n * 3);
+  if (n) {
+    HIPCHK(hipMemcpyAsync(hc.data(), c->ib_cell, sizeof(int) * n, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(hi.data(), c->ib_intp, sizeof(int) * 8 * n, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(hb.data(), c->ib_body, sizeof(int) * n, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(hco.data(), c->ib_coef, sizeof(double) * 8 * n, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(hh.data(), c->ib_hgdis, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(hn.data(), c->ib_nv, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(ha.data(), c->ib_attach, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+  }
+  for (int j = 0; j < n; ++j) {
+    if (cell) cell[j] = c->d2h[hc[j]];
+    for (int q = 0; q < 8; ++q) {
+      const int a = hi[(size_t)j * 8 + q];
+      if (intp) intp[(size_t)j * 8 + q] = a < 0 ? -1 : c->d2h[a];
+      if (coef) coef[(size_t)j * 8 + q] = hco[(size_t)j * 8 + q];
+    }
+    if (hgdis) hgdis[j] = hh[j];
+    for (int d = 0; d < 3; ++d) {
+      if (nv) nv[(size_t)j * 3 + d] = hn[(size_t)j * 3 + d];
+      if (attach) attach[(size_t)j * 3 + d] = ha[(size_t)j * 3 + d];
+    }
+    if (body) body[j] = hb[j];
+  }
+  return IBG_OK;
+}
+
+ibg_status ibg_get_patches(ibg_ctx *c, double *pc, double *nv, double *area, int32_t *intp,
+                           double *coef, double *hgdis) {
+  if (!c) return IBG_ERR_ARG;
+  if (!c->have_mesh) return IBG_ERR_STATE;
+  const int np = c->npatch;
+  std::vector<int> hi((size_t)np * 8);
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (pc) HIPCHK(hipMemcpy(pc, c->pat_pc, sizeof(double) * 3 * np, hipMemcpyDeviceToHost));
+  if (nv) HIPCHK(hipMemcpy(nv, c->pat_nv, sizeof(double) * 3 * np, hipMemcpyDeviceToHost));
+  if (area) HIPCHK(hipMemcpy(area, c->pat_area, sizeof(double) * np, hipMemcpyDeviceToHost));
+  if (coef) HIPCHK(hipMemcpy(coef, c->pat_coef, sizeof(double) * 8 * np, hipMemcpyDeviceToHost));
+  if (hgdis) HIPCHK(hipMemcpy(hgdis, c->pat_hgdis, sizeof(double) * np, hipMemcpyDeviceToHost));
+  if (intp) {
+    HIPCHK(hipMemcpy(hi.data(), c->pat_intp, sizeof(int) * 8 * np, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < hi.size(); ++k) intp[k] = hi[k] < 0 ? -1 : c->d2h[hi[k]];
+  }
   return IBG_OK;
 }
 

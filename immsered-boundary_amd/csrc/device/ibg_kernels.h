@@ -794,12 +794,14 @@ __global__ void k_prolong(int n, const int *ci, const int *fi, const int *nb27, 
 // (checked at upload), so one pass equals the reference's compute-all-then-
 // write-all.
 template <int D>
-__global__ void k_ib(int n, const int *cell, const int *intp, const double *coef,
+__global__ void k_ib(int n, const int *ndev, const int *cell, const int *intp, const double *coef,
                      const double *hgdis, const double *attach, const double *nvec, const int *body,
                      const int *ibcase, const double *signdis, const Motion *mot, int have_motion,
                      StateOut<D> o, const DevPhys *phys, unsigned long long *err) {
+  // n: list length, or its capacity when the length lives on the device
+  // (moving bodies rebuild the list every step)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || (ndev && i >= ndev[0])) return;
   const DevPhys &P = phys[ibcase[i]];
   double T = 0, p = 0, u = 0, v = 0, w = 0;
   for (int q = 0; q < 8; ++q) {

@@ -349,8 +349,9 @@ struct ibm_handle {
   std::vector<int32_t> res_ci, res_fi, res_blk, pro_ci, pro_fi, pro_blk;
   std::vector<int32_t> ib_cell, ib_intp, ib_body;
   std::vector<double> ib_coef, ib_hgdis, ib_nv, ib_attach;
-  std::vector<int32_t> pat_intp, pat_case;
+  std::vector<int32_t> pat_intp, pat_case, pat_close;
   std::vector<double> pat_coef, pat_hgdis, pat_nv, pat_pc, pat_area;
+  std::vector<int32_t> cell_patch;   // BoxtoWall::patch, global patch index
   std::vector<int32_t> body_case;
   std::vector<double> body_center;
   std::vector<ibm_case> cases;
@@ -669,6 +670,7 @@ void build_all(ibm_handle &H, std::vector<CaseMesh> &cms) {
         for (int q = 0; q < 8; ++q) { intp[q] = -1; coef[q] = 0.0; }
         if (!search_image(F, H, h, cs.origin, P.pc, P.nv, solid, H.ibidx, intp, coef, &hgdis, &cc))
           throw std::runtime_error("no valid image point for a surface patch");
+        H.pat_close.push_back(cc);
         for (int q = 0; q < 8; ++q) { H.pat_intp.push_back(intp[q]); H.pat_coef.push_back(coef[q]); }
         H.pat_hgdis.push_back(hgdis);
         for (int d = 0; d < 3; ++d) { H.pat_nv.push_back(P.nv[d]); H.pat_pc.push_back(P.pc[d]); }
@@ -678,7 +680,17 @@ void build_all(ibm_handle &H, std::vector<CaseMesh> &cms) {
       patch_base += (int)cm.patches.size();
     }
   }
-  (void)patch_base;
+  // closest patch of every cell as a global patch index
+  H.cell_patch.assign(N, -1);
+  {
+    std::vector<int> base(nc + 1, 0);
+    for (int c = 0; c < nc; ++c) base[c + 1] = base[c] + (cms[c].cs.nbody > 0 ? (int)cms[c].patches.size() : 0);
+    for (int c = 0; c < nc; ++c) {
+      if (cms[c].cs.nbody == 0) continue;
+      for (int g = H.blk_start[c * L]; g < H.blk_start[(c + 1) * L]; ++g)
+        if (cpatch[g] >= 0) H.cell_patch[g] = base[c] + cpatch[g];
+    }
+  }
 }
 
 void fill_soa(ibm_handle &H) {
@@ -748,6 +760,18 @@ const char *ibm_last_error(void) { return g_last_error.c_str(); }
 
 void ibm_free(ibm_handle *h) { delete h; }
 
+int32_t ibm_cell_patch(const ibm_handle *h, int32_t *out) {
+  if (!h || !out) return 1;
+  std::memcpy(out, h->cell_patch.data(), h->cell_patch.size() * sizeof(int32_t));
+  return 0;
+}
+
+int32_t ibm_patch_close(const ibm_handle *h, int32_t *out) {
+  if (!h || !out) return 1;
+  std::memcpy(out, h->pat_close.data(), h->pat_close.size() * sizeof(int32_t));
+  return 0;
+}
+
 int32_t ibm_cell_centers(const ibm_handle *h, double *xyz) {
   if (!h || !xyz) return 1;
   std::memcpy(xyz, h->center.data(), h->center.size() * sizeof(double));
@@ -815,21 +839,35 @@ int32_t ibm_init_state(const ibm_handle *h, const ibg_physics *phys, double *fv)
   return 0;
 }
 
-void ibm_naca4(double t, int32_t npts, double xc, double yc, double aoa_deg, double *xy) {
+void ibm_naca4_offsets(double t, int32_t npts, double *off) {
   const double pi = 3.14159265358979323846;
-  const double a = -aoa_deg * pi / 180.0;   // nose up: rotate clockwise
-  const double ca = std::cos(a), sa = std::sin(a);
   for (int s = 0; s < 2; ++s) {
     for (int i = 0; i < npts; ++i) {
       double x = 0.5 * (1.0 - std::cos(pi * i / (npts - 1)));
       double yt = 5.0 * t * (0.2969 * std::sqrt(x) - 0.1260 * x - 0.3516 * x * x +
                              0.2843 * x * x * x - 0.1036 * x * x * x * x);
-      double y = (s == 0) ? yt : -yt;
-      double px = x - 0.25, py = y;          // about the quarter chord
-      double rx = ca * px - sa * py, ry = sa * px + ca * py;
-      xy[2 * (s * npts + i)] = xc + rx;
-      xy[2 * (s * npts + i) + 1] = yc + ry;
+      off[2 * (s * npts + i)] = x - 0.25;    // about the quarter chord
+      off[2 * (s * npts + i) + 1] = (s == 0) ? yt : -yt;
     }
+  }
+}
+
+void ibm_pose(double xc, double yc, double aoa_deg, double *pose) {
+  const double pi = 3.14159265358979323846;
+  const double a = -aoa_deg * pi / 180.0;   // nose up: rotate clockwise
+  pose[0] = xc; pose[1] = yc; pose[2] = std::cos(a); pose[3] = std::sin(a);
+}
+
+void ibm_naca4(double t, int32_t npts, double xc, double yc, double aoa_deg, double *xy) {
+  std::vector<double> off((size_t)4 * npts);
+  double P[4];
+  ibm_naca4_offsets(t, npts, off.data());
+  ibm_pose(xc, yc, aoa_deg, P);
+  for (int k = 0; k < 2 * npts; ++k) {
+    const double px = off[2 * k], py = off[2 * k + 1];
+    const double rx = P[2] * px - P[3] * py, ry = P[3] * px + P[2] * py;
+    xy[2 * k] = P[0] + rx;
+    xy[2 * k + 1] = P[1] + ry;
   }
 }
 

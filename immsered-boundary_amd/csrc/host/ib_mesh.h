@@ -60,6 +60,11 @@ int32_t ibm_physics(const ibm_case *c, ibg_physics *out);
    NS_Solver.H:432-446), AoS [ncells][7]                                    */
 int32_t ibm_init_state(const ibm_handle *h, const ibg_physics *phys,
                        double *fv_aos);
+/* closest surface patch of every cell (BoxtoWall::patch, Box.H:68-106) as
+   an index into the mesh's pat_* arrays, -1 for cases without bodies        */
+int32_t ibm_cell_patch(const ibm_handle *h, int32_t *out /* [ncells] */);
+/* per surface patch: the cell holding its image point (HGCell::closecell) */
+int32_t ibm_patch_close(const ibm_handle *h, int32_t *out /* [npatch] */);
 /* cell centers [ncells][3]                                                 */
 int32_t ibm_cell_centers(const ibm_handle *h, double *xyz);
 /* error text of the last failed build                                      */
@@ -72,6 +77,11 @@ void ibm_free(ibm_handle *h);
    [2*npts][2].                                                             */
 void ibm_naca4(double t, int32_t npts, double xc, double yc, double aoa_deg,
                double *xy);
+/* The same points before the pose: offsets from the quarter chord at zero
+   angle, [2*npts][2]; ibm_naca4 = pivot + R(aoa) * offset with the pivot
+   and rotation that ibm_pose writes as {xc, yc, cos a, sin a}.             */
+void ibm_naca4_offsets(double t, int32_t npts, double *off);
+void ibm_pose(double xc, double yc, double aoa_deg, double *pose /* [4] */);
 
 #ifdef __cplusplus
 }

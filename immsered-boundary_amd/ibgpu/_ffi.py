@@ -69,6 +69,23 @@ class BodyMotion(C.Structure):
     _fields_ = [("tra_vel", C.c_double * 3), ("rot_vel", C.c_double * 3)]
 
 
+class Surface(C.Structure):
+    _fields_ = [
+        ("npoint", C.c_int32), ("pt_off", f64p), ("pat_pt0", i32p), ("pat_body", i32p),
+        ("pat_flip", i8p), ("pat_dz", f64p), ("body_p0", i32p), ("case_origin", f64p),
+        ("cell_patch", i32p), ("pat_close", i32p), ("ib_capacity", C.c_int32),
+    ]
+
+
+class BodyPose(C.Structure):
+    _fields_ = [("pivot", C.c_double * 2), ("cosa", C.c_double), ("sina", C.c_double)]
+
+
+class PitchHeave(C.Structure):
+    _fields_ = [("x0", C.c_double), ("y0", C.c_double), ("aoa0_deg", C.c_double),
+                ("amp_deg", C.c_double), ("omega", C.c_double), ("heave", C.c_double)]
+
+
 class Config(C.Structure):
     _fields_ = [
         ("mode", C.c_int32), ("nstage", C.c_int32), ("alpha", C.c_double * 4),
@@ -129,6 +146,14 @@ def host_lib():
         lib.ibm_naca4.restype = None
         lib.ibm_naca4.argtypes = [C.c_double, C.c_int32, C.c_double, C.c_double,
                                   C.c_double, f64p]
+        lib.ibm_naca4_offsets.restype = None
+        lib.ibm_naca4_offsets.argtypes = [C.c_double, C.c_int32, f64p]
+        lib.ibm_pose.restype = None
+        lib.ibm_pose.argtypes = [C.c_double, C.c_double, C.c_double, f64p]
+        lib.ibm_cell_patch.restype = C.c_int32
+        lib.ibm_cell_patch.argtypes = [C.c_void_p, i32p]
+        lib.ibm_patch_close.restype = C.c_int32
+        lib.ibm_patch_close.argtypes = [C.c_void_p, i32p]
         lib._typed = True
     return lib
 
@@ -163,6 +188,14 @@ GPU_FUNCS = {
     "ibg_set_dt": (C.c_int, [C.c_void_p, f64p]),
     "ibg_last_error": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
     "ibg_synchronize": (C.c_int, [C.c_void_p]),
+    "ibg_upload_surface": (C.c_int, [C.c_void_p, C.POINTER(MeshSoA), C.POINTER(Surface)]),
+    "ibg_move_bodies": (C.c_int, [C.c_void_p, C.POINTER(BodyPose)]),
+    "ibg_set_motion": (C.c_int, [C.c_void_p, C.POINTER(PitchHeave), C.c_double]),
+    "ibg_get_time": (C.c_int, [C.c_void_p, f64p]),
+    "ibg_get_geometry": (C.c_int, [C.c_void_p, f64p, i32p, i8p, i32p]),
+    "ibg_get_ib_list": (C.c_int, [C.c_void_p, C.c_int32, i32p, i32p, f64p, f64p, f64p, f64p,
+                                  i32p]),
+    "ibg_get_patches": (C.c_int, [C.c_void_p, f64p, f64p, f64p, i32p, f64p, f64p]),
     "ibg_last_timing": (C.c_int, [C.c_void_p, f64p, f64p, C.POINTER(C.c_int64)]),
     "ibg_bench_stage": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, f64p,
                                   C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),

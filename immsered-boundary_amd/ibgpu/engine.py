@@ -271,6 +271,73 @@ class Engine:
         self.lib.ibg_cell_stage_updates(self.ctx, C.byref(n))
         return n.value
 
+    # moving bodies ------------------------------------------------------------
+    def upload_surface(self, ib_capacity=0):
+        """Enable moving bodies (Body::MovingBody_Image between regrids)."""
+        self._surface = self.mesh.surface(ib_capacity)
+        self._chk(self.lib.ibg_upload_surface(self.ctx, self.mesh.soa_ptr,
+                                              C.byref(self._surface[0])), "ibg_upload_surface")
+
+    def move_bodies(self, poses):
+        """poses: per case {pivot x, pivot y, cos a, sin a} (Mesh.pose)."""
+        arr = (_ffi.BodyPose * self.mesh.ncases)()
+        for k, p in enumerate(poses):
+            arr[k].pivot[0], arr[k].pivot[1] = float(p[0]), float(p[1])
+            arr[k].cosa, arr[k].sina = float(p[2]), float(p[3])
+        self._chk(self.lib.ibg_move_bodies(self.ctx, arr), "ibg_move_bodies")
+
+    def set_motion(self, laws, t0=0.0):
+        """laws: per case (x0, y0, aoa0_deg, amp_deg, omega, heave) or None."""
+        if laws is None:
+            self._chk(self.lib.ibg_set_motion(self.ctx, None, 0.0), "ibg_set_motion")
+            return
+        arr = (_ffi.PitchHeave * self.mesh.ncases)()
+        for k, lw in enumerate(laws):
+            (arr[k].x0, arr[k].y0, arr[k].aoa0_deg, arr[k].amp_deg, arr[k].omega,
+             arr[k].heave) = [float(x) for x in lw]
+        self._chk(self.lib.ibg_set_motion(self.ctx, arr, float(t0)), "ibg_set_motion")
+
+    def get_time(self):
+        t = np.zeros(self.mesh.ncases)
+        self._chk(self.lib.ibg_get_time(self.ctx, t.ctypes.data_as(_ffi.f64p)), "ibg_get_time")
+        return t
+
+    def geometry(self):
+        """(signdis, closest patch, IB flag) per cell and the IB count."""
+        n = self.mesh.ncells
+        sd = np.zeros(n)
+        cp = np.zeros(n, np.int32)
+        ib = np.zeros(n, np.int8)
+        nib = C.c_int32()
+        self._chk(self.lib.ibg_get_geometry(self.ctx, sd.ctypes.data_as(_ffi.f64p),
+                                            cp.ctypes.data_as(_ffi.i32p),
+                                            ib.ctypes.data_as(_ffi.i8p), C.byref(nib)),
+                  "ibg_get_geometry")
+        return sd, cp, ib, nib.value
+
+    def ib_list(self, cap):
+        o = {"cell": np.zeros(cap, np.int32), "intp": np.zeros((cap, 8), np.int32),
+             "coef": np.zeros((cap, 8)), "hgdis": np.zeros(cap), "nv": np.zeros((cap, 3)),
+             "attach": np.zeros((cap, 3)), "body": np.zeros(cap, np.int32)}
+        self._chk(self.lib.ibg_get_ib_list(
+            self.ctx, cap, o["cell"].ctypes.data_as(_ffi.i32p), o["intp"].ctypes.data_as(_ffi.i32p),
+            o["coef"].ctypes.data_as(_ffi.f64p), o["hgdis"].ctypes.data_as(_ffi.f64p),
+            o["nv"].ctypes.data_as(_ffi.f64p), o["attach"].ctypes.data_as(_ffi.f64p),
+            o["body"].ctypes.data_as(_ffi.i32p)), "ibg_get_ib_list")
+        return o
+
+    def patches(self):
+        np_ = self.mesh.npatch
+        o = {"pc": np.zeros((np_, 3)), "nv": np.zeros((np_, 3)), "area": np.zeros(np_),
+             "intp": np.zeros((np_, 8), np.int32), "coef": np.zeros((np_, 8)),
+             "hgdis": np.zeros(np_)}
+        self._chk(self.lib.ibg_get_patches(
+            self.ctx, o["pc"].ctypes.data_as(_ffi.f64p), o["nv"].ctypes.data_as(_ffi.f64p),
+            o["area"].ctypes.data_as(_ffi.f64p), o["intp"].ctypes.data_as(_ffi.i32p),
+            o["coef"].ctypes.data_as(_ffi.f64p), o["hgdis"].ctypes.data_as(_ffi.f64p)),
+            "ibg_get_patches")
+        return o
+
     def synchronize(self):
         self._chk(self.lib.ibg_synchronize(self.ctx), "ibg_synchronize")
 

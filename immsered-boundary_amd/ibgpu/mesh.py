@@ -38,6 +38,7 @@ class CaseSpec:
     cfl: float = 0.5
     backpressure: float = 0.0
     span_patches: int = 1
+    pivot_y: float = 0.0          # y of the quarter chord (x is 0)
 
     def to_ctypes(self):
         c = _ffi.Case()
@@ -55,14 +56,14 @@ class CaseSpec:
             c.band_lo[i], c.band_hi[i] = a, b
         xy = np.zeros((2 * self.npts, 2), dtype=np.float64)
         host = _ffi.host_lib()
-        host.ibm_naca4(self.thickness, self.npts, 0.0, 0.0, self.aoa,
+        host.ibm_naca4(self.thickness, self.npts, 0.0, self.pivot_y, self.aoa,
                        xy.ctypes.data_as(_ffi.f64p))
         self._xy = np.ascontiguousarray(xy)       # keep alive
         c.nbody = 2
         c.body_npts[0] = self.npts
         c.body_npts[1] = self.npts
         c.body_xy = self._xy.ctypes.data_as(_ffi.f64p)
-        c.body_center[0], c.body_center[1], c.body_center[2] = 0.0, 0.0, 0.5 * self.span
+        c.body_center[0], c.body_center[1], c.body_center[2] = 0.0, self.pivot_y, 0.5 * self.span
         c.span = self.span
         c.span_patches = self.span_patches
         c.gama, c.Ma, c.Re, c.Pr = self.gama, self.Ma, self.Re, self.Pr
@@ -200,6 +201,69 @@ class Mesh:
         for b in range(self.ncases * self.nlevels):
             lv[self.blk_start[b]:self.blk_start[b + 1]] = b % self.nlevels
         return lv
+
+    def surface(self, ib_capacity=0):
+        """ibg_surface of the mesh's 2D sections for moving bodies: zero-pose
+        point offsets from each case's quarter chord (ibm_naca4_offsets), the
+        segment of every patch (the builder makes one patch per segment, upper
+        surface first), the patch range per body, the closest patch of every
+        cell and the image cell of every patch.  Returns (Surface, arrays)."""
+        if self.dim != 2:
+            raise ValueError("moving bodies: 2D cases only")
+        host = self._host
+        offs, pt0, pbody, flip, dz, bp0 = [], [], [], [], [], [0]
+        npt = 0
+        gb = 0
+        for c, sp in enumerate(self.specs):
+            if sp.span_patches != 1:
+                raise ValueError("moving bodies: one patch per segment")
+            o = np.zeros((2 * sp.npts, 2))
+            host.ibm_naca4_offsets(sp.thickness, sp.npts, o.ctypes.data_as(_ffi.f64p))
+            offs.append(o)
+            for b in range(2):
+                for i in range(sp.npts - 1):
+                    pt0.append(npt + b * sp.npts + i)
+                    pbody.append(gb)
+                    flip.append(1 if b % 2 == 1 else 0)
+                    dz.append(sp.span / sp.span_patches)
+                bp0.append(bp0[-1] + sp.npts - 1)
+                gb += 1
+            npt += 2 * sp.npts
+        if len(pt0) != self.npatch:
+            raise ValueError("patch count differs from one patch per segment")
+        a = {
+            "pt_off": np.ascontiguousarray(np.concatenate(offs)),
+            "pat_pt0": np.asarray(pt0, np.int32), "pat_body": np.asarray(pbody, np.int32),
+            "pat_flip": np.asarray(flip, np.int8), "pat_dz": np.asarray(dz, np.float64),
+            "body_p0": np.asarray(bp0, np.int32),
+            "case_origin": np.asarray([[sp.lo, sp.lo, 0.0] for sp in self.specs], np.float64),
+            "cell_patch": np.zeros(self.ncells, np.int32),
+            "pat_close": np.zeros(max(self.npatch, 1), np.int32),
+        }
+        host.ibm_cell_patch(self._h, a["cell_patch"].ctypes.data_as(_ffi.i32p))
+        host.ibm_patch_close(self._h, a["pat_close"].ctypes.data_as(_ffi.i32p))
+        s = _ffi.Surface()
+        s.npoint = npt
+        s.pt_off = a["pt_off"].ctypes.data_as(_ffi.f64p)
+        s.pat_pt0 = a["pat_pt0"].ctypes.data_as(_ffi.i32p)
+        s.pat_body = a["pat_body"].ctypes.data_as(_ffi.i32p)
+        s.pat_flip = a["pat_flip"].ctypes.data_as(_ffi.i8p)
+        s.pat_dz = a["pat_dz"].ctypes.data_as(_ffi.f64p)
+        s.body_p0 = a["body_p0"].ctypes.data_as(_ffi.i32p)
+        s.case_origin = a["case_origin"].ctypes.data_as(_ffi.f64p)
+        s.cell_patch = a["cell_patch"].ctypes.data_as(_ffi.i32p)
+        s.pat_close = a["pat_close"].ctypes.data_as(_ffi.i32p)
+        s.ib_capacity = int(ib_capacity)
+        s._keep = a
+        return s, a
+
+    @staticmethod
+    def pose(aoa_deg, pivot_y=0.0):
+        """{pivot x, pivot y, cos a, sin a} of a section at angle of attack
+        aoa_deg with its quarter chord at (0, pivot_y) (ibm_pose)."""
+        p = np.zeros(4)
+        _ffi.host_lib().ibm_pose(0.0, float(pivot_y), float(aoa_deg), p.ctypes.data_as(_ffi.f64p))
+        return p
 
     def init_state(self):
         fv = np.zeros((self.ncells, _ffi.IBG_NFV))

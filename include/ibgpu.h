@@ -260,6 +260,76 @@ ibg_status ibg_set_dt(ibg_ctx *ctx, const double *dt /* [ncases] */);
  * 888-894.                                                                  */
 ibg_status ibg_last_error(ibg_ctx *ctx, int64_t *cell, int32_t *code);
 
+/* ------------------------------------------------------------ moving bodies
+ * Rigid motion of the bodies of 2D cases on a fixed level set: the per-step
+ * part of Body::MovingBody_Image (Bodymove.cpp:37-125) that runs between
+ * regrids.  One move recomputes the surface patches (ComptNewLocation /
+ * ComptPatchParams :3-35), the closest patch and signed distance of every
+ * cell (RenewLevelDistance :127-258), the solid flags, the IB set
+ * (RemoveIBCells / InfectNewIBCells :345-767), the image points and weights
+ * of the IB cells (FindImageCell Body.cpp:999-1190) and of the wall-force
+ * patches (FindPatchHangingNode Body.cpp:1506-1575), and the stage masks.
+ * The reference point of every body of a case becomes the case's pivot.   */
+typedef struct ibg_surface {
+  int32_t npoint;
+  const double  *pt_off;       /* [npoint][2] surface point minus the pivot
+                                  at zero pose (Body::surfptoff)              */
+  const int32_t *pat_pt0;      /* [npatch] first point of the patch segment
+                                  (the mesh's pat_* order)                    */
+  const int32_t *pat_body;     /* [npatch] body of the patch                  */
+  const int8_t  *pat_flip;     /* [npatch] 1: outward normal is the right-
+                                  hand normal of the segment (lower surface)  */
+  const double  *pat_dz;       /* [npatch] z extent: area = length * dz       */
+  const int32_t *body_p0;      /* [nbody+1] patches of body b are
+                                  [body_p0[b], body_p0[b+1]), one polyline    */
+  const double  *case_origin;  /* [ncases][3] low corner of key (0,0,0)       */
+  const int32_t *cell_patch;   /* [ncells] BoxtoWall::patch, -1: no body      */
+  const int32_t *pat_close;    /* [npatch] HGCell::closecell of the patch     */
+  int32_t ib_capacity;         /* IB list capacity (0: 2*nib + 4096)          */
+} ibg_surface;
+
+/* Pose of the bodies of one case: point = pivot + R(a) * offset,
+ * R(a) = [[cosa, -sina], [sina, cosa]].                                    */
+typedef struct ibg_body_pose {
+  double pivot[2];
+  double cosa, sina;
+} ibg_body_pose;
+
+/* Prescribed pitch and heave of one case (the FSI configuration's motion):
+ * aoa(t) = aoa0 + amp sin(omega t) degrees nose up about the pivot,
+ * pivot y(t) = y0 + heave sin(omega t); the rotation is a = -aoa.        */
+typedef struct ibg_pitch_heave {
+  double x0, y0, aoa0_deg, amp_deg, omega, heave;
+} ibg_pitch_heave;
+
+/* Enable moving bodies for the uploaded mesh (2D only).  mesh is the mesh
+ * given to ibg_upload_mesh; the surface describes its bodies.             */
+ibg_status ibg_upload_surface(ibg_ctx *ctx, const ibg_mesh_soa *mesh,
+                              const ibg_surface *surf);
+/* Move the bodies to pose[ncases] now (stream-ordered; returns the
+ * reference's abort code 263 / 289 / 314 when the move is too large for
+ * the incremental update, 1082 for an IB cell on a ghost, 1090 when the
+ * IB list outgrows its capacity, 1183 / 1575 when no image point exists). */
+ibg_status ibg_move_bodies(ibg_ctx *ctx, const ibg_body_pose *pose);
+/* Move the bodies by law[ncases] at the start of every step of
+ * ibg_time_march (the pose at each case's time t, t0 at this call, t += dt
+ * after each step) with the law's wall velocity in the IB fill and, for
+ * motion == NULL, in ibg_forces.  law == NULL stops it.                    */
+ibg_status ibg_set_motion(ibg_ctx *ctx, const ibg_pitch_heave *law, double t0);
+ibg_status ibg_get_time(ibg_ctx *ctx, double *t /* [ncases] */);
+/* Probes of the moving-body geometry (host cell and patch indices).
+ * signdis / cell_patch / ib: [ncells] each (any may be NULL).             */
+ibg_status ibg_get_geometry(ibg_ctx *ctx, double *signdis, int32_t *cell_patch, int8_t *ib,
+                            int32_t *nib);
+/* The IB list (cap entries at most, cell order): IBCell::ci, intpcell[8],
+ * intpcoef[8], hgdis, patch normal, attach point, body.                    */
+ibg_status ibg_get_ib_list(ibg_ctx *ctx, int32_t cap, int32_t *cell, int32_t *intp,
+                           double *coef, double *hgdis, double *nv, double *attach,
+                           int32_t *body);
+/* Wall-force patches: centre, normal, area, image cells and weights.       */
+ibg_status ibg_get_patches(ibg_ctx *ctx, double *pc, double *nv, double *area, int32_t *intp,
+                           double *coef, double *hgdis);
+
 /* Wait for all work of the context.                                        */
 ibg_status ibg_synchronize(ibg_ctx *ctx);
 

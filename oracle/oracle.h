@@ -127,6 +127,45 @@ void orc_mlp_grad_f64(const double *p, const double *xs, const double *t, int64_
 int32_t orc_closure_features(const ibg_mesh_soa *m, const ibg_physics *ph, const double *fv,
                              const double *nut, int32_t cell, double *x, double *z);
 
+/* ------------------------------------------- moving bodies (retag.c)
+ * Surface of 2D sections: point offsets from the pivot at zero pose, the
+ * segment of each patch (mesh pat_* order) and the patch range per body.  */
+typedef struct orc_surface {
+  int32_t npoint, npatch, nbody;
+  const double *pt_off;        /* [npoint][2]                               */
+  const int32_t *pat_pt0;      /* [npatch] first point of the segment       */
+  const int32_t *pat_body;     /* [npatch] global body                      */
+  const int32_t *pat_case;     /* [npatch]                                  */
+  const int8_t *pat_flip;      /* [npatch] odd body: normal reversed        */
+  const double *pat_dz;        /* [npatch] z extent of the patch            */
+  const int32_t *body_p0;      /* [nbody+1] patch range of each body        */
+  const double *case_origin;   /* [ncases][3]                               */
+} orc_surface;
+
+/* In/out state of one move (caller-allocated).  signdis, cell_patch, ib
+ * (0/1 per cell) and pat_pc/pat_nv/pat_area hold the previous step on
+ * entry and the new one on return; the IB list comes back in cell order.  */
+typedef struct orc_retag_out {
+  double *signdis;
+  int32_t *cell_patch;
+  int8_t *ib;
+  double *pat_pc, *pat_nv, *pat_area;
+  int32_t ib_cap, nib;
+  int32_t *ib_cell, *ib_intp, *ib_body;
+  double *ib_coef, *ib_hgdis, *ib_nv, *ib_attach;
+  int32_t *pat_intp, *pat_close;
+  double *pat_coef, *pat_hgdis;
+  int32_t err_code;
+  int64_t err_cell;
+} orc_retag_out;
+
+/* Body::MovingBody_Image Bodymove.cpp:37-125 between regrids; pose
+ * [ncases][4] = {pivot x, pivot y, cos a, sin a}.  Returns 0 or the
+ * reference's abort code (263, 289, 314), 1082 (IB cell on a ghost), 1090
+ * (list capacity), 1183 / 1575 (no image point).                           */
+int32_t orc_move_bodies(const ibg_mesh_soa *m, const orc_surface *s, const double *pose,
+                        orc_retag_out *o);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,23 @@ class OrcCfg(C.Structure):
                 ("limiter", C.c_int32), ("viscous", C.c_int32)]
 
 
+class OrcSurface(C.Structure):
+    _fields_ = [("npoint", C.c_int32), ("npatch", C.c_int32), ("nbody", C.c_int32),
+                ("pt_off", f64p), ("pat_pt0", i32p), ("pat_body", i32p), ("pat_case", i32p),
+                ("pat_flip", C.POINTER(C.c_int8)), ("pat_dz", f64p), ("body_p0", i32p),
+                ("case_origin", f64p)]
+
+
+class OrcRetagOut(C.Structure):
+    _fields_ = [("signdis", f64p), ("cell_patch", i32p), ("ib", C.POINTER(C.c_int8)),
+                ("pat_pc", f64p), ("pat_nv", f64p), ("pat_area", f64p),
+                ("ib_cap", C.c_int32), ("nib", C.c_int32),
+                ("ib_cell", i32p), ("ib_intp", i32p), ("ib_body", i32p),
+                ("ib_coef", f64p), ("ib_hgdis", f64p), ("ib_nv", f64p), ("ib_attach", f64p),
+                ("pat_intp", i32p), ("pat_close", i32p), ("pat_coef", f64p), ("pat_hgdis", f64p),
+                ("err_code", C.c_int32), ("err_cell", C.c_int64)]
+
+
 _lib = None
 
 
@@ -81,6 +98,7 @@ def lib():
         "orc_mlp_loss_f64": (C.c_double, [f64p, f64p, f64p, C.c_int64]),
         "orc_mlp_grad_f64": (None, [f64p, f64p, f64p, C.c_int64, f64p, f64p]),
         "orc_closure_features": (C.c_int32, [M, P, f64p, f64p, C.c_int32, f64p, f64p]),
+        "orc_move_bodies": (C.c_int32, [M, C.POINTER(OrcSurface), f64p, C.POINTER(OrcRetagOut)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -142,6 +160,104 @@ def closure_features(mesh, fv, nut, cell):
     ok = lib().orc_closure_features(mesh.soa_ptr, mesh.phys, ptr(fv), ptr(nut), int(cell), ptr(x),
                                     C.byref(z))
     return bool(ok), x, z.value
+
+
+class MovingBodies:
+    """CPU restatement of the per-step moving-body update on a Mesh: holds
+    the geometry of the last move (signed distances, closest patches, IB
+    set, patches) and advances it with orc_move_bodies."""
+
+    def __init__(self, mesh, cap=None):
+        m = mesh
+        self.m = m
+        _, a = m.surface()
+        self.a = a
+        self.pat_case = np.ctypeslib.as_array(m.soa.pat_case, (m.npatch,)).astype(np.int32)
+        s = OrcSurface()
+        s.npoint = a["pt_off"].shape[0]
+        s.npatch = m.npatch
+        s.nbody = len(a["body_p0"]) - 1
+        s.pt_off = ptr(a["pt_off"])
+        s.pat_pt0 = a["pat_pt0"].ctypes.data_as(i32p)
+        s.pat_body = a["pat_body"].ctypes.data_as(i32p)
+        s.pat_case = self.pat_case.ctypes.data_as(i32p)
+        s.pat_flip = a["pat_flip"].ctypes.data_as(C.POINTER(C.c_int8))
+        s.pat_dz = ptr(a["pat_dz"])
+        s.body_p0 = a["body_p0"].ctypes.data_as(i32p)
+        s.case_origin = ptr(a["case_origin"])
+        self.s = s
+        n = m.ncells
+        self.signdis = np.array(m.signdis, dtype=np.float64)
+        self.cell_patch = a["cell_patch"].copy()
+        self.ib = (np.asarray(m.ibidx) >= 0).astype(np.int8)
+        v = np.ctypeslib.as_array
+        self.pat_pc = v(m.soa.pat_pc, (m.npatch, 3)).copy()
+        self.pat_nv = v(m.soa.pat_nv, (m.npatch, 3)).copy()
+        self.pat_area = v(m.soa.pat_area, (m.npatch,)).copy()
+        self.cap = cap or (2 * m.nib + 4096)
+
+    def move(self, poses):
+        m = self.m
+        cap, np_ = self.cap, m.npatch
+        r = {"cell": np.zeros(cap, np.int32), "intp": np.zeros((cap, 8), np.int32),
+             "body": np.zeros(cap, np.int32), "coef": np.zeros((cap, 8)), "hgdis": np.zeros(cap),
+             "nv": np.zeros((cap, 3)), "attach": np.zeros((cap, 3)),
+             "pat_intp": np.zeros((np_, 8), np.int32), "pat_close": np.zeros(np_, np.int32),
+             "pat_coef": np.zeros((np_, 8)), "pat_hgdis": np.zeros(np_)}
+        o = OrcRetagOut()
+        o.signdis = ptr(self.signdis)
+        o.cell_patch = self.cell_patch.ctypes.data_as(i32p)
+        o.ib = self.ib.ctypes.data_as(C.POINTER(C.c_int8))
+        o.pat_pc, o.pat_nv, o.pat_area = ptr(self.pat_pc), ptr(self.pat_nv), ptr(self.pat_area)
+        o.ib_cap = cap
+        o.ib_cell = r["cell"].ctypes.data_as(i32p)
+        o.ib_intp = r["intp"].ctypes.data_as(i32p)
+        o.ib_body = r["body"].ctypes.data_as(i32p)
+        o.ib_coef, o.ib_hgdis = ptr(r["coef"]), ptr(r["hgdis"])
+        o.ib_nv, o.ib_attach = ptr(r["nv"]), ptr(r["attach"])
+        o.pat_intp = r["pat_intp"].ctypes.data_as(i32p)
+        o.pat_close = r["pat_close"].ctypes.data_as(i32p)
+        o.pat_coef, o.pat_hgdis = ptr(r["pat_coef"]), ptr(r["pat_hgdis"])
+        pose = np.ascontiguousarray(np.asarray(poses, dtype=np.float64).reshape(m.ncases, 4))
+        code = lib().orc_move_bodies(m.soa_ptr, C.byref(self.s), ptr(pose), C.byref(o))
+        n = o.nib
+        for k in ("cell", "intp", "body", "coef", "hgdis", "nv", "attach"):
+            r[k] = r[k][:n]
+        r["code"], r["err_cell"], r["nib"] = int(code), int(o.err_cell), n
+        return r
+
+
+class MovedMesh:
+    """A Mesh whose immersed-boundary data are those of a move (the last
+    MovingBodies.move result): for oracle calls after the bodies moved."""
+
+    def __init__(self, mb, r):
+        m = mb.m
+        self._m = m
+        soa = type(m.soa).from_buffer_copy(m.soa)
+        n = r["nib"]
+        ibidx = np.full(m.ncells, -1, np.int32)
+        ibidx[r["cell"]] = np.arange(n, dtype=np.int32)
+        keep = {
+            "signdis": mb.signdis.copy(), "ibidx": ibidx,
+            "ib_cell": np.ascontiguousarray(r["cell"]), "ib_intp": np.ascontiguousarray(r["intp"]),
+            "ib_coef": np.ascontiguousarray(r["coef"]), "ib_hgdis": np.ascontiguousarray(r["hgdis"]),
+            "ib_nv": np.ascontiguousarray(r["nv"]), "ib_attach": np.ascontiguousarray(r["attach"]),
+            "ib_body": np.ascontiguousarray(r["body"]),
+            "pat_intp": r["pat_intp"].copy(), "pat_coef": r["pat_coef"].copy(),
+            "pat_hgdis": r["pat_hgdis"].copy(), "pat_nv": mb.pat_nv.copy(),
+            "pat_pc": mb.pat_pc.copy(), "pat_area": mb.pat_area.copy(),
+        }
+        for k, a in keep.items():
+            t = i32p if a.dtype == np.int32 else f64p
+            setattr(soa, k, a.ctypes.data_as(t))
+        soa.nib = n
+        self._keep = keep
+        self.soa = soa
+        self.soa_ptr = C.pointer(soa)
+
+    def __getattr__(self, k):
+        return getattr(self._m, k)
 
 
 def cfg(nstage=3, alpha=None, dt_var_step=1, temporal_refine=1, limiter=0, viscous=1):
