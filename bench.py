@@ -1,11 +1,14 @@
 """Throughput of the per-stage residual path on one MI355X.
 
-Workload: the mid-size 2D case with the SA closure — NACA0015 at Mach 0.3,
-Re 10000, in [-10,10]^2 with a 256x256 base grid and 5 AMR levels (6
+Workload: the mid-size 2D FSI case with the SA closure — NACA0015 at Mach
+0.3, Re 10000, in [-10,10]^2 with a 256x256 base grid and 5 AMR levels (6
 levels, ~2.6M leaf cells), 4-stage low-storage scheme (NS_Solver.H:204-208)
-with temporal sub-cycling, body held at its initial pose, Spalart-Allmaras
-working variable from a freestream of 5 (the reference's TURBULENCE build).
-A "step" is one InterfaceExchange + IBConditions_Image + TimeMarching
+with temporal sub-cycling, the section pitching 5 deg about its quarter
+chord and heaving 0.05 chord at reduced frequency 0.1 (the immersed
+boundary re-tagged every step on the fixed level set, Bodymove.cpp:37-125;
+--static holds it at its initial pose), Spalart-Allmaras working variable
+from a freestream of 5 (the reference's TURBULENCE build).  A "step" is one
+InterfaceExchange + IBConditions_Image (with the body move) + TimeMarching
 (main.cpp:87-97).
 
 Three forms run in the same process on the same mesh:
@@ -111,10 +114,19 @@ def updates_per_step(mesh, nstage, ts):
     return n * nstage
 
 
-def new_engine(mesh, mode, nstage):
+# Pitch and heave of the FSI case: aoa(t) = 5 deg sin(omega t) about the
+# quarter chord, y(t) = 0.05 sin(omega t), reduced frequency
+# k = omega c / (2 U) = 0.1 with c = U = 1.
+MOTION = (0.0, 0.0, 0.0, 5.0, 0.2, 0.05)
+
+
+def new_engine(mesh, mode, nstage, motion=True):
     eng = ibgpu.Engine(mesh, mode=mode, nstage=nstage, turbulence=True)
     eng.set_state(mesh.init_state())
     eng.set_nut(np.full(mesh.ncells, mesh.phys[0].nut_inf))
+    if motion:
+        eng.upload_surface()
+        eng.set_motion([MOTION] * mesh.ncases, t0=0.0)
     return eng
 
 
@@ -132,17 +144,18 @@ def timed_run(eng, mesh, mode, warmup, steps, reps, ml=False):
     forces = F.reshape(mesh.ncases, 6)[:, :2].copy()   # every case, before the timing launches
     state, nut = eng.get_state(), eng.get_nut()
     counts = eng.closure_counts() if ml else (0, 0)   # cells of the warmup + timed steps
+    t_end = float(eng.get_time()[0]) if getattr(eng, "_surface", None) else None
     # the stage kernel alone, launch by launch over the timed steps' mix
     ms, nbytes, ncell = stage_mix(eng, mesh, mode, warmup + steps, eng.cfg.nstage, reps)
     return {"seconds": t1 - t0, "updates": upd, "rate": upd / (t1 - t0), "cd": cd, "cl": cl,
             "stage_ms": ms, "stage_bytes": nbytes, "stage_cells": ncell, "state": state, "nut": nut,
-            "counts": counts, "forces": forces}
+            "counts": counts, "forces": forces, "t_end": t_end}
 
 
 def train_closure(mesh, nstage, args):
     """Network trained on cells sampled from the plain FP64 run."""
     t0 = time.perf_counter()
-    src = new_engine(mesh, "fp64", nstage)
+    src = new_engine(mesh, "fp64", nstage, not args.static)
     src.closure_init(args.samples, 20261202)
     period = 1 << (mesh.nlevels - 1)
     total, ts, k = 0, 0, 0
@@ -164,16 +177,28 @@ def train_closure(mesh, nstage, args):
                  "epochs": args.epochs, "final_loss": loss, "sample_s": t1 - t0, "train_s": t2 - t1}
 
 
-def cpu_baseline(mesh, nstage, budget_s):
+def cpu_baseline(mesh, nstage, budget_s, motion=True):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ffi
-    orc = oracle_ffi.Oracle(mesh, oracle_ffi.cfg(nstage=nstage, dt_var_step=1 << (mesh.nlevels - 1)))
+    ocfg = oracle_ffi.cfg(nstage=nstage, dt_var_step=1 << (mesh.nlevels - 1))
+    orc = oracle_ffi.Oracle(mesh, ocfg)
+    mb = oracle_ffi.MovingBodies(mesh) if motion else None
     fv = mesh.init_state()
     nut = np.full(mesh.ncells, mesh.phys[0].nut_inf)
     dt = np.array([mesh.phys[c].dt_init for c in range(mesh.ncases)])
-    upd, ts, t0 = 0, 0, time.perf_counter()
+    upd, ts, t0, t = 0, 0, time.perf_counter(), 0.0
     while True:
+        if mb is not None:
+            # the body move of the step (oracle/retag.c), then the step on
+            # the moved geometry
+            x0, y0, aoa0, amp, omega, heave = MOTION
+            a = np.deg2rad(aoa0 + amp * np.sin(omega * t))
+            r = mb.move([[x0, y0 + heave * np.sin(omega * t), np.cos(-a), np.sin(-a)]] * mesh.ncases)
+            if r["code"]:
+                raise RuntimeError(f"oracle move abort {r['code']}")
+            orc = oracle_ffi.Oracle(oracle_ffi.MovedMesh(mb, r), ocfg)
         code, _, _, _ = orc.step_sa(fv, nut, dt, ts)
+        t += float(dt[0])
         if code:
             raise RuntimeError(f"oracle abort {code}")
         upd += updates_per_step(mesh, nstage, ts)
@@ -184,7 +209,8 @@ def cpu_baseline(mesh, nstage, budget_s):
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": upd / el, "unit": "cell-stage updates/s", "cores": threads, "kind": "port",
             "sample": f"FP64 oracle with the SA closure (C restatement of the reference path, "
-                      f"OpenMP), same mesh, steps 0..{ts - 1} from freestream "
+                      f"OpenMP{', body move per step single-threaded' if motion else ''}), "
+                      f"same mesh, steps 0..{ts - 1} from freestream "
                       f"({upd} cell-stage updates in {el:.1f} s)"}
 
 
@@ -207,7 +233,7 @@ def secondary(args):
           f"{min(per_case)}..{max(per_case)} (total {sum(per_case)})", file=sys.stderr, flush=True)
     res, extra = {}, {}
     for mode in ("fp64", "mixed"):
-        eng = new_engine(mesh, mode, args.stages)
+        eng = new_engine(mesh, mode, args.stages, False)
         if args.workload == "ensemble" and mode == "mixed":
             best, ms = eng.autotune_tile()
             eng.set_state(mesh.init_state())
@@ -280,6 +306,8 @@ def main():
     ap.add_argument("--epochs", type=float, default=20.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--static", action="store_true",
+                    help="hold the body at its initial pose (no per-step re-tagging)")
     ap.add_argument("--workload", choices=("naca2d", "naca3d", "ensemble"), default="naca2d",
                     help="naca2d (default): the measured configuration with the learned closure; "
                          "naca3d, ensemble: plain FP64 vs mixed on the 3D and batched cases")
@@ -295,8 +323,9 @@ def main():
     leaf = int(sum(mesh.counts()[0]))
     print(f"[bench] mesh: {mesh.ncells} cells, normal per level {mesh.counts()[0]}", file=sys.stderr, flush=True)
     res = {}
+    motion = not args.static
     for mode in ("fp64", "mixed"):
-        eng = new_engine(mesh, mode, args.stages)
+        eng = new_engine(mesh, mode, args.stages, motion)
         res[mode] = timed_run(eng, mesh, mode, args.warmup, args.steps, args.reps)
         eng.close()
         r = res[mode]
@@ -307,7 +336,7 @@ def main():
     print(f"[bench] closure: {tinfo['samples']} samples, {tinfo['train_steps']} Adam steps "
           f"(batch {args.batch}), loss {tinfo['final_loss']:.3e}, sampling {tinfo['sample_s']:.1f} s, "
           f"training {tinfo['train_s']:.2f} s", file=sys.stderr, flush=True)
-    eng = new_engine(mesh, "mixed", args.stages)
+    eng = new_engine(mesh, "mixed", args.stages, motion)
     eng.closure_init(1 << 10)
     eng.closure_set_params(prm)
     eng.closure_enable(True)
@@ -366,8 +395,13 @@ def main():
                 "on cells of the plain FP64 run",
         "config": {
             "workload": "NACA0015 2D, M0.3 Re1e4, [-10,10]^2, 256^2 base + 5 AMR levels, "
-                        f"{leaf} leaf cells, 4-stage, temporal sub-cycling, body at initial pose, "
-                        "Spalart-Allmaras closure (plain FP64 vs mixed + learned)",
+                        f"{leaf} leaf cells, 4-stage, temporal sub-cycling, "
+                        + ("body at initial pose, " if args.static else
+                           "pitch 5 deg + heave 0.05c at k=0.1 re-tagged every step (no regrid), ")
+                        + "Spalart-Allmaras closure (plain FP64 vs mixed + learned)",
+            "motion": None if args.static else dict(zip(
+                ("x0", "y0", "aoa0_deg", "amp_deg", "omega", "heave"), MOTION),
+                t_end=ml["t_end"]),
             "cells": int(mesh.ncells),
             "leaf_cells": leaf,
             "levels": mesh.nlevels,
@@ -396,7 +430,7 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(mesh, args.stages, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(mesh, args.stages, args.cpu_seconds, motion)
     print(json.dumps(out))
 
 

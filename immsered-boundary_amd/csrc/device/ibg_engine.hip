@@ -115,22 +115,29 @@ struct ibg_ctx {
   // allocs with the mesh
   struct MoveState {
     bool on = false, law = false;
-    int npoint = 0, cap = 0, nblk256 = 0;
-    int *key = nullptr, *cpatch = nullptr, *cp1 = nullptr;
-    int8_t *kind = nullptr, *fin = nullptr, *l1 = nullptr, *isib = nullptr;
+    int npoint = 0, cap = 0, cap_s = 0, cap_l1 = 0, cap_io = 0;
+    int *key = nullptr, *cpatch = nullptr, *cp1 = nullptr, *nb8 = nullptr;
+    int8_t *kind = nullptr, *fin = nullptr;
     int16_t *cblk = nullptr;
-    double *origin = nullptr, *dist1 = nullptr, *sd1 = nullptr;
+    double *origin = nullptr, *sd1 = nullptr;
     double *pt_off = nullptr, *pat_dz = nullptr;
     int *pat_pt0 = nullptr, *pat_body = nullptr, *body_p0 = nullptr, *pat_cc = nullptr;
     int8_t *pat_flip = nullptr;
     Pose *pose = nullptr;
-    int *ib_n = nullptr, *bcount = nullptr, *boff = nullptr;
+    int *mk1 = nullptr, *mkib = nullptr;  // move stamps: layer 1, IB set
+    int8_t *cmix = nullptr;               // an in-plane neighbour's body differs
+    int *chg = nullptr;                   // cells whose body changed
+    int cap_chg = 0;
+    int *epoch = nullptr;                 // [1] move counter
+    int *cnt = nullptr;                   // [5] solid, layer-1, in->out, IB, changed
+    int *slist = nullptr, *l1list = nullptr, *iolist = nullptr;
+    int *gidx = nullptr;          // ghosts sorted by level, finest first
+    std::vector<int> gend;        // [l]: ghosts of levels >= l
     unsigned long long *upd = nullptr, *lvlcnt = nullptr;
     PitchHeave *lawd = nullptr;
     Motion *motc = nullptr;       // per case, for the wall force
     int *bcase_d = nullptr;       // case of each body
     double *t = nullptr;
-    std::vector<int> body_case;   // host: case of each body
   } mv;
   double *dt = nullptr, *cflpart = nullptr, *rpart = nullptr, *resid = nullptr, *probe = nullptr;
   unsigned long long *err = nullptr;
@@ -416,7 +423,7 @@ ibg_status launch_ib(ibg_ctx *c) {
   const int n = c->mv.on ? c->mv.cap : c->nib;
   if (n > 0) {
     hipLaunchKernelGGL((k_ib<D>), dim3(grid(n, 256)), dim3(256), 0, c->s, n,
-                       c->mv.on ? c->mv.ib_n : (const int *)nullptr, c->ib_cell,
+                       c->mv.on ? c->mv.cnt + 3 : (const int *)nullptr, c->ib_cell,
                        c->ib_intp, c->ib_coef, c->ib_hgdis, c->ib_attach, c->ib_nv, c->ib_body,
                        c->ib_case, c->signdis, c->motion, c->have_motion, c->sout<D>(), c->phys,
                        c->err);
@@ -430,57 +437,43 @@ ibg_status run_ib(ibg_ctx *c) { return c->D == 2 ? launch_ib<2>(c) : launch_ib<3
 MoveArgs move_args(ibg_ctx *c, unsigned long long *upd, int lmin) {
   MoveArgs a{};
   ibg_ctx::MoveState &v = c->mv;
-  a.N = c->N; a.Nn = c->Nn; a.npatch = c->npatch;
+  a.N = c->N; a.Nn = c->Nn; a.npatch = c->npatch; a.ncases = c->ncases; a.nbody = c->nbody;
+  a.nw = c->cell_end[lmin]; a.ngw = v.gend[lmin]; a.nflag = c->cell_end[lmin];
+  a.gidx = v.gidx;
   a.key = v.key; a.kind = v.kind; a.fin = v.fin; a.cblk = v.cblk; a.blk = c->blk;
-  a.origin = v.origin; a.nb27 = c->nb27;
-  a.signdis = c->signdis; a.sd32 = c->sd32;
-  a.cpatch = v.cpatch; a.cp1 = v.cp1; a.dist1 = v.dist1; a.sd1 = v.sd1;
-  a.l1 = v.l1; a.isib = v.isib; a.flags = c->flags;
+  a.origin = v.origin; a.nb8 = v.nb8; a.nb27 = c->nb27;
+  a.signdis = c->signdis; a.sd32 = c->sd32; a.cpatch = v.cpatch; a.cp1 = v.cp1; a.sd1 = v.sd1;
+  a.mk1 = v.mk1; a.mkib = v.mkib; a.epoch = v.epoch; a.cnt = v.cnt;
+  a.cmix = v.cmix; a.chg = v.chg; a.cap_chg = v.cap_chg;
+  a.cap_s = v.cap_s; a.cap_l1 = v.cap_l1; a.cap_io = v.cap_io; a.ib_cap = v.cap;
+  a.slist = v.slist; a.l1list = v.l1list; a.iolist = v.iolist;
+  a.flags = c->flags;
   a.pt_off = v.pt_off; a.pat_pt0 = v.pat_pt0; a.pat_body = v.pat_body; a.pat_case = c->pat_case;
-  a.pat_flip = v.pat_flip; a.pat_dz = v.pat_dz; a.body_p0 = v.body_p0; a.pose = v.pose;
+  a.pat_flip = v.pat_flip; a.pat_dz = v.pat_dz; a.body_p0 = v.body_p0; a.body_case = v.bcase_d;
+  a.pose = v.pose; a.use_law = v.law ? 1 : 0; a.law = v.lawd; a.t = v.t;
+  a.mot = c->motion; a.motc = v.motc; a.case_center = c->case_center;
   a.pat_pc = c->pat_pc; a.pat_nv = c->pat_nv; a.pat_area = c->pat_area;
   a.pat_intp = c->pat_intp; a.pat_cc = v.pat_cc; a.pat_coef = c->pat_coef; a.pat_hgdis = c->pat_hgdis;
-  a.ib_cap = v.cap; a.ib_n = v.ib_n; a.bcount = v.bcount; a.boff = v.boff;
   a.ib_cell = c->ib_cell; a.ib_intp = c->ib_intp; a.ib_body = c->ib_body; a.ib_case = c->ib_case;
   a.ib_coef = c->ib_coef; a.ib_hgdis = c->ib_hgdis; a.ib_attach = c->ib_attach; a.ib_nv = c->ib_nv;
-  a.err = c->err; a.upd = upd; a.lvlcnt = v.lvlcnt; a.upd_lmin = lmin; a.nstage = c->cfg.nstage;
+  a.err = c->err; a.upd = upd; a.lvlcnt = v.lvlcnt; a.nstage = c->cfg.nstage;
   return a;
 }
 
-// One move of the bodies to the poses in mv.pose: surface, distances, solid
-// and IB sets, image points, stage masks.  Stream-ordered, capturable.
-template <int D>
-ibg_status launch_move(ibg_ctx *c, unsigned long long *upd, int lmin) {
-  const MoveArgs a = move_args(c, upd, lmin);
-  const int nb = c->mv.nblk256;
-  hipLaunchKernelGGL(k_motion_center, dim3(grid(c->ncases, 64)), dim3(64), 0, c->s, c->ncases,
-                     (const Pose *)c->mv.pose, c->case_center);
-  hipLaunchKernelGGL(k_move_patches, dim3(grid(c->npatch, 256)), dim3(256), 0, c->s, a);
-  hipLaunchKernelGGL(k_move_walk, dim3(grid(c->N, 256)), dim3(256), 0, c->s, a);
-  hipLaunchKernelGGL((k_move_cycle<D>), dim3(grid(c->N, 256)), dim3(256), 0, c->s, a);
-  hipLaunchKernelGGL((k_move_l1<D>), dim3(grid(c->N, 256)), dim3(256), 0, c->s, a);
-  hipLaunchKernelGGL((k_move_ib<D>), dim3(nb), dim3(256), 0, c->s, a);
-  hipLaunchKernelGGL(k_move_scan, dim3(1), dim3(1024), 0, c->s, a, nb);
-  hipLaunchKernelGGL(k_move_list, dim3(nb), dim3(256), 0, c->s, a);
-  hipLaunchKernelGGL((k_move_image<D>), dim3(grid(c->mv.cap, 256)), dim3(256), 0, c->s, a);
-  hipLaunchKernelGGL((k_move_patch_image<D>), dim3(grid(c->npatch, 64)), dim3(64), 0, c->s, a);
-  hipLaunchKernelGGL(k_move_flags, dim3(grid(c->Nn, 256)), dim3(256), 0, c->s, a);
-  HIPCHK(hipGetLastError());
-  return IBG_OK;
-}
-
+// One move of the bodies (to the poses in mv.pose, or by the law at each
+// case's time): surface, distances, solid and IB sets, image points, stage
+// masks of the levels marched from lmin.  Stream-ordered, capturable.
 ibg_status run_move(ibg_ctx *c, unsigned long long *upd, int lmin) {
   if (c->D != 2) return IBG_ERR_STATE;
-  return launch_move<2>(c, upd, lmin);
-}
-
-// Pose and wall motion of the prescribed law at each case's time.
-ibg_status run_motion_law(ibg_ctx *c) {
-  ibg_ctx::MoveState &v = c->mv;
-  const int n = std::max(c->ncases, c->nbody);
-  hipLaunchKernelGGL(k_motion_pose, dim3(grid(n, 64)), dim3(64), 0, c->s, c->ncases,
-                     (const PitchHeave *)v.lawd, (const double *)v.t, v.pose, c->nbody,
-                     (const int *)v.bcase_d, c->motion, v.motc, (const double *)c->case_center);
+  const MoveArgs a = move_args(c, upd, lmin);
+  const int nw = a.nw + a.ngw;
+  hipLaunchKernelGGL(k_move_front, dim3(1), dim3(256), 0, c->s, a);
+  hipLaunchKernelGGL(k_move_dist, dim3(grid(nw, 256)), dim3(256), 0, c->s, a);
+  hipLaunchKernelGGL(k_move_mark1, dim3(grid(a.cap_s + a.cap_chg, 256)), dim3(256), 0, c->s, a);
+  hipLaunchKernelGGL(k_move_mark2, dim3(grid(a.cap_l1 + a.cap_io, 256)), dim3(256), 0, c->s, a);
+  const int ncommit = (int)std::min(grid(nw, 256), 2048u);
+  hipLaunchKernelGGL(k_move_finish, dim3(ncommit + grid(a.ib_cap + c->npatch, 256)), dim3(256), 0,
+                     c->s, a, ncommit);
   HIPCHK(hipGetLastError());
   return IBG_OK;
 }
@@ -507,7 +500,6 @@ ibg_status run_step(ibg_ctx *c, int ts) {
   if ((st = run_exchange(c, ts))) return st;
   if (c->mv.law) {
     // MovingBody_Image at the start of IBConditions_Image (Body.cpp:1353-1357)
-    if ((st = run_motion_law(c))) return st;
     if ((st = run_move(c, c->mv.upd, lmin_of(c, ts)))) return st;
   }
   if ((st = run_ib(c))) return st;
@@ -1506,15 +1498,54 @@ ibg_status ibg_upload_surface(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_surfa
   UP(v.pat_flip, flip); UP(v.pat_dz, dz); UP(v.body_p0, bp0); UP(v.pat_cc, cc);
   UP(v.bcase_d, bcase);
   {
-    std::vector<int8_t> isib0(N, 0);   // the current IB set
-    for (int j = 0; j < m->nib; ++j) isib0[c->h2d[m->ib_cell[j]]] = 1;
-    UP(v.isib, isib0);
+    // ghosts by level, finest first: the ghosts of the levels >= l are a
+    // prefix of this list
+    std::vector<int> gl;
+    v.gend.assign(L, 0);
+    for (int l = L - 1; l >= 0; --l) {
+      for (int i = c->Nn; i < N; ++i)
+        if (c->hcellblk[i] % L == l) gl.push_back(i);
+      v.gend[l] = (int)gl.size();
+    }
+    if (gl.empty()) gl.push_back(0);
+    UP(v.gidx, gl);
   }
-  AL(v.cp1, N); AL(v.dist1, N); AL(v.sd1, N); AL(v.l1, N);
+  {
+    // in-plane neighbours in SoA (Point_iterator_2d order without the cell)
+    static const int q8[kNb8] = {1, 4, 7, 10, 16, 19, 22, 25};
+    std::vector<int> nb8((size_t)kNb8 * N);
+    std::vector<int> nbh((size_t)N * 27);
+    HIPCHK(hipMemcpy(nbh.data(), c->nb27, sizeof(int) * nbh.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < kNb8; ++k)
+      for (int i = 0; i < N; ++i) nb8[(size_t)k * N + i] = nbh[(size_t)i * 27 + q8[k]];
+    UP(v.nb8, nb8);
+    // mixed-body neighbourhoods of the uploaded closest patches
+    std::vector<int8_t> cmix(N, 0);
+    for (int i = 0; i < N; ++i) {
+      if (cp[i] < 0) continue;
+      const int bi = sf->pat_body[cp[i]];
+      for (int k = 0; k < kNb8; ++k) {
+        const int y = nb8[(size_t)k * N + i];
+        if (y >= 0 && cp[y] >= 0 && sf->pat_body[cp[y]] != bi) { cmix[i] = 1; break; }
+      }
+    }
+    UP(v.cmix, cmix);
+    // move stamps: the uploaded IB set is move 1
+    std::vector<int> mkib(N, 0), one(1, 1);
+    for (int j = 0; j < m->nib; ++j) mkib[c->h2d[m->ib_cell[j]]] = 1;
+    UP(v.mkib, mkib);
+    UP(v.epoch, one);
+    // list capacities from the uploaded geometry: finest-level solid cells
+    int nsolid = 0;
+    for (int i = 0; i < N; ++i)
+      if (fin[i] && kind[i] != IBG_DMGHOST && m->signdis[c->d2h[i]] < 0.0) ++nsolid;
+    v.cap_s = 2 * nsolid + 4096;
+    v.cap_io = c->nib + 4096;
+    v.cap_chg = 8192;
+  }
+  AL(v.cp1, N); AL(v.sd1, N); AL(v.mk1, N);
   AL(v.pose, c->ncases); AL(v.motc, c->ncases); AL(v.lawd, c->ncases); AL(v.t, c->ncases);
-  v.nblk256 = (int)grid(N, 256);
-  AL(v.bcount, v.nblk256); AL(v.boff, v.nblk256);
-  AL(v.ib_n, 1); AL(v.upd, 1); AL(v.lvlcnt, IBG_MAX_LEVELS);
+  AL(v.upd, 1); AL(v.lvlcnt, IBG_MAX_LEVELS);
   // IB arrays at capacity, holding the current list
   const int cap = sf->ib_capacity > 0 ? sf->ib_capacity : 2 * c->nib + 4096;
   if (cap < c->nib) return fail("IB capacity below the current list");
@@ -1537,13 +1568,18 @@ ibg_status ibg_upload_surface(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_surfa
   }
   c->ib_cell = nc_; c->ib_intp = ni; c->ib_body = nbd; c->ib_case = ncs;
   c->ib_coef = nco; c->ib_hgdis = nh; c->ib_attach = nat; c->ib_nv = nnv;
-  {
-    const int n = c->nib;
-    HIPCHK(hipMemcpy(v.ib_n, &n, sizeof(int), hipMemcpyHostToDevice));
-  }
   v.cap = cap;
+  v.cap_l1 = cap;   // layer 1 is part of the IB set
+  {
+    ibg_status st2;
+    std::vector<int> cnt0 = {0, 0, 0, c->nib, 0};
+    if ((st2 = dupload(c, &v.cnt, cnt0))) return st2;
+    if ((st2 = dalloc(c, &v.chg, (size_t)v.cap_chg))) return st2;
+    if ((st2 = dalloc(c, &v.slist, (size_t)v.cap_s))) return st2;
+    if ((st2 = dalloc(c, &v.l1list, (size_t)v.cap_l1))) return st2;
+    if ((st2 = dalloc(c, &v.iolist, (size_t)v.cap_io))) return st2;
+  }
   v.npoint = sf->npoint;
-  v.body_case = c->hbody_case;
   v.on = true;
   // captured graphs hold the old IB launch
   for (int k = 0; k < 2; ++k)
@@ -1633,21 +1669,21 @@ ibg_status ibg_get_geometry(ibg_ctx *c, double *signdis, int32_t *cell_patch, in
   if (!c->have_mesh || !c->mv.on) return IBG_ERR_STATE;
   const int N = c->N;
   std::vector<double> sd(N);
-  std::vector<int> cp(N);
-  std::vector<int8_t> isib(N);
+  std::vector<int> cp(N), mk(N);
+  int n = 0, ep = 0;
   HIPCHK(hipMemcpyAsync(sd.data(), c->signdis, sizeof(double) * N, hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipMemcpyAsync(cp.data(), c->mv.cpatch, sizeof(int) * N, hipMemcpyDeviceToHost, c->s));
-  HIPCHK(hipMemcpyAsync(isib.data(), c->mv.isib, N, hipMemcpyDeviceToHost, c->s));
-  int n = 0;
-  HIPCHK(hipMemcpyAsync(&n, c->mv.ib_n, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemcpyAsync(mk.data(), c->mv.mkib, sizeof(int) * N, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemcpyAsync(&n, c->mv.cnt + 3, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemcpyAsync(&ep, c->mv.epoch, sizeof(int), hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   for (int i = 0; i < N; ++i) {
     const int g = c->d2h[i];
     if (signdis) signdis[g] = sd[i];
     if (cell_patch) cell_patch[g] = cp[i];
-    if (ib) ib[g] = isib[i] == 1 ? 1 : 0;
+    if (ib) ib[g] = mk[i] == ep ? 1 : 0;
   }
-  if (nib) *nib = n;
+  if (nib) *nib = std::min(n, c->mv.cap);
   return IBG_OK;
 }
 
@@ -1657,8 +1693,9 @@ ibg_status ibg_get_ib_list(ibg_ctx *c, int32_t cap, int32_t *cell, int32_t *intp
   if (!c->have_mesh) return IBG_ERR_STATE;
   int n = c->nib;
   if (c->mv.on) {
-    HIPCHK(hipMemcpyAsync(&n, c->mv.ib_n, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(&n, c->mv.cnt + 3, sizeof(int), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
+    n = std::min(n, c->mv.cap);
   }
   if (n > cap) return IBG_ERR_ARG;
   std::vector<int> hc(n), hi((size_t)n * 8), hb(n);

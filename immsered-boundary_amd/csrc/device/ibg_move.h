@@ -1,29 +1,34 @@
 // ibg_move.h — per-step immersed-boundary re-tagging of moving bodies on a
 // fixed level set (the part of Body::MovingBody_Image, Bodymove.cpp:37-125,
-// that runs between regrids).
+// that runs between regrids), 2D sections.  Five launches per move:
 //
-//   k_move_patches   new surface points and patch geometry
-//                    (ComptNewLocation Bodymove.cpp:3-23, ComptPatchParams
-//                    :25-35, the segment rules of the mesh builder)
-//   k_move_walk      closest patch by a local walk from the previous one
-//                    (RenewLevelDistance :136-155, FindClosePatchtoCell
-//                    Body.H:505-542)
-//   k_move_cycle     the closest patch of the other bodies of the case
-//                    (:156-223) and the solid flag (:224-249)
-//   k_move_l1 / k_move_ib / k_move_scan / k_move_list
-//                    the IB set.  RemoveIBCells / InfectNewIBCells
-//                    (:345-767) keep, after every move, the set of the
-//                    initial rule InfectWallBox_Init_Negative
-//                    (Body.cpp:1016-1082): non-solid cells with a solid
-//                    neighbour and their non-solid neighbours.  The list is
-//                    rebuilt in cell order.
-//   k_move_image     image points and multilinear weights of every IB cell
-//                    (FindImageCell Body.cpp:999-1190, ComptHgIntpCoef
-//                    Mesh.cpp:1403-1442)
-//   k_move_patch_image  the same for the wall-force patches
-//                    (Body.cpp:1506-1575)
-//   k_move_flags     the stage masks of the new solid / IB sets
-//                    (NS_Solver.cpp:856-983, 19-179)
+//   k_move_front   pose of every case (given, or from the pitch/heave law at
+//                  the case's time, with the wall motion of the IB fill and
+//                  the wall force), new surface points and patch geometry
+//                  (ComptNewLocation Bodymove.cpp:3-23, ComptPatchParams
+//                  :25-35, the segment rules of the mesh builder)
+//   k_move_dist    closest patch and signed distance of every cell of the
+//                  marched levels: the local walk from the previous patch
+//                  (RenewLevelDistance :136-155, FindClosePatchtoCell
+//                  Body.H:505-542) and the cross-body cycle (:156-223);
+//                  finest-level solid and in->out cells go to lists
+//   k_move_mark1   layer 1 of the IB set: non-solid neighbours of solid cells;
+//                  the mixed-body neighbourhood flags around changed cells
+//   k_move_mark2   the IB set: layer-1 cells and their non-solid neighbours
+//                  (RemoveIBCells / InfectNewIBCells :345-767 keep, after
+//                  every move, the set of the initial rule of Body.cpp:
+//                  1016-1082); the in->out check (abort 314)
+//   k_move_finish  image points and weights of every IB cell (FindImageCell
+//                  Body.cpp:999-1190, Mesh.cpp:1403-1442) and of the
+//                  wall-force patches (Body.cpp:1506-1575); new distances in
+//                  place, the stage masks of the marched cells
+//                  (NS_Solver.cpp:856-983, 19-179), update count
+//
+// The IB set is stamped with a move counter (a cell is an IB cell when its
+// stamp equals the current move), so marking grows from the solid cells of
+// the finest level and costs what the wall region holds, not the grid.
+// The list order follows the atomic appends; every IB cell is filled
+// independently of the others, so results do not depend on it.
 //
 // Bit-exactness.  Every expression is evaluated in the order the host mesh
 // builder and the reference evaluate it, with floating-point contraction
@@ -40,21 +45,46 @@ struct Pose {
   double px, py, c, s;
 };
 
+// Prescribed pitch and heave of the bodies of one case about a pivot:
+// angle of attack aoa(t) = aoa0 + amp sin(omega t) (nose up), pivot
+// y(t) = y0 + heave sin(omega t).  The rotation applied to the zero-pose
+// offsets is a = -aoa (the section rotates clockwise for a nose-up angle).
+struct PitchHeave {
+  double x0, y0, aoa0, amp, omega, heave;   // angles in radians
+};
+
+// In-plane neighbour slots of Point_iterator_2d (i outer, j inner, k = 1)
+// without the cell itself: nb27 slots 1, 4, 7, 10, 16, 19, 22, 25.
+constexpr int kNb8 = 8;
+
+// Cell sets of one move: the normal cells [0, nw) of the device order plus
+// the ghosts gidx[0, ngw) (ghosts sorted by level, finest first) -- the
+// levels the step marches, i.e. every level the step's kernels read.
 struct MoveArgs {
-  int N, Nn, npatch;
+  int N, Nn, npatch, ncases, nbody;
+  int nw, ngw, nflag;
+  const int *gidx;
   const int *key;        // [N][3]
   const int8_t *kind;    // [N] ibg_cell_kind
   const int8_t *fin;     // [N] finest level of a case with bodies
   const int16_t *cblk;   // [N] block (case, level)
   const BlkInfo *blk;
   const double *origin;  // [ncases][3]
-  const int *nb27;
+  const int *nb8;        // [8][N] in-plane neighbours, SoA
+  const int *nb27;       // [N][27]
   double *signdis;
   float *sd32;
   int *cpatch;           // [N] closest patch (global index), -1: no body
-  int *cp1;              // [N] after the walk
-  double *dist1, *sd1;   // [N]
-  int8_t *l1, *isib;     // [N]
+  int *cp1;              // [N] new closest patch
+  double *sd1;           // [N] new signed distance
+  int *mk1, *mkib;       // [N] move stamps: layer 1, IB set
+  int8_t *cmix;          // [N] an in-plane neighbour's closest body differs
+  int *chg;              // cells whose closest body changed in this move
+  int cap_chg;
+  int *epoch;            // [1] move counter
+  int *cnt;              // [5] solid, layer-1, in->out, IB, changed list lengths
+  int cap_s, cap_l1, cap_io, ib_cap;
+  int *slist, *l1list, *iolist;
   uint8_t *flags;        // [Nn]
   // surface
   const double *pt_off;  // [npoint][2]
@@ -64,23 +94,48 @@ struct MoveArgs {
   const int8_t *pat_flip;// [npatch] odd body: normal reversed
   const double *pat_dz;  // [npatch] z extent
   const int *body_p0;    // [nbody+1] patches of body b: [body_p0[b], body_p0[b+1])
-  const Pose *pose;      // [ncases]
+  const int *body_case;  // [nbody]
+  Pose *pose;            // [ncases]
+  int use_law;
+  const PitchHeave *law; // [ncases]
+  const double *t;       // [ncases]
+  Motion *mot, *motc;    // per body (IB fill), per case (wall force)
+  double *case_center;   // [ncases][3]
   double *pat_pc, *pat_nv, *pat_area;
   int *pat_intp, *pat_cc;
   double *pat_coef, *pat_hgdis;
   // IB list
-  int ib_cap;
-  int *ib_n;             // [1] cells in the list (clamped to ib_cap)
-  int *bcount, *boff;    // [ceil(N/256)] per-block IB counts / offsets
   int *ib_cell, *ib_intp, *ib_body, *ib_case;
   double *ib_coef, *ib_hgdis, *ib_attach, *ib_nv;
   unsigned long long *err;
   unsigned long long *upd;     // [1] += stage updates of this step (may be null)
   unsigned long long *lvlcnt;  // [L] per-level update counts
-  int upd_lmin, nstage;
+  int nstage;
 };
 
 constexpr double kCriticDis = 9.5;   // critic_dis, Bodymove.cpp:132
+
+__device__ __forceinline__ int set_cell(const MoveArgs &a, int t, int n) {
+  return t < n ? t : a.gidx[t - n];
+}
+
+// Append val to list (wave-aggregated: one atomic per wave); every lane of
+// the wave calls it.  A full list is the capacity error 1090.
+__device__ __forceinline__ void wave_append(int *cnt, bool v, int val, int *list, int cap,
+                                            unsigned long long *err) {
+  const unsigned long long m = __ballot(v);
+  if (!m) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(cnt, __popcll(m));
+  base = __shfl(base, leader);
+  if (v) {
+    const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+    if (pos < cap) list[pos] = val;
+    else report_error(err, val, 1090);
+  }
+}
 
 __device__ __forceinline__ void cell_center_dev(const MoveArgs &a, int i, double *x) {
 #pragma clang fp contract(off)
@@ -101,7 +156,8 @@ __device__ __forceinline__ void patch_dist(const MoveArgs &a, const double *x, i
 
 // FindClosePatchtoCell Body.H:505-542 on the body that owns `patch`: the
 // neighbours of a patch of a 2D section are the previous and the next
-// segment of the same body (CollectPatchNeib Body.cpp:822-892).
+// segment of the same body (CollectPatchNeib Body.cpp:822-892).  The walk
+// never leaves the body it starts on.
 __device__ void walk_body(const MoveArgs &a, const double *x, int &patch, double &dist,
                           double &sd) {
 #pragma clang fp contract(off)
@@ -121,211 +177,245 @@ __device__ void walk_body(const MoveArgs &a, const double *x, int &patch, double
   sd = nv[0] * (x[0] - pc[0]) + nv[1] * (x[1] - pc[1]) + nv[2] * (x[2] - pc[2]);
 }
 
-// New surface: points of the patch segment at the case's pose, then the
-// segment centre, outward normal and area of the host mesh builder
-// (ib_mesh.cpp build_segments / build_patches).
-__global__ void k_move_patches(MoveArgs a) {
-#pragma clang fp contract(off)
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.npatch) return;
-  const Pose P = a.pose[a.pat_case[p]];
-  const int i0 = a.pat_pt0[p];
-  const double ox0 = a.pt_off[2 * i0], oy0 = a.pt_off[2 * i0 + 1];
-  const double ox1 = a.pt_off[2 * i0 + 2], oy1 = a.pt_off[2 * i0 + 3];
-  const double x0 = P.px + (P.c * ox0 - P.s * oy0), y0 = P.py + (P.s * ox0 + P.c * oy0);
-  const double x1 = P.px + (P.c * ox1 - P.s * oy1), y1 = P.py + (P.s * ox1 + P.c * oy1);
-  const double dx = x1 - x0, dy = y1 - y0;
-  const double len = sqrt(dx * dx + dy * dy);
-  double nx = -dy / len, ny = dx / len;
-  if (a.pat_flip[p]) { nx = -nx; ny = -ny; }
-  a.pat_pc[(size_t)p * 3] = 0.5 * (x0 + x1);
-  a.pat_pc[(size_t)p * 3 + 1] = 0.5 * (y0 + y1);
-  a.pat_nv[(size_t)p * 3] = nx;
-  a.pat_nv[(size_t)p * 3 + 1] = ny;
-  a.pat_area[p] = len * a.pat_dz[p];
+__device__ __forceinline__ Motion law_motion(const PitchHeave &Lw, double t, double zc) {
+  const double wt = Lw.omega * t;
+  const double sw = sin(wt), cw = cos(wt);
+  Motion M;
+  M.tra[0] = 0.0; M.tra[1] = Lw.heave * Lw.omega * cw; M.tra[2] = 0.0;
+  M.rot[0] = 0.0; M.rot[1] = 0.0; M.rot[2] = -Lw.amp * Lw.omega * cw;
+  M.center[0] = Lw.x0; M.center[1] = Lw.y0 + Lw.heave * sw; M.center[2] = zc;
+  return M;
 }
 
-// RenewLevelDistance first loop (Bodymove.cpp:136-155): cells closer than
-// critic_dis walk from their previous closest patch.
-__global__ void k_move_walk(MoveArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.N) return;
-  int p = a.cpatch[i];
-  const double sd0 = a.signdis[i];
-  if (p < 0 || !(sd0 < kCriticDis)) {
-    a.cp1[i] = p; a.sd1[i] = sd0; a.dist1[i] = 999.0;
+// One workgroup: move counter and list lengths, the poses (and, under the
+// law, the rigid-body velocity of Body::Surface_Point_Velocity Body.H:
+// 709-721 about the pivot), the reference point of each case (the pivot),
+// then the patches.
+__global__ void __launch_bounds__(256) k_move_front(MoveArgs a) {
+#pragma clang fp contract(off)
+  __shared__ Pose P[IBG_MAX_CASES];
+  if (threadIdx.x == 0) {
+    a.epoch[0] += 1;
+    for (int k = 0; k < 5; ++k) a.cnt[k] = 0;
+  }
+  for (int k = threadIdx.x; k < a.ncases; k += blockDim.x) {
+    Pose q;
+    if (a.use_law) {
+      const PitchHeave Lw = a.law[k];
+      const double sw = sin(Lw.omega * a.t[k]);
+      const double ang = -(Lw.aoa0 + Lw.amp * sw);
+      q.px = Lw.x0;
+      q.py = Lw.y0 + Lw.heave * sw;
+      q.c = cos(ang);
+      q.s = sin(ang);
+      a.pose[k] = q;
+      a.motc[k] = law_motion(Lw, a.t[k], a.case_center[(size_t)k * 3 + 2]);
+    } else {
+      q = a.pose[k];
+    }
+    P[k] = q;
+    a.case_center[(size_t)k * 3] = q.px;
+    a.case_center[(size_t)k * 3 + 1] = q.py;
+  }
+  if (a.use_law)
+    for (int b = threadIdx.x; b < a.nbody; b += blockDim.x) {
+      const int cs = a.body_case[b];
+      a.mot[b] = law_motion(a.law[cs], a.t[cs], a.case_center[(size_t)cs * 3 + 2]);
+    }
+  __syncthreads();
+  for (int p = threadIdx.x; p < a.npatch; p += blockDim.x) {
+    const Pose Q = P[a.pat_case[p]];
+    const int i0 = a.pat_pt0[p];
+    const double ox0 = a.pt_off[2 * i0], oy0 = a.pt_off[2 * i0 + 1];
+    const double ox1 = a.pt_off[2 * i0 + 2], oy1 = a.pt_off[2 * i0 + 3];
+    const double x0 = Q.px + (Q.c * ox0 - Q.s * oy0), y0 = Q.py + (Q.s * ox0 + Q.c * oy0);
+    const double x1 = Q.px + (Q.c * ox1 - Q.s * oy1), y1 = Q.py + (Q.s * ox1 + Q.c * oy1);
+    const double dx = x1 - x0, dy = y1 - y0;
+    const double len = sqrt(dx * dx + dy * dy);
+    double nx = -dy / len, ny = dx / len;
+    if (a.pat_flip[p]) { nx = -nx; ny = -ny; }
+    a.pat_pc[(size_t)p * 3] = 0.5 * (x0 + x1);
+    a.pat_pc[(size_t)p * 3 + 1] = 0.5 * (y0 + y1);
+    a.pat_nv[(size_t)p * 3] = nx;
+    a.pat_nv[(size_t)p * 3 + 1] = ny;
+    a.pat_area[p] = len * a.pat_dz[p];
+  }
+}
+
+// RenewLevelDistance (Bodymove.cpp:136-249) for one cell: the walk from its
+// previous patch for cells closer than critic_dis, then its entries of the
+// cycle list -- the neighbours whose closest body differs, in neighbour
+// order, each starting from that neighbour's patch after the neighbour's
+// own walk (recomputed here: a walk keeps the body, so the bodies of the
+// previous step decide which neighbours enter the list).  distance_to_dm is
+// 9999 without wall boundaries on the domain, so signdis = min(signdis,
+// distance_to_dm) leaves it unchanged.  On the finest level: solid cells to
+// the solid list, in->out cells to their list, and an out->in cell that was
+// not an IB cell is the reference's abort 263 (Bodymove.cpp:494-544).
+__global__ void __launch_bounds__(256) k_move_dist(MoveArgs a) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = t < a.nw + a.ngw;
+  const int i = in ? set_cell(a, t, a.nw) : 0;
+  bool solid = false, io = false, chg = false;
+  if (in) {
+    int p = a.cpatch[i];
+    const double s0 = a.signdis[i];
+    double sd = s0;
+    if (p >= 0 && s0 < kCriticDis) {
+      double x[3], dist;
+      cell_center_dev(a, i, x);
+      walk_body(a, x, p, dist, sd);
+      // cells whose neighbours all share their body have no cycle entries
+      if (sd < kCriticDis && a.cmix[i]) {
+        const int myb0 = a.pat_body[p];
+        int myb = myb0;
+        for (int k = 0; k < kNb8; ++k) {
+          const int an = a.nb8[(size_t)k * a.N + i];
+          if (an < 0) continue;
+          const int q0 = a.cpatch[an];
+          if (q0 < 0) continue;
+          const int nbb = a.pat_body[q0];
+          if (nbb == myb0) continue;      // pushed only when it differs (:172-179)
+          if (nbb == myb) continue;       // processed only when it still differs (:193)
+          int q1 = q0;
+          if (a.signdis[an] < kCriticDis) {
+            double xa[3], da, sa;
+            cell_center_dev(a, an, xa);
+            walk_body(a, xa, q1, da, sa);
+          }
+          double dn, sn;
+          walk_body(a, x, q1, dn, sn);
+          bool change = false;
+          if (sn > 0.0 && sd < 0.0) change = true;
+          else if (dn < dist) {
+            if (sd * sn > 0.0) change = true;
+            else if (sn > 0.0) change = true;
+          }
+          if (change) { myb = nbb; p = q1; dist = dn; sd = sn; }
+        }
+      }
+    }
+    a.cp1[i] = p;
+    a.sd1[i] = sd;
+    const int p0 = a.cpatch[i];
+    chg = p0 >= 0 && a.pat_body[p] != a.pat_body[p0];
+    if (a.fin[i] && a.kind[i] != IBG_DMGHOST) {
+      solid = sd < 0.0;
+      io = s0 < 0.0 && !(sd < 0.0);
+      if (!(s0 < 0.0) && sd < 0.0 && a.mkib[i] != a.epoch[0] - 1) report_error(a.err, i, 263);
+    }
+  }
+  wave_append(a.cnt + 0, solid, i, a.slist, a.cap_s, a.err);
+  wave_append(a.cnt + 2, io, i, a.iolist, a.cap_io, a.err);
+  wave_append(a.cnt + 4, chg, i, a.chg, a.cap_chg, a.err);
+}
+
+// cmix of cell x from the new closest patches of x and its neighbours
+__device__ __forceinline__ void refresh_cmix(const MoveArgs &a, int x) {
+  const int px = a.cp1[x];
+  int8_t v = 0;
+  if (px >= 0) {
+    const int bx = a.pat_body[px];
+    for (int k = 0; k < kNb8 && !v; ++k) {
+      const int y = a.nb8[(size_t)k * a.N + x];
+      if (y < 0) continue;
+      const int py = a.cp1[y];
+      if (py >= 0 && a.pat_body[py] != bx) v = 1;
+    }
+  }
+  a.cmix[x] = v;
+}
+
+// Layer 1: the non-solid, non-domain-ghost neighbours of each solid cell of
+// the finest level (2D: the in-plane 3x3).
+__global__ void __launch_bounds__(256) k_move_mark1(MoveArgs a) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.cap_s) {
+    // the neighbourhood flags around cells whose body changed
+    const int r = j - a.cap_s;
+    if (r >= min(a.cnt[4], a.cap_chg)) return;
+    const int c = a.chg[r];
+    refresh_cmix(a, c);
+    for (int k = 0; k < kNb8; ++k) {
+      const int y = a.nb8[(size_t)k * a.N + c];
+      if (y >= 0) refresh_cmix(a, y);
+    }
     return;
   }
-  double x[3], dist, sd;
-  cell_center_dev(a, i, x);
-  walk_body(a, x, p, dist, sd);
-  a.cp1[i] = p; a.sd1[i] = sd; a.dist1[i] = dist;
-}
-
-// RenewLevelDistance cycle and solid loops (Bodymove.cpp:156-249).  The
-// cycle list holds, per cell, its neighbours whose closest body differs, in
-// neighbour order; a cell's entries only change that cell, so one thread
-// per cell processes its entries in order.  distance_to_dm is 9999 without
-// wall boundaries on the domain, so signdis = min(signdis, distance_to_dm)
-// leaves it unchanged.  An in->out cell of the finest level needs a solid
-// neighbour (InfectNewIBCells :575-650, abort 314).
-template <int D>
-__global__ void k_move_cycle(MoveArgs a) {
-#pragma clang fp contract(off)
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.N) return;
-  int p = a.cp1[i];
-  double sd = a.sd1[i], dist = a.dist1[i];
-  const double sd_old = a.signdis[i];
-  if (p >= 0 && sd < kCriticDis) {
-    const int myb0 = a.pat_body[p];
-    int myb = myb0;
-    double x[3];
-    bool have_x = false;
-    for (int q = 0; q < 27; ++q) {
-      if (D == 2 && q % 3 != 1) continue;
-      const int an = a.nb27[(size_t)i * 27 + q];
-      if (an < 0) continue;
-      const int np = a.cp1[an];
-      if (np < 0) continue;
-      const int nbb = a.pat_body[np];
-      if (nbb == myb0) continue;      // pushed only when it differs (:172-179)
-      if (myb == nbb) continue;       // processed only when it still differs (:193)
-      if (!have_x) { cell_center_dev(a, i, x); have_x = true; }
-      int pn = np;
-      double dn, sn;
-      walk_body(a, x, pn, dn, sn);
-      bool change = false;
-      if (sn > 0.0 && sd < 0.0) change = true;
-      else if (dn < dist) {
-        if (sd * sn > 0.0) change = true;
-        else if (sn > 0.0) change = true;
-      }
-      if (change) { myb = nbb; p = pn; dist = dn; sd = sn; }
+  const int ns = min(a.cnt[0], a.cap_s);
+  if (j >= ns) return;
+  const int ep = a.epoch[0];
+  const int s = a.slist[j];
+  for (int k = 0; k < kNb8; ++k) {
+    const int n = a.nb8[(size_t)k * a.N + s];
+    if (n < 0 || a.kind[n] == IBG_DMGHOST || a.sd1[n] < 0.0) continue;
+    if (atomicMax(&a.mk1[n], ep) < ep) {
+      const int pos = atomicAdd(&a.cnt[1], 1);
+      if (pos < a.cap_l1) a.l1list[pos] = n;
+      else report_error(a.err, n, 1090);
     }
   }
-  a.cpatch[i] = p;
-  a.signdis[i] = sd;
-  a.sd32[i] = (float)sd;
-  // out->in cells of the finest level were IB cells (RemoveIBCells abort
-  // 263, Bodymove.cpp:494-544); in->out cells are checked by k_move_l1
-  const bool chk = a.fin[i] && a.kind[i] != IBG_DMGHOST;
-  if (chk && !(sd_old < 0.0) && sd < 0.0 && a.isib[i] != 1) report_error(a.err, i, 263);
-  a.isib[i] = (chk && sd_old < 0.0 && !(sd < 0.0)) ? 2 : 0;
 }
 
-// IB layer 1: non-solid, non-domain-ghost finest cells with a solid
-// non-domain-ghost neighbour (2D: the in-plane 3x3; 3D: 27).  An in->out
-// cell without one is the reference's abort 314 (the move step is too
-// large for the incremental update, Bodymove.cpp:604-650).
-template <int D>
-__global__ void k_move_l1(MoveArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.N) return;
-  int8_t v = 0;
-  if (a.fin[i] && a.kind[i] != IBG_DMGHOST && !(a.signdis[i] < 0.0)) {
-    for (int q = 0; q < 27 && !v; ++q) {
-      if (D == 2 && q % 3 != 1) continue;
-      const int an = a.nb27[(size_t)i * 27 + q];
-      if (an >= 0 && a.signdis[an] < 0.0 && a.kind[an] != IBG_DMGHOST) v = 1;
+__device__ __forceinline__ void mark_ib(const MoveArgs &a, int m, int ep) {
+  if (atomicMax(&a.mkib[m], ep) < ep) {
+    if (a.kind[m] != IBG_NORMAL) report_error(a.err, m, 1082);
+    const int pos = atomicAdd(&a.cnt[3], 1);
+    if (pos < a.ib_cap) a.ib_cell[pos] = m;
+    else report_error(a.err, m, 1090);
+  }
+}
+
+// The IB set: every layer-1 cell and its non-solid, non-domain-ghost
+// neighbours.  Threads past the layer-1 capacity check the in->out cells:
+// each must be a layer-1 cell (InfectNewIBCells Bodymove.cpp:575-650,
+// abort 314 -- the move step is too large for the incremental update).
+__global__ void __launch_bounds__(256) k_move_mark2(MoveArgs a) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ep = a.epoch[0];
+  if (j < a.cap_l1) {
+    if (j >= min(a.cnt[1], a.cap_l1)) return;
+    const int n = a.l1list[j];
+    mark_ib(a, n, ep);
+    for (int k = 0; k < kNb8; ++k) {
+      const int m = a.nb8[(size_t)k * a.N + n];
+      if (m < 0 || a.kind[m] == IBG_DMGHOST || a.sd1[m] < 0.0) continue;
+      mark_ib(a, m, ep);
     }
+  } else {
+    const int r = j - a.cap_l1;
+    if (r >= min(a.cnt[2], a.cap_io)) return;
+    const int c = a.iolist[r];
+    if (a.mk1[c] != ep) report_error(a.err, c, 314);
   }
-  a.l1[i] = v;
-  if (a.isib[i] == 2 && !v) report_error(a.err, i, 314);
-}
-
-// IB set: layer-1 cells and the non-solid, non-domain-ghost neighbours of
-// layer-1 cells; per-block counts for the compaction.
-template <int D>
-__global__ void __launch_bounds__(256) k_move_ib(MoveArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int v = 0;
-  if (i < a.N && a.fin[i] && a.kind[i] != IBG_DMGHOST && !(a.signdis[i] < 0.0)) {
-    if (a.l1[i]) v = 1;
-    for (int q = 0; q < 27 && !v; ++q) {
-      if (D == 2 && q % 3 != 1) continue;
-      const int an = a.nb27[(size_t)i * 27 + q];
-      if (an >= 0 && a.l1[an]) v = 1;
-    }
-    if (v && a.kind[i] != IBG_NORMAL) report_error(a.err, i, 1082);
-  }
-  if (i < a.N) a.isib[i] = (int8_t)v;
-  const int n = __syncthreads_count(v);
-  if (threadIdx.x == 0) a.bcount[blockIdx.x] = n;
-}
-
-// Exclusive scan of the per-block counts (one workgroup).
-__global__ void __launch_bounds__(1024) k_move_scan(MoveArgs a, int nblocks) {
-  __shared__ int part[1024];
-  __shared__ int carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (int base = 0; base < nblocks; base += 1024) {
-    const int j = base + (int)threadIdx.x;
-    const int v = j < nblocks ? a.bcount[j] : 0;
-    part[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      const int t = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
-      __syncthreads();
-      part[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (j < nblocks) a.boff[j] = carry + part[threadIdx.x] - v;
-    __syncthreads();
-    if (threadIdx.x == 1023) carry += part[1023];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    if (carry > a.ib_cap) report_error(a.err, a.N, 1090);
-    a.ib_n[0] = carry < a.ib_cap ? carry : a.ib_cap;
-  }
-}
-
-// IB list in cell order.
-__global__ void __launch_bounds__(256) k_move_list(MoveArgs a) {
-  __shared__ int wsum[4];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool v = i < a.N && a.isib[i];
-  const unsigned long long m = __ballot(v);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) wsum[w] = __popcll(m);
-  __syncthreads();
-  if (!v) return;
-  int pos = a.boff[blockIdx.x];
-  for (int k = 0; k < w; ++k) pos += wsum[k];
-  pos += __popcll(m & ((1ull << lane) - 1ull));
-  if (pos < a.ib_cap) a.ib_cell[pos] = i;
 }
 
 // Cell of the finest level holding key ik, reached from `home` through the
 // neighbour table (one step per axis at a time); -1 when the path leaves the
 // level.
-template <int D>
 __device__ int locate_dev(const MoveArgs &a, int home, const int *ik) {
   int cur = home;
-  int k0 = a.key[(size_t)cur * 3], k1 = a.key[(size_t)cur * 3 + 1], k2 = a.key[(size_t)cur * 3 + 2];
-  if (D == 2 && ik[2] != k2) return -1;
+  int k0 = a.key[(size_t)cur * 3], k1 = a.key[(size_t)cur * 3 + 1];
+  if (ik[2] != a.key[(size_t)cur * 3 + 2]) return -1;
   for (int guard = 0; guard < 256; ++guard) {
-    if (k0 == ik[0] && k1 == ik[1] && k2 == ik[2]) return cur;
+    if (k0 == ik[0] && k1 == ik[1]) return cur;
     const int sx = (ik[0] > k0) - (ik[0] < k0), sy = (ik[1] > k1) - (ik[1] < k1);
-    const int sz = D == 2 ? 0 : (ik[2] > k2) - (ik[2] < k2);
-    const int nx = a.nb27[(size_t)cur * 27 + (1 + sx) * 9 + (1 + sy) * 3 + (1 + sz)];
+    const int nx = a.nb27[(size_t)cur * 27 + (1 + sx) * 9 + (1 + sy) * 3 + 1];
     if (nx < 0) return -1;
     cur = nx;
-    k0 += sx; k1 += sy; k2 += sz;
+    k0 += sx; k1 += sy;
   }
   return -1;
 }
 
 // Image-point search of the host mesh builder (Body.cpp:1044-1183,
 // 1527-1575): march from `start` along nv in steps of one cell in the
-// dominant direction until the 2x2(x2) bracketing cells are neither solid
-// nor IB cells, then the weights of Mesh::ComptHgIntpCoef (Mesh.cpp:
-// 1403-1442).  `home` is a finest-level cell near start; on success it is
-// the cell holding the image point.
-template <int D>
-__device__ bool search_image_dev(const MoveArgs &a, const double *start, const double *nv,
+// dominant direction until the 2x2 bracketing cells are neither solid nor
+// IB cells, then the weights of Mesh::ComptHgIntpCoef (Mesh.cpp:1403-1442).
+// `home` is a finest-level cell near start; on success it is the cell
+// holding the image point.
+__device__ bool search_image_dev(const MoveArgs &a, int ep, const double *start, const double *nv,
                                  const double *h, const double *org, int &home, int *intp,
                                  double *coef, double &hgdis_out) {
 #pragma clang fp contract(off)
@@ -344,7 +434,7 @@ __device__ bool search_image_dev(const MoveArgs &a, const double *start, const d
     for (int d = 0; d < 3; ++d) pt[d] = start[d] + nv[d] * hgdis;
     int ik[3];
     for (int d = 0; d < 3; ++d) ik[d] = (int)floor((pt[d] - org[d]) / h[d]);
-    const int cc = locate_dev<D>(a, home, ik);
+    const int cc = locate_dev(a, home, ik);
     if (cc < 0) return false;
     if (a.kind[cc] == IBG_DMGHOST) return false;
     home = cc;
@@ -352,7 +442,7 @@ __device__ bool search_image_dev(const MoveArgs &a, const double *start, const d
     cell_center_dev(a, cc, cen);
     double dkt[3];
     for (int d = 0; d < 3; ++d) dkt[d] = (pt[d] - cen[d]) / h[d];
-    if (D == 2) dkt[2] = 0.0;
+    dkt[2] = 0.0;   // one z layer: the image point is in it
     int hgdir[3];
     for (int d = 0; d < 3; ++d) hgdir[d] = dkt[d] > 0.0 ? 1 : 0;
     bool ok = true;
@@ -365,7 +455,7 @@ __device__ bool search_image_dev(const MoveArgs &a, const double *start, const d
           intp[q] = an;
           coef[q] = 0.0;
           if (an < 0) { ok = false; continue; }
-          if (a.signdis[an] < 0.0 || a.isib[an]) ok = false;
+          if (a.sd1[an] < 0.0 || a.mkib[an] == ep) ok = false;
           coef[q] = (ca[nx] + cb[nx] * fabs(dkt[0])) * (ca[ny] + cb[ny] * fabs(dkt[1])) *
                     (ca[nz] + cb[nz] * fabs(dkt[2]));
         }
@@ -375,50 +465,47 @@ __device__ bool search_image_dev(const MoveArgs &a, const double *start, const d
 }
 
 // Image point, weights, attach point (Body.cpp:1073) and normal of every IB
-// cell; failure is the reference's abort of FindImageCell.
-template <int D>
-__global__ void k_move_image(MoveArgs a) {
+// cell, then (threads past the list capacity) the image points of the
+// wall-force patches, searched from each patch centre (pat_cc holds a
+// finest-level cell near the previous image point).  Failure is the
+// reference's abort of FindImageCell (1183) / FindPatchHangingNode (1575).
+__device__ void move_image(const MoveArgs &a, int j) {
 #pragma clang fp contract(off)
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.ib_n[0]) return;
-  const int g = a.ib_cell[j];
-  const int p = a.cpatch[g];
-  if (p < 0) { report_error(a.err, g, 1183); return; }
-  const BlkInfo &B = a.blk[a.cblk[g]];
-  const double *org = a.origin + (size_t)B.case_id * 3;
-  const double nv[3] = {a.pat_nv[(size_t)p * 3], a.pat_nv[(size_t)p * 3 + 1],
-                        a.pat_nv[(size_t)p * 3 + 2]};
-  const double sd = a.signdis[g];
-  double cen[3];
-  cell_center_dev(a, g, cen);
+  const int ep = a.epoch[0];
   int intp[8];
   double coef[8], hgdis = 0.0;
-  int home = g;
-  if (!search_image_dev<D>(a, cen, nv, B.h, org, home, intp, coef, hgdis)) {
-    report_error(a.err, g, 1183);
-    for (int q = 0; q < 8; ++q) { intp[q] = -1; coef[q] = 0.0; }
+  if (j < a.ib_cap) {
+    if (j >= min(a.cnt[3], a.ib_cap)) return;
+    const int g = a.ib_cell[j];
+    const int p = a.cp1[g];
+    if (p < 0) { report_error(a.err, g, 1183); return; }
+    const BlkInfo &B = a.blk[a.cblk[g]];
+    const double *org = a.origin + (size_t)B.case_id * 3;
+    const double nv[3] = {a.pat_nv[(size_t)p * 3], a.pat_nv[(size_t)p * 3 + 1],
+                          a.pat_nv[(size_t)p * 3 + 2]};
+    const double sd = a.sd1[g];
+    double cen[3];
+    cell_center_dev(a, g, cen);
+    int home = g;
+    if (!search_image_dev(a, ep, cen, nv, B.h, org, home, intp, coef, hgdis)) {
+      report_error(a.err, g, 1183);
+      for (int q = 0; q < 8; ++q) { intp[q] = -1; coef[q] = 0.0; }
+    }
+    for (int q = 0; q < 8; ++q) {
+      const bool z0 = (q & 1) == 0;   // zero-weight z layer of the 2D stencil
+      a.ib_intp[(size_t)j * 8 + q] = z0 ? -1 : intp[q];
+      a.ib_coef[(size_t)j * 8 + q] = z0 ? 0.0 : coef[q];
+    }
+    a.ib_hgdis[j] = hgdis;
+    for (int d = 0; d < 3; ++d) {
+      a.ib_nv[(size_t)j * 3 + d] = nv[d];
+      a.ib_attach[(size_t)j * 3 + d] = cen[d] - nv[d] * sd;
+    }
+    a.ib_body[j] = a.pat_body[p];
+    a.ib_case[j] = B.case_id;
+    return;
   }
-  for (int q = 0; q < 8; ++q) {
-    int an = intp[q];
-    double w = coef[q];
-    if (D == 2 && (q & 1) == 0) { an = -1; w = 0.0; }   // zero-weight z layer
-    a.ib_intp[(size_t)j * 8 + q] = an;
-    a.ib_coef[(size_t)j * 8 + q] = w;
-  }
-  a.ib_hgdis[j] = hgdis;
-  for (int d = 0; d < 3; ++d) {
-    a.ib_nv[(size_t)j * 3 + d] = nv[d];
-    a.ib_attach[(size_t)j * 3 + d] = cen[d] - nv[d] * sd;
-  }
-  a.ib_body[j] = a.pat_body[p];
-  a.ib_case[j] = B.case_id;
-}
-
-// Image points of the wall-force patches, searched from each patch centre;
-// pat_cc holds a finest-level cell near the previous image point.
-template <int D>
-__global__ void k_move_patch_image(MoveArgs a) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = j - a.ib_cap;
   if (p >= a.npatch) return;
   int home = a.pat_cc[p];
   const BlkInfo &B = a.blk[a.cblk[home]];
@@ -427,50 +514,69 @@ __global__ void k_move_patch_image(MoveArgs a) {
                         a.pat_pc[(size_t)p * 3 + 2]};
   const double nv[3] = {a.pat_nv[(size_t)p * 3], a.pat_nv[(size_t)p * 3 + 1],
                         a.pat_nv[(size_t)p * 3 + 2]};
-  int intp[8];
-  double coef[8], hgdis = 0.0;
-  if (!search_image_dev<D>(a, pc, nv, B.h, org, home, intp, coef, hgdis)) {
+  if (!search_image_dev(a, ep, pc, nv, B.h, org, home, intp, coef, hgdis)) {
     report_error(a.err, a.pat_cc[p], 1575);
     for (int q = 0; q < 8; ++q) { intp[q] = -1; coef[q] = 0.0; }
   } else {
     a.pat_cc[p] = home;
   }
   for (int q = 0; q < 8; ++q) {
-    int an = intp[q];
-    double w = coef[q];
-    if (D == 2 && (q & 1) == 0) { an = -1; w = 0.0; }
-    a.pat_intp[(size_t)p * 8 + q] = an;
-    a.pat_coef[(size_t)p * 8 + q] = w;
+    const bool z0 = (q & 1) == 0;
+    a.pat_intp[(size_t)p * 8 + q] = z0 ? -1 : intp[q];
+    a.pat_coef[(size_t)p * 8 + q] = z0 ? 0.0 : coef[q];
   }
   a.pat_hgdis[p] = hgdis;
 }
 
-// Stage masks of the new solid / IB sets, as at upload; with `upd` set,
-// the stage updates of this step (levels >= upd_lmin) are accumulated.
-__global__ void __launch_bounds__(256) k_move_flags(MoveArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// New distances and patches in place (the FP32 copy for the mixed SA
+// source), the stage masks of the marched normal cells as at upload; with
+// `upd` set, the stage updates of this step are accumulated (one atomic
+// per workgroup, grid-stride).
+__device__ void move_commit(const MoveArgs &a, int nblk) {
+  const int ep = a.epoch[0];
+  const int n = a.nw + a.ngw;
   int cnt = 0;
-  if (i < a.Nn) {
-    const double sd = a.signdis[i];
-    const bool ib = a.isib[i] != 0;
-    const bool solid = sd < 0.0;
-    uint8_t f;
-    if (!a.fin[i]) {
-      f = F_UPDATE | F_RESID | ((sd > 0.0) ? F_PRIM : F_FREE);
-    } else {
-      f = F_FINEST;
-      if (!solid && !ib) f |= F_UPDATE;
-      if (sd > 0.0 && !ib) f |= F_PRIM | F_RESID;
-      else if (!ib) f |= F_FREE;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += nblk * blockDim.x) {
+    const int i = set_cell(a, t, a.nw);
+    const double sd = a.sd1[i];
+    a.cpatch[i] = a.cp1[i];
+    a.signdis[i] = sd;
+    a.sd32[i] = (float)sd;
+    if (t < a.nflag) {
+      const bool ib = a.mkib[i] == ep;
+      const bool solid = sd < 0.0;
+      uint8_t f;
+      if (!a.fin[i]) {
+        f = F_UPDATE | F_RESID | ((sd > 0.0) ? F_PRIM : F_FREE);
+      } else {
+        f = F_FINEST;
+        if (!solid && !ib) f |= F_UPDATE;
+        if (sd > 0.0 && !ib) f |= F_PRIM | F_RESID;
+        else if (!ib) f |= F_FREE;
+      }
+      if (sd >= 0.0) f |= F_SA;
+      a.flags[i] = f;
+      cnt += (f & F_UPDATE) ? 1 : 0;
     }
-    if (sd >= 0.0) f |= F_SA;
-    a.flags[i] = f;
-    cnt = (f & F_UPDATE) && a.blk[a.cblk[i]].level >= a.upd_lmin;
   }
   if (a.upd) {
-    const int n = __syncthreads_count(cnt);
-    if (threadIdx.x == 0 && n) atomicAdd(a.upd, (unsigned long long)n * a.nstage);
+    __shared__ int part[4];
+    int v = cnt;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int s = part[0] + part[1] + part[2] + part[3];
+      if (s) atomicAdd(a.upd, (unsigned long long)s * a.nstage);
+    }
   }
+}
+
+// Commit and image search in one launch: workgroups [0, ncommit) commit,
+// the others search (they read only the new arrays, not the committed ones).
+__global__ void __launch_bounds__(256) k_move_finish(MoveArgs a, int ncommit) {
+  if ((int)blockIdx.x < ncommit) move_commit(a, ncommit);
+  else move_image(a, ((int)blockIdx.x - ncommit) * blockDim.x + threadIdx.x);
 }
 
 // Per-level count of the cells the stage update touches.
@@ -482,61 +588,6 @@ __global__ void __launch_bounds__(256) k_move_levelcount(MoveArgs a, int L) {
   if (i < a.Nn && (a.flags[i] & F_UPDATE)) atomicAdd(&cnt[a.blk[a.cblk[i]].level], 1u);
   __syncthreads();
   if (threadIdx.x < L && cnt[threadIdx.x]) atomicAdd(&a.lvlcnt[threadIdx.x], (unsigned long long)cnt[threadIdx.x]);
-}
-
-// Prescribed pitch and heave of the bodies of one case about a pivot:
-// angle of attack aoa(t) = aoa0 + amp sin(omega t) (nose up), pivot
-// y(t) = y0 + heave sin(omega t).  The rotation applied to the zero-pose
-// offsets is a = -aoa (the section rotates clockwise for a nose-up angle).
-struct PitchHeave {
-  double x0, y0, aoa0, amp, omega, heave;   // angles in radians
-};
-
-// Pose, reference point and wall motion of every case at its time t
-// (the rigid-body velocity of Body::Surface_Point_Velocity, Body.H:709-721:
-// translation plus rotation about the body centre, here the pivot).
-__device__ __forceinline__ Motion law_motion(const PitchHeave &Lw, double t, double zc) {
-  const double wt = Lw.omega * t;
-  const double sw = sin(wt), cw = cos(wt);
-  Motion M;
-  M.tra[0] = 0.0; M.tra[1] = Lw.heave * Lw.omega * cw; M.tra[2] = 0.0;
-  M.rot[0] = 0.0; M.rot[1] = 0.0; M.rot[2] = -Lw.amp * Lw.omega * cw;
-  M.center[0] = Lw.x0; M.center[1] = Lw.y0 + Lw.heave * sw; M.center[2] = zc;
-  return M;
-}
-
-// mot: per body (the IB fill); motc: per case (the wall force); case_center
-// z is kept.
-__global__ void k_motion_pose(int ncases, const PitchHeave *law, const double *t, Pose *pose,
-                              int nbody, const int *body_case, Motion *mot, Motion *motc,
-                              const double *case_center) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < nbody) {
-    const int cs = body_case[k];
-    mot[k] = law_motion(law[cs], t[cs], case_center[(size_t)cs * 3 + 2]);
-  }
-  if (k < ncases) {
-    const PitchHeave Lw = law[k];
-    const double sw = sin(Lw.omega * t[k]);
-    const double a = -(Lw.aoa0 + Lw.amp * sw);
-    Pose P;
-    P.px = Lw.x0;
-    P.py = Lw.y0 + Lw.heave * sw;
-    P.c = cos(a);
-    P.s = sin(a);
-    pose[k] = P;
-    motc[k] = law_motion(Lw, t[k], case_center[(size_t)k * 3 + 2]);
-  }
-}
-
-// Reference point of each case after the bodies moved: the pivot.  Runs
-// after k_motion_pose (one launch per step, ordered on the stream).
-__global__ void k_motion_center(int ncases, const Pose *pose, double *case_center) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < ncases) {
-    case_center[(size_t)k * 3] = pose[k].px;
-    case_center[(size_t)k * 3 + 1] = pose[k].py;
-  }
 }
 
 // t += dt of the step just marched.

@@ -142,8 +142,16 @@ def test_prescribed_motion_march(tiny, mode):
     eng.set_motion(law, t0=0.0)
     eng.set_state(fv0)
     mb = oracle_ffi.MovingBodies(m)
+    level = m.level_of()
+    L = m.nlevels
     t = 0.0
     for ts in range(nsteps):
+        # the march re-tags the levels the step marches (the ones its kernels
+        # read); the oracle re-tags every level
+        lmin = L - 1
+        while lmin > 0 and ts % (1 << (L - lmin)) == 0:
+            lmin -= 1
+        fresh = level >= lmin
         a = np.deg2rad(aoa0 + amp * np.sin(omega * t))
         pose = [x0, y0 + heave * np.sin(omega * t), np.cos(-a), np.sin(-a)]
         r = mb.move([pose])
@@ -151,7 +159,7 @@ def test_prescribed_motion_march(tiny, mode):
         eng.TimeMarching(ts, 1, use_graph=False)
         sd, cp, ib, nib = eng.geometry()
         assert np.array_equal(ib, mb.ib) and nib == r["nib"]
-        assert np.allclose(sd, mb.signdis, rtol=1e-12, atol=1e-13)
+        assert np.allclose(sd[fresh], mb.signdis[fresh], rtol=1e-12, atol=1e-13)
         t = eng.get_time()[0]
     assert t > 0.0
     assert np.array_equal(eng.get_state(), out[0][0])
