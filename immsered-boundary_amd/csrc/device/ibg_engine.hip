@@ -1663,6 +1663,14 @@ ibg_status ibg_get_time(ibg_ctx *c, double *t) {
   return IBG_OK;
 }
 
+ibg_status ibg_set_time(ibg_ctx *c, const double *t) {
+  if (!c || !t) return IBG_ERR_ARG;
+  if (!c->have_mesh || !c->mv.on) return IBG_ERR_STATE;
+  HIPCHK(hipMemcpyAsync(c->mv.t, t, sizeof(double) * c->ncases, hipMemcpyHostToDevice, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return IBG_OK;
+}
+
 ibg_status ibg_get_geometry(ibg_ctx *c, double *signdis, int32_t *cell_patch, int8_t *ib,
                             int32_t *nib) {
   if (!c) return IBG_ERR_ARG;

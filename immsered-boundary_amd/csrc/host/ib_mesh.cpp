@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -759,6 +760,179 @@ const ibg_mesh_soa *ibm_mesh(const ibm_handle *h) { return h ? &h->soa : nullptr
 const char *ibm_last_error(void) { return g_last_error.c_str(); }
 
 void ibm_free(ibm_handle *h) { delete h; }
+
+// AMR::ComptDatainNewBox (AMR.H:148-215) after a regrid, on canonical keys:
+// a cell whose key was a normal cell of the same level keeps its data, and
+// so does a ghost whose key existed; a
+// cell whose old region was refined takes the restriction of the old leaves
+// below it (AverageCell TwoLevelOp.H:45-87, level by level, the SA variable
+// averaged); a cell whose old parent
+// was a normal cell takes its prolongation (Prolongation TwoLevelOp.H:
+// 183-215: conserved variables U_p +- 1/4 of their central differences over
+// the parent's face neighbours, ComptFineCellData :12-44), or a copy of a
+// ghost parent (AMR.H:208-213); otherwise the nearest older cell at the same
+// key or above is copied.
+int32_t ibm_transfer(const ibm_handle *oh, const ibg_physics *ph, const double *fv_old,
+                     const double *nut_old, const ibm_handle *nh, double *fv_new,
+                     double *nut_new) {
+  try {
+    if (!oh || !nh || !ph || !fv_old || !fv_new) throw std::runtime_error("null argument");
+    if (oh->ncases != nh->ncases || oh->L != nh->L || oh->dim != nh->dim)
+      throw std::runtime_error("level sets of different cases");
+    if ((nut_old == nullptr) != (nut_new == nullptr)) throw std::runtime_error("nut in or out missing");
+    const int L = oh->L, nc = oh->ncases, dim = oh->dim;
+    const int NF = IBG_NFV;
+    if (oh->blk_twod != nh->blk_twod) throw std::runtime_error("level sets refine differently");
+    std::vector<std::unordered_map<int64_t, int>> omap((size_t)nc * L);
+    for (int b = 0; b < nc * L; ++b) {
+      auto &mp = omap[b];
+      mp.reserve((size_t)(oh->blk_start[b + 1] - oh->blk_start[b]) * 2);
+      for (int g = oh->blk_start[b]; g < oh->blk_start[b + 1]; ++g)
+        mp[pack_key(oh->key[(size_t)g * 3], oh->key[(size_t)g * 3 + 1], oh->key[(size_t)g * 3 + 2])] = g;
+    }
+    auto old_at = [&](int c, int l, int ix, int iy, int iz) -> int {
+      if (l < 0 || l >= L) return -1;
+      const auto &mp = omap[(size_t)c * L + l];
+      auto it = mp.find(pack_key(ix, iy, iz));
+      return it == mp.end() ? -1 : it->second;
+    };
+    auto fdiv2 = [](int k) { return k >= 0 ? k / 2 : -((-k + 1) / 2); };
+    auto put = [&](double *f, const ibg_physics &P, double r, double u, double v, double w, double pr) {
+      // FlowVariables (roe,u,v,w,p,T,e), T and e by Get_T / Get_E (FlowVariables.H:23,33)
+      f[0] = r; f[1] = u; f[2] = v; f[3] = w; f[4] = pr;
+      f[5] = pr / (r * P.Rg);
+      f[6] = P.Cv * f[5] + 0.5 * (u * u + v * v + w * w);
+    };
+    auto cons = [&](int a, double *U) {
+      const double *f = fv_old + (size_t)a * NF;
+      U[0] = f[0]; U[1] = f[0] * f[1]; U[2] = f[0] * f[2]; U[3] = f[0] * f[3]; U[4] = f[0] * f[6];
+    };
+    // value of an old region: a normal cell, or the restriction (AverageCell)
+    // of the values of its sons when every son resolves
+    std::function<bool(int, int, int, int, int, double &, double &, double &, double &, double &,
+                       double &)>
+        leafval = [&](int c, int l, int kx, int ky, int kz, double &r, double &u, double &v,
+                      double &w, double &pr, double &nv) -> bool {
+      const int a = old_at(c, l, kx, ky, kz);
+      if (a >= 0 && oh->kind[a] == IBG_NORMAL) {
+        const double *o = fv_old + (size_t)a * NF;
+        r = o[0]; u = o[1]; v = o[2]; w = o[3]; pr = o[4];
+        nv = nut_old ? nut_old[a] : 0.0;
+        return true;
+      }
+      if (l + 1 >= L) return false;
+      const ibg_physics &P = ph[c];
+      const bool tw = oh->blk_twod[(size_t)c * L + l + 1] != 0;
+      double rs = 0, rus = 0, rvs = 0, rws = 0, res = 0, ns = 0;
+      int cnt = 0;
+      for (int si = 0; si < 2; ++si)
+        for (int sj = 0; sj < 2; ++sj)
+          for (int sk = 0; sk < (tw ? 1 : 2); ++sk) {
+            double r2, u2, v2, w2, p2, n2;
+            if (!leafval(c, l + 1, 2 * kx + si, 2 * ky + sj, tw ? kz : 2 * kz + sk, r2, u2, v2, w2,
+                         p2, n2))
+              return false;
+            const double T = p2 / (r2 * P.Rg);
+            const double e = P.Cv * T + 0.5 * (u2 * u2 + v2 * v2 + w2 * w2);
+            rs += r2; rus += u2 * r2; rvs += v2 * r2; rws += w2 * r2; res += e * r2; ns += n2;
+            ++cnt;
+          }
+      u = rus / rs; v = rvs / rs; w = rws / rs;
+      const double e = res / rs;
+      r = rs / cnt;
+      pr = (e - 0.5 * (u * u + v * v + w * w)) * (P.gama - 1.0) * r;
+      nv = ns / cnt;
+      return true;
+    };
+    const int N = nh->blk_start[(size_t)nc * L];
+    for (int b = 0; b < nc * L; ++b) {
+      const int c = b / L, l = b % L;
+      const ibg_physics &P = ph[c];
+      const bool twod_l = nh->blk_twod[b] != 0;                  // level l over l-1
+      for (int g = nh->blk_start[b]; g < nh->blk_start[b + 1]; ++g) {
+        const int kx = nh->key[(size_t)g * 3], ky = nh->key[(size_t)g * 3 + 1], kz = nh->key[(size_t)g * 3 + 2];
+        double *f = fv_new + (size_t)g * NF;
+        double *nu = nut_new ? nut_new + g : nullptr;
+        int a = old_at(c, l, kx, ky, kz);
+        // a kept cell, or a ghost whose key existed (ghosts are refilled by
+        // the ghost fills of the next step)
+        if (a >= 0 && (oh->kind[a] == IBG_NORMAL || nh->kind[g] != IBG_NORMAL)) {
+          std::memcpy(f, fv_old + (size_t)a * NF, sizeof(double) * NF);
+          if (nu) *nu = nut_old[a];
+          continue;
+        }
+        // restriction: the average of the conserved variables over the old
+        // leaves below (sons, or theirs where a son was itself refined)
+        {
+          double r, u, v, w, pr, nv = 0.0;
+          if (leafval(c, l, kx, ky, kz, r, u, v, w, pr, nv)) {
+            put(f, P, r, u, v, w, pr);
+            if (nu) *nu = nv;
+            continue;
+          }
+        }
+        // prolongation of the old parent
+        if (l > 0) {
+          const int px = fdiv2(kx), py = fdiv2(ky), pz = twod_l ? kz : fdiv2(kz);
+          const int pa = old_at(c, l - 1, px, py, pz);
+          if (pa >= 0) {
+            const int *nb = oh->nb.data() + (size_t)pa * 27;
+            const int ax0 = nb[0 * 9 + 1 * 3 + 1], ax1 = nb[2 * 9 + 1 * 3 + 1];
+            const int ay0 = nb[1 * 9 + 0 * 3 + 1], ay1 = nb[1 * 9 + 2 * 3 + 1];
+            const int az0 = nb[1 * 9 + 1 * 3 + 0], az1 = nb[1 * 9 + 1 * 3 + 2];
+            const bool full = ax0 >= 0 && ax1 >= 0 && ay0 >= 0 && ay1 >= 0 && az0 >= 0 && az1 >= 0;
+            if (oh->kind[pa] == IBG_NORMAL && full) {
+              double Uc[5], X0[5], X1[5], Y0[5], Y1[5], Z0[5], Z1[5];
+              cons(pa, Uc); cons(ax0, X0); cons(ax1, X1); cons(ay0, Y0); cons(ay1, Y1);
+              cons(az0, Z0); cons(az1, Z1);
+              const double finc[2] = {-0.25, 0.25};
+              const int si = kx - 2 * px, sj = ky - 2 * py, sk = twod_l ? 0 : kz - 2 * pz;
+              double U[5];
+              for (int t = 0; t < 5; ++t) {
+                const double gx = (X1[t] - X0[t]) * 0.5, gy = (Y1[t] - Y0[t]) * 0.5,
+                             gz = (Z1[t] - Z0[t]) * 0.5;
+                U[t] = twod_l ? Uc[t] + finc[si] * gx + finc[sj] * gy
+                              : Uc[t] + finc[si] * gx + finc[sj] * gy + finc[sk] * gz;
+              }
+              const double r = U[0], u = U[1] / U[0], v = U[2] / U[0], w = U[3] / U[0], e = U[4] / U[0];
+              const double ke = (dim == 3) ? 0.5 * (u * u + v * v + w * w) : 0.5 * (u * u + v * v);
+              put(f, P, r, u, v, w, (e - ke) * (P.gama - 1.0) * r);
+              if (nu) {
+                const double gx = (nut_old[ax1] - nut_old[ax0]) * 0.5, gy = (nut_old[ay1] - nut_old[ay0]) * 0.5,
+                             gz = (nut_old[az1] - nut_old[az0]) * 0.5;
+                *nu = twod_l ? nut_old[pa] + finc[si] * gx + finc[sj] * gy
+                             : nut_old[pa] + finc[si] * gx + finc[sj] * gy + finc[sk] * gz;
+              }
+              continue;
+            }
+            if (a < 0) a = pa;   // a ghost parent is copied
+          }
+        }
+        // the same key as a ghost, or the nearest older ancestor
+        for (int up = 1; a < 0 && l - up >= 0; ++up) {
+          int ax = kx, ay = ky, az = kz;
+          for (int k = 0; k < up; ++k) {
+            const bool tw = nh->blk_twod[(size_t)c * L + l - k] != 0;
+            ax = fdiv2(ax); ay = fdiv2(ay); if (!tw) az = fdiv2(az);
+          }
+          a = old_at(c, l - up, ax, ay, az);
+        }
+        if (a < 0)
+          throw std::runtime_error("cell (case " + std::to_string(c) + ", level " + std::to_string(l) +
+                                   ", key " + std::to_string(kx) + " " + std::to_string(ky) + " " +
+                                   std::to_string(kz) + ", kind " + std::to_string((int)nh->kind[g]) +
+                                   ") of the new level set has no old cell above it");
+        std::memcpy(f, fv_old + (size_t)a * NF, sizeof(double) * NF);
+        if (nu) *nu = nut_old[a];
+      }
+    }
+    (void)N;
+    return 0;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return 1;
+  }
+}
 
 int32_t ibm_cell_patch(const ibm_handle *h, int32_t *out) {
   if (!h || !out) return 1;

@@ -65,6 +65,13 @@ int32_t ibm_init_state(const ibm_handle *h, const ibg_physics *phys,
 int32_t ibm_cell_patch(const ibm_handle *h, int32_t *out /* [ncells] */);
 /* per surface patch: the cell holding its image point (HGCell::closecell) */
 int32_t ibm_patch_close(const ibm_handle *h, int32_t *out /* [npatch] */);
+/* State of a new level set of the same cases from the state of the old one
+   after a regrid (AMR::ComptDatainNewBox AMR.H:148-215): AoS FlowVariables
+   [ncells][7] in and out, the SA variable [ncells] in and out or both
+   NULL.  Returns 0, or 1 with ibm_last_error.                             */
+int32_t ibm_transfer(const ibm_handle *old_mesh, const ibg_physics *phys /* [ncases] */,
+                     const double *fv_old, const double *nut_old,
+                     const ibm_handle *new_mesh, double *fv_new, double *nut_new);
 /* cell centers [ncells][3]                                                 */
 int32_t ibm_cell_centers(const ibm_handle *h, double *xyz);
 /* error text of the last failed build                                      */

@@ -150,6 +150,9 @@ def host_lib():
         lib.ibm_naca4_offsets.argtypes = [C.c_double, C.c_int32, f64p]
         lib.ibm_pose.restype = None
         lib.ibm_pose.argtypes = [C.c_double, C.c_double, C.c_double, f64p]
+        lib.ibm_transfer.restype = C.c_int32
+        lib.ibm_transfer.argtypes = [C.c_void_p, C.POINTER(Physics), f64p, f64p, C.c_void_p, f64p,
+                                     f64p]
         lib.ibm_cell_patch.restype = C.c_int32
         lib.ibm_cell_patch.argtypes = [C.c_void_p, i32p]
         lib.ibm_patch_close.restype = C.c_int32
@@ -192,6 +195,7 @@ GPU_FUNCS = {
     "ibg_move_bodies": (C.c_int, [C.c_void_p, C.POINTER(BodyPose)]),
     "ibg_set_motion": (C.c_int, [C.c_void_p, C.POINTER(PitchHeave), C.c_double]),
     "ibg_get_time": (C.c_int, [C.c_void_p, f64p]),
+    "ibg_set_time": (C.c_int, [C.c_void_p, f64p]),
     "ibg_get_geometry": (C.c_int, [C.c_void_p, f64p, i32p, i8p, i32p]),
     "ibg_get_ib_list": (C.c_int, [C.c_void_p, C.c_int32, i32p, i32p, f64p, f64p, f64p, f64p,
                                   i32p]),

@@ -290,12 +290,40 @@ class Engine:
         """laws: per case (x0, y0, aoa0_deg, amp_deg, omega, heave) or None."""
         if laws is None:
             self._chk(self.lib.ibg_set_motion(self.ctx, None, 0.0), "ibg_set_motion")
+            self._law = None
             return
+        self._law = [tuple(lw) for lw in laws]
         arr = (_ffi.PitchHeave * self.mesh.ncases)()
         for k, lw in enumerate(laws):
             (arr[k].x0, arr[k].y0, arr[k].aoa0_deg, arr[k].amp_deg, arr[k].omega,
              arr[k].heave) = [float(x) for x in lw]
         self._chk(self.lib.ibg_set_motion(self.ctx, arr, float(t0)), "ibg_set_motion")
+
+    def regrid(self, new_mesh):
+        """Regrid (AMR::PerformAdaptive AMR.cpp:6-209) to `new_mesh`, a rebuild
+        of the same cases (for a moving body: at its current pose): the state
+        moves over by canonical keys (Mesh.transfer_to) and replaces the
+        uploaded mesh; dt, the time and a prescribed motion carry on."""
+        fv = self.get_state()
+        nut = self.get_nut() if self.turbulence else None
+        dt = self.get_dt()
+        moving = getattr(self, "_surface", None) is not None
+        t = self.get_time() if moving else None
+        fv2, nut2 = self.mesh.transfer_to(new_mesh, fv, nut)
+        self._chk(self.lib.ibg_upload_mesh(self.ctx, new_mesh.soa_ptr, new_mesh.phys),
+                  "ibg_upload_mesh")
+        self.mesh = new_mesh
+        self._surface = None
+        self.set_state(fv2)
+        if nut2 is not None:
+            self.set_nut(nut2)
+        self.set_dt(dt)
+        if moving:
+            self.upload_surface()
+            if getattr(self, "_law", None) is not None:
+                self.set_motion(self._law)
+                self._chk(self.lib.ibg_set_time(self.ctx, np.ascontiguousarray(t).ctypes.data_as(_ffi.f64p)),
+                          "ibg_set_time")
 
     def get_time(self):
         t = np.zeros(self.mesh.ncases)

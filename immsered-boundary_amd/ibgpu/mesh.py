@@ -265,6 +265,22 @@ class Mesh:
         _ffi.host_lib().ibm_pose(0.0, float(pivot_y), float(aoa_deg), p.ctypes.data_as(_ffi.f64p))
         return p
 
+    def transfer_to(self, new, fv, nut=None):
+        """State on `new` (a rebuild of the same cases after a regrid) from the
+        state `fv` [ncells][7] (and SA variable `nut`) on this level set:
+        AMR::ComptDatainNewBox (AMR.H:148-215) on canonical keys."""
+        fv = np.ascontiguousarray(fv, dtype=np.float64)
+        out = np.zeros((new.ncells, _ffi.IBG_NFV))
+        nut_in = None if nut is None else np.ascontiguousarray(nut, dtype=np.float64)
+        nut_out = None if nut is None else np.zeros(new.ncells)
+        rc = self._host.ibm_transfer(self._h, self.phys, fv.ctypes.data_as(_ffi.f64p),
+                                     None if nut_in is None else nut_in.ctypes.data_as(_ffi.f64p),
+                                     new._h, out.ctypes.data_as(_ffi.f64p),
+                                     None if nut_out is None else nut_out.ctypes.data_as(_ffi.f64p))
+        if rc != 0:
+            raise RuntimeError("state transfer failed: " + self._host.ibm_last_error().decode())
+        return out, nut_out
+
     def init_state(self):
         fv = np.zeros((self.ncells, _ffi.IBG_NFV))
         self._host.ibm_init_state(self._h, self.phys, fv.ctypes.data_as(_ffi.f64p))

@@ -317,6 +317,8 @@ ibg_status ibg_move_bodies(ibg_ctx *ctx, const ibg_body_pose *pose);
  * motion == NULL, in ibg_forces.  law == NULL stops it.                    */
 ibg_status ibg_set_motion(ibg_ctx *ctx, const ibg_pitch_heave *law, double t0);
 ibg_status ibg_get_time(ibg_ctx *ctx, double *t /* [ncases] */);
+/* The law's time of every case (carried over a regrid).                    */
+ibg_status ibg_set_time(ibg_ctx *ctx, const double *t /* [ncases] */);
 /* Probes of the moving-body geometry (host cell and patch indices).
  * signdis / cell_patch / ib: [ncells] each (any may be NULL).             */
 ibg_status ibg_get_geometry(ibg_ctx *ctx, double *signdis, int32_t *cell_patch, int8_t *ib,
