@@ -84,4 +84,11 @@ struct Motion {
   double tra[3], rot[3], center[3];
 };
 
+// A domain-wall ghost (BC -81): the reference cell's inward neighbour, the
+// wall normal and the wall distances of the reference cell and neighbour.
+struct DmgWall {
+  double nmv[3], dref, dnbr;
+  int nbr, pad;
+};
+
 }  // namespace ibg

@@ -101,6 +101,8 @@ struct ibg_ctx {
   BlkInfo *blk = nullptr;
   DevPhys *phys = nullptr;
   int4 *dmg = nullptr;
+  DmgWall *dmgw = nullptr;      // wall ghosts (BC -81), parallel to dmg; null without walls
+  double *dist_dm = nullptr;    // [N] distance to the domain walls; null without walls
   int *res_ci = nullptr, *res_fi = nullptr, *pro_ci = nullptr, *pro_fi = nullptr;
   int *ib_cell = nullptr, *ib_intp = nullptr, *ib_body = nullptr, *ib_case = nullptr;
   double *ib_coef = nullptr, *ib_hgdis = nullptr, *ib_attach = nullptr, *ib_nv = nullptr,
@@ -372,12 +374,14 @@ template <int D>
 ibg_status launch_bc(ibg_ctx *c) {
   if (c->ndmg_edge > 0) {
     hipLaunchKernelGGL((k_bc<D>), dim3(grid(c->ndmg_edge, 256)), dim3(256), 0, c->s, c->ndmg_edge,
-                       c->dmg, c->sout<D>(), c->phys);
+                       c->dmg, (const DmgWall *)c->dmgw, c->sout<D>(), c->phys);
     HIPCHK(hipGetLastError());
   }
   if (c->ndmg_corner > 0) {
     hipLaunchKernelGGL((k_bc<D>), dim3(grid(c->ndmg_corner, 256)), dim3(256), 0, c->s,
-                       c->ndmg_corner, c->dmg + c->ndmg_edge, c->sout<D>(), c->phys);
+                       c->ndmg_corner, c->dmg + c->ndmg_edge,
+                       c->dmgw ? (const DmgWall *)c->dmgw + c->ndmg_edge : (const DmgWall *)nullptr,
+                       c->sout<D>(), c->phys);
     HIPCHK(hipGetLastError());
   }
   return IBG_OK;
@@ -441,7 +445,7 @@ MoveArgs move_args(ibg_ctx *c, unsigned long long *upd, int lmin) {
   a.nw = c->cell_end[lmin]; a.ngw = v.gend[lmin]; a.nflag = c->cell_end[lmin];
   a.gidx = v.gidx;
   a.key = v.key; a.kind = v.kind; a.fin = v.fin; a.cblk = v.cblk; a.blk = c->blk;
-  a.origin = v.origin; a.nb8 = v.nb8; a.nb27 = c->nb27;
+  a.origin = v.origin; a.nb8 = v.nb8; a.nb27 = c->nb27; a.dist_dm = c->dist_dm;
   a.signdis = c->signdis; a.sd32 = c->sd32; a.cpatch = v.cpatch; a.cp1 = v.cp1; a.sd1 = v.sd1;
   a.mk1 = v.mk1; a.mkib = v.mkib; a.epoch = v.epoch; a.cnt = v.cnt;
   a.cmix = v.cmix; a.chg = v.chg; a.cap_chg = v.cap_chg;
@@ -617,6 +621,13 @@ ibg_status validate_mesh(const ibg_mesh_soa *m) {
   }
   for (int i = 0; i < m->ndmg; ++i)
     if (!in(m->dmg_cell[i]) || !in(m->dmg_ref[i])) return fail("domain ghost out of range");
+  if ((m->dmg_nbr == nullptr) != (m->dist_dm == nullptr) || (m->dmg_nbr == nullptr) != (m->dmg_nmv == nullptr))
+    return fail("domain wall arrays incomplete");
+  for (int i = 0; i < m->ndmg; ++i) {
+    if (m->dmg_type[i] != -81) continue;
+    if (!m->dmg_nbr) return fail("wall ghost without its reference neighbour");
+    if (!in(m->dmg_nbr[i])) return fail("wall ghost reference neighbour out of range");
+  }
   for (int i = 0; i < m->nib; ++i) {
     if (!in(m->ib_cell[i])) return fail("IB cell out of range");
     if (m->ib_body[i] < 0 || m->ib_body[i] >= m->nbody) return fail("IB body out of range");
@@ -701,6 +712,15 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
     }
   }
   if (m->ncases > IBG_MAX_CASES || m->nlevels > IBG_MAX_LEVELS) return IBG_ERR_ARG;
+  if (!c->turb)
+    for (int i = 0; i < m->ndmg; ++i)
+      if (m->dmg_type[i] == -81) {
+        // the laminar build has no wall boundary: BCBoxTreat stops with its
+        // unidentified-type abort (BCValues.H:170-176)
+        c->errmsg = "domain wall (-81) needs the SA closure";
+        fprintf(stderr, "ibgpu: mesh rejected: %s\n", c->errmsg.c_str());
+        return IBG_ERR_ARG;
+      }
   if (c->turb) {
     // the SA source takes its face neighbours from the face list: check that
     // they are the Box::neib neighbours ComptMutSource reads (NS_Solver.cpp:
@@ -926,16 +946,29 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
   // ---- domain ghosts: edge entries first, then corners (whose refcell is an
   // edge ghost)
   std::vector<int4> dmg;
+  std::vector<DmgWall> dmgw;
   {
     std::vector<int4> edge, corner;
+    std::vector<DmgWall> wedge, wcorner;
     for (int i = 0; i < m->ndmg; ++i) {
       const int g = m->dmg_cell[i], r = m->dmg_ref[i];
       const int4 e = make_int4(h2d[g], h2d[r], m->dmg_type[i], hblk[g] / L);
-      if (m->kind[r] == IBG_DMGHOST) corner.push_back(e);
-      else edge.push_back(e);
+      DmgWall w{};
+      w.nbr = -1;
+      if (m->dmg_nbr && m->dmg_type[i] == -81) {
+        const int a = m->dmg_nbr[i];
+        w.nbr = h2d[a];
+        for (int k = 0; k < 3; ++k) w.nmv[k] = m->dmg_nmv[(size_t)i * 3 + k];
+        w.dref = m->dist_dm[r];
+        w.dnbr = m->dist_dm[a];
+      }
+      if (m->kind[r] == IBG_DMGHOST) { corner.push_back(e); wcorner.push_back(w); }
+      else { edge.push_back(e); wedge.push_back(w); }
     }
     dmg = edge;
     dmg.insert(dmg.end(), corner.begin(), corner.end());
+    dmgw = wedge;
+    dmgw.insert(dmgw.end(), wcorner.begin(), wcorner.end());
     c->ndmg_edge = (int)edge.size();
     c->ndmg_corner = (int)corner.size();
   }
@@ -1062,6 +1095,14 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
   UP(c->blk, hb);
   UP(c->phys, c->hphys);
   UP(c->dmg, dmg);
+  c->dmgw = nullptr;
+  c->dist_dm = nullptr;
+  if (m->dmg_nbr) {
+    UP(c->dmgw, dmgw);
+    std::vector<double> ddm(N);
+    for (int i = 0; i < N; ++i) ddm[i] = m->dist_dm[order[i]];
+    UP(c->dist_dm, ddm);
+  }
   UP(c->res_ci, res_ci); UP(c->res_fi, res_fi); UP(c->pro_ci, pro_ci); UP(c->pro_fi, pro_fi);
   UP(c->ib_cell, ib_cell); UP(c->ib_intp, ib_intp); UP(c->ib_body, ib_body); UP(c->ib_case, ib_case);
   UP(c->ib_coef, ib_coef); UP(c->ib_hgdis, ib_hgdis); UP(c->ib_attach, ib_att); UP(c->ib_nv, ib_nvh);

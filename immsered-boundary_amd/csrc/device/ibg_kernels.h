@@ -663,17 +663,54 @@ __global__ void k_copy_back(int n, Pack<R, D> src, Pack<R, D> dst, const R *nusr
 
 // ---------------------------------------------------------- ghost fills
 // BCBoxTreat BCValues.H:51-188: -13 pressure inlet (:191-206), -12 pressure
-// outlet (:208-214), -50 outflow (:216-220); entries (cell, refcell, type,
-// case).  SA working variable: copied with the state for -12/-50, the inlet
-// value for -13 (:203-205).
+// outlet (:208-214), -50 outflow (:216-220), -81 wall of the TURBULENCE
+// build (:89-125); entries (cell, refcell, type, case).  SA working
+// variable: copied with the state for -12/-50, the inlet value for -13
+// (:203-205), kappa y+ at a wall.
 template <int D>
-__global__ void k_bc(int n, const int4 *dmg, StateOut<D> o, const DevPhys *phys) {
+__global__ void k_bc(int n, const int4 *dmg, const DmgWall *wall, StateOut<D> o,
+                     const DevPhys *phys) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int4 e = dmg[i];
   const int g = e.x, ty = e.z;
   const DevPhys &P = phys[e.w];
   const Prim<double, D> q = pload<double, D>(o.m, e.y);
+  if (ty == -81) {
+    // the wall of the TURBULENCE build (BCValues.H:89-125); nothing in the
+    // laminar build, where the branch does not exist
+    if (!o.nm || !wall) return;
+    const DmgWall W = wall[i];
+    if (W.dnbr > 1.0) {
+      put_all<D>(o, g, q.r, q.u, q.v, q.w, q.p);
+      put_nu<D>(o, g, o.nm[e.y]);
+      return;
+    }
+    // ImageViseddy_Domain Turbulence.cpp:140-195 with the reference cell's
+    // neighbour as the reference point and a wall at rest
+    const Prim<double, D> s = pload<double, D>(o.m, W.nbr);
+    const double *nm = W.nmv;
+    const double Tr = s.p / (s.r * P.Rg);
+    const double ib_rdis = W.dref / W.dnbr;
+    const double hvn = s.u * nm[0] + s.v * nm[1] + s.w * nm[2];
+    const double hn[3] = {hvn * nm[0], hvn * nm[1], hvn * nm[2]};
+    const double ht[3] = {s.u - hn[0], s.v - hn[1], s.w - hn[2]};
+    const double df_velt = sqrt(ht[0] * ht[0] + ht[1] * ht[1] + ht[2] * ht[2]);
+    const double hg_mu = sqrt(Tr * Tr * Tr) * ((1.0 + P.SoT) / (Tr + P.SoT));
+    const double hg_vis = hg_mu / s.r;
+    const double hg_re = df_velt * W.dnbr / hg_vis * P.Re;
+    const double hg_yplus = sa_yplus(hg_re);
+    const double ut = hg_yplus * hg_vis / W.dnbr / P.Re;
+    const double yplus = ib_rdis * hg_yplus;
+    const double uplus = sa_uplus(yplus);
+    const double ib_velt = uplus * ut;
+    const double ib_r0 = (fabs(df_velt) < 0.00000001) ? 0.0 : ib_velt / df_velt;
+    double ou[3];
+    for (int k = 0; k < 3; ++k) ou[k] = hn[k] * ib_rdis + ht[k] * ib_r0;
+    put_all<D>(o, g, s.p / (P.Rg * Tr), ou[0], ou[1], (D == 3) ? ou[2] : 0.0, s.p);
+    put_nu<D>(o, g, SA::kappa * yplus);
+    return;
+  }
   if (ty == -50) {
     put_all<D>(o, g, q.r, q.u, q.v, q.w, q.p);
     if (o.nm) put_nu<D>(o, g, o.nm[e.y]);

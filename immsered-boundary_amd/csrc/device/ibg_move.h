@@ -72,6 +72,7 @@ struct MoveArgs {
   const double *origin;  // [ncases][3]
   const int *nb8;        // [8][N] in-plane neighbours, SoA
   const int *nb27;       // [N][27]
+  const double *dist_dm; // [N] distance to the domain walls, or null
   double *signdis;
   float *sd32;
   int *cpatch;           // [N] closest patch (global index), -1: no body
@@ -247,9 +248,9 @@ __global__ void __launch_bounds__(256) k_move_front(MoveArgs a) {
 // cycle list -- the neighbours whose closest body differs, in neighbour
 // order, each starting from that neighbour's patch after the neighbour's
 // own walk (recomputed here: a walk keeps the body, so the bodies of the
-// previous step decide which neighbours enter the list).  distance_to_dm is
-// 9999 without wall boundaries on the domain, so signdis = min(signdis,
-// distance_to_dm) leaves it unchanged.  On the finest level: solid cells to
+// previous step decide which neighbours enter the list), then signdis =
+// min(signdis, distance_to_dm) for walls on the domain.  On the finest
+// level: solid cells to
 // the solid list, in->out cells to their list, and an out->in cell that was
 // not an IB cell is the reference's abort 263 (Bodymove.cpp:494-544).
 __global__ void __launch_bounds__(256) k_move_dist(MoveArgs a) {
@@ -294,6 +295,11 @@ __global__ void __launch_bounds__(256) k_move_dist(MoveArgs a) {
           }
           if (change) { myb = nbb; p = q1; dist = dn; sd = sn; }
         }
+      }
+      // solid loop (:224-249): signdis = min(signdis, distance_to_dm)
+      if (a.dist_dm && sd < kCriticDis) {
+        const double dm = a.dist_dm[i];
+        sd = (dm < sd) ? dm : sd;
       }
     }
     a.cp1[i] = p;

@@ -346,7 +346,9 @@ struct ibm_handle {
   std::vector<double> signdis, center;
   std::vector<int32_t> blk_face_start, face_cells;
   std::vector<int8_t> face_dir;
-  std::vector<int32_t> dmg_cell, dmg_ref, dmg_type;
+  std::vector<int32_t> dmg_cell, dmg_ref, dmg_type, dmg_nbr;
+  std::vector<double> dmg_nmv, dist_dm;
+  bool walls = false;
   std::vector<int32_t> res_ci, res_fi, res_blk, pro_ci, pro_fi, pro_blk;
   std::vector<int32_t> ib_cell, ib_intp, ib_body;
   std::vector<double> ib_coef, ib_hgdis, ib_nv, ib_attach;
@@ -549,7 +551,9 @@ void build_all(ibm_handle &H, std::vector<CaseMesh> &cms) {
     const ibm_case &cs = cms[c].cs;
     int b = c * L + 0;
     int s = H.blk_start[b], e = H.blk_start[b + 1];
-    const int ftype[4] = {-13, -12, -50, -50};
+    const int fdefault[4] = {-13, -12, -50, -50};
+    int ftype[4];
+    for (int j = 0; j < 4; ++j) ftype[j] = cs.face_bc[j] ? cs.face_bc[j] : fdefault[j];
     const int fdir[4][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}};
     for (int pass = 0; pass < 2; ++pass)
       for (int g = s; g < e; ++g) {
@@ -573,6 +577,18 @@ void build_all(ibm_handle &H, std::vector<CaseMesh> &cms) {
         H.dmg_cell.push_back(g);
         H.dmg_ref.push_back(ref);
         H.dmg_type.push_back(ftype[j]);
+        // a wall ghost's reference neighbour: one step further inside from
+        // the reference cell (BCValues.H:98-104), and the inward wall normal
+        int nbr = -1;
+        if (ftype[j] == -81) {
+          nbr = H.nb[(size_t)ref * 27 + (1 + fdir[j][0]) * 9 + (1 + fdir[j][1]) * 3 + 1];
+          if (nbr < 0) throw std::runtime_error("wall ghost without a reference neighbour");
+          H.walls = true;
+        }
+        H.dmg_nbr.push_back(nbr);
+        H.dmg_nmv.push_back((double)fdir[j][0]);
+        H.dmg_nmv.push_back((double)fdir[j][1]);
+        H.dmg_nmv.push_back(0.0);
       }
   }
   // coarse/fine pairs, global indices: per coarse level f_res_ghost[0],
@@ -681,6 +697,34 @@ void build_all(ibm_handle &H, std::vector<CaseMesh> &cms) {
       patch_base += (int)cm.patches.size();
     }
   }
+  // Mesh::ComptDistancetoDomain (Mesh.cpp:1638-1686, 1688-1770): the distance
+  // of normal cells to the nearest wall face (|(c - fc) . n| on the plane
+  // faces of the box), except IB cells of the finest level; signdis becomes
+  // min(signdis, distance_to_dm)
+  H.dist_dm.assign(N, 9999.0);
+  for (int c = 0; c < nc && H.walls; ++c) {
+    const ibm_case &cs = cms[c].cs;
+    bool wall[4];
+    bool any = false;
+    for (int j = 0; j < 4; ++j) { wall[j] = cs.face_bc[j] == -81; any = any || wall[j]; }
+    if (!any) continue;
+    const double xlo = cs.origin[0], xhi = cs.origin[0] + cs.nx * cs.dh0[0];
+    const double ylo = cs.origin[1], yhi = cs.origin[1] + cs.ny * cs.dh0[1];
+    for (int l = 0; l < L; ++l) {
+      const int b = c * L + l;
+      for (int g = H.blk_start[b]; g < H.blk_start[b] + H.blk_nnormal[b]; ++g) {
+        if (l == L - 1 && cs.nbody > 0 && H.ibidx[g] >= 0) continue;
+        const double *x = &H.center[(size_t)g * 3];
+        double d = 9999.0;
+        if (wall[0]) d = std::min(d, std::fabs(x[0] - xlo));
+        if (wall[1]) d = std::min(d, std::fabs(x[0] - xhi));
+        if (wall[2]) d = std::min(d, std::fabs(x[1] - ylo));
+        if (wall[3]) d = std::min(d, std::fabs(x[1] - yhi));
+        H.dist_dm[g] = d;
+        H.signdis[g] = std::min(H.signdis[g], d);
+      }
+    }
+  }
   // closest patch of every cell as a global patch index
   H.cell_patch.assign(N, -1);
   {
@@ -722,6 +766,9 @@ void fill_soa(ibm_handle &H) {
   m.pat_case = H.pat_case.data();
   m.nbody = (int32_t)H.body_case.size();
   m.body_case = H.body_case.data(); m.body_center = H.body_center.data();
+  if (H.walls) {
+    m.dmg_nbr = H.dmg_nbr.data(); m.dmg_nmv = H.dmg_nmv.data(); m.dist_dm = H.dist_dm.data();
+  }
 }
 
 }  // namespace

@@ -48,6 +48,10 @@ typedef struct ibm_case {
   int32_t span_patches;         /* patches per segment along z (>= 1)         */
   /* flow */
   double gama, Ma, Re, Pr, cfl, backpressure, dt_init;
+  /* BC code of the domain faces x-, x+, y-, y+; 0 keeps the case file's
+     -13 inlet, -12 outlet, -50, -50 (Userfunction_cy_naca15.cpp:283-346);
+     -81 is the wall of the TURBULENCE build (BCValues.H:89-125)          */
+  int32_t face_bc[4];
 } ibm_case;
 
 typedef struct ibm_handle ibm_handle;

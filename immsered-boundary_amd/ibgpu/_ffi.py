@@ -46,6 +46,7 @@ class MeshSoA(C.Structure):
         ("pat_hgdis", f64p), ("pat_nv", f64p), ("pat_pc", f64p),
         ("pat_area", f64p), ("pat_case", i32p),
         ("nbody", C.c_int32), ("body_case", i32p), ("body_center", f64p),
+        ("dmg_nbr", i32p), ("dmg_nmv", f64p), ("dist_dm", f64p),
     ]
 
 
@@ -107,7 +108,7 @@ class Case(C.Structure):
         ("span", C.c_double), ("span_patches", C.c_int32),
         ("gama", C.c_double), ("Ma", C.c_double), ("Re", C.c_double),
         ("Pr", C.c_double), ("cfl", C.c_double), ("backpressure", C.c_double),
-        ("dt_init", C.c_double),
+        ("dt_init", C.c_double), ("face_bc", C.c_int32 * 4),
     ]
 
 

@@ -39,6 +39,7 @@ class CaseSpec:
     backpressure: float = 0.0
     span_patches: int = 1
     pivot_y: float = 0.0          # y of the quarter chord (x is 0)
+    face_bc: tuple = (0, 0, 0, 0)  # BC codes of x-, x+, y-, y+ (0: -13, -12, -50, -50)
 
     def to_ctypes(self):
         c = _ffi.Case()
@@ -70,6 +71,8 @@ class CaseSpec:
         c.cfl = self.cfl
         c.backpressure = self.backpressure
         c.dt_init = 1.0e30
+        for j in range(4):
+            c.face_bc[j] = int(self.face_bc[j])
         return c
 
 

@@ -121,6 +121,13 @@ typedef struct ibg_mesh_soa {
   int32_t nbody;
   const int32_t *body_case;    /* [nbody]     */
   const double  *body_center;  /* [nbody][3]  bodycenter                     */
+  /* domain walls (BC -81 of the TURBULENCE build, BCValues.H:89-125); all
+     three NULL without walls                                               */
+  const int32_t *dmg_nbr;      /* [ndmg] the reference cell's neighbour one
+                                  step further inside (-1: not a wall ghost)  */
+  const double  *dmg_nmv;      /* [ndmg][3] Domainghost::nmv, wall normal     */
+  const double  *dist_dm;      /* [ncells] BoxtoWall::distance_to_dm
+                                  (Mesh.cpp:1638-1757), 9999 away from walls  */
 } ibg_mesh_soa;
 
 /* Nondimensional constants of one case, FlowConstants.cpp:5-44 and the case

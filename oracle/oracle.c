@@ -534,8 +534,14 @@ int32_t orc_update(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *
 /* BCBoxTreat BCValues.H:51-188 for the types the cases use: -13 pressure
  * inlet (:191-206), -12 pressure outlet (:208-214), -50 outflow (:216-220),
  * -99 periodic (no-op).  Applied on marched levels only.                   */
-int32_t orc_bc_fill(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *cfg,
-                    double *fv, int32_t ts) {
+static int32_t wall_fill(const ibg_mesh_soa *m, const ibg_physics *P, int i, double *fv,
+                         double *nut);
+
+/* nut == NULL: the laminar build, which has no -81 branch (abort 52).  With
+   nut, the SA working variable of every ghost too, in the same pass (a
+   corner ghost reads an edge ghost that may be a wall ghost).            */
+static int32_t bc_fill(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *cfg,
+                       double *fv, double *nut, int32_t ts) {
   const int L = m->nlevels;
   for (int i = 0; i < m->ndmg; ++i) {
     int g = m->dmg_cell[i], r = m->dmg_ref[i], ty = m->dmg_type[i];
@@ -549,11 +555,14 @@ int32_t orc_bc_fill(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg 
     if (ty == -99) {
     } else if (ty == -50) {
       memcpy(bc, rf, sizeof(double) * NF);
+      if (nut) nut[g] = nut[r];
     } else if (ty == -12) {
       memcpy(bc, rf, sizeof(double) * NF);
       bc[PP] = P->outlet_p;
       bc[RO] = bc[PP] / (P->Rg * bc[TT]);
+      if (nut) nut[g] = nut[r];
     } else if (ty == -13) {
+      if (nut) nut[g] = P->nut_inlet;
       double speed0 = sqrt(rf[UU] * rf[UU] + rf[VV] * rf[VV] + rf[WW] * rf[WW]);
       bc[UU] = speed0 * cos(P->inlet_angle);
       bc[VV] = speed0 * sin(P->inlet_angle);
@@ -562,11 +571,19 @@ int32_t orc_bc_fill(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg 
       bc[PP] = P->inlet_p0 * pow((bc[TT] / P->inlet_T0), 3.5);
       bc[RO] = bc[PP] / (P->Rg * bc[TT]);
       bc[EE] = P->Cv * bc[TT] + (0.5 * (bc[UU] * bc[UU] + bc[VV] * bc[VV] + bc[WW] * bc[WW]));
+    } else if (ty == -81 && nut) {
+      const int32_t code = wall_fill(m, P, i, fv, nut);
+      if (code) return code;
     } else {
       return 52;   /* unidentified boundary type, BCValues.H:170-176 */
     }
   }
   return 0;
+}
+
+int32_t orc_bc_fill(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *cfg,
+                    double *fv, int32_t ts) {
+  return bc_fill(m, ph, cfg, fv, NULL, ts);
 }
 
 /* AMR::InterfaceExchange AMRInterface.cpp:3-35: restriction of every due
@@ -936,27 +953,54 @@ void orc_sa_update(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *
   }
 }
 
-/* viseddy of the domain ghosts (BCValues.H:203-205 inlet = facevar, -12
- * and -50 copy the reference cell with the rest of the state).            */
-int32_t orc_sa_bc_fill(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *cfg,
-                       double *nut, int32_t ts) {
-  const int L = m->nlevels;
-  for (int i = 0; i < m->ndmg; ++i) {
-    int g = m->dmg_cell[i], r = m->dmg_ref[i], ty = m->dmg_type[i];
-    int c = 0, l = 0;
-    for (int b = 0; b < m->ncases * L; ++b)
-      if (g >= m->blk_start[b] && g < m->blk_start[b + 1]) { c = b / L; l = b % L; break; }
-    if (!orc_marched(L, l, ts, cfg->temporal_refine)) continue;
-    if (ty == -99) {
-    } else if (ty == -50 || ty == -12) {
-      nut[g] = nut[r];
-    } else if (ty == -13) {
-      nut[g] = ph[c].nut_inlet;
-    } else {
-      return 52;
-    }
+/* A -81 wall ghost (BCValues.H:89-125): a copy of the reference cell when
+ * its inward neighbour lies more than 1.0 from the wall, else
+ * ImageViseddy_Domain (Turbulence.cpp:140-195) from that neighbour at the
+ * reference cell's wall distance, wall at rest.                          */
+static int32_t wall_fill(const ibg_mesh_soa *m, const ibg_physics *P, int i, double *fv,
+                         double *nut) {
+  if (!m->dmg_nbr || !m->dist_dm) return 52;
+  const int g = m->dmg_cell[i], r = m->dmg_ref[i], a = m->dmg_nbr[i];
+  double *bc = &fv[(size_t)g * NF];
+  if (m->dist_dm[a] > 1.0) {
+    memcpy(bc, &fv[(size_t)r * NF], sizeof(double) * NF);
+    nut[g] = nut[r];
+    return 0;
   }
+  const double *rv = &fv[(size_t)a * NF];
+  const double *nm = &m->dmg_nmv[(size_t)i * 3];
+  const double refdis = m->dist_dm[a], targetdis = m->dist_dm[r];
+  const double ib_rdis = targetdis / refdis;
+  const double hvn = rv[UU] * nm[0] + rv[VV] * nm[1] + rv[WW] * nm[2];
+  const double hn[3] = {hvn * nm[0], hvn * nm[1], hvn * nm[2]};
+  const double ht[3] = {rv[UU] - hn[0], rv[VV] - hn[1], rv[WW] - hn[2]};
+  const double df_velt = sqrt(ht[0] * ht[0] + ht[1] * ht[1] + ht[2] * ht[2]);
+  const double T = rv[PP] / (rv[RO] * P->Rg);
+  const double hg_mu = sqrt(T * T * T) * ((1.0 + P->S_over_T_ref) / (T + P->S_over_T_ref));
+  const double hg_vis = hg_mu / rv[RO];
+  const double hg_re = df_velt * refdis / hg_vis * P->Re;
+  const double hg_yplus = orc_sa_yplus(hg_re);
+  const double ut = hg_yplus * hg_vis / refdis / P->Re;
+  const double yplus = ib_rdis * hg_yplus;
+  const double ib_velt = orc_sa_uplus(yplus) * ut;
+  const double ib_r0 = (fabs(df_velt) < 0.00000001) ? 0.0 : ib_velt / df_velt;
+  bc[UU] = hn[0] * ib_rdis + ht[0] * ib_r0;
+  bc[VV] = hn[1] * ib_rdis + ht[1] * ib_r0;
+  bc[WW] = m->dim == 3 ? hn[2] * ib_rdis + ht[2] * ib_r0 : 0.0;
+  bc[PP] = rv[PP];
+  bc[TT] = T;
+  bc[RO] = bc[PP] / (P->Rg * bc[TT]);
+  bc[EE] = P->Cv * bc[TT] + 0.5 * (bc[UU] * bc[UU] + bc[VV] * bc[VV] + bc[WW] * bc[WW]);
+  nut[g] = ORC_SA_KAPPA * yplus;
   return 0;
+}
+
+/* BCBoxTreat of the TURBULENCE build: the state (as orc_bc_fill) and the
+ * viseddy of every domain ghost (BCValues.H:203-205 inlet = facevar, -12
+ * and -50 copy the reference cell), and the -81 walls, in one pass.      */
+int32_t orc_sa_bc_fill(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *cfg,
+                       double *fv, double *nut, int32_t ts) {
+  return bc_fill(m, ph, cfg, fv, nut, ts);
 }
 
 /* viseddy of the AMR transfer: AverageCell[_2d] TwoLevelOp.H:129-179 (plain
@@ -1172,8 +1216,7 @@ int32_t orc_step_sa(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg 
   if (!code) {
     orc_old_cons(m, cfg, fv, ts, cvo);
     memcpy(nuo, nut, sizeof(double) * N);   /* c_vis_old */
-    code = orc_bc_fill(m, ph, cfg, fv, ts);
-    if (!code) code = orc_sa_bc_fill(m, ph, cfg, nut, ts);
+    code = bc_fill(m, ph, cfg, fv, nut, ts);
   }
   if (!code) orc_viscous_flux_sa(m, ph, cfg, fv, nut, ts, vs, mf);
   for (int st = 0; st < cfg->nstage && !code; ++st) {

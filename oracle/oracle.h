@@ -101,7 +101,7 @@ void orc_sa_update(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *
                    const double *nut_old, const double *mutsource, const double *dt,
                    int32_t ts, int32_t subt, double *nut);
 int32_t orc_sa_bc_fill(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *cfg,
-                       double *nut, int32_t ts);
+                       double *fv, double *nut, int32_t ts);
 void orc_sa_exchange(const ibg_mesh_soa *m, const orc_cfg *cfg, double *nut, int32_t ts);
 int32_t orc_ib_fill_sa(const ibg_mesh_soa *m, const ibg_physics *ph, double *fv, double *nut,
                        const ibg_body_motion *motion);

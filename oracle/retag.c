@@ -253,7 +253,12 @@ int32_t orc_move_bodies(const ibg_mesh_soa *m, const orc_surface *s, const doubl
     }
     free(cb);
   }
-  /* solid flags (:224-249); distance_to_dm is 9999 without domain walls */
+  /* solid loop (:224-249): signdis = min(signdis, distance_to_dm) */
+  for (int g = 0; g < N; ++g) {
+    if (!m->dist_dm || o->cell_patch[g] < 0 || !(o->signdis[g] < CRITIC_DIS)) continue;
+    const double dm = m->dist_dm[g];
+    if (dm < o->signdis[g]) o->signdis[g] = dm;
+  }
   int8_t *solid = (int8_t *)malloc((size_t)N);
   for (int g = 0; g < N; ++g) solid[g] = o->signdis[g] < 0.0;
   /* RemoveIBCells / InfectNewIBCells on the finest level of each case */

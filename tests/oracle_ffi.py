@@ -85,7 +85,7 @@ def lib():
         "orc_viscous_flux_sa": (None, [M, P, G, f64p, f64p, C.c_int32, f64p, f64p]),
         "orc_mut_source": (None, [M, P, G, f64p, f64p, f64p, C.c_int32, f64p]),
         "orc_sa_update": (None, [M, P, G, f64p, f64p, f64p, C.c_int32, C.c_int32, f64p]),
-        "orc_sa_bc_fill": (C.c_int32, [M, P, G, f64p, C.c_int32]),
+        "orc_sa_bc_fill": (C.c_int32, [M, P, G, f64p, f64p, C.c_int32]),
         "orc_sa_exchange": (None, [M, G, f64p, C.c_int32]),
         "orc_ib_fill_sa": (C.c_int32, [M, P, f64p, f64p, BM]),
         "orc_forces_sa": (None, [M, P, f64p, BM, f64p]),
@@ -321,8 +321,9 @@ class Oracle:
     def ib_fill_sa(self, fv, nut, motion=None):
         return self.L.orc_ib_fill_sa(self.m.soa_ptr, self.m.phys, ptr(fv), ptr(nut), motion)
 
-    def sa_bc_fill(self, nut, ts):
-        return self.L.orc_sa_bc_fill(self.m.soa_ptr, self.m.phys, C.byref(self.cfg), ptr(nut), ts)
+    def sa_bc_fill(self, fv, nut, ts):
+        return self.L.orc_sa_bc_fill(self.m.soa_ptr, self.m.phys, C.byref(self.cfg), ptr(fv),
+                                     ptr(nut), ts)
 
     def sa_exchange(self, nut, ts):
         self.L.orc_sa_exchange(self.m.soa_ptr, C.byref(self.cfg), ptr(nut), ts)

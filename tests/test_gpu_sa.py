@@ -126,7 +126,7 @@ def test_sa_ghost_fills_fp64(tiny, phase):
     elif phase == "bc":
         eng.BCBoxTreat(0)
         assert orc.bc_fill(cpu, 0) == 0
-        assert orc.sa_bc_fill(cnut, 0) == 0
+        assert orc.sa_bc_fill(cpu, cnut, 0) == 0
     else:
         P = 1 << (tiny.nlevels - 1)
         for ts in range(P):
@@ -208,3 +208,43 @@ def test_single_stage_copy_back(tiny):
     eng.set_state(fv0)
     eng.TimeMarching(0, 4, use_graph=False)
     assert rel(eng.get_state()[:, :5], cpu[:, :5]) < TOL
+
+
+@pytest.fixture(scope="module")
+def walled():
+    import dataclasses
+    return ibgpu.Mesh(dataclasses.replace(ibgpu.tiny_case(), face_bc=(0, 0, -81, 0)))
+
+
+def test_wall_bc_fill_fp64(walled):
+    """Domain wall ghosts (BC -81, ImageViseddy_Domain) against the oracle."""
+    print("domain wall fill: tiny case, wall at y = lo", flush=True)
+    fv, nut = perturbed_state(walled), seeded_nut(walled)
+    eng = engine(walled, fv, nut)
+    orc = oracle_ffi.Oracle(walled)
+    cpu, cnut = fv.copy(), nut.copy()
+    eng.BCBoxTreat(0)
+    assert orc.sa_bc_fill(cpu, cnut, 0) == 0
+    assert rel(eng.get_state()[:, :5], cpu[:, :5]) < TOL
+    assert rel(eng.get_nut(), cnut) < TOL
+
+
+def test_wall_march_fp64(walled):
+    """Steps with a domain wall against the oracle with the SA closure."""
+    print("domain wall march: tiny case, 6 steps", flush=True)
+    fv, nut = perturbed_state(walled), seeded_nut(walled)
+    orc = oracle_ffi.Oracle(walled, oracle_ffi.cfg(nstage=3, dt_var_step=1 << (walled.nlevels - 1)))
+    cpu, cnut = fv.copy(), nut.copy()
+    dt = np.array([walled.phys[0].dt_init])
+    for ts in range(6):
+        code, _, _, _ = orc.step_sa(cpu, cnut, dt, ts)
+        assert code == 0, code
+    eng = engine(walled, fv, nut)
+    eng.TimeMarching(0, 6, use_graph=True)
+    assert rel(eng.get_state()[:, :5], cpu[:, :5]) < TOL
+    assert rel(eng.get_nut(), cnut) < TOL
+
+
+def test_wall_needs_the_closure(walled):
+    with pytest.raises(ibgpu.IBGError):
+        ibgpu.Engine(walled, mode="fp64")
