@@ -981,6 +981,212 @@ int32_t ibm_transfer(const ibm_handle *oh, const ibg_physics *ph, const double *
   }
 }
 
+// ------------------------------------------------------- Tecplot output
+// Pltoutput::Output_Grid / Output_Soln (Pltoutput.cpp:3-100, 417-450,
+// Pltoutput.H:354-380): one FEBRICK zone "N-<case> L-<level>" per level,
+// the 8 corners of every cell as nodes (shared corners merged), the
+// solution cell-centred in BLOCK packing with the reference's variables.
+// Every cell of the level is an element, ghosts included (the reference's
+// PRINTGHOST / INPUT_GHOST form), so a solution file is also an exact
+// restart (values printed with 17 significant digits).
+namespace {
+struct ZoneNodes {
+  std::vector<int32_t> conn;    // [ncell][8] 1-based node numbers
+  std::vector<double> xyz;      // [nnode][3]
+};
+
+ZoneNodes zone_nodes(const ibm_handle &H, int b) {
+  const int L = H.L, c = b / L;
+  const ibm_case &cs = H.cases[c];
+  const double *h = &H.blk_h[(size_t)b * 3];
+  ZoneNodes z;
+  std::unordered_map<int64_t, int32_t> id;
+  const int s = H.blk_start[b], e = H.blk_start[b + 1];
+  z.conn.reserve((size_t)(e - s) * 8);
+  // FEBRICK corner order: bottom face counter-clockwise, then the top face
+  const int ci[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                        {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+  for (int g = s; g < e; ++g) {
+    for (int k = 0; k < 8; ++k) {
+      const int x = H.key[(size_t)g * 3] + ci[k][0], y = H.key[(size_t)g * 3 + 1] + ci[k][1],
+                zz = H.key[(size_t)g * 3 + 2] + ci[k][2];
+      const int64_t key = pack_key(x, y, zz);
+      auto it = id.find(key);
+      int32_t n;
+      if (it == id.end()) {
+        n = (int32_t)(z.xyz.size() / 3) + 1;
+        id.emplace(key, n);
+        z.xyz.push_back(cs.origin[0] + x * h[0]);
+        z.xyz.push_back(cs.origin[1] + y * h[1]);
+        z.xyz.push_back(cs.origin[2] + zz * h[2]);
+      } else {
+        n = it->second;
+      }
+      z.conn.push_back(n);
+    }
+  }
+  return z;
+}
+
+void print_block(FILE *f, const std::vector<double> &v) {
+  for (size_t i = 0; i < v.size(); ++i)
+    std::fprintf(f, (i % 5 == 4 || i + 1 == v.size()) ? "%.17g\n" : "%.17g ", v[i]);
+}
+}  // namespace
+
+int32_t ibm_write_grid(const ibm_handle *h, const char *path) {
+  if (!h || !path) return 1;
+  FILE *f = std::fopen(path, "w");
+  if (!f) { g_last_error = std::string("cannot open ") + path; return 1; }
+  std::fprintf(f, "TITLE = \"Boxgrid\"\nFILETYPE = GRID\nVARIABLES = \"x\",\"y\",\"z\"\n");
+  for (int b = 0; b < h->ncases * h->L; ++b) {
+    const ZoneNodes z = zone_nodes(*h, b);
+    const int ne = h->blk_start[b + 1] - h->blk_start[b];
+    std::fprintf(f, "zone T = \"N-%d L-%d\", NODES=%d, ELEMENTS=%d, DATAPACKING=BLOCK, ZONETYPE=FEBRICK\n",
+                 b / h->L, b % h->L, (int)(z.xyz.size() / 3), ne);
+    for (int d = 0; d < 3; ++d) {
+      std::vector<double> col(z.xyz.size() / 3);
+      for (size_t n = 0; n < col.size(); ++n) col[n] = z.xyz[n * 3 + d];
+      print_block(f, col);
+    }
+    for (int g = 0; g < ne; ++g) {
+      for (int k = 0; k < 8; ++k) std::fprintf(f, k < 7 ? "%d " : "%d\n", z.conn[(size_t)g * 8 + k]);
+    }
+  }
+  std::fclose(f);
+  return 0;
+}
+
+int32_t ibm_write_solution(const ibm_handle *h, const ibg_physics *ph, const double *fv,
+                           const double *nut, int32_t ts, const double *dt, const double *t,
+                           const char *path) {
+  if (!h || !ph || !fv || !path) return 1;
+  FILE *f = std::fopen(path, "w");
+  if (!f) { g_last_error = std::string("cannot open ") + path; return 1; }
+  const int NF = IBG_NFV;
+  std::fprintf(f, "TITLE = \"Flow Solution\"\nFILETYPE = SOLUTION\n");
+  if (nut)
+    std::fprintf(f, "VARIABLES = \"roe\",\"u\",\"v\",\"w\",\"p\",\"T\",\"viseddy\",\"Ma\",\"dis\",\"dmdis\",\"i_index\",\"j_index\",\"k_index\",\"Omegaz\"\n");
+  else
+    std::fprintf(f, "VARIABLES = \"roe\",\"u\",\"v\",\"w\",\"p\",\"T\",\"Ma\",\"dis\",\"dmdis\",\"i_index\",\"j_index\",\"k_index\",\"Omegaz\"\n");
+  std::fprintf(f, "DATASETAUXDATA ts=\"%d\"\n", ts);
+  for (int c = 0; c < h->ncases; ++c) {
+    if (dt) std::fprintf(f, "DATASETAUXDATA dt%d=\"%.17g\"\n", c, dt[c]);
+    if (t) std::fprintf(f, "DATASETAUXDATA t%d=\"%.17g\"\n", c, t[c]);
+  }
+  const int nvar = nut ? 14 : 13;
+  for (int b = 0; b < h->ncases * h->L; ++b) {
+    const int c = b / h->L;
+    const ibg_physics &P = ph[c];
+    const ZoneNodes z = zone_nodes(*h, b);
+    const int s = h->blk_start[b], e = h->blk_start[b + 1], ne = e - s;
+    std::fprintf(f, "zone T = \"N-%d L-%d\", NODES=%d, ELEMENTS=%d, DATAPACKING=BLOCK, ZONETYPE=FEBRICK\n",
+                 c, b % h->L, (int)(z.xyz.size() / 3), ne);
+    std::fprintf(f, "VARLOCATION = ([1-%d]=CELLCENTERED)\n", nvar);
+    std::vector<double> col(ne);
+    for (int v = 0; v < 6; ++v) {
+      for (int i = 0; i < ne; ++i) col[i] = fv[(size_t)(s + i) * NF + v];
+      print_block(f, col);
+    }
+    if (nut) {
+      for (int i = 0; i < ne; ++i) col[i] = nut[s + i];
+      print_block(f, col);
+    }
+    for (int i = 0; i < ne; ++i) {   // Ma = |u| / sqrt(gama Rg T)
+      const double *q = fv + (size_t)(s + i) * NF;
+      col[i] = std::sqrt(q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) / std::sqrt(P.gama * P.Rg * q[5]);
+    }
+    print_block(f, col);
+    for (int i = 0; i < ne; ++i) col[i] = h->signdis[s + i];
+    print_block(f, col);
+    for (int i = 0; i < ne; ++i) col[i] = h->dist_dm.empty() ? 9999.0 : h->dist_dm[s + i];
+    print_block(f, col);
+    for (int d = 0; d < 3; ++d) {
+      for (int i = 0; i < ne; ++i) col[i] = h->key[(size_t)(s + i) * 3 + d];
+      print_block(f, col);
+    }
+    // Omegaz = dv/dx - du/dy by central differences over the face
+    // neighbours (0 where one is missing)
+    const double *hh = &h->blk_h[(size_t)b * 3];
+    for (int i = 0; i < ne; ++i) {
+      const int g = s + i;
+      const int *nb = &h->nb[(size_t)g * 27];
+      const int xm = nb[0 * 9 + 1 * 3 + 1], xp = nb[2 * 9 + 1 * 3 + 1];
+      const int ym = nb[1 * 9 + 0 * 3 + 1], yp = nb[1 * 9 + 2 * 3 + 1];
+      col[i] = (xm < 0 || xp < 0 || ym < 0 || yp < 0)
+                   ? 0.0
+                   : (fv[(size_t)xp * NF + 2] - fv[(size_t)xm * NF + 2]) / (2.0 * hh[0]) -
+                         (fv[(size_t)yp * NF + 1] - fv[(size_t)ym * NF + 1]) / (2.0 * hh[1]);
+    }
+    print_block(f, col);
+  }
+  std::fclose(f);
+  return 0;
+}
+
+// InputSoln / ReadLevelSolution (input_soln.cpp:12-180): the zones of a
+// solution file back into the state of the same level set (element count =
+// cells of the level, abort 65 otherwise); e = Cv T + |u|^2/2.
+int32_t ibm_read_solution(const ibm_handle *h, const ibg_physics *ph, const char *path, double *fv,
+                          double *nut, int32_t *ts, double *dt, double *t) {
+  if (!h || !ph || !path || !fv) return 1;
+  FILE *f = std::fopen(path, "r");
+  if (!f) { g_last_error = std::string("cannot open ") + path; return 1; }
+  const int NF = IBG_NFV;
+  char line[4096];
+  bool has_nut = false;
+  int zones = 0;
+  int32_t rc = 0;
+  auto fail = [&](const std::string &msg) { g_last_error = msg; rc = 1; };
+  while (rc == 0 && std::fgets(line, sizeof(line), f)) {
+    std::string sl(line);
+    if (sl.rfind("VARIABLES", 0) == 0) { has_nut = sl.find("viseddy") != std::string::npos; continue; }
+    if (sl.rfind("DATASETAUXDATA", 0) == 0) {
+      int k; double v; int iv;
+      if (ts && std::sscanf(line, "DATASETAUXDATA ts=\"%d\"", &iv) == 1) *ts = iv;
+      else if (dt && std::sscanf(line, "DATASETAUXDATA dt%d=\"%lf\"", &k, &v) == 2 && k >= 0 && k < h->ncases) dt[k] = v;
+      else if (t && std::sscanf(line, "DATASETAUXDATA t%d=\"%lf\"", &k, &v) == 2 && k >= 0 && k < h->ncases) t[k] = v;
+      continue;
+    }
+    if (sl.rfind("zone", 0) != 0) continue;
+    int c = -1, l = -1, nn = 0, ne = 0;
+    if (std::sscanf(line, "zone T = \"N-%d L-%d\", NODES=%d, ELEMENTS=%d", &c, &l, &nn, &ne) != 4 ||
+        c < 0 || c >= h->ncases || l < 0 || l >= h->L) {
+      fail("bad zone header");
+      break;
+    }
+    const int b = c * h->L + l, s = h->blk_start[b];
+    if (ne != h->blk_start[b + 1] - s) { fail("zone element count differs from the level (abort 65)"); break; }
+    if (!std::fgets(line, sizeof(line), f)) { fail("truncated file"); break; }   // VARLOCATION
+    const int nv = has_nut ? 7 : 6;
+    std::vector<double> col(ne);
+    for (int v = 0; v < nv && rc == 0; ++v) {
+      for (int i = 0; i < ne; ++i)
+        if (std::fscanf(f, "%lf", &col[i]) != 1) { fail("truncated zone data"); break; }
+      if (rc) break;
+      for (int i = 0; i < ne; ++i) {
+        if (v < 6) fv[(size_t)(s + i) * NF + v] = col[i];
+        else if (nut) nut[s + i] = col[i];
+      }
+    }
+    // skip the derived variables (Ma, dis, dmdis, indices, Omegaz)
+    for (int v = 0; v < 7 && rc == 0; ++v)
+      for (int i = 0; i < ne; ++i)
+        if (std::fscanf(f, "%lf", &col[0]) != 1) { fail("truncated zone data"); break; }
+    if (rc) break;
+    const ibg_physics &P = ph[c];
+    for (int i = 0; i < ne; ++i) {
+      double *q = fv + (size_t)(s + i) * NF;
+      q[6] = P.Cv * q[5] + 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    }
+    ++zones;
+  }
+  std::fclose(f);
+  if (rc == 0 && zones != h->ncases * h->L) { g_last_error = "missing zones"; rc = 1; }
+  if (rc == 0 && nut && !has_nut) { g_last_error = "no viseddy in the file"; rc = 1; }
+  return rc;
+}
+
 int32_t ibm_cell_patch(const ibm_handle *h, int32_t *out) {
   if (!h || !out) return 1;
   std::memcpy(out, h->cell_patch.data(), h->cell_patch.size() * sizeof(int32_t));

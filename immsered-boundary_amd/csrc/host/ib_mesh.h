@@ -76,6 +76,19 @@ int32_t ibm_patch_close(const ibm_handle *h, int32_t *out /* [npatch] */);
 int32_t ibm_transfer(const ibm_handle *old_mesh, const ibg_physics *phys /* [ncases] */,
                      const double *fv_old, const double *nut_old,
                      const ibm_handle *new_mesh, double *fv_new, double *nut_new);
+/* Tecplot output (Pltoutput::Output_Grid / Output_Soln, Pltoutput.cpp:3-100,
+   417-450): one FEBRICK zone "N-<case> L-<level>" per level, every cell of
+   the level an element (ghosts included), the solution cell-centred with
+   the reference's variables (roe u v w p T [viseddy] Ma dis dmdis i j k
+   Omegaz).  17 significant digits, ts / dt / t as DATASETAUXDATA: the
+   solution file is also an exact restart (ibm_read_solution, InputSoln
+   input_soln.cpp:12-180).  nut, dt, t may be NULL.                        */
+int32_t ibm_write_grid(const ibm_handle *h, const char *path);
+int32_t ibm_write_solution(const ibm_handle *h, const ibg_physics *phys, const double *fv,
+                           const double *nut, int32_t ts, const double *dt, const double *t,
+                           const char *path);
+int32_t ibm_read_solution(const ibm_handle *h, const ibg_physics *phys, const char *path,
+                          double *fv, double *nut, int32_t *ts, double *dt, double *t);
 /* cell centers [ncells][3]                                                 */
 int32_t ibm_cell_centers(const ibm_handle *h, double *xyz);
 /* error text of the last failed build                                      */

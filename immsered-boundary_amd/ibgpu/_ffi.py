@@ -154,6 +154,14 @@ def host_lib():
         lib.ibm_transfer.restype = C.c_int32
         lib.ibm_transfer.argtypes = [C.c_void_p, C.POINTER(Physics), f64p, f64p, C.c_void_p, f64p,
                                      f64p]
+        lib.ibm_write_grid.restype = C.c_int32
+        lib.ibm_write_grid.argtypes = [C.c_void_p, C.c_char_p]
+        lib.ibm_write_solution.restype = C.c_int32
+        lib.ibm_write_solution.argtypes = [C.c_void_p, C.POINTER(Physics), f64p, f64p, C.c_int32,
+                                           f64p, f64p, C.c_char_p]
+        lib.ibm_read_solution.restype = C.c_int32
+        lib.ibm_read_solution.argtypes = [C.c_void_p, C.POINTER(Physics), C.c_char_p, f64p, f64p,
+                                          C.POINTER(C.c_int32), f64p, f64p]
         lib.ibm_cell_patch.restype = C.c_int32
         lib.ibm_cell_patch.argtypes = [C.c_void_p, i32p]
         lib.ibm_patch_close.restype = C.c_int32

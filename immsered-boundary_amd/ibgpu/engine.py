@@ -325,6 +325,28 @@ class Engine:
                 self._chk(self.lib.ibg_set_time(self.ctx, np.ascontiguousarray(t).ctypes.data_as(_ffi.f64p)),
                           "ibg_set_time")
 
+    def write_solution(self, path, ts):
+        """Tecplot solution of the current state (Pltoutput::Output_Soln) with
+        ts, dt and the motion time: a restart point."""
+        moving = getattr(self, "_surface", None) is not None
+        self.mesh.write_solution(path, self.get_state(),
+                                 self.get_nut() if self.turbulence else None, ts,
+                                 self.get_dt(), self.get_time() if moving else None)
+
+    def read_solution(self, path):
+        """Restart from a solution file of this level set (InputSoln): state,
+        SA variable, dt and, for a prescribed motion, its time; returns ts."""
+        fv, nut, ts, dt, t = self.mesh.read_solution(path, self.turbulence)
+        self.set_state(fv)
+        if nut is not None:
+            self.set_nut(nut)
+        if np.all(np.isfinite(dt)):
+            self.set_dt(dt)
+        if getattr(self, "_surface", None) is not None and np.all(np.isfinite(t)):
+            self._chk(self.lib.ibg_set_time(self.ctx, np.ascontiguousarray(t).ctypes.data_as(_ffi.f64p)),
+                      "ibg_set_time")
+        return ts
+
     def get_time(self):
         t = np.zeros(self.mesh.ncases)
         self._chk(self.lib.ibg_get_time(self.ctx, t.ctypes.data_as(_ffi.f64p)), "ibg_get_time")

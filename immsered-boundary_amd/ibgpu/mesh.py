@@ -284,6 +284,38 @@ class Mesh:
             raise RuntimeError("state transfer failed: " + self._host.ibm_last_error().decode())
         return out, nut_out
 
+    def write_grid(self, path):
+        """Tecplot grid file of every level (Pltoutput::Output_Grid)."""
+        if self._host.ibm_write_grid(self._h, str(path).encode()) != 0:
+            raise RuntimeError(self._host.ibm_last_error().decode())
+
+    def write_solution(self, path, fv, nut=None, ts=0, dt=None, t=None):
+        """Tecplot solution file (Pltoutput::Output_Soln), also a restart."""
+        fv = np.ascontiguousarray(fv, dtype=np.float64)
+        arr = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        nut, dt, t = arr(nut), arr(dt), arr(t)
+        p = lambda a: None if a is None else a.ctypes.data_as(_ffi.f64p)
+        if self._host.ibm_write_solution(self._h, self.phys, p(fv), p(nut), int(ts), p(dt), p(t),
+                                         str(path).encode()) != 0:
+            raise RuntimeError(self._host.ibm_last_error().decode())
+
+    def read_solution(self, path, with_nut=False):
+        """State (and SA variable, ts, dt, t) from a solution file of this
+        level set (InputSoln)."""
+        fv = self.init_state()
+        nut = np.zeros(self.ncells) if with_nut else None
+        ts = C.c_int32(0)
+        dt = np.full(self.ncases, np.nan)
+        t = np.full(self.ncases, np.nan)
+        rc = self._host.ibm_read_solution(self._h, self.phys, str(path).encode(),
+                                          fv.ctypes.data_as(_ffi.f64p),
+                                          None if nut is None else nut.ctypes.data_as(_ffi.f64p),
+                                          C.byref(ts), dt.ctypes.data_as(_ffi.f64p),
+                                          t.ctypes.data_as(_ffi.f64p))
+        if rc != 0:
+            raise RuntimeError("solution file rejected: " + self._host.ibm_last_error().decode())
+        return fv, nut, ts.value, dt, t
+
     def init_state(self):
         fv = np.zeros((self.ncells, _ffi.IBG_NFV))
         self._host.ibm_init_state(self._h, self.phys, fv.ctypes.data_as(_ffi.f64p))

@@ -198,3 +198,38 @@ def test_regrid_during_prescribed_motion(tiny):
         assert np.all(np.isfinite(st))
         runs.append(eng.ComptWallForce()[:2])
     assert np.allclose(runs[1], runs[0], rtol=1e-3), runs
+
+
+@pytest.mark.parametrize("moving", [False, True])
+def test_restart_from_solution_file_is_exact(tiny, tmp_path, moving):
+    """Stop after 8 steps, write the Tecplot solution (state, SA variable,
+    dt, motion time), restart a fresh context from it: the next 8 steps give
+    the bits of the uninterrupted run."""
+    spec, m = tiny
+    print(f"restart from a solution file: tiny case, SA, moving={moving}", flush=True)
+    law = [(0.0, 0.0, spec.aoa, 1.0, 10.0, 0.01)]
+    fv0 = perturbed_state(m)
+    nut0 = np.full(m.ncells, m.phys[0].nut_inf)
+
+    def start():
+        eng = ibgpu.Engine(m, mode="fp64", nstage=3, turbulence=True)
+        if moving:
+            eng.upload_surface()
+            eng.set_motion(law, t0=0.0)
+        return eng
+
+    a = start()
+    a.set_state(fv0)
+    a.set_nut(nut0)
+    a.TimeMarching(0, 8, use_graph=True)
+    path = tmp_path / "amr_soln-8.plt"
+    a.write_solution(path, 8)
+    a.TimeMarching(8, 8, use_graph=True)
+    b = start()
+    ts = b.read_solution(path)
+    assert ts == 8
+    b.TimeMarching(ts, 8, use_graph=True)
+    assert np.array_equal(b.get_state(), a.get_state())
+    assert np.array_equal(b.get_nut(), a.get_nut())
+    if moving:
+        assert np.array_equal(b.get_time(), a.get_time())
