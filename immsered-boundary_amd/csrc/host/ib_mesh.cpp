@@ -121,6 +121,7 @@ void build_segments(CaseMesh &m) {
       m.seg_ny.push_back(ny);
       m.seg_len.push_back(len);
       m.seg_body.push_back(b);
+      m.seg_first.push_back(off + i);
     }
     off += n;
   }
@@ -355,6 +356,12 @@ struct ibm_handle {
   std::vector<int32_t> pat_intp, pat_case, pat_close;
   std::vector<double> pat_coef, pat_hgdis, pat_nv, pat_pc, pat_area;
   std::vector<int32_t> cell_patch;   // BoxtoWall::patch, global patch index
+  // surface of the moving-body update (ibm_surface)
+  std::vector<int32_t> sf_pt0, sf_body, sf_body_p0;
+  std::vector<int8_t> sf_flip;
+  std::vector<double> sf_dz, sf_origin;
+  int sf_npoint = 0;
+  bool sf_ok = true;
   std::vector<int32_t> body_case;
   std::vector<double> body_center;
   std::vector<ibm_case> cases;
@@ -724,6 +731,38 @@ void build_all(ibm_handle &H, std::vector<CaseMesh> &cms) {
         H.signdis[g] = std::min(H.signdis[g], d);
       }
     }
+  }
+  // the segment, body and flip of every patch, for the moving-body update
+  // (one patch per segment of a 2D section)
+  {
+    int pbase = 0, gbody = 0;
+    H.sf_body_p0.assign(1, 0);
+    for (int c = 0; c < nc; ++c) {
+      const CaseMesh &cm = cms[c];
+      const ibm_case &cs = cm.cs;
+      if (cs.nbody == 0) continue;
+      if (std::max(cs.span_patches, 1) != 1 || dim != 2) H.sf_ok = false;
+      for (int b = 0; b < cs.nbody; ++b) {
+        int n = 0;
+        for (size_t sgi = 0; sgi < cm.seg_body.size(); ++sgi)
+          if (cm.seg_body[sgi] == b) {
+            H.sf_pt0.push_back(H.sf_npoint + cm.seg_first[sgi]);
+            H.sf_body.push_back(gbody + b);
+            H.sf_flip.push_back((int8_t)(b % 2 == 1));
+            H.sf_dz.push_back(cs.span / std::max(cs.span_patches, 1));
+            ++n;
+          }
+        H.sf_body_p0.push_back(H.sf_body_p0.back() + n);
+      }
+      int npts = 0;
+      for (int b = 0; b < cs.nbody; ++b) npts += cs.body_npts[b];
+      H.sf_npoint += npts;
+      gbody += cs.nbody;
+      pbase += (int)cm.patches.size();
+    }
+    (void)pbase;
+    for (int c = 0; c < nc; ++c)
+      for (int d = 0; d < 3; ++d) H.sf_origin.push_back(cms[c].cs.origin[d]);
   }
   // closest patch of every cell as a global patch index
   H.cell_patch.assign(N, -1);
@@ -1185,6 +1224,24 @@ int32_t ibm_read_solution(const ibm_handle *h, const ibg_physics *ph, const char
   if (rc == 0 && zones != h->ncases * h->L) { g_last_error = "missing zones"; rc = 1; }
   if (rc == 0 && nut && !has_nut) { g_last_error = "no viseddy in the file"; rc = 1; }
   return rc;
+}
+
+int32_t ibm_surface(const ibm_handle *h, const double *pt_off, int32_t npoint, ibg_surface *out) {
+  if (!h || !pt_off || !out) return 1;
+  if (!h->sf_ok) { g_last_error = "moving bodies: 2D sections with one patch per segment"; return 1; }
+  if (npoint != h->sf_npoint) { g_last_error = "surface point count differs from the bodies"; return 1; }
+  std::memset(out, 0, sizeof(*out));
+  out->npoint = npoint;
+  out->pt_off = pt_off;
+  out->pat_pt0 = h->sf_pt0.data();
+  out->pat_body = h->sf_body.data();
+  out->pat_flip = h->sf_flip.data();
+  out->pat_dz = h->sf_dz.data();
+  out->body_p0 = h->sf_body_p0.data();
+  out->case_origin = h->sf_origin.data();
+  out->cell_patch = h->cell_patch.data();
+  out->pat_close = h->pat_close.data();
+  return 0;
 }
 
 int32_t ibm_cell_patch(const ibm_handle *h, int32_t *out) {

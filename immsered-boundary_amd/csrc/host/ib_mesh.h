@@ -67,6 +67,12 @@ int32_t ibm_init_state(const ibm_handle *h, const ibg_physics *phys,
 /* closest surface patch of every cell (BoxtoWall::patch, Box.H:68-106) as
    an index into the mesh's pat_* arrays, -1 for cases without bodies        */
 int32_t ibm_cell_patch(const ibm_handle *h, int32_t *out /* [ncells] */);
+/* The ibg_surface of the mesh's 2D sections for ibg_upload_surface: pt_off
+   (caller-owned, [npoint][2]) are the zero-pose offsets of every body's
+   points from its pivot in the order the bodies were given (for the NACA
+   sections: ibm_naca4_offsets); the other arrays live in the handle.     */
+int32_t ibm_surface(const ibm_handle *h, const double *pt_off, int32_t npoint,
+                    ibg_surface *out);
 /* per surface patch: the cell holding its image point (HGCell::closecell) */
 int32_t ibm_patch_close(const ibm_handle *h, int32_t *out /* [npatch] */);
 /* State of a new level set of the same cases from the state of the old one

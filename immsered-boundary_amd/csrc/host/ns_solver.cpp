@@ -1,7 +1,11 @@
 // ns_solver.cpp — C++ driver in the shape of the reference's main.cpp.
 //
-//   ibg_run [steps] [fp64|mixed] [stages] [laminar|sa|ml]
+//   ibg_run [steps] [fp64|mixed] [stages] [laminar|sa|ml] [static|move] [solution.plt]
 //
+// move pitches the section 5 deg about its quarter chord and heaves it 0.05
+// chord at reduced frequency 0.1 (the IB re-tagged every step,
+// Bodymove.cpp:37-125); a last argument writes the final Tecplot solution
+// (Pltoutput::Output_Soln, a restart file).
 // sa runs the TURBULENCE build (SA closure, working variable from its
 // freestream, Userfunction_cy_naca15.cpp:368-370); ml trains the learned
 // closure on cells of an FP64 SA run of the same case first (mixed mode).
@@ -27,6 +31,8 @@ int main(int argc, char **argv) {
   const int stages = argc > 3 ? std::atoi(argv[3]) : 3;
   const std::string closure = argc > 4 ? std::string(argv[4]) : std::string("laminar");
   const bool turb = closure != "laminar";
+  const bool move = argc > 5 && std::string(argv[5]) == "move";
+  const char *soln = argc > 6 ? argv[6] : nullptr;
 
   const int npts = 100;
   std::vector<double> xy(4 * npts);
@@ -89,6 +95,20 @@ int main(int argc, char **argv) {
       solver.ClosureSetParams(net);
       solver.ClosureEnable(true);
     }
+    std::vector<double> pt_off;
+    if (move) {
+      pt_off.resize(4 * npts);
+      ibm_naca4_offsets(0.12, npts, pt_off.data());
+      ibg_surface surf;
+      if (ibm_surface(h, pt_off.data(), 2 * npts, &surf) != 0) {
+        std::fprintf(stderr, "surface: %s\n", ibm_last_error());
+        ibm_free(h);
+        return 1;
+      }
+      solver.UploadSurface(*m, surf);
+      const ibg_pitch_heave law = {0.0, 0.0, 2.0, 5.0, 0.2, 0.05};
+      solver.SetMotion(std::vector<ibg_pitch_heave>(1, law), 0.0);
+    }
     std::vector<double> linf, l2, init_res;
     for (int ts = 0; ts < steps; ++ts) {
       solver.TimeMarching(ts, 1, false);
@@ -103,6 +123,20 @@ int main(int argc, char **argv) {
     const std::vector<double> F = solver.ComptWallForce();
     std::printf("C_D %.8f C_L %.8f C_M %.8f\n", F[0] / phys[0].force_ref, F[1] / phys[0].force_ref,
                 F[5] / phys[0].moment_ref);
+    if (move) std::printf("body time %.8e\n", solver.GetTime()[0]);
+    if (soln) {
+      const std::vector<double> st = solver.GetState();
+      const std::vector<double> nut = turb ? solver.GetViseddy() : std::vector<double>();
+      const std::vector<double> dt = solver.GetDt();
+      const std::vector<double> t = move ? solver.GetTime() : std::vector<double>();
+      if (ibm_write_solution(h, phys.data(), st.data(), turb ? nut.data() : nullptr, steps, dt.data(),
+                             move ? t.data() : nullptr, soln) != 0) {
+        std::fprintf(stderr, "solution: %s\n", ibm_last_error());
+        ibm_free(h);
+        return 1;
+      }
+      std::printf("solution written to %s\n", soln);
+    }
   } catch (const ibgpu::Error &e) {
     std::fprintf(stderr, "%s\n", e.what());
     ibm_free(h);

@@ -102,6 +102,29 @@ class NS_Solver {
     check(ibg_residual_nut(ctx_, linf.data(), l2.data()), "ibg_residual_nut");
   }
 
+  // Body::MovingBody_Image (Bodymove.cpp:37-125) between regrids: the
+  // surface once, then a move per step to the structural solver's pose, or
+  // a prescribed pitch/heave law applied by TimeMarching itself
+  void UploadSurface(const ibg_mesh_soa &mesh, const ibg_surface &surf) {
+    check(ibg_upload_surface(ctx_, &mesh, &surf), "ibg_upload_surface");
+  }
+  void MovingBody_Image(const std::vector<ibg_body_pose> &pose) {
+    check(ibg_move_bodies(ctx_, pose.data()), "MovingBody_Image");
+  }
+  void SetMotion(const std::vector<ibg_pitch_heave> &law, double t0) {
+    check(ibg_set_motion(ctx_, law.data(), t0), "ibg_set_motion");
+  }
+  std::vector<double> GetTime() {
+    std::vector<double> t(ncases_);
+    check(ibg_get_time(ctx_, t.data()), "ibg_get_time");
+    return t;
+  }
+  std::vector<double> GetDt() {
+    std::vector<double> dt(ncases_);
+    check(ibg_get_dt(ctx_, dt.data()), "ibg_get_dt");
+    return dt;
+  }
+
   // Learned closure (no reference counterpart): sample the current state,
   // fit the statistics, train, and switch the stage to the network.
   void ClosureInit(long long capacity, unsigned long long seed) {

@@ -162,6 +162,8 @@ def host_lib():
         lib.ibm_read_solution.restype = C.c_int32
         lib.ibm_read_solution.argtypes = [C.c_void_p, C.POINTER(Physics), C.c_char_p, f64p, f64p,
                                           C.POINTER(C.c_int32), f64p, f64p]
+        lib.ibm_surface.restype = C.c_int32
+        lib.ibm_surface.argtypes = [C.c_void_p, f64p, C.c_int32, C.POINTER(Surface)]
         lib.ibm_cell_patch.restype = C.c_int32
         lib.ibm_cell_patch.argtypes = [C.c_void_p, i32p]
         lib.ibm_patch_close.restype = C.c_int32

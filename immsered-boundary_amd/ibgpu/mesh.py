@@ -206,57 +206,36 @@ class Mesh:
         return lv
 
     def surface(self, ib_capacity=0):
-        """ibg_surface of the mesh's 2D sections for moving bodies: zero-pose
-        point offsets from each case's quarter chord (ibm_naca4_offsets), the
-        segment of every patch (the builder makes one patch per segment, upper
-        surface first), the patch range per body, the closest patch of every
-        cell and the image cell of every patch.  Returns (Surface, arrays)."""
+        """ibg_surface of the mesh's 2D sections for moving bodies (ibm_surface):
+        the zero-pose point offsets from each case's quarter chord
+        (ibm_naca4_offsets) with the builder's patch segments, patch ranges,
+        closest patch of every cell and image cell of every patch.  Returns
+        (Surface, arrays) with the arrays as numpy views."""
         if self.dim != 2:
             raise ValueError("moving bodies: 2D cases only")
         host = self._host
-        offs, pt0, pbody, flip, dz, bp0 = [], [], [], [], [], [0]
-        npt = 0
-        gb = 0
-        for c, sp in enumerate(self.specs):
-            if sp.span_patches != 1:
-                raise ValueError("moving bodies: one patch per segment")
+        offs = []
+        for sp in self.specs:
             o = np.zeros((2 * sp.npts, 2))
             host.ibm_naca4_offsets(sp.thickness, sp.npts, o.ctypes.data_as(_ffi.f64p))
             offs.append(o)
-            for b in range(2):
-                for i in range(sp.npts - 1):
-                    pt0.append(npt + b * sp.npts + i)
-                    pbody.append(gb)
-                    flip.append(1 if b % 2 == 1 else 0)
-                    dz.append(sp.span / sp.span_patches)
-                bp0.append(bp0[-1] + sp.npts - 1)
-                gb += 1
-            npt += 2 * sp.npts
-        if len(pt0) != self.npatch:
-            raise ValueError("patch count differs from one patch per segment")
-        a = {
-            "pt_off": np.ascontiguousarray(np.concatenate(offs)),
-            "pat_pt0": np.asarray(pt0, np.int32), "pat_body": np.asarray(pbody, np.int32),
-            "pat_flip": np.asarray(flip, np.int8), "pat_dz": np.asarray(dz, np.float64),
-            "body_p0": np.asarray(bp0, np.int32),
-            "case_origin": np.asarray([[sp.lo, sp.lo, 0.0] for sp in self.specs], np.float64),
-            "cell_patch": np.zeros(self.ncells, np.int32),
-            "pat_close": np.zeros(max(self.npatch, 1), np.int32),
-        }
-        host.ibm_cell_patch(self._h, a["cell_patch"].ctypes.data_as(_ffi.i32p))
-        host.ibm_patch_close(self._h, a["pat_close"].ctypes.data_as(_ffi.i32p))
+        pt_off = np.ascontiguousarray(np.concatenate(offs))
         s = _ffi.Surface()
-        s.npoint = npt
-        s.pt_off = a["pt_off"].ctypes.data_as(_ffi.f64p)
-        s.pat_pt0 = a["pat_pt0"].ctypes.data_as(_ffi.i32p)
-        s.pat_body = a["pat_body"].ctypes.data_as(_ffi.i32p)
-        s.pat_flip = a["pat_flip"].ctypes.data_as(_ffi.i8p)
-        s.pat_dz = a["pat_dz"].ctypes.data_as(_ffi.f64p)
-        s.body_p0 = a["body_p0"].ctypes.data_as(_ffi.i32p)
-        s.case_origin = a["case_origin"].ctypes.data_as(_ffi.f64p)
-        s.cell_patch = a["cell_patch"].ctypes.data_as(_ffi.i32p)
-        s.pat_close = a["pat_close"].ctypes.data_as(_ffi.i32p)
+        if host.ibm_surface(self._h, pt_off.ctypes.data_as(_ffi.f64p), pt_off.shape[0],
+                            C.byref(s)) != 0:
+            raise ValueError(host.ibm_last_error().decode())
         s.ib_capacity = int(ib_capacity)
+        v = np.ctypeslib.as_array
+        nb = len([0 for sp in self.specs for _ in range(2)])
+        a = {
+            "pt_off": pt_off,
+            "pat_pt0": v(s.pat_pt0, (self.npatch,)), "pat_body": v(s.pat_body, (self.npatch,)),
+            "pat_flip": v(s.pat_flip, (self.npatch,)), "pat_dz": v(s.pat_dz, (self.npatch,)),
+            "body_p0": v(s.body_p0, (nb + 1,)),
+            "case_origin": v(s.case_origin, (self.ncases, 3)),
+            "cell_patch": v(s.cell_patch, (self.ncells,)),
+            "pat_close": v(s.pat_close, (max(self.npatch, 1),)),
+        }
         s._keep = a
         return s, a
 

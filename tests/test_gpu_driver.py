@@ -42,3 +42,36 @@ def test_driver_closures_run(closure):
     assert cd == cd and cl == cl and abs(cd) < 10.0   # finite, physical magnitude
     if closure == "ml":
         assert "closure:" in out
+
+
+def test_driver_moving_body_restart_file_matches_the_python_engine(tmp_path):
+    """ibg_run ... sa move soln.plt pitches and heaves the section under the
+    prescribed law for 12 steps and writes the final solution; the same case,
+    law and step loop through the Python engine give the same state, SA
+    variable and body time bit for bit (the file holds %.17g)."""
+    import numpy as np
+
+    import ibgpu
+    from ibgpu import mesh as M
+
+    soln = tmp_path / "driver.plt"
+    cd, cl, steps, out = run(12, "fp64", 3, "sa", "move", soln)
+    assert cd == cd and cl == cl and abs(cd) < 10.0
+    t_drv = float(re.search(r"body time (\S+)", out).group(1))
+    assert t_drv > 0.0
+
+    mesh = M.Mesh(M.config1())
+    eng = ibgpu.Engine(mesh, mode="fp64", nstage=3, turbulence=True)
+    eng.set_state(mesh.init_state())
+    eng.set_nut(np.full(mesh.ncells, mesh.phys[0].nut_inf))
+    eng.upload_surface()
+    eng.set_motion([(0.0, 0.0, 2.0, 5.0, 0.2, 0.05)], t0=0.0)
+    for ts in range(12):
+        eng.TimeMarching(ts, 1, False)
+    fv, nut, ts, dt, t = mesh.read_solution(soln, with_nut=True)
+    assert ts == 12
+    assert np.array_equal(fv, eng.get_state())
+    assert np.array_equal(nut, eng.get_nut())
+    assert np.array_equal(t, eng.get_time())
+    assert np.array_equal(dt, eng.get_dt())
+    assert t[0] == pytest.approx(t_drv, rel=1e-8)
