@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(256) k_closure_train(TrainArgs a) {
     for (int i = 0; i < 64; ++i) s += red[i];
     a.lossp[blockIdx.x] = s;
   }
-  // dZ2 = bf16(dy w3 silu'(Z2)), in place of S2
+  // dZ2 = bf16(dy w3 relu'(Z2)), in place of S2
   for (int i = threadIdx.x; i < 64 * 64; i += blockDim.x) {
     const int row = i >> 6, j = i & 63;
     sS2[row * kH + j] = (__bf16)(sy[row] * sw3[j] * (float)sS2[row * kH + j]);
@@ -331,7 +331,7 @@ __global__ void __launch_bounds__(256) k_closure_train(TrainArgs a) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) g[kOffW2 + (16 * w + 4 * q + v) * 64 + 16 * nt + r] = acc[v];
   }
-  // dZ1 = bf16((dZ2 W2^T) silu'(Z1)) for this wave's rows, in place of S1
+  // dZ1 = bf16((dZ2 W2^T) relu'(Z1)) for this wave's rows, in place of S1
   {
     floatx4_t acc[4];
 #pragma unroll

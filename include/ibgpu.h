@@ -380,8 +380,7 @@ ibg_status ibg_cell_stage_updates(ibg_ctx *ctx, int64_t *count);
  * has no learned component, so these calls replace nothing there and are
  * the training / inference surface of the closure (the reference's
  * per-cell evaluation stays the fallback and the measure of its error).
- * Network 8 -> 64 -> 64 -> 1 with SiLU units in base 2, g(z) = z/(1+2^-z)
- * = SiLU(z ln2)/ln2 (the scalings fold into the adjacent layers), BF16
+ * Network 8 -> 64 -> 64 -> 1 with ReLU units, BF16
  * weights and activations, FP32 accumulation, trained on cells sampled
  * from this context's own state.
  * Parameter vector (floats): W1[8][64] b1[64] W2[64][64] b2[64] w3[64] b3,

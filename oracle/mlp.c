@@ -40,18 +40,10 @@ float orc_bf16(float x) {
   return y;
 }
 
-/* The hidden units are SiLU units in base 2: g(z) = z / (1 + 2^-z)
- * = SiLU(z ln2) / ln2.  The scalings ln2 and 1/ln2 fold into the adjacent
- * layers (W1, b1, b2 times ln2 and w3 over ln2 give the network with
- * g replaced by SiLU), so the function class is that of a SiLU network;
- * the GPU evaluates 2^-z with one v_exp_f32 and no multiply.
- * g'(z) = s + z ln2 s (1 - s), s = 1 / (1 + 2^-z). */
-#define LN2 0.69314718055994530942
-static float silu_f(float z) { return z / (1.0f + exp2f(-z)); }
-static float dsilu_f(float z) {
-  float s = 1.0f / (1.0f + exp2f(-z));
-  return s + z * (float)LN2 * s * (1.0f - s);
-}
+/* The hidden units are ReLU units; the derivative is taken as 0 at z = 0
+ * (as the GPU's z > 0 test). */
+static float relu_f(float z) { return z > 0.0f ? z : 0.0f; }
+static float drelu_f(float z) { return z > 0.0f ? 1.0f : 0.0f; }
 
 /* prm: parameters then fmean[8] fstd[8] flo[8] fhi[8] zmean zstd */
 static void fwd_row(const float *prm, const float *xraw, float *xb, float *h1, float *s1,
@@ -66,16 +58,16 @@ static void fwd_row(const float *prm, const float *xraw, float *xb, float *h1, f
     double a = 0.0;
     for (int k = 0; k < 8; ++k) a += (double)xb[k] * (double)orc_bf16(prm[OW1 + k * 64 + j]);
     float z = (float)(a + prm[OB1 + j]);
-    h1[j] = orc_bf16(silu_f(z));
-    s1[j] = orc_bf16(dsilu_f(z));
+    h1[j] = orc_bf16(relu_f(z));
+    s1[j] = orc_bf16(drelu_f(z));
   }
   double yy = 0.0;
   for (int j = 0; j < 64; ++j) {
     double a = 0.0;
     for (int i = 0; i < 64; ++i) a += (double)h1[i] * (double)orc_bf16(prm[OW2 + i * 64 + j]);
     float z = (float)(a + prm[OB2 + j]);
-    h2[j] = orc_bf16(silu_f(z));
-    s2[j] = orc_bf16(dsilu_f(z));
+    h2[j] = orc_bf16(relu_f(z));
+    s2[j] = orc_bf16(drelu_f(z));
     yy += (double)h2[j] * (double)orc_bf16(prm[OW3 + j]);
   }
   *y = (float)(yy + prm[OB3]);
@@ -126,11 +118,8 @@ void orc_mlp_grad(const float *prm, const float *x, const float *z, int64_t n, d
 }
 
 /* plain FP64 network on standardised inputs xs[n][8] and targets t[n] */
-static double silu_d(double z) { return z / (1.0 + exp2(-z)); }
-static double dsilu_d(double z) {
-  double s = 1.0 / (1.0 + exp2(-z));
-  return s + z * LN2 * s * (1.0 - s);
-}
+static double relu_d(double z) { return z > 0.0 ? z : 0.0; }
+static double drelu_d(double z) { return z > 0.0 ? 1.0 : 0.0; }
 
 double orc_mlp_loss_f64(const double *p, const double *xs, const double *t, int64_t n) {
   double L = 0.0;
@@ -139,12 +128,12 @@ double orc_mlp_loss_f64(const double *p, const double *xs, const double *t, int6
     for (int j = 0; j < 64; ++j) {
       double a = p[OB1 + j];
       for (int k = 0; k < 8; ++k) a += xs[r * 8 + k] * p[OW1 + k * 64 + j];
-      h1[j] = silu_d(a);
+      h1[j] = relu_d(a);
     }
     for (int j = 0; j < 64; ++j) {
       double a = p[OB2 + j];
       for (int i = 0; i < 64; ++i) a += h1[i] * p[OW2 + i * 64 + j];
-      y += silu_d(a) * p[OW3 + j];
+      y += relu_d(a) * p[OW3 + j];
     }
     L += (y - t[r]) * (y - t[r]);
   }
@@ -161,13 +150,13 @@ void orc_mlp_grad_f64(const double *p, const double *xs, const double *t, int64_
       double a = p[OB1 + j];
       for (int k = 0; k < 8; ++k) a += xs[r * 8 + k] * p[OW1 + k * 64 + j];
       z1[j] = a;
-      h1[j] = silu_d(a);
+      h1[j] = relu_d(a);
     }
     for (int j = 0; j < 64; ++j) {
       double a = p[OB2 + j];
       for (int i = 0; i < 64; ++i) a += h1[i] * p[OW2 + i * 64 + j];
       z2[j] = a;
-      h2[j] = silu_d(a);
+      h2[j] = relu_d(a);
       y += h2[j] * p[OW3 + j];
     }
     double e = y - t[r];
@@ -176,7 +165,7 @@ void orc_mlp_grad_f64(const double *p, const double *xs, const double *t, int64_
     double dz2[64];
     for (int j = 0; j < 64; ++j) {
       grad[OW3 + j] += h2[j] * dy;
-      dz2[j] = dy * p[OW3 + j] * dsilu_d(z2[j]);
+      dz2[j] = dy * p[OW3 + j] * drelu_d(z2[j]);
       grad[OB2 + j] += dz2[j];
     }
     grad[OB3] += dy;
@@ -186,7 +175,7 @@ void orc_mlp_grad_f64(const double *p, const double *xs, const double *t, int64_
         grad[OW2 + i * 64 + j] += h1[i] * dz2[j];
         a += dz2[j] * p[OW2 + i * 64 + j];
       }
-      double dz1 = a * dsilu_d(z1[i]);
+      double dz1 = a * drelu_d(z1[i]);
       grad[OB1 + i] += dz1;
       for (int k = 0; k < 8; ++k) grad[OW1 + k * 64 + i] += xs[r * 8 + k] * dz1;
     }
