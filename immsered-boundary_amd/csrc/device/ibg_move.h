@@ -260,15 +260,24 @@ __global__ void __launch_bounds__(256) k_move_dist(MoveArgs a) {
   const int i = in ? set_cell(a, t, a.nw) : 0;
   bool solid = false, io = false, chg = false;
   if (in) {
-    int p = a.cpatch[i];
+    // every per-cell operand up front: the walk below is a chain of
+    // dependent loads, and loads placed after it would add one more
+    // memory round trip to every wave
+    const int p0 = a.cpatch[i];
     const double s0 = a.signdis[i];
+    const double dm = a.dist_dm ? a.dist_dm[i] : 0.0;
+    const bool mixed = a.cmix[i] != 0;
+    const bool fin = a.fin[i] && a.kind[i] != IBG_DMGHOST;
+    const int mk = a.mkib[i];
+    const int b0 = p0 >= 0 ? a.pat_body[p0] : -1;
+    int p = p0;
     double sd = s0;
     if (p >= 0 && s0 < kCriticDis) {
       double x[3], dist;
       cell_center_dev(a, i, x);
       walk_body(a, x, p, dist, sd);
       // cells whose neighbours all share their body have no cycle entries
-      if (sd < kCriticDis && a.cmix[i]) {
+      if (sd < kCriticDis && mixed) {
         const int myb0 = a.pat_body[p];
         int myb = myb0;
         for (int k = 0; k < kNb8; ++k) {
@@ -297,19 +306,15 @@ __global__ void __launch_bounds__(256) k_move_dist(MoveArgs a) {
         }
       }
       // solid loop (:224-249): signdis = min(signdis, distance_to_dm)
-      if (a.dist_dm && sd < kCriticDis) {
-        const double dm = a.dist_dm[i];
-        sd = (dm < sd) ? dm : sd;
-      }
+      if (a.dist_dm && sd < kCriticDis) sd = (dm < sd) ? dm : sd;
     }
     a.cp1[i] = p;
     a.sd1[i] = sd;
-    const int p0 = a.cpatch[i];
-    chg = p0 >= 0 && a.pat_body[p] != a.pat_body[p0];
-    if (a.fin[i] && a.kind[i] != IBG_DMGHOST) {
+    chg = p0 >= 0 && a.pat_body[p] != b0;
+    if (fin) {
       solid = sd < 0.0;
       io = s0 < 0.0 && !(sd < 0.0);
-      if (!(s0 < 0.0) && sd < 0.0 && a.mkib[i] != a.epoch[0] - 1) report_error(a.err, i, 263);
+      if (!(s0 < 0.0) && sd < 0.0 && mk != a.epoch[0] - 1) report_error(a.err, i, 263);
     }
   }
   wave_append(a.cnt + 0, solid, i, a.slist, a.cap_s, a.err);
