@@ -260,8 +260,9 @@ constexpr int kRpart = 12;   // per tile: max[5], sumsq[5], SA max, SA sumsq
 constexpr int kMlSlots = 64, kMlSlotStride = 16;
 
 // LDS of the learned-closure part of k_stage (aliases the face arrays,
-// which are dead by then): the weights (W1t, W2p, b1, b2, w3).
-constexpr size_t kStageMlSmem = (size_t)64 * kX * 2 + (size_t)64 * kH * 2 + 3 * 64 * 4;
+// which are dead by then): the weights (W1t as 64 rows of its 8 used
+// columns, W2p, b1, b2, w3).
+constexpr size_t kStageMlSmem = (size_t)64 * 8 * 2 + (size_t)64 * kH * 2 + 3 * 64 * 4;
 
 // One workgroup per tile of Morton-consecutive normal cells of one
 // (case, level).  Phase 1: every face touching the tile — MUSCL + all-speed
@@ -439,35 +440,33 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R, D> &a) {
   if constexpr (ML) {
     if (threadIdx.x == 0) { mlc[0] = 0u; mlc[1] = 0u; }
     __syncthreads();   // every phase-2a read of the face arrays is done
-    __bf16 *mW1t = reinterpret_cast<__bf16 *>(smem_raw);
-    __bf16 *mW2p = mW1t + 64 * kX;
+    __bf16 *mW1t = reinterpret_cast<__bf16 *>(smem_raw);   // [64][8]
+    __bf16 *mW2p = mW1t + 64 * 8;
     float *mb1 = reinterpret_cast<float *>(mW2p + 64 * kH), *mb2 = mb1 + 64;
     __bf16 *mw3 = reinterpret_cast<__bf16 *>(mb2 + 64);
     {
-      // weights to LDS in 16-byte chunks (W1t then W2p), all loads issued
-      // before the stores
-      constexpr int kW1Chunks = 64 * kX * 2 / 16, kWChunks = kW1Chunks + 64 * kH * 2 / 16;
-      static_assert(kW1Chunks == 320 && kWChunks == 896, "weight chunk counts");
+      // weights to LDS in 16-byte chunks, all loads issued before the
+      // stores: the first 16 bytes of each W1t row (its 8 used columns),
+      // then W2p
+      constexpr int kW1Rows = 64, kWChunks = kW1Rows + 64 * kH * 2 / 16;
+      static_assert(kWChunks == 640, "weight chunk count");
       uint4 *dst = reinterpret_cast<uint4 *>(mW1t);   // W2p follows W1t in LDS
-      const uint4 *s1 = reinterpret_cast<const uint4 *>(a.mlp.W1t);   // chunks [0, 320)
-      const uint4 *s2 = reinterpret_cast<const uint4 *>(a.mlp.W2p);   // chunks [320, 896)
+      const uint4 *s2 = reinterpret_cast<const uint4 *>(a.mlp.W2p);   // chunks [64, 640)
       // chunk id of W1t or W2p; both indices stay in range whichever is read
       auto chunk = [&](int id) {
-        const uint4 x1 = s1[id < kW1Chunks ? id : 0];
-        const uint4 x2 = s2[id < kW1Chunks ? 0 : id - kW1Chunks];
-        return id < kW1Chunks ? x1 : x2;
+        const uint4 x1 = *reinterpret_cast<const uint4 *>(a.mlp.W1t + (size_t)(id < kW1Rows ? id : 0) * kX);
+        const uint4 x2 = s2[id < kW1Rows ? 0 : id - kW1Rows];
+        return id < kW1Rows ? x1 : x2;
       };
       if (blockDim.x == 256) {
-        // chunks t, t+256, t+512 always exist; t+768 for t < 128
+        // chunks t, t+256 always exist; t+512 for t < 128
         const int t = threadIdx.x;
         const uint4 c0 = chunk(t);
-        const uint4 c1 = chunk(t + 256);
-        const uint4 c2 = s2[t + 512 - kW1Chunks];
-        const uint4 c3 = s2[(t < kWChunks - 768 ? t + 768 : 768) - kW1Chunks];
+        const uint4 c1 = s2[t + 256 - kW1Rows];
+        const uint4 c2 = s2[(t < kWChunks - 512 ? t + 512 : 512) - kW1Rows];
         dst[t] = c0;
         dst[t + 256] = c1;
-        dst[t + 512] = c2;
-        if (t < kWChunks - 768) dst[t + 768] = c3;
+        if (t < kWChunks - 512) dst[t + 512] = c2;
       } else {
         for (int id = threadIdx.x; id < kWChunks; id += blockDim.x) dst[id] = chunk(id);
       }
@@ -490,7 +489,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R, D> &a) {
     for (int sb = 0; sb < 4; ++sb) {
       asm volatile("" ::: "memory");
       if (((on >> (16 * sb)) & 0xffffull) == 0ull) continue;
-      const float y = mlp_fwd_reg(xf, sb, mW1t, mW2p, mb1, mb2, mw3);
+      const float y = mlp_fwd_reg<8>(xf, sb, mW1t, mW2p, mb1, mb2, mw3);
       if ((lane >> 4) == sb) yown = y;
     }
     if (use_ml) {

@@ -180,6 +180,8 @@ __device__ __forceinline__ void mlp_fwd16(const __bf16 *Xs, __bf16 *H1s, __bf16 
 // the 4 lanes of a cell.  Returns y (b3 not included) of cell r in every
 // lane (r, q).  No LDS for activations, no barriers.  W1t/W2p/b1/b2 and the
 // bf16 w3 come from LDS.
+// W1S: row stride (bf16) of W1t; only the first 8 columns of a row are read.
+template <int W1S = kX>
 __device__ __forceinline__ float mlp_fwd_reg(bf16x8_t xf, int sb, const __bf16 *W1t,
                                              const __bf16 *W2p, const float *b1, const float *b2,
                                              const __bf16 *w3b) {
@@ -192,7 +194,7 @@ __device__ __forceinline__ float mlp_fwd_reg(bf16x8_t xf, int sb, const __bf16 *
   for (int nt = 0; nt < 4; ++nt) {
     const float4 bb = *reinterpret_cast<const float4 *>(b1 + 16 * nt + 4 * q);
     h1[nt] = floatx4_t{bb.x, bb.y, bb.z, bb.w};
-    const bf16x8_t aw = *reinterpret_cast<const bf16x8_t *>(W1t + (16 * nt + r) * kX);
+    const bf16x8_t aw = *reinterpret_cast<const bf16x8_t *>(W1t + (16 * nt + r) * W1S);
     h1[nt] = mfma16(aw, xm, h1[nt]);
   }
   // relu -> bf16, packed as the layer-2 B fragments (kt = 0: tiles 0,1; kt = 1: tiles 2,3)
