@@ -143,6 +143,7 @@ def test_prescribed_motion_march(tiny, mode):
     eng.set_state(fv0)
     mb = oracle_ffi.MovingBodies(m)
     level = m.level_of()
+    normal = np.asarray(m.kind)
     L = m.nlevels
     t = 0.0
     for ts in range(nsteps):
@@ -160,6 +161,12 @@ def test_prescribed_motion_march(tiny, mode):
         sd, cp, ib, nib = eng.geometry()
         assert np.array_equal(ib, mb.ib) and nib == r["nib"]
         assert np.allclose(sd[fresh], mb.signdis[fresh], rtol=1e-12, atol=1e-13)
+        # cell-stage updates counted on the device: every marched normal cell
+        # below the finest level, and the finest level's cells that are
+        # neither solid nor IB cells (NS_Solver.cpp:856-983), per stage
+        upd_cell = (level < L - 1) | ((sd >= 0.0) & (ib == 0))
+        want = 3 * int(np.count_nonzero(fresh & (normal == 0) & upd_cell))
+        assert eng.cell_stage_updates() == want, (ts, eng.cell_stage_updates(), want)
         t = eng.get_time()[0]
     assert t > 0.0
     assert np.array_equal(eng.get_state(), out[0][0])
