@@ -1348,7 +1348,8 @@ ibg_status ibg_time_march(ibg_ctx *c, int32_t ts0, int32_t nsteps, int32_t use_g
   c->last_updates = upd;
   if (c->mv.law) {
     // the moving-body masks change every step: counted on the device
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->s, c->mv.upd, 1, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid(kUpdSlots * kUpdSlotStride, 256)), dim3(256), 0, c->s,
+                       c->mv.upd, kUpdSlots * kUpdSlotStride, 0ull);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(c->ev0, c->s));
@@ -1387,10 +1388,13 @@ ibg_status ibg_time_march(ibg_ctx *c, int32_t ts0, int32_t nsteps, int32_t use_g
   HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_total_ms = ms;
   if (c->mv.law) {
-    unsigned long long u = 0;
-    HIPCHK(hipMemcpyAsync(&u, c->mv.upd, sizeof(u), hipMemcpyDeviceToHost, c->s));
+    std::vector<unsigned long long> u((size_t)kUpdSlots * kUpdSlotStride);
+    HIPCHK(hipMemcpyAsync(u.data(), c->mv.upd, sizeof(unsigned long long) * u.size(),
+                          hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
-    c->last_updates = (int64_t)u;
+    unsigned long long sum = 0;
+    for (int k = 0; k < kUpdSlots; ++k) sum += u[(size_t)k * kUpdSlotStride];
+    c->last_updates = (int64_t)sum;
   }
   int64_t cell;
   int32_t code;
@@ -1586,7 +1590,7 @@ ibg_status ibg_upload_surface(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_surfa
   }
   AL(v.cp1, N); AL(v.sd1, N); AL(v.mk1, N);
   AL(v.pose, c->ncases); AL(v.motc, c->ncases); AL(v.lawd, c->ncases); AL(v.t, c->ncases);
-  AL(v.upd, 1); AL(v.lvlcnt, IBG_MAX_LEVELS);
+  AL(v.upd, (size_t)kUpdSlots * kUpdSlotStride); AL(v.lvlcnt, IBG_MAX_LEVELS);
   // IB arrays at capacity, holding the current list
   const int cap = sf->ib_capacity > 0 ? sf->ib_capacity : 2 * c->nib + 4096;
   if (cap < c->nib) return fail("IB capacity below the current list");

@@ -53,6 +53,11 @@ struct PitchHeave {
   double x0, y0, aoa0, amp, omega, heave;   // angles in radians
 };
 
+// The update count of a move is summed into kUpdSlots counters one 128-byte
+// line apart (workgroup b adds into slot b % kUpdSlots; the reader sums the
+// slots): thousands of workgroups adding into one address serialise on it.
+constexpr int kUpdSlots = 64, kUpdSlotStride = 16;
+
 // In-plane neighbour slots of Point_iterator_2d (i outer, j inner, k = 1)
 // without the cell itself: nb27 slots 1, 4, 7, 10, 16, 19, 22, 25.
 constexpr int kNb8 = 8;
@@ -578,7 +583,9 @@ __device__ void move_commit(const MoveArgs &a, int nblk) {
     __syncthreads();
     if (threadIdx.x == 0) {
       const int s = part[0] + part[1] + part[2] + part[3];
-      if (s) atomicAdd(a.upd, (unsigned long long)s * a.nstage);
+      if (s)
+        atomicAdd(a.upd + (size_t)(blockIdx.x % kUpdSlots) * kUpdSlotStride,
+                  (unsigned long long)s * a.nstage);
     }
   }
 }
