@@ -143,6 +143,37 @@ __device__ __forceinline__ void wave_append(int *cnt, bool v, int val, int *list
   }
 }
 
+// Append up to K values per lane (bit k of set: val[k]) with one atomic per
+// wave.  The lanes of the wave that reach the call take part; the counts and
+// offsets are wave-uniform ballots.
+template <int K>
+__device__ __forceinline__ void wave_append_set(int *cnt, unsigned set, const int *val, int *list,
+                                                int cap, unsigned long long *err) {
+  unsigned long long m[K];
+  int total = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    m[k] = __ballot((set >> k) & 1u);
+    total += __popcll(m[k]);
+  }
+  if (total == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)__ballot(1)) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(cnt, total);
+  base = __shfl(base, leader);
+  const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if ((set >> k) & 1u) {
+      const int pos = base + __popcll(m[k] & below);
+      if (pos < cap) list[pos] = val[k];
+      else report_error(err, val[k], 1090);
+    }
+    base += __popcll(m[k]);
+  }
+}
+
 __device__ __forceinline__ void cell_center_dev(const MoveArgs &a, int i, double *x) {
 #pragma clang fp contract(off)
   const BlkInfo &B = a.blk[a.cblk[i]];
@@ -359,28 +390,28 @@ __global__ void __launch_bounds__(256) k_move_mark1(MoveArgs a) {
     }
     return;
   }
+  // (the lanes past the solid list's capacity have left: those that reach
+  // the append are the low lanes of the wave)
   const int ns = min(a.cnt[0], a.cap_s);
-  if (j >= ns) return;
   const int ep = a.epoch[0];
-  const int s = a.slist[j];
-  for (int k = 0; k < kNb8; ++k) {
-    const int n = a.nb8[(size_t)k * a.N + s];
-    if (n < 0 || a.kind[n] == IBG_DMGHOST || a.sd1[n] < 0.0) continue;
-    if (atomicMax(&a.mk1[n], ep) < ep) {
-      const int pos = atomicAdd(&a.cnt[1], 1);
-      if (pos < a.cap_l1) a.l1list[pos] = n;
-      else report_error(a.err, n, 1090);
+  int nb[kNb8];
+  unsigned fresh = 0;
+  if (j < ns) {
+    const int s = a.slist[j];
+    bool ok[kNb8];
+#pragma unroll
+    for (int k = 0; k < kNb8; ++k) {
+      const int n = a.nb8[(size_t)k * a.N + s];
+      nb[k] = n;
+      ok[k] = n >= 0 && a.kind[n] != IBG_DMGHOST && !(a.sd1[n] < 0.0);
     }
+    // the eight stamps are independent: issued together, not one round
+    // trip after another
+#pragma unroll
+    for (int k = 0; k < kNb8; ++k)
+      if (ok[k] && atomicMax(&a.mk1[nb[k]], ep) < ep) fresh |= 1u << k;
   }
-}
-
-__device__ __forceinline__ void mark_ib(const MoveArgs &a, int m, int ep) {
-  if (atomicMax(&a.mkib[m], ep) < ep) {
-    if (a.kind[m] != IBG_NORMAL) report_error(a.err, m, 1082);
-    const int pos = atomicAdd(&a.cnt[3], 1);
-    if (pos < a.ib_cap) a.ib_cell[pos] = m;
-    else report_error(a.err, m, 1090);
-  }
+  wave_append_set<kNb8>(a.cnt + 1, fresh, nb, a.l1list, a.cap_l1, a.err);
 }
 
 // The IB set: every layer-1 cell and its non-solid, non-domain-ghost
@@ -391,14 +422,29 @@ __global__ void __launch_bounds__(256) k_move_mark2(MoveArgs a) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int ep = a.epoch[0];
   if (j < a.cap_l1) {
-    if (j >= min(a.cnt[1], a.cap_l1)) return;
-    const int n = a.l1list[j];
-    mark_ib(a, n, ep);
-    for (int k = 0; k < kNb8; ++k) {
-      const int m = a.nb8[(size_t)k * a.N + n];
-      if (m < 0 || a.kind[m] == IBG_DMGHOST || a.sd1[m] < 0.0) continue;
-      mark_ib(a, m, ep);
+    // a layer-1 cell and its eligible neighbours: nine independent stamps,
+    // then one append per wave (the lanes that reach it are the low lanes)
+    int c[kNb8 + 1];
+    unsigned fresh = 0;
+    if (j < min(a.cnt[1], a.cap_l1)) {
+      const int n = a.l1list[j];
+      bool ok[kNb8 + 1];
+      c[0] = n;
+      ok[0] = true;
+#pragma unroll
+      for (int k = 0; k < kNb8; ++k) {
+        const int m = a.nb8[(size_t)k * a.N + n];
+        c[k + 1] = m;
+        ok[k + 1] = m >= 0 && a.kind[m] != IBG_DMGHOST && !(a.sd1[m] < 0.0);
+      }
+#pragma unroll
+      for (int k = 0; k <= kNb8; ++k)
+        if (ok[k] && atomicMax(&a.mkib[c[k]], ep) < ep) {
+          fresh |= 1u << k;
+          if (a.kind[c[k]] != IBG_NORMAL) report_error(a.err, c[k], 1082);
+        }
     }
+    wave_append_set<kNb8 + 1>(a.cnt + 3, fresh, c, a.ib_cell, a.ib_cap, a.err);
   } else {
     const int r = j - a.cap_l1;
     if (r >= min(a.cnt[2], a.cap_io)) return;
