@@ -67,9 +67,11 @@ __device__ __forceinline__ float mlp_dact(float z) { return z > 0.0f ? 1.0f : 0.
 // Two units to a bf16 pair with the ReLU applied after the rounding, as a
 // signed 16-bit max against 0 (v_cvt_pk_bf16_f32 + v_pk_max_i16): a
 // negative value rounds to a bf16 with the sign bit set (or -0), a negative
-// integer, so bf16(relu(z)) = max_i16(bf16(z), 0) bit for bit.  fmaxf on the
-// FP32 values costs two v_max_f32 each (IEEE-mode quieting of the MFMA
-// result, then the max).
+// integer, so for finite z, bf16(relu(z)) = max_i16(bf16(z), 0) bit for
+// bit.  Non-finite z differ: a NaN with the sign bit clear stays NaN here
+// while fmaxf(NaN, 0) = 0 in the training path (mlp_act); both only arise
+// from diverged weights.  fmaxf on the FP32 values costs two v_max_f32 each
+// (IEEE-mode quieting of the MFMA result, then the max).
 typedef short s16x2_t __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ bf16x2_t mlp_act_pk(float a, float b) {
   typedef float f32x2_t __attribute__((ext_vector_type(2)));

@@ -1176,6 +1176,7 @@ int32_t ibm_read_solution(const ibm_handle *h, const ibg_physics *ph, const char
   bool has_nut = false;
   int zones = 0;
   int32_t rc = 0;
+  std::vector<char> seen((size_t)h->ncases * h->L, 0);   // one zone per (case, level)
   auto fail = [&](const std::string &msg) { g_last_error = msg; rc = 1; };
   while (rc == 0 && std::fgets(line, sizeof(line), f)) {
     std::string sl(line);
@@ -1195,6 +1196,8 @@ int32_t ibm_read_solution(const ibm_handle *h, const ibg_physics *ph, const char
       break;
     }
     const int b = c * h->L + l, s = h->blk_start[b];
+    if (seen[b]) { fail("zone N-" + std::to_string(c) + " L-" + std::to_string(l) + " repeated"); break; }
+    seen[b] = 1;
     if (ne != h->blk_start[b + 1] - s) { fail("zone element count differs from the level (abort 65)"); break; }
     if (!std::fgets(line, sizeof(line), f)) { fail("truncated file"); break; }   // VARLOCATION
     const int nv = has_nut ? 7 : 6;
@@ -1221,6 +1224,11 @@ int32_t ibm_read_solution(const ibm_handle *h, const ibg_physics *ph, const char
     ++zones;
   }
   std::fclose(f);
+  for (size_t b = 0; rc == 0 && b < seen.size(); ++b)
+    if (!seen[b]) {
+      g_last_error = "missing zone N-" + std::to_string(b / h->L) + " L-" + std::to_string(b % h->L);
+      rc = 1;
+    }
   if (rc == 0 && zones != h->ncases * h->L) { g_last_error = "missing zones"; rc = 1; }
   if (rc == 0 && nut && !has_nut) { g_last_error = "no viseddy in the file"; rc = 1; }
   return rc;

@@ -260,6 +260,33 @@ class MovedMesh:
         return getattr(self._m, k)
 
 
+class PatchedMesh:
+    """A Mesh with some SoA arrays (and counts) replaced, for pins that plant
+    cell classes (solid, IB, zero distance) or a synthetic wall surface.
+    Arrays are passed by field name; `counts` sets integer fields."""
+
+    def __init__(self, mesh, counts=None, **arrays):
+        self._m = mesh
+        soa = type(mesh.soa).from_buffer_copy(mesh.soa)
+        keep = {}
+        for k, a in arrays.items():
+            if a.dtype == np.int32:
+                a = np.ascontiguousarray(a, dtype=np.int32)
+                setattr(soa, k, a.ctypes.data_as(i32p))
+            else:
+                a = np.ascontiguousarray(a, dtype=np.float64)
+                setattr(soa, k, a.ctypes.data_as(f64p))
+            keep[k] = a
+        for k, v in (counts or {}).items():
+            setattr(soa, k, int(v))
+        self._keep = keep
+        self.soa = soa
+        self.soa_ptr = C.pointer(soa)
+
+    def __getattr__(self, k):
+        return getattr(self._m, k)
+
+
 def cfg(nstage=3, alpha=None, dt_var_step=1, temporal_refine=1, limiter=0, viscous=1):
     c = OrcCfg()
     c.nstage = nstage

@@ -63,3 +63,18 @@ def test_other_level_set_is_refused(tiny, tmp_path):
     other.write_solution(path, other.init_state())
     with pytest.raises(RuntimeError, match="element count"):
         tiny.read_solution(path)
+
+
+def test_repeated_zone_is_refused(tmp_path):
+    """a file that repeats one (case, level) zone and leaves out another is
+    refused, even when the element counts match"""
+    two = M.Mesh([M.tiny_case(), M.tiny_case()])
+    path = tmp_path / "amr_soln-0.plt"
+    two.write_solution(path, two.init_state())
+    text = path.read_text()
+    assert text.count('"N-1 L-0"') == 1
+    bad = tmp_path / "amr_soln-bad.plt"
+    bad.write_text(text.replace('"N-1 L-0"', '"N-0 L-0"'))
+    with pytest.raises(RuntimeError, match="repeated"):
+        two.read_solution(bad)
+    two.read_solution(path)
