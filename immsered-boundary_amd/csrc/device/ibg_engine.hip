@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <type_traits>
@@ -43,6 +44,7 @@ using namespace ibg;
 
 struct ibg_ctx {
   ibg_config cfg{};
+  int dev = 0;                 // HIP device current at ibg_create
   hipStream_t s = nullptr;
   bool mixed = false;
   int D = 2, NV = 4, ncases = 0, L = 0, nblk = 0;
@@ -188,6 +190,19 @@ struct ibg_ctx {
     return o;
   }
 };
+
+// Staging memory of ibg_closure_infer_noalloc: one file-scope device array
+// per GPU, held until the process ends, so that entry point calls no
+// allocator.  Host code never passes the array's name to a kernel or copy:
+// it uses the device address that ibg_create looks up once per device with
+// hipGetSymbolAddress (g_scratch_dev).  Every context on a device shares
+// that device's array; calls that overlap on one device race.
+constexpr long long kScratchRows = 1 << 16;                  // feature rows per call
+constexpr size_t kScratchFloats = (size_t)kScratchRows * (kMlpIn + 1);   // x[rows][8], then z[rows]
+static __device__ __align__(256) float scratch[kScratchFloats];
+constexpr int kMaxDevices = 64;
+static float *g_scratch_dev[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
 
 namespace {
 
@@ -673,8 +688,24 @@ ibg_status ibg_create(const ibg_config *cfg, ibg_ctx **out) {
     fprintf(stderr, "ibgpu: no HIP device\n");
     return IBG_ERR_HIP;
   }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return IBG_ERR_HIP;
+  {
+    // the staging array's address on this device (once per device)
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+      void *p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p) {
+        fprintf(stderr, "ibgpu: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+        return IBG_ERR_HIP;
+      }
+      g_scratch_dev[dev] = (float *)p;
+    }
+  }
   ibg_ctx *c = new ibg_ctx();
   c->cfg = *cfg;
+  c->dev = dev;
   c->mixed = (cfg->mode == IBG_MODE_MIXED);
   c->turb = (cfg->turbulence != 0);
   c->nbuf = (cfg->nstage == 1) ? 2 : 3;
@@ -2292,6 +2323,27 @@ ibg_status ibg_closure_infer(ibg_ctx *c, const float *x, float *z, int64_t n) {
   (void)hipFree(dx);
   (void)hipFree(dz);
   return e == hipSuccess ? IBG_OK : IBG_ERR_HIP;
+}
+
+ibg_status ibg_closure_infer_noalloc(ibg_ctx *c, const float *x, float *z, int64_t n) {
+  if (!c || !x || !z || !c->cl.ready || n < 1) return IBG_ERR_ARG;
+  if (n > kScratchRows) {
+    fprintf(stderr, "ibgpu: ibg_closure_infer_noalloc: %lld rows exceed the staging limit of %lld\n",
+            (long long)n, kScratchRows);
+    return IBG_ERR_ARG;
+  }
+  float *dx = g_scratch_dev[c->dev];   // set by ibg_create on this device
+  if (!dx) return IBG_ERR_STATE;
+  float *dz = dx + (size_t)kScratchRows * kMlpIn;
+  // rows [0, n) of x are copied in and rows [0, n) of z written: nothing
+  // left from an earlier call is read
+  HIPCHK(hipMemcpyAsync(dx, x, sizeof(float) * n * kMlpIn, hipMemcpyHostToDevice, c->s));
+  hipLaunchKernelGGL(k_closure_infer, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->s, c->mlpdev(),
+                     (const float *)dx, dz, (long long)n);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(z, dz, sizeof(float) * n, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return IBG_OK;
 }
 
 ibg_status ibg_closure_grad(ibg_ctx *c, const float *x, const float *z, int64_t n, float *grad,

@@ -23,6 +23,7 @@ f32p = C.POINTER(C.c_float)
 i64p = C.POINTER(C.c_int64)
 IBG_CLOSURE_NPARAM = 4801
 IBG_CLOSURE_NFEAT = 8
+IBG_CLOSURE_NOALLOC_MAX_ROWS = 65536
 
 
 class MeshSoA(C.Structure):
@@ -226,6 +227,7 @@ GPU_FUNCS = {
     "ibg_closure_params": (C.c_int, [C.c_void_p, f32p, C.c_int64]),
     "ibg_closure_set_params": (C.c_int, [C.c_void_p, f32p, C.c_int64]),
     "ibg_closure_infer": (C.c_int, [C.c_void_p, f32p, f32p, C.c_int64]),
+    "ibg_closure_infer_noalloc": (C.c_int, [C.c_void_p, f32p, f32p, C.c_int64]),
     "ibg_closure_grad": (C.c_int, [C.c_void_p, f32p, f32p, C.c_int64, f32p, f64p]),
     "ibg_closure_dataset": (C.c_int, [C.c_void_p, f32p, f32p, i64p, C.c_int64]),
     "ibg_closure_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),

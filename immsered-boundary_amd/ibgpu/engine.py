@@ -222,6 +222,16 @@ class Engine:
                   "ibg_closure_infer")
         return z
 
+    def closure_infer_noalloc(self, x):
+        """closure_infer staged through the library's fixed per-device
+        array (no device allocation; at most 65536 rows)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        z = np.zeros(x.shape[0], dtype=np.float32)
+        self._chk(self.lib.ibg_closure_infer_noalloc(self.ctx, x.ctypes.data_as(_ffi.f32p),
+                                                     z.ctypes.data_as(_ffi.f32p), x.shape[0]),
+                  "ibg_closure_infer_noalloc")
+        return z
+
     def closure_grad(self, x, z):
         x = np.ascontiguousarray(x, dtype=np.float32)
         z = np.ascontiguousarray(z, dtype=np.float32)

@@ -405,6 +405,17 @@ ibg_status ibg_closure_params(ibg_ctx *ctx, float *out, int64_t n);
 ibg_status ibg_closure_set_params(ibg_ctx *ctx, const float *in, int64_t n);
 /* network output z (destandardised) for n raw feature rows x[n][8]          */
 ibg_status ibg_closure_infer(ibg_ctx *ctx, const float *x, float *z, int64_t n);
+/* ibg_closure_infer without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * x and z are host buffers as in ibg_closure_infer; they are staged through
+ * a fixed per-device array of the library (2.25 MiB, held until the process
+ * ends) instead of per-call allocations.  n is at most
+ * IBG_CLOSURE_NOALLOC_MAX_ROWS (IBG_ERR_ARG otherwise).  Every context on a
+ * device shares that array: calls that overlap on one device race.  Same
+ * kernel and results as ibg_closure_infer.                                  */
+#define IBG_CLOSURE_NOALLOC_MAX_ROWS 65536
+ibg_status ibg_closure_infer_noalloc(ibg_ctx *ctx, const float *x, float *z, int64_t n);
 /* gradient of the batch loss (1/n) sum (y - (z - zmean)/zstd)^2 w.r.t. the
  * parameter vector for n explicit rows; loss = that mean                    */
 ibg_status ibg_closure_grad(ibg_ctx *ctx, const float *x, const float *z, int64_t n,
