@@ -78,14 +78,13 @@ def stage_bytes(mode, dim, turb, cells, faces, last=False, first=False):
     return cells * per_cell + faces * per_face
 
 
-def stage_mix(eng, mesh, mode, ts0, nstage, reps):
-    """Stage-kernel launches of one sub-cycling period (every stage of the
-    steps ts0 .. ts0 + 2^(L-1) - 1, the launch mix of the timed steps),
-    each timed with HIP events on the engine's stream: average launch
-    duration, algorithmic bytes and cells per launch."""
-    period = 1 << (mesh.nlevels - 1)
+def stage_mix(eng, mesh, mode, ts0, nsteps, nstage, reps):
+    """Stage-kernel launches of the timed steps ts0 .. ts0 + nsteps - 1 (every
+    stage of each, the launch mix of the timed region), each timed with HIP
+    events on the engine's stream: average launch duration, algorithmic
+    bytes and cells per launch."""
     ms_sum, by_sum, c_sum, n = 0.0, 0.0, 0, 0
-    for ts in range(ts0, ts0 + period):
+    for ts in range(ts0, ts0 + nsteps):
         for k in range(nstage):
             ms, nc, nf = eng.bench_stage(ts, k, reps)
             ms_sum += ms
@@ -131,7 +130,12 @@ def new_engine(mesh, mode, nstage, motion=True):
 
 
 def timed_run(eng, mesh, mode, warmup, steps, reps, ml=False):
+    """W warmup steps, then exactly K timed steps replayed from HIP graphs
+    captured beforehand (ibg_prepare_march: the timed region holds no
+    capture, instantiation or direct launch), stream synchronised on both
+    sides; then the stage kernel alone over the same steps' launch mix."""
     eng.TimeMarching(0, warmup, use_graph=True)
+    eng.prepare_march(warmup, steps)
     eng.synchronize()
     t0 = time.perf_counter()
     eng.TimeMarching(warmup, steps, use_graph=True)
@@ -146,35 +150,30 @@ def timed_run(eng, mesh, mode, warmup, steps, reps, ml=False):
     counts = eng.closure_counts() if ml else (0, 0)   # cells of the warmup + timed steps
     t_end = float(eng.get_time()[0]) if getattr(eng, "_surface", None) else None
     # the stage kernel alone, launch by launch over the timed steps' mix
-    ms, nbytes, ncell = stage_mix(eng, mesh, mode, warmup + steps, eng.cfg.nstage, reps)
+    ms, nbytes, ncell = stage_mix(eng, mesh, mode, warmup, steps, eng.cfg.nstage, reps)
     return {"seconds": t1 - t0, "updates": upd, "rate": upd / (t1 - t0), "cd": cd, "cl": cl,
             "stage_ms": ms, "stage_bytes": nbytes, "stage_cells": ncell, "state": state, "nut": nut,
             "counts": counts, "forces": forces, "t_end": t_end}
 
 
+def accuracy_run(eng, mesh, nsteps):
+    """The form's run from the start to nsteps (graph replay, untimed):
+    C_L, C_D, state and SA variable there, for the comparison between forms
+    at the configuration's horizon."""
+    eng.TimeMarching(0, nsteps, use_graph=True)
+    P = mesh.phys[0]
+    F = eng.ComptWallForce()
+    t_end = float(eng.get_time()[0]) if getattr(eng, "_surface", None) else None
+    return {"cd": F[0] / P.force_ref, "cl": F[1] / P.force_ref, "state": eng.get_state(),
+            "nut": eng.get_nut(), "t_end": t_end}
+
+
 def train_closure(mesh, nstage, args):
-    """Network trained on cells sampled from the plain FP64 run."""
-    t0 = time.perf_counter()
-    src = new_engine(mesh, "fp64", nstage, not args.static)
-    src.closure_init(args.samples, 20261202)
-    period = 1 << (mesh.nlevels - 1)
-    total, ts, k = 0, 0, 0
-    frac = min(1.0, args.samples / (args.sample_rounds * 2.5e6))
-    while total < args.samples and k < 4 * args.sample_rounds:
-        src.TimeMarching(ts, period, use_graph=True)
-        ts += period
-        total = src.closure_sample(frac, 1000 + k)
-        k += 1
-    src.closure_fit_stats(0.0)
-    t1 = time.perf_counter()
-    steps = max(1, int(args.epochs * total / args.batch))
-    loss = src.closure_train(steps, batch=args.batch, lr=1e-3, seed=7)
-    src.synchronize()
-    t2 = time.perf_counter()
-    prm = src.closure_params()
-    src.close()
-    return prm, {"samples": int(total), "sampled_steps": ts, "train_steps": steps, "batch": args.batch,
-                 "epochs": args.epochs, "final_loss": loss, "sample_s": t1 - t0, "train_s": t2 - t1}
+    """Network trained on cells sampled from the plain FP64 run
+    (ibgpu.train_closure: the configuration's sampling and Adam recipe)."""
+    return ibgpu.train_closure(mesh, nstage=nstage, samples=args.samples,
+                               sample_rounds=args.sample_rounds, batch=args.batch,
+                               epochs=args.epochs, motion=None if args.static else MOTION)
 
 
 def cpu_baseline(mesh, nstage, budget_s, motion=True):
@@ -293,6 +292,33 @@ def secondary(args):
     print(json.dumps(out))
 
 
+def _vs(r, fp):
+    return {"CL": r["cl"], "CD": r["cd"],
+            "dCL_rel": abs(r["cl"] - fp["cl"]) / max(abs(fp["cl"]), 1e-30),
+            "dCD_rel": abs(r["cd"] - fp["cd"]) / max(abs(fp["cd"]), 1e-30),
+            "state_rel_L2": float(np.linalg.norm(r["state"][:, :5] - fp["state"][:, :5]) /
+                                  np.linalg.norm(fp["state"][:, :5])),
+            "nut_rel_L2": float(np.linalg.norm(r["nut"] - fp["nut"]) / np.linalg.norm(fp["nut"]))}
+
+
+def accuracy_block(acc, fp, mx, ml, args):
+    """C_L / C_D and state of the mixed forms against plain FP64: at the
+    configuration's horizon (acc_steps, own untimed runs) and after the
+    timed region's steps."""
+    tr = {"after_steps": args.warmup + args.steps, "CL_fp64": fp["cl"], "CD_fp64": fp["cd"],
+          "mixed": _vs(mx, fp), "mixed_ml": _vs(ml, fp)}
+    if not acc:
+        return dict(vs="plain FP64 after the same steps", **tr)
+    a = acc["fp64"]
+    out = {"vs": "plain FP64 after the same steps", "after_steps": args.acc_steps,
+           "t_end": a["t_end"], "CL_fp64": a["cl"], "CD_fp64": a["cd"],
+           "mixed": _vs(acc["mixed"], a), "mixed_ml": _vs(acc["ml"], a),
+           "mixed_ml_counts": {"network_cells": acc["ml"]["counts"][0],
+                               "fallback_cells": acc["ml"]["counts"][1]},
+           "timed_region": tr}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -300,6 +326,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=32)
     ap.add_argument("--stages", type=int, default=4)
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--acc-steps", type=int, default=2000,
+                    help="horizon of the C_L/C_D comparison between the forms (0: the timed steps)")
     ap.add_argument("--samples", type=int, default=5_000_000)
     ap.add_argument("--sample-rounds", type=int, default=8)
     ap.add_argument("--batch", type=int, default=8192)
@@ -324,6 +352,7 @@ def main():
     print(f"[bench] mesh: {mesh.ncells} cells, normal per level {mesh.counts()[0]}", file=sys.stderr, flush=True)
     res = {}
     motion = not args.static
+    acc = {}
     for mode in ("fp64", "mixed"):
         eng = new_engine(mesh, mode, args.stages, motion)
         res[mode] = timed_run(eng, mesh, mode, args.warmup, args.steps, args.reps)
@@ -332,6 +361,10 @@ def main():
         print(f"[bench] {mode}+SA: {r['rate']:.4e} cell-stage/s, {1e3 * r['seconds'] / args.steps:.3f} ms/step, "
               f"CL {r['cl']:.6f} CD {r['cd']:.6f}, stage kernel {r['stage_ms'] * 1e3:.1f} us",
               file=sys.stderr, flush=True)
+        if args.acc_steps:
+            eng = new_engine(mesh, mode, args.stages, motion)
+            acc[mode] = accuracy_run(eng, mesh, args.acc_steps)
+            eng.close()
     prm, tinfo = train_closure(mesh, args.stages, args)
     print(f"[bench] closure: {tinfo['samples']} samples, {tinfo['train_steps']} Adam steps "
           f"(batch {args.batch}), loss {tinfo['final_loss']:.3e}, sampling {tinfo['sample_s']:.1f} s, "
@@ -342,6 +375,14 @@ def main():
     eng.closure_enable(True)
     res["ml"] = timed_run(eng, mesh, "mixed", args.warmup, args.steps, args.reps, ml=True)
     on_net, fallback = res["ml"]["counts"]
+    if args.acc_steps:
+        e2 = new_engine(mesh, "mixed", args.stages, motion)
+        e2.closure_init(1 << 10)
+        e2.closure_set_params(prm)
+        e2.closure_enable(True)
+        acc["ml"] = accuracy_run(e2, mesh, args.acc_steps)
+        acc["ml"]["counts"] = e2.closure_counts()
+        e2.close()
     # host boundary: the AoS state in and out through the C ABI (PCIe + layout
     # conversion), which a caller pays once around a run of steps
     fv_host = mesh.init_state()
@@ -371,14 +412,6 @@ def main():
     ml_bytes, ml_gbs = ml["stage_bytes"], kernel_gbs(ml)
     mx_gbs, fp_gbs = kernel_gbs(mx), kernel_gbs(fp)
 
-    def acc(r):
-        return {"CL": r["cl"], "CD": r["cd"],
-                "dCL_rel": abs(r["cl"] - fp["cl"]) / max(abs(fp["cl"]), 1e-30),
-                "dCD_rel": abs(r["cd"] - fp["cd"]) / max(abs(fp["cd"]), 1e-30),
-                "state_rel_L2": float(np.linalg.norm(r["state"][:, :5] - fp["state"][:, :5]) /
-                                      np.linalg.norm(fp["state"][:, :5])),
-                "nut_rel_L2": float(np.linalg.norm(r["nut"] - fp["nut"]) / np.linalg.norm(fp["nut"]))}
-
     out = {
         "metric": "cell-stage updates/s, mixed+ML vs plain FP64 at equal C_L/C_D error",
         "value": ml["rate"],
@@ -406,6 +439,8 @@ def main():
             "leaf_cells": leaf,
             "levels": mesh.nlevels,
             "stages": args.stages,
+            "closure_network": "8-64-64-1, ReLU hidden units (config's SiLU measured slower at "
+                               "equal closure error), BF16 weights, FP32 accumulation",
             "parallelism": "1 GPU",
         },
         "plain_fp64": {"value": fp["rate"], "ms_per_step": 1e3 * fp["seconds"] / args.steps,
@@ -414,9 +449,7 @@ def main():
                                 "stage_kernel_ms": mx["stage_ms"], "stage_kernel_GBs": mx_gbs},
         "ml_over_fp64": ml["rate"] / fp["rate"],
         "mixed_over_fp64": mx["rate"] / fp["rate"],
-        "accuracy": {"vs": "plain FP64 after the same steps", "after_steps": args.warmup + args.steps,
-                     "CL_fp64": fp["cl"], "CD_fp64": fp["cd"],
-                     "mixed": acc(mx), "mixed_ml": acc(ml)},
+        "accuracy": accuracy_block(acc, fp, mx, ml, args),
         "closure": dict(tinfo, network_cells=on_net, fallback_cells=fallback),
         "host_boundary": {"state_in_out_ms": 1e3 * host_io_s,
                           "value_incl_state_in_out": ml["updates"] / (ml["seconds"] + host_io_s)},
@@ -424,6 +457,10 @@ def main():
                      "frac": ml_gbs / HBM_PEAK_GBS,
                      "traffic": (TRAFFIC_BYTES_PER_CELL * ml["stage_cells"]
                                  if TRAFFIC_BYTES_PER_CELL else None),
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of this kernel in a "
+                                       "profiled run of this script (profiles/k_stage_naca2d_sa_"
+                                       "traffic.md), bytes per cell x this run's cells per launch; "
+                                       "not counted in this run",
                      "kernel": "k_stage_ml<2>", "bytes_per_launch": ml_bytes,
                      "avg_launch_ms": ml["stage_ms"], "cells_per_launch": ml["stage_cells"],
                      "copy_probe_GBs": copy_gbs},

@@ -69,7 +69,26 @@ struct StateOut {
   double *nm, *nd1, *nd2;
   float *nf0, *nf1, *nf2;
   int nd, nf;
+  const unsigned long long *err;   // error word: a fill does nothing once it is set
 };
+
+// A march stops at its first error: every kernel that writes the state,
+// the time or dt returns at once when an earlier launch has set the error
+// word (~0 when clear), so the state is the one of the failing launch (the
+// reference aborts there, NS_Solver.cpp:888-894, NS_Solver.H:355-375).
+__device__ __forceinline__ bool halted(const unsigned long long *err) {
+  return err && *err != ~0ull;
+}
+// The same decision for a whole workgroup (one read, shared through LDS),
+// for kernels whose waves cooperate through LDS: a launch that sets the
+// word while some of the group's waves have read it and others not would
+// otherwise leave a cooperative phase half done.
+__device__ __forceinline__ bool wg_halted(const unsigned long long *err) {
+  __shared__ int flag;
+  if (threadIdx.x == 0) flag = halted(err) ? 1 : 0;
+  __syncthreads();
+  return flag != 0;
+}
 
 struct BlkInfo {
   double h[3], invh[3], area[3], vol, step_num;

@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -146,7 +147,10 @@ struct ibg_ctx {
   double *dt = nullptr, *cflpart = nullptr, *rpart = nullptr, *resid = nullptr, *probe = nullptr;
   unsigned long long *err = nullptr;
   int last_ntiles = 0;
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  // captured step ranges of the time march, keyed by graph_key(): the
+  // launch sequence of a range depends only on its first step's phase in
+  // the level schedule, its length and whether it starts with a dt update
+  std::map<long long, hipGraphExec_t> gexec;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_total_ms = 0.0, last_stage_ms = 0.0;
   int64_t last_stage_launches = 0, last_updates = 0;
@@ -175,6 +179,7 @@ struct ibg_ctx {
   template <typename R> R *nubuf(int k) const { return (R *)NUs[k]; }
   template <int D> StateOut<D> sout() const {
     StateOut<D> o{};
+    o.err = err;
     o.m = w64<D>();
     o.nm = turb ? nu64 : nullptr;
     if (!mixed) {
@@ -241,9 +246,13 @@ ibg_status dupload(ibg_ctx *c, T **p, const std::vector<T> &v) {
   return IBG_OK;
 }
 
+void drop_graphs(ibg_ctx *c) {
+  for (auto &g : c->gexec) (void)hipGraphExecDestroy(g.second);
+  c->gexec.clear();
+}
+
 void free_all(ibg_ctx *c) {
-  for (int k = 0; k < 2; ++k)
-    if (c->gexec[k]) { (void)hipGraphExecDestroy(c->gexec[k]); c->gexec[k] = nullptr; }
+  drop_graphs(c);
   for (void *p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
   c->have_mesh = false;
@@ -299,6 +308,7 @@ ViscArgs<R, D> visc_args(ibg_ctx *c) {
   a.mutf = (R *)c->mutf;
   a.viscous = c->cfg.viscous;
   a.turb = c->turb ? 1 : 0;
+  a.err = c->err;
   return a;
 }
 
@@ -343,7 +353,7 @@ ibg_status launch_stage(ibg_ctx *c, int ntile, int ncell, int subt, bool last, d
     // single-stage scheme: the marched cells go back into the home buffer
     hipLaunchKernelGGL((k_copy_back<R, D>), dim3(grid(ncell, 256)), dim3(256), 0, c->s, ncell,
                        c->sbuf<R, D>(bout), c->sbuf<R, D>(0), c->nubuf<R>(bout),
-                       c->turb ? c->nubuf<R>(0) : (R *)nullptr);
+                       c->turb ? c->nubuf<R>(0) : (R *)nullptr, (const unsigned long long *)c->err);
     HIPCHK(hipGetLastError());
   }
   return IBG_OK;
@@ -508,7 +518,7 @@ ibg_status run_dt(ibg_ctx *c) {
     HIPCHK(hipGetLastError());
   }
   hipLaunchKernelGGL(k_dt_final, dim3(c->ncases), dim3(256), 0, c->s, c->L, c->case_tiles,
-                     c->cflpart, c->dt, c->phys, c->ncases);
+                     c->cflpart, c->dt, c->phys, c->ncases, (const unsigned long long *)c->err);
   HIPCHK(hipGetLastError());
   return IBG_OK;
 }
@@ -532,7 +542,7 @@ ibg_status run_step(ibg_ctx *c, int ts) {
     if ((st = run_stage(c, ts, k, nullptr, nullptr, fuse))) return st;
   if (c->mv.law) {
     hipLaunchKernelGGL(k_motion_advance, dim3(grid(c->ncases, 64)), dim3(64), 0, c->s, c->ncases,
-                       c->mv.t, (const double *)c->dt);
+                       c->mv.t, (const double *)c->dt, (const unsigned long long *)c->err);
     HIPCHK(hipGetLastError());
   }
   return IBG_OK;
@@ -1238,6 +1248,7 @@ ibg_status ibg_set_state(ibg_ctx *c, const double *fv, int64_t n) {
   HIPCHK(hipMemcpy(c->W64a, a, sizeof(D4) * N, hipMemcpyHostToDevice));
   if (c->D == 3) HIPCHK(hipMemcpy(c->W64e, e, sizeof(double) * N, hipMemcpyHostToDevice));
   ibg_status st;
+  if ((st = reset_err(c))) return st;   // a new state clears an earlier error
   if (c->D == 2) st = c->mixed ? fill_stencil_from_master<float, 2>(c) : fill_stencil_from_master<double, 2>(c);
   else st = c->mixed ? fill_stencil_from_master<float, 3>(c) : fill_stencil_from_master<double, 3>(c);
   if (st) return st;
@@ -1367,6 +1378,92 @@ ibg_status ibg_stage(ibg_ctx *c, int32_t ts, int32_t subt) {
   return run_stage(c, ts, subt, last ? c->probe : nullptr, last ? c->nuprobe : nullptr);
 }
 
+}  // extern "C"
+
+namespace {
+
+// A run of steps that one captured graph replays: from ts up to the next
+// boundary of the level schedule's period (2^(L-1) steps under temporal
+// sub-cycling), within [ts, end).  Its launch sequence depends only on the
+// phase of ts in the period, its length and the dt updates it holds; a
+// range is captured when those are fixed by its first step (dt_var_step a
+// multiple of the period: only a range's first step can update dt; or
+// dt_var_step 1: every step does).
+struct MarchRange {
+  int ts, n, dtflag;
+  bool capturable;
+};
+
+MarchRange next_range(const ibg_ctx *c, int ts, int end) {
+  const int P = c->cfg.temporal_refine ? (1 << (c->L - 1)) : 1;
+  MarchRange r;
+  r.ts = ts;
+  r.n = std::min(end - ts, P - ts % P);
+  const int dv = c->cfg.dt_var_step;
+  if (dv <= 0) { r.dtflag = 0; r.capturable = true; }
+  else if (dv == 1) { r.dtflag = 2; r.capturable = true; }
+  else if (dv % P == 0) { r.dtflag = (ts % dv == 0) ? 1 : 0; r.capturable = true; }
+  else { r.dtflag = 0; r.capturable = false; }
+  return r;
+}
+
+long long graph_key(const ibg_ctx *c, const MarchRange &r) {
+  const int P = c->cfg.temporal_refine ? (1 << (c->L - 1)) : 1;
+  return ((long long)(r.ts % P) * (P + 1) + r.n) * 4 + r.dtflag;
+}
+
+// the graph of a range from the cache (captured on first use, without
+// executing anything); launched on the stream when launch is set
+ibg_status range_graph(ibg_ctx *c, const MarchRange &r, bool launch) {
+  const long long key = graph_key(c, r);
+  auto it = c->gexec.find(key);
+  if (it == c->gexec.end()) {
+    const int keep_ntiles = c->last_ntiles;
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < r.n; ++k) {
+      const ibg_status st = run_step(c, r.ts + k);
+      if (st) { hipGraph_t tmp; (void)hipStreamEndCapture(c->s, &tmp); if (tmp) (void)hipGraphDestroy(tmp); return st; }
+    }
+    HIPCHK(hipStreamEndCapture(c->s, &g));
+    hipGraphExec_t ge = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      fprintf(stderr, "ibgpu: hipGraphInstantiate failed: %s\n", hipGetErrorString(e));
+      return IBG_ERR_HIP;
+    }
+    it = c->gexec.emplace(key, ge).first;
+    c->last_ntiles = keep_ntiles;   // nothing ran
+  }
+  if (launch) {
+    HIPCHK(hipGraphLaunch(it->second, c->s));
+    c->last_ntiles = c->tile_end[lmin_of(c, r.ts + r.n - 1)];
+  }
+  return IBG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ibg_status ibg_prepare_march(ibg_ctx *c, int32_t ts0, int32_t nsteps) {
+  if (!c || !c->have_mesh) return IBG_ERR_STATE;
+  if (nsteps < 0) return IBG_ERR_ARG;
+  const int keep_motion = c->have_motion;
+  c->have_motion = c->mv.law ? 1 : 0;   // as ibg_time_march sets it
+  int ts = ts0;
+  const int end = ts0 + nsteps;
+  ibg_status st = IBG_OK;
+  while (ts < end && !st) {
+    const MarchRange r = next_range(c, ts, end);
+    if (r.capturable) st = range_graph(c, r, false);
+    ts += r.n;
+  }
+  c->have_motion = keep_motion;
+  return st;
+}
+
 ibg_status ibg_time_march(ibg_ctx *c, int32_t ts0, int32_t nsteps, int32_t use_graph) {
   if (!c || !c->have_mesh) return IBG_ERR_STATE;
   if (nsteps < 0) return IBG_ERR_ARG;
@@ -1386,33 +1483,16 @@ ibg_status ibg_time_march(ibg_ctx *c, int32_t ts0, int32_t nsteps, int32_t use_g
   HIPCHK(hipEventRecord(c->ev0, c->s));
   int ts = ts0;
   const int end = ts0 + nsteps;
-  if (use_graph) {
-    // eager up to a period boundary, then one captured graph per period of
-    // the level schedule (the launch sequence repeats every 2^(L-1) steps)
-    while (ts < end && ts % P != 0) { if ((st = run_step(c, ts))) return st; ++ts; }
-    while (end - ts >= P) {
-      const int with_dt = (c->cfg.dt_var_step > 0 && ts % c->cfg.dt_var_step == 0) ? 1 : 0;
-      bool ok = true;
-      for (int k = 1; k < P; ++k)
-        if (c->cfg.dt_var_step > 0 && (ts + k) % c->cfg.dt_var_step == 0) ok = false;
-      if (!ok) break;
-      if (!c->gexec[with_dt]) {
-        hipGraph_t g;
-        HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < P; ++k) {
-          st = run_step(c, ts + k);
-          if (st) { hipGraph_t tmp; (void)hipStreamEndCapture(c->s, &tmp); return st; }
-        }
-        HIPCHK(hipStreamEndCapture(c->s, &g));
-        HIPCHK(hipGraphInstantiate(&c->gexec[with_dt], g, nullptr, nullptr, 0));
-        (void)hipGraphDestroy(g);
-      }
-      HIPCHK(hipGraphLaunch(c->gexec[with_dt], c->s));
-      c->last_ntiles = c->tile_end[lmin_of(c, ts + P - 1)];
-      ts += P;
+  while (ts < end) {
+    const MarchRange r = next_range(c, ts, end);
+    if (use_graph && r.capturable) {
+      if ((st = range_graph(c, r, true))) return st;
+    } else {
+      for (int k = 0; k < r.n; ++k)
+        if ((st = run_step(c, ts + k))) return st;
     }
+    ts += r.n;
   }
-  while (ts < end) { if ((st = run_step(c, ts))) return st; ++ts; }
   HIPCHK(hipEventRecord(c->ev1, c->s));
   HIPCHK(hipEventSynchronize(c->ev1));
   float ms = 0.f;
@@ -1658,8 +1738,7 @@ ibg_status ibg_upload_surface(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_surfa
   v.npoint = sf->npoint;
   v.on = true;
   // captured graphs hold the old IB launch
-  for (int k = 0; k < 2; ++k)
-    if (c->gexec[k]) { (void)hipGraphExecDestroy(c->gexec[k]); c->gexec[k] = nullptr; }
+  drop_graphs(c);
   return IBG_OK;
 }
 
@@ -1709,8 +1788,7 @@ ibg_status ibg_move_bodies(ibg_ctx *c, const ibg_body_pose *pose) {
 ibg_status ibg_set_motion(ibg_ctx *c, const ibg_pitch_heave *law, double t0) {
   if (!c) return IBG_ERR_ARG;
   if (!c->have_mesh || !c->mv.on) return IBG_ERR_STATE;
-  for (int k = 0; k < 2; ++k)
-    if (c->gexec[k]) { (void)hipGraphExecDestroy(c->gexec[k]); c->gexec[k] = nullptr; }
+  drop_graphs(c);
   if (!law) { c->mv.law = false; return IBG_OK; }
   const double d2r = 3.14159265358979323846 / 180.0;
   std::vector<PitchHeave> h(c->ncases);
@@ -2096,8 +2174,7 @@ ibg_status ibg_closure_init(ibg_ctx *c, int64_t capacity, uint64_t seed) {
   // captured time-march graphs may hold the closure buffers about to be
   // freed (the network variant of the stage kernel): drop them
   HIPCHK(hipStreamSynchronize(c->s));
-  for (int k = 0; k < 2; ++k)
-    if (c->gexec[k]) { (void)hipGraphExecDestroy(c->gexec[k]); c->gexec[k] = nullptr; }
+  drop_graphs(c);
   for (void *p : c->cl.allocs) (void)hipFree(p);
   c->cl = ibg_ctx::Closure();
   ibg_ctx::Closure &L = c->cl;
@@ -2280,8 +2357,7 @@ ibg_status ibg_closure_enable(ibg_ctx *c, int32_t on) {
     if (!c->mixed || !c->turb) return IBG_ERR_ARG;
   }
   if (c->cl.on != (on != 0)) {
-    for (int k = 0; k < 2; ++k)
-      if (c->gexec[k]) { (void)hipGraphExecDestroy(c->gexec[k]); c->gexec[k] = nullptr; }
+    drop_graphs(c);
   }
   c->cl.on = (on != 0);
   return IBG_OK;

@@ -181,11 +181,21 @@ __device__ __forceinline__ void roe_flux(const Prim<R, D> &t0, const Prim<R, D> 
 __device__ __forceinline__ float f_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float f_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// van Albada in FP32 through the ratio r (as the reference forms it, with
+// its guards): r > 0 here, phi = (r^2 + r)/(r^2 + 1).  The one-reciprocal
+// form pr (dqi + dqii) / (dqi^2 + dqii^2) overflows in FP32 once both
+// differences fall below ~1e-19 (the square sum below 2^-126).
+__device__ __forceinline__ float fast_albada(float num, float den) {
+  float r = num * f_rcp(den);
+  if (!(r <= 1.0e10f)) r = (num > 0.0f) ? 1000000.0f : 0.0f;   // also NaN / inf
+  return (r * r + r) * f_rcp(r * r + 1.0f);
+}
+
 __device__ __forceinline__ float fast_left(float q_i, float qi, float qi1, int lim) {
   const float dqi = qi - q_i, dqii = qi1 - qi;
   const float pr = dqi * dqii;
   if (!(pr > 0.0f)) return qi;
-  if (lim == 1) return qi + 0.5f * pr * (dqi + dqii) * f_rcp(dqi * dqi + dqii * dqii);
+  if (lim == 1) return qi + 0.5f * dqi * fast_albada(dqii, dqi);
   return qi + pr * f_rcp(dqi + dqii);
 }
 
@@ -193,7 +203,7 @@ __device__ __forceinline__ float fast_right(float q_i, float qi, float qi1, int 
   const float dqi = qi - q_i, dqii = qi1 - qi;
   const float pr = dqi * dqii;
   if (!(pr > 0.0f)) return qi;
-  if (lim == 1) return qi - 0.5f * pr * (dqi + dqii) * f_rcp(dqi * dqi + dqii * dqii);
+  if (lim == 1) return qi - 0.5f * dqii * fast_albada(dqi, dqii);
   return qi - pr * f_rcp(dqi + dqii);
 }
 

@@ -106,6 +106,7 @@ struct ViscArgs {
   const R *nu;               // SA working variable (turb)
   R *mutf;                   // [Nn] frozen face-flux term (turb)
   int viscous, turb;
+  const unsigned long long *err;
 };
 
 // Once per step (ComptViscousStress NS_Solver.cpp:365-629 is frozen across
@@ -201,6 +202,7 @@ __device__ __forceinline__ void visc_prep_tile(const ViscArgs<R, D> &a, const Ti
 template <typename R, int D>
 __global__ void __launch_bounds__(kTile) k_visc_prep(ViscArgs<R, D> a) {
   extern __shared__ __align__(32) unsigned char smem_raw[];
+  if (wg_halted(a.err)) return;
   const TileInfo ti = a.tiles[xcd_tile()];
   visc_prep_tile<R, D>(a, ti, smem_raw);
 }
@@ -287,6 +289,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R, D> &a) {
   constexpr bool kMixed = std::is_same<R, float>::value;
   using V = typename V4<R>::T;
   extern __shared__ __align__(32) unsigned char smem_raw[];
+  if (wg_halted(a.err)) return;
   const int tile = xcd_tile();
   const TileInfo ti = a.tiles[tile];
   if constexpr (kFuseVisc<R, D>) {
@@ -429,8 +432,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R, D> &a) {
         sa_nu = nu;
         bool ok = sa_features<R>(me.u, me.v, me.w, me.r, me.p, mu0, nu, sd, g, olen, Re,
                                  (R)P.init[4], ml_x, ml_scale);
+        // the range test without short-circuit branches (a chain of
+        // exec-mask branches, one per bound, otherwise)
 #pragma unroll
-        for (int k = 0; k < kMlpIn; ++k) ok = ok && ml_x[k] >= a.mlp.flo[k] && ml_x[k] <= a.mlp.fhi[k];
+        for (int k = 0; k < kMlpIn; ++k)
+          ok = ok & (ml_x[k] >= a.mlp.flo[k]) & (ml_x[k] <= a.mlp.fhi[k]);
         use_ml = ok;
         if (!ok) snu = (double)(sa_rest + sa_pointwise<R>(nu, mu0, me.r, sd, olen, Re));
       }
@@ -582,10 +588,17 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R, D> &a) {
       }
       if (a.nusrc) a.nusrc[c] = snu;
       if (fg & F_RESID) {
+        bool nan = (snu != snu);
 #pragma unroll
-        for (int k = 0; k < NV; ++k) { mx[k] = fabs(s[k]); sq[k] = s[k] * s[k]; }
+        for (int k = 0; k < NV; ++k) {
+          mx[k] = fabs(s[k]);
+          sq[k] = s[k] * s[k];
+          nan = nan || (s[k] != s[k]);
+        }
         nmx = fabs(snu);
         nsq = snu * snu;
+        // a NaN residual aborts the march (ResinDomain, NS_Solver.H:355-375)
+        if (nan) report_error(a.err, c, 268);
       }
     }
   }
@@ -652,9 +665,10 @@ k_stage_ml(StageArgs<float, D> a) {
 // Stages with a single substep write into buffer 1; this copies the marched
 // cells back into the home buffer afterwards.
 template <typename R, int D>
-__global__ void k_copy_back(int n, Pack<R, D> src, Pack<R, D> dst, const R *nusrc, R *nudst) {
+__global__ void k_copy_back(int n, Pack<R, D> src, Pack<R, D> dst, const R *nusrc, R *nudst,
+                            const unsigned long long *err) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
+  if (c >= n || halted(err)) return;
   dst.a[c] = src.a[c];
   if (D == 3) dst.e[c] = src.e[c];
   if (nudst) nudst[c] = nusrc[c];
@@ -670,7 +684,7 @@ template <int D>
 __global__ void k_bc(int n, const int4 *dmg, const DmgWall *wall, StateOut<D> o,
                      const DevPhys *phys) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || halted(o.err)) return;
   const int4 e = dmg[i];
   const int g = e.x, ty = e.z;
   const DevPhys &P = phys[e.w];
@@ -734,7 +748,7 @@ template <int D>
 __global__ void k_restrict(int n, const int *ci, const int *fi, StateOut<D> o, int twod,
                            const DevPhys *phys, const int *cellcase) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || halted(o.err)) return;
   const int c = ci[i];
   const DevPhys &P = phys[cellcase[c]];
   double r = 0, ru = 0, rv = 0, rw = 0, re = 0, nu = 0;
@@ -767,7 +781,7 @@ template <int D>
 __global__ void k_prolong(int n, const int *ci, const int *fi, const int *nb27, StateOut<D> o,
                           int twod, const DevPhys *phys, const int *cellcase) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || halted(o.err)) return;
   const int c = ci[i];
   const DevPhys &P = phys[cellcase[c]];
   const int *nb = nb27 + (size_t)c * 27;
@@ -837,7 +851,7 @@ __global__ void k_ib(int n, const int *ndev, const int *cell, const int *intp, c
   // n: list length, or its capacity when the length lives on the device
   // (moving bodies rebuild the list every step)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || (ndev && i >= ndev[0])) return;
+  if (i >= n || (ndev && i >= ndev[0]) || halted(err)) return;
   const DevPhys &P = phys[ibcase[i]];
   double T = 0, p = 0, u = 0, v = 0, w = 0;
   for (int q = 0; q < 8; ++q) {
@@ -922,6 +936,7 @@ __global__ void __launch_bounds__(kTile) k_cfl(const TileInfo *tiles, Pack<doubl
                                                const BlkInfo *blk, const DevPhys *phys,
                                                double *cflpart, unsigned long long *err) {
   __shared__ double red[kTile / 64];
+  if (wg_halted(err)) return;
   const TileInfo ti = tiles[blockIdx.x];
   const BlkInfo &bi = blk[ti.blk];
   const DevPhys &P = phys[bi.case_id];
@@ -950,10 +965,11 @@ __global__ void __launch_bounds__(kTile) k_cfl(const TileInfo *tiles, Pack<doubl
 // (NS_Solver.cpp:198, 263).  One workgroup per case over the case's tile
 // range of each level (ct[case][level] = {first, end}).
 __global__ void k_dt_final(int L, const int *ct, const double *cflpart, double *dt,
-                           const DevPhys *phys, int ncases) {
+                           const DevPhys *phys, int ncases, const unsigned long long *err) {
   const int cs = blockIdx.x;
   if (cs >= ncases) return;
   __shared__ double red[256];
+  if (wg_halted(err)) return;
   double m = 10000.0;
   for (int l = 0; l < L; ++l) {
     const int a = ct[(cs * L + l) * 2], b = ct[(cs * L + l) * 2 + 1];

@@ -231,6 +231,7 @@ __device__ __forceinline__ Motion law_motion(const PitchHeave &Lw, double t, dou
 __global__ void __launch_bounds__(256) k_move_front(MoveArgs a) {
 #pragma clang fp contract(off)
   __shared__ Pose P[IBG_MAX_CASES];
+  if (wg_halted(a.err)) return;
   if (threadIdx.x == 0) {
     a.epoch[0] += 1;
     for (int k = 0; k < 5; ++k) a.cnt[k] = 0;
@@ -291,6 +292,7 @@ __global__ void __launch_bounds__(256) k_move_front(MoveArgs a) {
 // not an IB cell is the reference's abort 263 (Bodymove.cpp:494-544).
 __global__ void __launch_bounds__(256) k_move_dist(MoveArgs a) {
 #pragma clang fp contract(off)
+  if (halted(a.err)) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const bool in = t < a.nw + a.ngw;
   const int i = in ? set_cell(a, t, a.nw) : 0;
@@ -377,6 +379,7 @@ __device__ __forceinline__ void refresh_cmix(const MoveArgs &a, int x) {
 // Layer 1: the non-solid, non-domain-ghost neighbours of each solid cell of
 // the finest level (2D: the in-plane 3x3).
 __global__ void __launch_bounds__(256) k_move_mark1(MoveArgs a) {
+  if (halted(a.err)) return;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= a.cap_s) {
     // the neighbourhood flags around cells whose body changed
@@ -419,6 +422,7 @@ __global__ void __launch_bounds__(256) k_move_mark1(MoveArgs a) {
 // each must be a layer-1 cell (InfectNewIBCells Bodymove.cpp:575-650,
 // abort 314 -- the move step is too large for the incremental update).
 __global__ void __launch_bounds__(256) k_move_mark2(MoveArgs a) {
+  if (halted(a.err)) return;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int ep = a.epoch[0];
   if (j < a.cap_l1) {
@@ -639,6 +643,7 @@ __device__ void move_commit(const MoveArgs &a, int nblk) {
 // Commit and image search in one launch: workgroups [0, ncommit) commit,
 // the others search (they read only the new arrays, not the committed ones).
 __global__ void __launch_bounds__(256) k_move_finish(MoveArgs a, int ncommit) {
+  if (wg_halted(a.err)) return;
   if ((int)blockIdx.x < ncommit) move_commit(a, ncommit);
   else move_image(a, ((int)blockIdx.x - ncommit) * blockDim.x + threadIdx.x);
 }
@@ -655,9 +660,10 @@ __global__ void __launch_bounds__(256) k_move_levelcount(MoveArgs a, int L) {
 }
 
 // t += dt of the step just marched.
-__global__ void k_motion_advance(int ncases, double *t, const double *dt) {
+__global__ void k_motion_advance(int ncases, double *t, const double *dt,
+                                 const unsigned long long *err) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < ncases) t[k] += dt[k];
+  if (k < ncases && !halted(err)) t[k] += dt[k];
 }
 
 }  // namespace ibg

@@ -191,6 +191,7 @@ GPU_FUNCS = {
     "ibg_viscous_flux": (C.c_int, [C.c_void_p, C.c_int32]),
     "ibg_stage": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "ibg_time_march": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
+    "ibg_prepare_march": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "ibg_residual": (C.c_int, [C.c_void_p, f64p, f64p]),
     "ibg_residual_nut": (C.c_int, [C.c_void_p, f64p, f64p]),
     "ibg_forces": (C.c_int, [C.c_void_p, C.POINTER(BodyMotion), f64p]),

@@ -116,6 +116,11 @@ class Engine:
     def Stage(self, ts, subt):
         self._chk(self.lib.ibg_stage(self.ctx, ts, subt), "ibg_stage")
 
+    def prepare_march(self, ts0, nsteps):
+        """Capture the graphs TimeMarching(ts0, nsteps, use_graph=True)
+        replays, without running anything."""
+        self._chk(self.lib.ibg_prepare_march(self.ctx, ts0, nsteps), "ibg_prepare_march")
+
     def TimeMarching(self, ts0, nsteps=1, use_graph=False):
         self._chk(self.lib.ibg_time_march(self.ctx, ts0, nsteps, 1 if use_graph else 0),
                   "ibg_time_march")

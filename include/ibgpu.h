@@ -228,10 +228,17 @@ ibg_status ibg_stage(ibg_ctx *ctx, int32_t ts, int32_t subt);
 /* NS_Solver::TimeMarching(AMR&) NS_Solver.H:657-702 (dt, U_old, BC, viscous,
  * all stages, residual) for nsteps steps starting at ts0, each preceded by
  * InterfaceExchange and IBConditions_Image as in main.cpp:84-99.  When
- * use_graph is nonzero the per-step launch sequence is captured in a HIP
- * graph and replayed.                                                      */
+ * use_graph is nonzero the steps are replayed from captured HIP graphs, one
+ * per run of steps up to a boundary of the level schedule's period
+ * (2^(L-1) steps; any ts0 and nsteps), captured on first use; with a
+ * dt_var_step that is neither 1 nor a multiple of the period the steps run
+ * as direct launches.                                                      */
 ibg_status ibg_time_march(ibg_ctx *ctx, int32_t ts0, int32_t nsteps,
                           int32_t use_graph);
+/* Capture (without running anything) the graphs ibg_time_march(ts0, nsteps,
+ * 1) will replay, so that the call itself is pure replay.  No counterpart
+ * in the reference (its march has no launch sequence to prepare).          */
+ibg_status ibg_prepare_march(ibg_ctx *ctx, int32_t ts0, int32_t nsteps);
 /* Residual of the last stage.  NS_Solver::ResinDomain() NS_Solver.H:264-378:
  * linf[case*5+v] = max |source[0][v]| over the marched cells (unnormalised),
  * l2[case*5+v]   = sqrt(sum source[0][v]^2) over the same cells.            */

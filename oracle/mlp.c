@@ -190,6 +190,14 @@ void orc_mlp_grad_f64(const double *p, const double *xs, const double *t, int64_
  * Returns 0 when the cell is outside the network's domain.                */
 int32_t orc_closure_features(const ibg_mesh_soa *m, const ibg_physics *ph, const double *fv,
                              const double *nut, int32_t cell, double *x, double *z) {
+  double scale;
+  return orc_closure_cell(m, ph, fv, nut, cell, x, z, &scale);
+}
+
+/* The same, also returning the production scale nu * S~ that turns the
+ * network's output into the point-wise SA term (term = z * scale).        */
+int32_t orc_closure_cell(const ibg_mesh_soa *m, const ibg_physics *ph, const double *fv,
+                         const double *nut, int32_t cell, double *x, double *z, double *scale_out) {
   const int L = m->nlevels;
   int b = 0;
   for (int q = 0; q < m->ncases * L; ++q)
@@ -226,6 +234,7 @@ int32_t orc_closure_features(const ibg_mesh_soa *m, const ibg_physics *ph, const
   x[6] = p / P->initvar[4];
   x[7] = log(nul);
   double scale = nu * shat;
+  *scale_out = scale;
   if (!(d > eps && nu > eps && scale > 0.0)) return 0;
   /* point-wise SA term: the full cell source minus its derivative terms,
      evaluated with zero gradients of nu and zero face term               */
@@ -237,4 +246,13 @@ int32_t orc_closure_features(const ibg_mesh_soa *m, const ibg_physics *ph, const
                                 1.0 / h[1], 1.0 / h[2], P->S_over_T_ref, P->Re, 0.0);
   *z = q / scale;
   return 1;
+}
+
+/* orc_closure_cell over a list of cells (ok[k] = its return value). */
+void orc_closure_cells(const ibg_mesh_soa *m, const ibg_physics *ph, const double *fv,
+                       const double *nut, int64_t n, const int32_t *cells, double *x, double *z,
+                       double *scale, int8_t *ok) {
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < n; ++k)
+    ok[k] = (int8_t)orc_closure_cell(m, ph, fv, nut, cells[k], x + 8 * k, z + k, scale + k);
 }

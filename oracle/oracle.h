@@ -126,6 +126,11 @@ void orc_mlp_grad_f64(const double *p, const double *xs, const double *t, int64_
                       double *grad, double *loss);
 int32_t orc_closure_features(const ibg_mesh_soa *m, const ibg_physics *ph, const double *fv,
                              const double *nut, int32_t cell, double *x, double *z);
+int32_t orc_closure_cell(const ibg_mesh_soa *m, const ibg_physics *ph, const double *fv,
+                         const double *nut, int32_t cell, double *x, double *z, double *scale);
+void orc_closure_cells(const ibg_mesh_soa *m, const ibg_physics *ph, const double *fv,
+                       const double *nut, int64_t n, const int32_t *cells, double *x, double *z,
+                       double *scale, int8_t *ok);
 
 /* ------------------------------------------- moving bodies (retag.c)
  * Surface of 2D sections: point offsets from the pivot at zero pose, the

@@ -98,6 +98,8 @@ def lib():
         "orc_mlp_loss_f64": (C.c_double, [f64p, f64p, f64p, C.c_int64]),
         "orc_mlp_grad_f64": (None, [f64p, f64p, f64p, C.c_int64, f64p, f64p]),
         "orc_closure_features": (C.c_int32, [M, P, f64p, f64p, C.c_int32, f64p, f64p]),
+        "orc_closure_cells": (None, [M, P, f64p, f64p, C.c_int64, i32p, f64p, f64p, f64p,
+                                     C.POINTER(C.c_int8)]),
         "orc_move_bodies": (C.c_int32, [M, C.POINTER(OrcSurface), f64p, C.POINTER(OrcRetagOut)]),
     }
     for name, (res, args) in sig.items():
@@ -160,6 +162,20 @@ def closure_features(mesh, fv, nut, cell):
     ok = lib().orc_closure_features(mesh.soa_ptr, mesh.phys, ptr(fv), ptr(nut), int(cell), ptr(x),
                                     C.byref(z))
     return bool(ok), x, z.value
+
+
+def closure_cells(mesh, fv, nut, cells):
+    """Features, plain-closure target and production scale of many cells
+    (orc_closure_cells): x [n][8], z [n], scale [n], ok [n] (bool)."""
+    cells = np.ascontiguousarray(cells, dtype=np.int32)
+    fv = np.ascontiguousarray(fv, dtype=np.float64)
+    nut = np.ascontiguousarray(nut, dtype=np.float64)
+    n = cells.shape[0]
+    x, z, sc = np.zeros((n, 8)), np.zeros(n), np.zeros(n)
+    ok = np.zeros(n, np.int8)
+    lib().orc_closure_cells(mesh.soa_ptr, mesh.phys, ptr(fv), ptr(nut), n, cells.ctypes.data_as(i32p),
+                            ptr(x), ptr(z), ptr(sc), ok.ctypes.data_as(C.POINTER(C.c_int8)))
+    return x, z, sc, ok.astype(bool)
 
 
 class MovingBodies:

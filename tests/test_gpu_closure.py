@@ -15,7 +15,7 @@ import pytest
 
 import ibgpu
 import oracle_ffi as O
-from test_gpu_parity import rel
+from test_gpu_parity import perturbed_state, rel
 
 pytestmark = pytest.mark.gpu
 
@@ -196,3 +196,115 @@ def test_network_gain_over_no_closure(tiny, net):
     dd = lambda F: abs(F[0] - F_ref[0]) / abs(F_ref[0])
     assert dd(F_ml) < 0.01, dd(F_ml)
     assert dd(F_ml) < dd(F_lam), (dd(F_ml), dd(F_lam))
+
+
+# ------------------------------------------------ the network inside the stage
+
+
+def fp32_state(mesh, fv, nut):
+    """The state the mixed-mode stencil buffers hold: roe, u, v, w, p and the
+    SA variable rounded to FP32 (T and e derived again in FP64)."""
+    f = fv.copy()
+    f[:, :5] = fv[:, :5].astype(np.float32).astype(np.float64)
+    for c in range(mesh.ncases):
+        s, e = mesh.blk_start[c * mesh.nlevels], mesh.blk_start[(c + 1) * mesh.nlevels]
+        P = mesh.phys[c]
+        f[s:e, 5] = f[s:e, 4] / (f[s:e, 0] * P.Rg)
+        f[s:e, 6] = P.Cv * f[s:e, 5] + 0.5 * (f[s:e, 1] ** 2 + f[s:e, 2] ** 2 + f[s:e, 3] ** 2)
+    return f, nut.astype(np.float32).astype(np.float64)
+
+
+def range_params(x, z, seed=3, scale=0.3, q=(2.0, 98.0)):
+    """Random weights with statistics fitted to feature rows x and targets z;
+    the network's range is the q-percentile box of the features, so that a
+    part of the cells takes the plain closure."""
+    rng = np.random.default_rng(seed)
+    w = rng.uniform(-scale, scale, O.NPARAM)
+    lo, hi = np.percentile(x, q[0], axis=0), np.percentile(x, q[1], axis=0)
+    stats = np.concatenate([x.mean(0), x.std(0) + 1e-3, lo, hi, [z.mean(), z.std() + 1e-12]])
+    return np.concatenate([w, stats]).astype(np.float32)
+
+
+def network_stage_check(mesh, fv, nut, prm=None, sample=None, seed=7):
+    """One stage of the learned-closure kernel (k_stage_ml) against the CPU
+    mirror, cell by cell: the SA source of every sampled SA cell is either
+    the plain closure (cells outside the network's range) or the plain
+    closure with its point-wise term replaced by network output x production
+    scale; the network/fallback counts equal the oracle's range test.
+    Returns (params, statistics)."""
+    e = ibgpu.Engine(mesh, mode="mixed", nstage=4, turbulence=True)
+    e.set_state(fv)
+    e.set_nut(nut)
+    ts = 0
+    e.InterfaceExchange(ts)
+    e.IBConditions_Image()
+    e.Comptdt(ts)
+    e.BCBoxTreat(ts)
+    e.ComptViscousStress(ts)
+    f32, n32 = fp32_state(mesh, e.get_state(), e.get_nut())   # what the stage reads
+    normal = np.asarray(mesh.kind) == 0
+    sa = normal & (np.asarray(mesh.signdis) >= 0.0) & (np.asarray(mesh.ibidx) < 0)
+    cells = np.flatnonzero(sa).astype(np.int32)
+    x, zt, sc, ok = O.closure_cells(mesh, f32, n32, cells)
+    if prm is None:
+        prm = range_params(x[ok], zt[ok], seed=seed)
+    e.closure_init(1 << 10)
+    e.closure_set_params(prm)
+    e.closure_enable(True)
+    e.closure_counts()
+    e.Stage(ts, 3)
+    g = e.nut_source()
+    on_net, fallback = e.closure_counts()
+    e.close()
+    # plain closure of the oracle on the same FP32-rounded state
+    orc = O.Oracle(mesh, O.cfg(nstage=4))
+    _, mutflux = orc.viscous_flux_sa(f32, n32, ts)
+    plain = orc.mut_source(f32, n32, mutflux, ts)[cells]
+    flo, fhi = prm[O.NPARAM + 16:O.NPARAM + 24], prm[O.NPARAM + 24:O.NPARAM + 32]
+    inside = ok & np.all((x >= flo) & (x <= fhi), axis=1)
+    # a cell whose FP64 feature lies within FP32 rounding of a bound may
+    # fall on either side on the device
+    tol = 1e-5 * np.maximum(1.0, np.abs(x))
+    near = ok & np.any((np.abs(x - flo) <= tol) | (np.abs(x - fhi) <= tol), axis=1)
+    assert on_net + fallback == cells.size, (on_net, fallback, cells.size)
+    assert abs(on_net - int(inside.sum())) <= int(near.sum()), (on_net, int(inside.sum()), int(near.sum()))
+    rng = np.random.default_rng(seed)
+    net_cells = np.flatnonzero(inside & ~near)
+    fb_cells = np.flatnonzero(~inside & ~near)
+    if sample is not None:
+        net_cells = rng.choice(net_cells, min(sample, net_cells.size), replace=False)
+        fb_cells = rng.choice(fb_cells, min(sample, fb_cells.size), replace=False)
+    gs = g[cells]
+    # fallback cells: the FP32 plain closure against the FP64 one
+    ref = 1e-3 * np.median(np.abs(plain)) + 1e-30
+    err_fb = np.abs(gs[fb_cells] - plain[fb_cells]) / (np.abs(plain[fb_cells]) + ref)
+    # network cells: plain - target * scale + network * scale
+    znet = O.mlp_forward(prm, x[net_cells].astype(np.float32)).astype(np.float64)
+    want = plain[net_cells] - zt[net_cells] * sc[net_cells] + znet * sc[net_cells]
+    scale_term = np.abs(znet * sc[net_cells]) + np.abs(plain[net_cells]) + ref
+    err_net = np.abs(gs[net_cells] - want) / scale_term
+    stats = {"cells": int(cells.size), "on_net": on_net, "fallback": fallback,
+             "near_bound": int(near.sum()), "net_checked": int(net_cells.size),
+             "fb_checked": int(fb_cells.size),
+             "fb_max": float(err_fb.max()) if err_fb.size else 0.0,
+             "net_p99": float(np.percentile(err_net, 99)) if err_net.size else 0.0,
+             "net_max": float(err_net.max()) if err_net.size else 0.0,
+             "net_rel_l2": rel(gs[net_cells], want)}
+    print("network stage check:", stats, flush=True)
+    assert net_cells.size > 0 and fb_cells.size > 0
+    assert stats["fb_max"] < 1e-3, stats
+    # BF16 inputs: a feature within FP32 rounding of a BF16 rounding
+    # boundary can round the other way on the device (rare); the bulk
+    # agrees to FP32 accumulation order
+    assert stats["net_p99"] < 1e-3, stats
+    assert np.mean(err_net > 2e-2) < 2e-3, stats
+    assert stats["net_rel_l2"] < 2e-3, stats
+    return prm, stats
+
+
+def test_network_in_stage_matches_cpu_mirror(tiny):
+    print("network inside the stage kernel: tiny case", flush=True)
+    fv = perturbed_state(tiny)
+    rng = np.random.default_rng(4)
+    nut = tiny.phys[0].nut_inf * (1.0 + 0.5 * rng.uniform(-1.0, 1.0, tiny.ncells))
+    network_stage_check(tiny, fv, nut)

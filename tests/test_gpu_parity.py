@@ -266,6 +266,49 @@ def test_ghost_fills_fp64(tiny, phase):
     assert changed.any()
 
 
+@pytest.mark.parametrize("use_graph", [False, True])
+@pytest.mark.parametrize("fault", ["nan", "dt"])
+def test_march_stops_at_first_error(tiny, fault, use_graph):
+    """The march stops at its first error, as the reference aborts there: a
+    NaN planted in one fluid cell is the NaN-residual abort (268) at the
+    last stage of the first step; an oversized time step is the negative
+    density abort (523/611/37) in its first stage.  Every later launch does
+    nothing: the state (NaNs included) and dt after three requested steps
+    equal those after one, with the same code and cell."""
+    fv0 = perturbed_state(tiny)
+    dt_var = 1 << (tiny.nlevels - 1)
+    if fault == "nan":
+        cand = np.flatnonzero((tiny.kind == 0) & (tiny.ibidx < 0) & (tiny.signdis > 0.5))
+        fv0[cand[len(cand) // 2], 0] = np.nan
+    out = []
+    for n in (1, 3):
+        print(f"first error: {fault}, {n} step(s), graph={use_graph}", flush=True)
+        eng = ibgpu.Engine(tiny, mode="fp64", nstage=3, dt_var_step=0 if fault == "dt" else dt_var)
+        eng.set_state(fv0)
+        if fault == "dt":
+            eng.set_dt(np.array([5.0]))
+        with pytest.raises(ibgpu.IBGError) as ei:
+            eng.TimeMarching(0, n, use_graph=use_graph)
+        out.append((ei.value.code, ei.value.cell, eng.get_state(), eng.get_dt()))
+        eng.close()
+    assert out[0][0] == out[1][0] and out[0][1] == out[1][1] >= 0
+    assert out[0][0] == 268 if fault == "nan" else out[0][0] in (523, 611, 37)
+    assert np.array_equal(out[0][2], out[1][2], equal_nan=True)
+    assert np.array_equal(out[0][3], out[1][3])
+    # a new state clears the error: the same context marches again
+    eng = ibgpu.Engine(tiny, mode="fp64", nstage=3, dt_var_step=0 if fault == "dt" else dt_var)
+    eng.set_state(fv0)
+    if fault == "dt":
+        eng.set_dt(np.array([5.0]))
+    with pytest.raises(ibgpu.IBGError):
+        eng.TimeMarching(0, 1, use_graph=use_graph)
+    eng.set_state(perturbed_state(tiny))
+    if fault == "dt":
+        eng.Comptdt(0)   # back to the CFL time step
+    eng.TimeMarching(0, 2, use_graph=use_graph)
+    assert np.all(np.isfinite(eng.get_state()))
+
+
 def test_negative_density_reports_code(tiny):
     """An oversized time step drives a density negative: the engine returns
     the reference's abort code (523/611) with the offending cell instead of
