@@ -79,18 +79,24 @@ def stage_bytes(mode, dim, turb, cells, faces, last=False, first=False):
 
 
 def stage_mix(eng, mesh, mode, ts0, nsteps, nstage, reps):
-    """Stage-kernel launches of the timed steps ts0 .. ts0 + nsteps - 1 (every
-    stage of each, the launch mix of the timed region), each timed with HIP
-    events on the engine's stream: average launch duration, algorithmic
-    bytes and cells per launch."""
-    ms_sum, by_sum, c_sum, n = 0.0, 0.0, 0, 0
+    """Stage-kernel launches of the timed steps ts0 .. ts0 + nsteps - 1, the
+    launch mix of the timed region: a step's launches depend only on its
+    phase in the level schedule (ts mod 2^(L-1)), so each phase present is
+    timed once (every stage, HIP events on the engine's stream) and
+    weighted by how often it occurs.  Returns the average launch duration,
+    algorithmic bytes and cells per launch."""
+    period = 1 << (mesh.nlevels - 1)
+    count = {}
     for ts in range(ts0, ts0 + nsteps):
+        count.setdefault(ts % period, [ts, 0])[1] += 1
+    ms_sum, by_sum, c_sum, n = 0.0, 0.0, 0.0, 0
+    for ts, w in count.values():
         for k in range(nstage):
             ms, nc, nf = eng.bench_stage(ts, k, reps)
-            ms_sum += ms
-            by_sum += stage_bytes(mode, mesh.dim, True, nc, nf, last=(k == nstage - 1), first=(k == 0))
-            c_sum += nc
-            n += 1
+            ms_sum += w * ms
+            by_sum += w * stage_bytes(mode, mesh.dim, True, nc, nf, last=(k == nstage - 1), first=(k == 0))
+            c_sum += w * nc
+            n += w
     return ms_sum / n, by_sum / n, c_sum / n
 
 
@@ -166,6 +172,28 @@ def accuracy_run(eng, mesh, nsteps):
     t_end = float(eng.get_time()[0]) if getattr(eng, "_surface", None) else None
     return {"cd": F[0] / P.force_ref, "cl": F[1] / P.force_ref, "state": eng.get_state(),
             "nut": eng.get_nut(), "t_end": t_end}
+
+
+def regrid_cost(eng, spec, every=20):
+    """One regrid (AMR::PerformAdaptive AMR.cpp:6-209 as a rebuild of the
+    level set at the section's current pose, Engine.regrid) timed from the
+    device time to the context marching on the new mesh, and the cell
+    counts before and after."""
+    import dataclasses
+    eng.synchronize()
+    t0 = time.perf_counter()
+    t = float(eng.get_time()[0])
+    x0, y0, aoa0, amp, omega, heave = MOTION
+    sw = np.sin(omega * t)
+    new = ibgpu.Mesh(dataclasses.replace(spec, aoa=aoa0 + amp * sw, pivot_y=y0 + heave * sw))
+    t1 = time.perf_counter()
+    n0 = eng.mesh.ncells
+    eng.regrid(new)
+    eng.synchronize()
+    t2 = time.perf_counter()
+    return {"every_steps": every, "seconds": t2 - t0, "build_s": t1 - t0, "transfer_upload_s": t2 - t1,
+            "cells_before": int(n0), "cells_after": int(new.ncells), "t": t,
+            "note": "host rebuild of the level set at the pose (not incremental); not in value"}
 
 
 def train_closure(mesh, nstage, args):
@@ -393,6 +421,13 @@ def main():
     _ = eng.get_state()
     _ = eng.get_nut()
     host_io_s = time.perf_counter() - t0
+    regrid = None
+    if motion:
+        # the configuration regrids every 20 steps: one regrid of the moving
+        # section at the law's pose of the device time (host rebuild of the
+        # level set, state moved over by canonical keys, re-upload), timed
+        eng.TimeMarching(0, 32, use_graph=True)
+        regrid = regrid_cost(eng, ibgpu.config2())
     eng.close()
     r = res["ml"]
     print(f"[bench] mixed+ML: {r['rate']:.4e} cell-stage/s, {1e3 * r['seconds'] / args.steps:.3f} ms/step, "
@@ -451,6 +486,9 @@ def main():
         "mixed_over_fp64": mx["rate"] / fp["rate"],
         "accuracy": accuracy_block(acc, fp, mx, ml, args),
         "closure": dict(tinfo, network_cells=on_net, fallback_cells=fallback),
+        "regrid": (dict(regrid, value_incl_regrid=ml["updates"] / (
+            ml["seconds"] + regrid["seconds"] * args.steps / regrid["every_steps"]))
+            if regrid else None),
         "host_boundary": {"state_in_out_ms": 1e3 * host_io_s,
                           "value_incl_state_in_out": ml["updates"] / (ml["seconds"] + host_io_s)},
         "roofline": {"bound": "hbm", "achieved": ml_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",

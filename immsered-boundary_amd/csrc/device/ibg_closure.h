@@ -174,13 +174,15 @@ __global__ void k_closure_pack(const float *p, __bf16 *W1t, __bf16 *W2t, __bf16 
     const int a = t / kH, b = t % kH;
     W2t[t] = (__bf16)(b < 64 ? p[kOffW2 + b * 64 + a] : 0.0f);   // W2t[j][i] = W2[i][j]
     W2[t] = (__bf16)(b < 64 ? p[kOffW2 + a * 64 + b] : 0.0f);    // W2[i][j]
-    // W2p[j][k] = W2[pi(k)][j], pi(32 kt + 8 q + e) = 16 (2 kt + e/4) + 4 q + e%4
-    int pk = 0;
-    if (b < 64) {
-      const int kt = b / 32, q = (b % 32) / 8, e = b % 8;
-      pk = 16 * (2 * kt + e / 4) + 4 * q + e % 4;
-    }
-    W2p[t] = (__bf16)(b < 64 ? p[kOffW2 + pk * 64 + a] : 0.0f);
+  }
+  if (t < 64 * 64) {
+    // W2p row j, physical 16-byte chunk pc holds logical chunk pc ^ (j % 8);
+    // logical column 16 kt + 8 h + e = W2[pi(kt, h, e)][j] with
+    // pi(kt, h, e) = 32 (kt/2) + 8 (2 (kt%2) + e/4) + 4 h + e%4 (mlp_fwd_reg)
+    const int j = t / 64, pc = (t % 64) / 8, e = t % 8;
+    const int lc = pc ^ (j & 7), kt = lc / 2, hh = lc % 2;
+    const int pk = 32 * (kt / 2) + 8 * (2 * (kt % 2) + e / 4) + 4 * hh + e % 4;
+    W2p[t] = (__bf16)p[kOffW2 + pk * 64 + j];
   }
   if (t < 64) {
     w3[t] = (float)(__bf16)p[kOffW3 + t];
@@ -195,11 +197,11 @@ __global__ void k_closure_pack(const float *p, __bf16 *W1t, __bf16 *W2t, __bf16 
 // one row per lane.
 __global__ void __launch_bounds__(256) k_closure_infer(MlpDev M, const float *X, float *Zout, long long n) {
   __shared__ __align__(16) __bf16 sW1t[64 * kX];
-  __shared__ __align__(16) __bf16 sW2p[64 * kH];
+  __shared__ __align__(16) __bf16 sW2p[64 * 64];
   __shared__ __align__(16) float sb1[64], sb2[64];
   __shared__ __align__(16) __bf16 sw3[64];
   for (int i = threadIdx.x; i < 64 * kX; i += blockDim.x) sW1t[i] = M.W1t[i];
-  for (int i = threadIdx.x; i < 64 * kH; i += blockDim.x) sW2p[i] = M.W2p[i];
+  for (int i = threadIdx.x; i < 64 * 64; i += blockDim.x) sW2p[i] = M.W2p[i];
   for (int i = threadIdx.x; i < 64; i += blockDim.x) { sw3[i] = (__bf16)M.w3[i]; sb1[i] = M.b1[i]; sb2[i] = M.b2[i]; }
   __syncthreads();
   const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -210,9 +212,9 @@ __global__ void __launch_bounds__(256) k_closure_infer(MlpDev M, const float *X,
   const int lane = threadIdx.x & 63;
   float yown = 0.0f;
 #pragma unroll 1
-  for (int sb = 0; sb < 4; ++sb) {
+  for (int sb = 0; sb < 2; ++sb) {
     const float y = mlp_fwd_reg(xf, sb, sW1t, sW2p, sb1, sb2, sw3);
-    if ((lane >> 4) == sb) yown = y;
+    if ((lane >> 5) == sb) yown = y;
   }
   if (row < n) Zout[row] = M.zst[0] + M.zst[1] * (yown + M.b3[0]);
 }

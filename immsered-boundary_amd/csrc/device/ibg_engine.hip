@@ -46,6 +46,8 @@ using namespace ibg;
 struct ibg_ctx {
   ibg_config cfg{};
   int dev = 0;                 // HIP device current at ibg_create
+  int ncu = 0;                 // its compute units
+  std::map<std::pair<const void *, size_t>, int> occ;   // resident workgroups per CU
   hipStream_t s = nullptr;
   bool mixed = false;
   int D = 2, NV = 4, ncases = 0, L = 0, nblk = 0;
@@ -127,6 +129,7 @@ struct ibg_ctx {
     double *origin = nullptr, *sd1 = nullptr;
     double *pt_off = nullptr, *pat_dz = nullptr;
     int *pat_pt0 = nullptr, *pat_body = nullptr, *body_p0 = nullptr, *pat_cc = nullptr;
+    int2 *pat_lh = nullptr;    // [npatch] patch range of the patch's body
     int8_t *pat_flip = nullptr;
     Pose *pose = nullptr;
     int *mk1 = nullptr, *mkib = nullptr;  // move stamps: layer 1, IB set
@@ -211,6 +214,14 @@ static std::mutex g_scratch_mu;
 
 namespace {
 
+// byte offset of the learned closure's weights under the persistent
+// schedule: after the stage's face arrays of the largest tile
+size_t ml_resident_offset(const ibg_ctx *c) {
+  size_t per = 4 * sizeof(float) + (c->D == 3 ? sizeof(float) : 0);
+  if (c->turb) per += 2 * sizeof(int);
+  return ((size_t)c->max_tile_faces * per + 15) & ~(size_t)15;
+}
+
 #define HIPCHK(x)                                                        \
   do {                                                                   \
     hipError_t e_ = (x);                                                 \
@@ -282,7 +293,15 @@ size_t tile_smem(const ibg_ctx *c, size_t rsize, bool stage) {
   size_t per = 4 * rsize + (c->D == 3 ? rsize : 0);
   if (c->turb) per += stage ? std::max<size_t>(2 * sizeof(int), rsize) : rsize;
   const size_t b = (size_t)c->max_tile_faces * per;
-  if (stage && c->cl.on) return std::max(b, kStageMlSmem);
+  if (stage && c->cl.on) {
+    // the network's weights: after every face array, resident across the
+    // tiles (persistent schedule), or after the face fluxes over the
+    // face-cell indices (one workgroup per tile)
+    if (c->cfg.schedule == 1) return ml_resident_offset(c) + kStageMlSmem;
+    const size_t m = (c->D == 2 ? ml_smem_offset<float, 2>(c->max_tile_faces)
+                                : ml_smem_offset<float, 3>(c->max_tile_faces)) + kStageMlSmem;
+    return std::max(b, m);
+  }
   return b;
 }
 
@@ -344,10 +363,31 @@ ibg_status launch_stage(ibg_ctx *c, int ntile, int ncell, int subt, bool last, d
   // the step's first stage reads the home buffer S0, the state k_visc_prep reads
   a.fuse_visc = (fuse && subt == 0 && kFuseVisc<R, D>) ? 1 : 0;
   a.va = visc_args<R, D>(c);
-  if constexpr (ML)
-    hipLaunchKernelGGL((k_stage_ml<D>), dim3(ntile), dim3(c->tile), tile_smem(c, sizeof(R), true), c->s, a);
-  else
-    hipLaunchKernelGGL((k_stage<R, D, false>), dim3(ntile), dim3(c->tile), tile_smem(c, sizeof(R), true), c->s, a);
+  const size_t smem = tile_smem(c, sizeof(R), true);
+  a.ntile = ntile;
+  a.mlw_off = (unsigned)ml_resident_offset(c);
+  if (c->cfg.schedule == 1 && ntile >= 8) {
+    // persistent schedule: as many workgroups as the device holds at once
+    // (at least one per XCD, so each XCD's run of tiles has a worker)
+    const void *kfn = ML ? (const void *)k_stage_ml<D, true> : (const void *)k_stage<R, D, false, true>;
+    auto key = std::make_pair(kfn, smem);
+    auto it = c->occ.find(key);
+    if (it == c->occ.end()) {
+      int nb = 0;
+      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kfn, c->tile, smem));
+      it = c->occ.emplace(key, std::max(nb, 1)).first;
+    }
+    const int grid_n = std::max(8, std::min(ntile, it->second * c->ncu));
+    if constexpr (ML)
+      hipLaunchKernelGGL((k_stage_ml<D, true>), dim3(grid_n), dim3(c->tile), smem, c->s, a);
+    else
+      hipLaunchKernelGGL((k_stage<R, D, false, true>), dim3(grid_n), dim3(c->tile), smem, c->s, a);
+  } else {
+    if constexpr (ML)
+      hipLaunchKernelGGL((k_stage_ml<D, false>), dim3(ntile), dim3(c->tile), smem, c->s, a);
+    else
+      hipLaunchKernelGGL((k_stage<R, D, false, false>), dim3(ntile), dim3(c->tile), smem, c->s, a);
+  }
   HIPCHK(hipGetLastError());
   if (c->cfg.nstage == 1) {
     // single-stage scheme: the marched cells go back into the home buffer
@@ -479,6 +519,7 @@ MoveArgs move_args(ibg_ctx *c, unsigned long long *upd, int lmin) {
   a.flags = c->flags;
   a.pt_off = v.pt_off; a.pat_pt0 = v.pat_pt0; a.pat_body = v.pat_body; a.pat_case = c->pat_case;
   a.pat_flip = v.pat_flip; a.pat_dz = v.pat_dz; a.body_p0 = v.body_p0; a.body_case = v.bcase_d;
+  a.pat_lh = v.pat_lh;
   a.pose = v.pose; a.use_law = v.law ? 1 : 0; a.law = v.lawd; a.t = v.t;
   a.mot = c->motion; a.motc = v.motc; a.case_center = c->case_center;
   a.pat_pc = c->pat_pc; a.pat_nv = c->pat_nv; a.pat_area = c->pat_area;
@@ -691,6 +732,7 @@ ibg_status ibg_create(const ibg_config *cfg, ibg_ctx **out) {
   if (cfg->mode != IBG_MODE_FP64 && cfg->mode != IBG_MODE_MIXED) return IBG_ERR_ARG;
   if (cfg->turbulence != 0 && cfg->turbulence != 1) return IBG_ERR_ARG;
   if (cfg->turbulence && !cfg->viscous) return IBG_ERR_ARG;
+  if (cfg->schedule != 0 && cfg->schedule != 1) return IBG_ERR_ARG;
   const int tile = cfg->tile > 0 ? cfg->tile : kTile;
   if (tile % 64 != 0 || tile > kTile) return IBG_ERR_ARG;
   int ndev = 0;
@@ -716,6 +758,9 @@ ibg_status ibg_create(const ibg_config *cfg, ibg_ctx **out) {
   ibg_ctx *c = new ibg_ctx();
   c->cfg = *cfg;
   c->dev = dev;
+  if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      c->ncu < 1)
+    c->ncu = 1;
   c->mixed = (cfg->mode == IBG_MODE_MIXED);
   c->turb = (cfg->turbulence != 0);
   c->nbuf = (cfg->nstage == 1) ? 2 : 3;
@@ -1654,6 +1699,14 @@ ibg_status ibg_upload_surface(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_surfa
   UP(v.pat_flip, flip); UP(v.pat_dz, dz); UP(v.body_p0, bp0); UP(v.pat_cc, cc);
   UP(v.bcase_d, bcase);
   {
+    std::vector<int2> lh(np);
+    for (int p = 0; p < np; ++p) {
+      const int b = sf->pat_body[p];
+      lh[p] = make_int2(sf->body_p0[b], sf->body_p0[b + 1]);
+    }
+    UP(v.pat_lh, lh);
+  }
+  {
     // ghosts by level, finest first: the ghosts of the levels >= l are a
     // prefix of this list
     std::vector<int> gl;
@@ -2027,6 +2080,9 @@ ibg_status ibg_bench_stage(ibg_ctx *c, int32_t ts, int32_t subt, int32_t reps, d
     for (const TileInfo &t : h) { nc += t.ncell; nf += t.nface; }
   }
   ibg_status st;
+  // a measurement, not a march: the repeated stages must not stop at an
+  // error they produce themselves (the state is not advanced consistently)
+  if ((st = reset_err(c))) return st;
   HIPCHK(hipEventRecord(c->ev0, c->s));
   for (int r = 0; r < reps; ++r)
     if ((st = run_stage(c, ts, subt, nullptr, nullptr, fused_visc(c)))) return st;
@@ -2085,23 +2141,28 @@ ibg_status ibg_autotune_tile(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physic
       nu[g] = ph[cs].nut_inf;
     }
   double tbest = 1e300;
-  int kbest = 0;
+  int kbest = 0, sbest = 0;
   ibg_status st;
   for (int k = 0; k < ncand; ++k) {
     c->cfg.tile = cand[k];
     c->tile = cand[k];
+    c->cfg.schedule = 0;
     if ((st = ibg_upload_mesh(c, m, ph))) return st;
     if ((st = ibg_set_state(c, fv.data(), m->ncells))) return st;
     if (c->turb && (st = ibg_set_nut(c, nu.data(), m->ncells))) return st;
     if ((st = run_dt(c))) return st;
-    double t = 0.0;
-    if ((st = ibg_bench_stage(c, 0, 0, 2, &t, nullptr, nullptr))) return st;   // warm
-    if ((st = ibg_bench_stage(c, 0, 0, reps, &t, nullptr, nullptr))) return st;
-    if (ms) ms[k] = t;
-    if (t < tbest) { tbest = t; kbest = k; }
+    for (int sch = 0; sch < 2; ++sch) {
+      c->cfg.schedule = sch;
+      double t = 0.0;
+      if ((st = ibg_bench_stage(c, 0, 0, 2, &t, nullptr, nullptr))) return st;   // warm
+      if ((st = ibg_bench_stage(c, 0, 0, reps, &t, nullptr, nullptr))) return st;
+      if (ms) ms[2 * k + sch] = t;
+      if (t < tbest) { tbest = t; kbest = k; sbest = sch; }
+    }
   }
   c->cfg.tile = cand[kbest];
   c->tile = cand[kbest];
+  c->cfg.schedule = sbest;
   if ((st = ibg_upload_mesh(c, m, ph))) return st;
   if (best) *best = cand[kbest];
   return IBG_OK;
@@ -2195,7 +2256,7 @@ ibg_status ibg_closure_init(ibg_ctx *c, int64_t capacity, uint64_t seed) {
   if ((st = cl_alloc(c, &L.W1t, (size_t)64 * kX))) return st;
   if ((st = cl_alloc(c, &L.W2t, (size_t)64 * kH))) return st;
   if ((st = cl_alloc(c, &L.W2, (size_t)64 * kH))) return st;
-  if ((st = cl_alloc(c, &L.W2p, (size_t)64 * kH))) return st;
+  if ((st = cl_alloc(c, &L.W2p, (size_t)64 * 64))) return st;
   if ((st = cl_alloc(c, &L.w3, 64))) return st;
   if ((st = cl_alloc(c, &L.b1, 64))) return st;
   if ((st = cl_alloc(c, &L.b2, 64))) return st;

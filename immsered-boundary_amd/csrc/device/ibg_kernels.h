@@ -114,9 +114,11 @@ struct ViscArgs {
 // LDS, then per cell Rv = sum_d (Fv_hi - Fv_lo).  With the SA closure the
 // face flux of the working variable (mutflux, :534-542) is reduced the same
 // way into sum_d (mutflux_hi - mutflux_lo) / V / Re (:791-800).
+// halt: an LDS flag, set by thread 0 before the call from the error word,
+// tested after the face phase's barrier: a halted workgroup writes nothing.
 template <typename R, int D>
 __device__ __forceinline__ void visc_prep_tile(const ViscArgs<R, D> &a, const TileInfo &ti,
-                                               unsigned char *smem_raw) {
+                                               unsigned char *smem_raw, const int *halt) {
   constexpr int NV = D + 2;
   using V = typename V4<R>::T;
   V *fl4 = reinterpret_cast<V *>(smem_raw);
@@ -160,6 +162,7 @@ __device__ __forceinline__ void visc_prep_tile(const ViscArgs<R, D> &a, const Ti
     }
   }
   __syncthreads();
+  if (*halt) return;
   const int i = threadIdx.x;
   if (i >= ti.ncell) return;   // no barrier follows in this function
   const int c = ti.cell0 + i;
@@ -202,9 +205,10 @@ __device__ __forceinline__ void visc_prep_tile(const ViscArgs<R, D> &a, const Ti
 template <typename R, int D>
 __global__ void __launch_bounds__(kTile) k_visc_prep(ViscArgs<R, D> a) {
   extern __shared__ __align__(32) unsigned char smem_raw[];
-  if (wg_halted(a.err)) return;
+  __shared__ int halt;
+  if (threadIdx.x == 0) halt = halted(a.err) ? 1 : 0;
   const TileInfo ti = a.tiles[xcd_tile()];
-  visc_prep_tile<R, D>(a, ti, smem_raw);
+  visc_prep_tile<R, D>(a, ti, smem_raw, &halt);
 }
 
 template <typename R, int D>
@@ -243,6 +247,12 @@ struct StageArgs {
   // (the work of k_visc_prep, into the same Rv / mutf arrays)
   int fuse_visc;
   ViscArgs<R, D> va;
+  // persistent schedule (the P instantiations): gridDim.x workgroups loop
+  // over the ntile tiles of the launch (XCD-contiguous runs), with the
+  // learned closure's weights staged once per workgroup at byte mlw_off of
+  // the dynamic LDS
+  int ntile;
+  unsigned mlw_off;
 };
 
 // The fused first stage for the 2D FP64 variant only.  There the viscous
@@ -261,10 +271,17 @@ constexpr int kRpart = 12;   // per tile: max[5], sumsq[5], SA max, SA sumsq
 // the reader sums the slots
 constexpr int kMlSlots = 64, kMlSlotStride = 16;
 
-// LDS of the learned-closure part of k_stage (aliases the face arrays,
-// which are dead by then): the weights (W1t as 64 rows of its 8 used
-// columns, W2p, b1, b2, w3).
-constexpr size_t kStageMlSmem = (size_t)64 * 8 * 2 + (size_t)64 * kH * 2 + 3 * 64 * 4;
+// LDS of the learned-closure part of k_stage, after the face fluxes (it
+// overwrites the face-cell indices, dead by then; the fluxes are read after
+// the network): W1t as 64 rows of its 8 used columns, W2p (64 rows of 64,
+// 16-byte chunks XOR-swizzled by the row, see mlp_fwd_reg), b1, b2 and the
+// bf16 w3.
+constexpr size_t kStageMlSmem = (size_t)64 * 8 * 2 + (size_t)64 * 64 * 2 + 2 * 64 * 4 + 64 * 2;
+// byte offset of that region for a tile of nface faces (16-byte aligned)
+template <typename R, int D>
+__host__ __device__ constexpr size_t ml_smem_offset(int nface) {
+  return ((sizeof(typename V4<R>::T) + (D == 3 ? sizeof(R) : 0)) * (size_t)nface + 15) & ~(size_t)15;
+}
 
 // One workgroup per tile of Morton-consecutive normal cells of one
 // (case, level).  Phase 1: every face touching the tile — MUSCL + all-speed
@@ -283,21 +300,67 @@ constexpr size_t kStageMlSmem = (size_t)64 * 8 * 2 + (size_t)64 * kH * 2 + 3 * 6
 // one has already advanced within the launch.  U_old is cons(W64): the
 // master holds the step-start state until the last stage rewrites it, and
 // each cell's master entry is read and written by the same thread.
-template <typename R, int D, bool ML>
-__device__ __forceinline__ void stage_tile(const StageArgs<R, D> &a) {
+// The network's weights to LDS at dst (W1t rows' 8 used columns, then W2p
+// as stored, b1, b2, the bf16 w3), in 16-byte chunks with all loads issued
+// before the stores.  Cooperative over the workgroup; the caller syncs.
+__device__ __forceinline__ void ml_fill_weights(const MlpDev &M, unsigned char *dst_raw) {
+  __bf16 *mW1t = reinterpret_cast<__bf16 *>(dst_raw);   // [64][8]
+  __bf16 *mW2p = mW1t + 64 * 8;                          // [64][64]
+  float *mb1 = reinterpret_cast<float *>(mW2p + 64 * 64), *mb2 = mb1 + 64;
+  __bf16 *mw3 = reinterpret_cast<__bf16 *>(mb2 + 64);
+  constexpr int kW1Rows = 64, kWChunks = kW1Rows + 64 * 64 * 2 / 16;
+  static_assert(kWChunks == 576, "weight chunk count");
+  uint4 *dst = reinterpret_cast<uint4 *>(mW1t);   // W2p follows W1t in LDS
+  const uint4 *s2 = reinterpret_cast<const uint4 *>(M.W2p);   // chunks [64, 576)
+  // chunk id of W1t or W2p; both indices stay in range whichever is read
+  auto chunk = [&](int id) {
+    const uint4 x1 = *reinterpret_cast<const uint4 *>(M.W1t + (size_t)(id < kW1Rows ? id : 0) * kX);
+    const uint4 x2 = s2[id < kW1Rows ? 0 : id - kW1Rows];
+    return id < kW1Rows ? x1 : x2;
+  };
+  if (blockDim.x == 256) {
+    // chunks t, t+256 always exist; t+512 for t < 64
+    const int t = threadIdx.x;
+    const uint4 c0 = chunk(t);
+    const uint4 c1 = s2[t + 256 - kW1Rows];
+    const uint4 c2 = s2[(t < kWChunks - 512 ? t + 512 : 512) - kW1Rows];
+    dst[t] = c0;
+    dst[t + 256] = c1;
+    if (t < kWChunks - 512) dst[t + 512] = c2;
+  } else {
+    for (int id = threadIdx.x; id < kWChunks; id += blockDim.x) dst[id] = chunk(id);
+  }
+  for (int i = threadIdx.x; i < 64; i += blockDim.x) {
+    mw3[i] = (__bf16)M.w3[i]; mb1[i] = M.b1[i]; mb2[i] = M.b2[i];
+  }
+}
+
+// One tile.  WRES: the network's weights are resident at a.mlw_off
+// (persistent schedule); otherwise the ML instantiation stages them after
+// the face fluxes, over the face-cell indices (dead by then).
+template <typename R, int D, bool ML, bool WRES>
+__device__ __forceinline__ void stage_tile(const StageArgs<R, D> &a, int tile) {
+  constexpr bool wres = WRES;
   constexpr int NV = D + 2;
   constexpr bool kMixed = std::is_same<R, float>::value;
   using V = typename V4<R>::T;
   extern __shared__ __align__(32) unsigned char smem_raw[];
-  if (wg_halted(a.err)) return;
-  const int tile = xcd_tile();
+  // the error word is read while the face phase runs and tested after its
+  // barrier (the face phase writes only LDS)
+  __shared__ int halt;
+  __shared__ unsigned mlc[2];   // learned closure: network / fallback cells of this tile
+  if (threadIdx.x == 0) {
+    halt = halted(a.err) ? 1 : 0;
+    if (ML) { mlc[0] = 0u; mlc[1] = 0u; }
+  }
   const TileInfo ti = a.tiles[tile];
   if constexpr (kFuseVisc<R, D>) {
     if (a.fuse_visc) {
       // Rv and mutf of this tile's cells from the step-start state (the
       // buffer this stage reads); each thread reads back only its own
       // cell's entries in phase 2, so no other workgroup is involved
-      visc_prep_tile<R, D>(a.va, ti, smem_raw);
+      visc_prep_tile<R, D>(a.va, ti, smem_raw, &halt);
+      if (halt) return;  // uniform: read after the barrier in visc_prep_tile
       __syncthreads();   // the face arrays are reused by phase 1
     }
   }
@@ -308,7 +371,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R, D> &a) {
   int *fL = reinterpret_cast<int *>(smem_raw + (sizeof(V) + (D == 3 ? sizeof(R) : 0)) * (size_t)ti.nface);
   int *fR = fL + ti.nface;
   __shared__ double red[kRpart][kTile / 64];
-  __shared__ unsigned mlc[2];   // learned closure: network / fallback cells of this tile
 
   const BlkInfo &bi = a.blk[ti.blk];
   const DevPhys &P = a.phys[bi.case_id];
@@ -338,6 +400,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R, D> &a) {
     }
   }
   __syncthreads();
+  if (halt) return;   // uniform: the flag was written before the barrier
 
   const double level_ts = a.alpha * a.dt[bi.case_id] * bi.step_num;
   const double inv_vol = 1.0 / bi.vol;
@@ -358,8 +421,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R, D> &a) {
   bool use_ml = false;
   R ml_x[kMlpIn];
   R ml_scale = R(0), sa_rest = R(0), sa_nu = R(0);
-  if (live) {
-    int sl[2 * D];
+  // the cell's face slots in the tile's face list
+  auto load_slots = [&](int *sl) {
     if (D == 2) {
       const ushort4 s4 = *reinterpret_cast<const ushort4 *>(a.slots + (size_t)c * 4);
       sl[0] = s4.x; sl[1] = s4.y; sl[2] = s4.z; sl[3] = s4.w;
@@ -367,8 +430,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R, D> &a) {
 #pragma unroll
       for (int q = 0; q < 2 * D; ++q) sl[q] = a.slots[(size_t)c * 2 * D + q];
     }
-    // ComptSource: per direction (F_lo - F_hi), plus the frozen viscous
-    // residual sum_d (Fv_hi - Fv_lo)
+  };
+  int sl[2 * D];
+  if (live) load_slots(sl);
+  // ComptSource: per direction (F_lo - F_hi), plus the frozen viscous
+  // residual sum_d (Fv_hi - Fv_lo).  The learned-closure instantiation
+  // forms it after the network (slots read again), so that neither the sum
+  // nor the slots are held in registers across the network (they spill).
+  auto residual = [&]() {
+    if constexpr (ML) load_slots(sl);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const V lo = fl4[sl[2 * d]], hi = fl4[sl[2 * d + 1]];
@@ -378,12 +448,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R, D> &a) {
       s[3] += (double)lo.w - (double)hi.w;
       if (D == 3) s[4] += (double)fl1[sl[2 * d]] - (double)fl1[sl[2 * d + 1]];
     }
-    {
-      R rv[NV];
-      vload<R, D>(a.Rv, c, rv);
+    R rv[NV];
+    vload<R, D>(a.Rv, c, rv);
 #pragma unroll
-      for (int k = 0; k < NV; ++k) s[k] += (double)rv[k];
-    }
+    for (int k = 0; k < NV; ++k) s[k] += (double)rv[k];
+  };
+  if (live) {
+    if constexpr (!ML) residual();
     // ComptMutSource for a non-solid updated cell (zero otherwise: the
     // reference never writes mutsource of solid cells)
     if (a.turb && (fg & F_UPDATE) && (fg & F_SA)) {
@@ -444,59 +515,32 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R, D> &a) {
   }
   // ---- learned closure: the network for the cells in its range
   if constexpr (ML) {
-    if (threadIdx.x == 0) { mlc[0] = 0u; mlc[1] = 0u; }
-    __syncthreads();   // every phase-2a read of the face arrays is done
-    __bf16 *mW1t = reinterpret_cast<__bf16 *>(smem_raw);   // [64][8]
-    __bf16 *mW2p = mW1t + 64 * 8;
-    float *mb1 = reinterpret_cast<float *>(mW2p + 64 * kH), *mb2 = mb1 + 64;
-    __bf16 *mw3 = reinterpret_cast<__bf16 *>(mb2 + 64);
-    {
-      // weights to LDS in 16-byte chunks, all loads issued before the
-      // stores: the first 16 bytes of each W1t row (its 8 used columns),
-      // then W2p
-      constexpr int kW1Rows = 64, kWChunks = kW1Rows + 64 * kH * 2 / 16;
-      static_assert(kWChunks == 640, "weight chunk count");
-      uint4 *dst = reinterpret_cast<uint4 *>(mW1t);   // W2p follows W1t in LDS
-      const uint4 *s2 = reinterpret_cast<const uint4 *>(a.mlp.W2p);   // chunks [64, 640)
-      // chunk id of W1t or W2p; both indices stay in range whichever is read
-      auto chunk = [&](int id) {
-        const uint4 x1 = *reinterpret_cast<const uint4 *>(a.mlp.W1t + (size_t)(id < kW1Rows ? id : 0) * kX);
-        const uint4 x2 = s2[id < kW1Rows ? 0 : id - kW1Rows];
-        return id < kW1Rows ? x1 : x2;
-      };
-      if (blockDim.x == 256) {
-        // chunks t, t+256 always exist; t+512 for t < 128
-        const int t = threadIdx.x;
-        const uint4 c0 = chunk(t);
-        const uint4 c1 = s2[t + 256 - kW1Rows];
-        const uint4 c2 = s2[(t < kWChunks - 512 ? t + 512 : 512) - kW1Rows];
-        dst[t] = c0;
-        dst[t + 256] = c1;
-        if (t < kWChunks - 512) dst[t + 512] = c2;
-      } else {
-        for (int id = threadIdx.x; id < kWChunks; id += blockDim.x) dst[id] = chunk(id);
-      }
-      for (int i = threadIdx.x; i < 64; i += blockDim.x) {
-        mw3[i] = (__bf16)a.mlp.w3[i]; mb1[i] = a.mlp.b1[i]; mb2[i] = a.mlp.b2[i];
-      }
+    unsigned char *wbase = smem_raw + (wres ? (size_t)a.mlw_off : ml_smem_offset<R, D>(ti.nface));
+    if (!wres) {
+      __syncthreads();   // every phase-2a read of the face-cell indices is done
+      ml_fill_weights(a.mlp, wbase);
+      __syncthreads();
     }
-    __syncthreads();
+    __bf16 *mW1t = reinterpret_cast<__bf16 *>(wbase);   // [64][8]
+    __bf16 *mW2p = mW1t + 64 * 8;                        // [64][64]
+    float *mb1 = reinterpret_cast<float *>(mW2p + 64 * 64), *mb2 = mb1 + 64;
+    __bf16 *mw3 = reinterpret_cast<__bf16 *>(mb2 + 64);
     const int lane = threadIdx.x & 63;
     bf16x8_t xf;
 #pragma unroll
     for (int k = 0; k < kMlpIn; ++k)
       xf[k] = (__bf16)(use_ml ? ((float)ml_x[k] - a.mlp.fmean[k]) * a.mlp.finv[k] : 0.0f);
     float yown = 0.0f;
-    // one 16-cell block per pass, skipped (wave-uniformly) when none of its
+    // one 32-cell block per pass, skipped (wave-uniformly) when none of its
     // cells is on the network; the compiler barrier keeps the weight
-    // fragments from being hoisted out of the loop (96 registers if they were)
+    // fragments from being hoisted out of the loop
     const unsigned long long on = __ballot(use_ml);
 #pragma unroll 1
-    for (int sb = 0; sb < 4; ++sb) {
+    for (int sb = 0; sb < 2; ++sb) {
       asm volatile("" ::: "memory");
-      if (((on >> (16 * sb)) & 0xffffull) == 0ull) continue;
+      if (((on >> (32 * sb)) & 0xffffffffull) == 0ull) continue;
       const float y = mlp_fwd_reg<8>(xf, sb, mW1t, mW2p, mb1, mb2, mw3);
-      if ((lane >> 4) == sb) yown = y;
+      if ((lane >> 5) == sb) yown = y;
     }
     if (use_ml) {
       const float z = a.mlp.zst[0] + a.mlp.zst[1] * (yown + a.mlp.b3[0]);
@@ -511,6 +555,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R, D> &a) {
         atomicAdd(&mlc[1], (unsigned)__popcll(nf));
       }
     }
+    if (live) residual();   // the face fluxes are still in LDS
   }
   // ---- phase 2b: update
   if (live) {
@@ -648,18 +693,49 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R, D> &a) {
   }
 }
 
-template <typename R, int D, bool ML>
-__global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
-  stage_tile<R, D, ML>(a);
+// Schedule P (a separate instantiation each, so that the loop costs the
+// one-workgroup-per-tile kernels no registers).  Persistent: XCD x (blocks
+// b = x mod 8) owns the x-th contiguous run of the launch's tiles and its
+// workgroups take them in turn, so concurrently processed tiles are
+// Morton neighbours sharing that XCD's L2.
+template <typename R, int D, bool ML, bool P>
+__device__ __forceinline__ void stage_launch(const StageArgs<R, D> &a) {
+  if constexpr (!P) {
+    stage_tile<R, D, ML, false>(a, xcd_tile());
+    return;
+  }
+  extern __shared__ __align__(32) unsigned char smem_raw[];
+  if constexpr (ML) {
+    ml_fill_weights(a.mlp, smem_raw + a.mlw_off);
+    __syncthreads();
+  }
+  constexpr int kXcd = 8;
+  const int b = (int)blockIdx.x, x = b % kXcd, G = (int)gridDim.x;
+  const int per = a.ntile / kXcd, rem = a.ntile % kXcd;
+  const int t0 = x * per + (x < rem ? x : rem), t1 = t0 + per + (x < rem ? 1 : 0);
+  const int nwk = (G - x + kXcd - 1) / kXcd;   // workgroups of XCD x
+  for (int t = t0 + b / kXcd; t < t1; t += nwk) {
+    // nothing is hoisted out of the loop: values kept live across the
+    // tiles would cost the loop body registers (and occupancy)
+    asm volatile("" ::: "memory");
+    stage_tile<R, D, ML, true>(a, t);
+    __syncthreads();   // the next tile rewrites the face arrays
+  }
 }
 
-// The learned-closure stage with the scalar registers held to what eight
-// waves per SIMD allow (its many parameter pointers otherwise take 102
-// SGPRs, which caps the occupancy at seven).
-template <int D>
-__global__ void __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(8, 8)))
+template <typename R, int D, bool ML, bool P = false>
+__global__ void __launch_bounds__(kTile) k_stage(StageArgs<R, D> a) {
+  stage_launch<R, D, ML, P>(a);
+}
+
+// The learned-closure stage at seven waves per SIMD: its network needs more
+// than the 64 registers eight waves allow (held to 64, the compiler spills
+// to scratch and the launch takes 112 instead of 93 us on the 2.6M-cell
+// NACA0015 case).
+template <int D, bool P = false>
+__global__ void __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(7, 8)))
 k_stage_ml(StageArgs<float, D> a) {
-  stage_tile<float, D, true>(a);
+  stage_launch<float, D, true, P>(a);
 }
 
 // Stages with a single substep write into buffer 1; this copies the marched

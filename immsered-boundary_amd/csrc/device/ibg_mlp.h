@@ -49,7 +49,7 @@ struct MlpDev {
   const __bf16 *W1t;   // [64][kX]  W1 transposed, inputs padded to 32
   const __bf16 *W2t;   // [64][kH]  W2 transposed (forward B operand)
   const __bf16 *W2;    // [64][kH]  W2 (backward B operand)
-  const __bf16 *W2p;   // [64][kH]  W2p[j][k] = W2[pi(k)][j] (transposed forward, see mlp_fwd_reg)
+  const __bf16 *W2p;   // [64][64]  W2 permuted and swizzled for the register forward (mlp_fwd_reg)
   const float *w3;     // [64] bf16-rounded values
   const float *b1, *b2, *b3;
   const float *fmean, *fstd;   // [8] feature standardisation
@@ -168,65 +168,96 @@ __device__ __forceinline__ void mlp_fwd16(const __bf16 *Xs, __bf16 *H1s, __bf16 
   __syncthreads();
 }
 
-// Transposed forward with the activations in registers (the stage kernel).
-// The wave handles 16 cells per call: the cells of lanes 16*sb .. 16*sb+15,
-// each lane holding its own cell's 8 standardised features (bf16x8 xf).
-// Layer 1 computes Z1^T = W1^T X^T: every K group of the A operand holds W1
-// and only the K group sb of the B operand is nonzero (lane (r, q) = lane
-// r + 16 q contributes its own feature vector when q == sb).  The output of lane (r, q)
-// is hidden units 16 nt + 4 q + v of cell r of the block.  Layer 2 takes its
-// B operand straight from those registers: the K order of layer 2 is the
-// permutation pi(32 kt + 8 q + e) = 16 (2 kt + e/4) + 4 q + e%4 of the
-// hidden units, applied to W2 once (W2p).  Layer 3 takes the 16 products of
-// a lane in bf16 pairs (v_dot2_f32_bf16, FP32 accumulate) and reduces over
-// the 4 lanes of a cell.  Returns y (b3 not included) of cell r in every
-// lane (r, q).  No LDS for activations, no barriers.  W1t/W2p/b1/b2 and the
-// bf16 w3 come from LDS.
-// W1S: row stride (bf16) of W1t; only the first 8 columns of a row are read.
+// Transposed forward with the activations in registers (the stage kernel),
+// on v_mfma_f32_32x32x16_bf16 (32x32 output tiles, K = 16):
+//   A fragment  lane l: row l%32, k = 8*(l/32) .. +7
+//   B fragment  lane l: col l%32, k = 8*(l/32) .. +7
+//   D           lane l: col l%32, rows 8*g + 4*(l/32) + v (acc[4g+v], g,v < 4)
+// The wave's 64 cells are two blocks of 32: block sb is the cells of the
+// lanes with l/32 == sb, each lane holding its own cell's 8 standardised
+// features (bf16x8 xf).  Lane (n, h) = lane n + 32 h.
+// Layer 1, Z1^T = W1^T X^T per 32 hidden units t: both K halves of the A
+// operand hold W1 and only the K half sb of the B operand is nonzero, so
+// column n is cell 32 sb + n.  Lane (n, h) receives hidden units
+// 32 t + 8 g + 4 h + v of that cell; ReLU and bf16 pairs turn them straight
+// into the layer-2 B fragments: K chunk kt (16 inputs) holds, in lane
+// (n, h), the units pi(kt, h, e) = 32 (kt/2) + 8 (2 (kt%2) + e/4) + 4 h + e%4,
+// e < 8, the order in which W2p stores each row: logical column
+// 16 kt + 8 h + e of row j holds W2[pi(kt, h, e)][j], and the 16-byte chunk
+// of logical index q of row j sits at chunk q ^ (j % 8) (rows of 128 bytes:
+// the eight rows a group of eight lanes reads then fall on distinct banks).
+// Layer 2 runs two output tiles of 32 units one after the other, each followed by its ReLU and the output layer as bf16 v_dot2
+// with FP32 accumulation; the two halves of a cell are summed across lanes
+// n and n + 32.  Returns y (b3 not included) of cell 32 sb + n in lanes
+// (n, 0) and (n, 1).  Per block: 10 MFMA, weights from LDS (W1t rows of
+// stride W1S of which the first 8 columns are read, W2p as above),
+// biases and the bf16 w3 from LDS; no LDS for activations, no barriers.
+typedef float floatx16_t __attribute__((ext_vector_type(16)));
+__device__ __forceinline__ floatx16_t mfma32(bf16x8_t a, bf16x8_t b, floatx16_t c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+// 16 units of a lane: acc[4g + v] of 32-unit tile `base` start at unit
+// base + 8 g + 4 h + v (bias and w3 reads, four 4-vectors)
+__device__ __forceinline__ floatx16_t mlp_bias16(const float *b, int base, int h) {
+  floatx16_t c;
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const float4 bb = *reinterpret_cast<const float4 *>(b + base + 8 * g + 4 * h);
+    c[4 * g] = bb.x; c[4 * g + 1] = bb.y; c[4 * g + 2] = bb.z; c[4 * g + 3] = bb.w;
+  }
+  return c;
+}
+
 template <int W1S = kX>
 __device__ __forceinline__ float mlp_fwd_reg(bf16x8_t xf, int sb, const __bf16 *W1t,
                                              const __bf16 *W2p, const float *b1, const float *b2,
                                              const __bf16 *w3b) {
   const int lane = threadIdx.x & 63;
-  const int r = lane & 15, q = lane >> 4;
+  const int n = lane & 31, h = lane >> 5;
   const bf16x8_t zero8 = {};
-  const bf16x8_t xm = (q == sb) ? xf : zero8;
-  floatx4_t h1[4];
+  const bf16x8_t xm = (h == sb) ? xf : zero8;
+  // layer 1 -> the four layer-2 B fragments
+  bf16x8_t b2f[4];
 #pragma unroll
-  for (int nt = 0; nt < 4; ++nt) {
-    const float4 bb = *reinterpret_cast<const float4 *>(b1 + 16 * nt + 4 * q);
-    h1[nt] = floatx4_t{bb.x, bb.y, bb.z, bb.w};
-    const bf16x8_t aw = *reinterpret_cast<const bf16x8_t *>(W1t + (16 * nt + r) * W1S);
-    h1[nt] = mfma16(aw, xm, h1[nt]);
-  }
-  // relu -> bf16, packed as the layer-2 B fragments (kt = 0: tiles 0,1; kt = 1: tiles 2,3)
-  bf16x8_t b2f[2];
+  for (int t = 0; t < 2; ++t) {
+    // compiler barriers between the tiles hold the LDS reads of one tile
+    // back until the previous tile is done (hoisted, they exceed the
+    // register budget of eight waves per SIMD)
+    asm volatile("" ::: "memory");
+    const bf16x8_t aw = *reinterpret_cast<const bf16x8_t *>(W1t + (32 * t + n) * W1S);
+    const floatx16_t z = mfma32(aw, xm, mlp_bias16(b1, 32 * t, h));
 #pragma unroll
-  for (int kt = 0; kt < 2; ++kt) {
-    const floatx4_t &u = h1[2 * kt], &w = h1[2 * kt + 1];
-    const bf16x2_t p0 = mlp_act_pk(u[0], u[1]), p1 = mlp_act_pk(u[2], u[3]);
-    const bf16x2_t p2 = mlp_act_pk(w[0], w[1]), p3 = mlp_act_pk(w[2], w[3]);
-    b2f[kt] = __builtin_shufflevector(__builtin_shufflevector(p0, p1, 0, 1, 2, 3),
-                                      __builtin_shufflevector(p2, p3, 0, 1, 2, 3), 0, 1, 2, 3, 4,
-                                      5, 6, 7);
+    for (int c2 = 0; c2 < 2; ++c2) {
+      const int o = 8 * c2;
+      const bf16x2_t p0 = mlp_act_pk(z[o], z[o + 1]), p1 = mlp_act_pk(z[o + 2], z[o + 3]);
+      const bf16x2_t p2 = mlp_act_pk(z[o + 4], z[o + 5]), p3 = mlp_act_pk(z[o + 6], z[o + 7]);
+      b2f[2 * t + c2] = __builtin_shufflevector(__builtin_shufflevector(p0, p1, 0, 1, 2, 3),
+                                                __builtin_shufflevector(p2, p3, 0, 1, 2, 3), 0, 1,
+                                                2, 3, 4, 5, 6, 7);
+    }
   }
+  // layer 2 and the output layer, one 32-unit tile at a time
   float y = 0.0f;
 #pragma unroll
-  for (int nt = 0; nt < 4; ++nt) {
-    const float4 bb = *reinterpret_cast<const float4 *>(b2 + 16 * nt + 4 * q);
-    floatx4_t acc = floatx4_t{bb.x, bb.y, bb.z, bb.w};
+  for (int nt = 0; nt < 2; ++nt) {
+    asm volatile("" ::: "memory");
+    floatx16_t acc = mlp_bias16(b2, 32 * nt, h);
 #pragma unroll
-    for (int kt = 0; kt < 2; ++kt) {
-      const bf16x8_t aw = *reinterpret_cast<const bf16x8_t *>(W2p + (16 * nt + r) * kH + 32 * kt + 8 * q);
-      acc = mfma16(aw, b2f[kt], acc);
+    for (int kt = 0; kt < 4; ++kt) {
+      const bf16x8_t aw =
+          *reinterpret_cast<const bf16x8_t *>(W2p + (32 * nt + n) * 64 + 8 * ((2 * kt + h) ^ (n & 7)));
+      acc = mfma32(aw, b2f[kt], acc);
     }
-    const bf16x2_t *ww = reinterpret_cast<const bf16x2_t *>(w3b + 16 * nt + 4 * q);
-    const bf16x2_t h01 = mlp_act_pk(acc[0], acc[1]);
-    const bf16x2_t h23 = mlp_act_pk(acc[2], acc[3]);
-    y = __builtin_amdgcn_fdot2_f32_bf16(h01, ww[0], y, false);
-    y = __builtin_amdgcn_fdot2_f32_bf16(h23, ww[1], y, false);
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const bf16x2_t *ww = reinterpret_cast<const bf16x2_t *>(w3b + 32 * nt + 8 * g + 4 * h);
+      const bf16x2_t h01 = mlp_act_pk(acc[4 * g], acc[4 * g + 1]);
+      const bf16x2_t h23 = mlp_act_pk(acc[4 * g + 2], acc[4 * g + 3]);
+      y = __builtin_amdgcn_fdot2_f32_bf16(h01, ww[0], y, false);
+      y = __builtin_amdgcn_fdot2_f32_bf16(h23, ww[1], y, false);
+    }
   }
-  y += __shfl_xor(y, 16, 64);
   y += __shfl_xor(y, 32, 64);
   return y;
 }

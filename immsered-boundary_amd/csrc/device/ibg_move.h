@@ -100,6 +100,7 @@ struct MoveArgs {
   const int8_t *pat_flip;// [npatch] odd body: normal reversed
   const double *pat_dz;  // [npatch] z extent
   const int *body_p0;    // [nbody+1] patches of body b: [body_p0[b], body_p0[b+1])
+  const int2 *pat_lh;    // [npatch] {body_p0[b], body_p0[b+1]} of the patch's body b
   const int *body_case;  // [nbody]
   Pose *pose;            // [ncases]
   int use_law;
@@ -195,14 +196,51 @@ __device__ __forceinline__ void patch_dist(const MoveArgs &a, const double *x, i
 // neighbours of a patch of a 2D section are the previous and the next
 // segment of the same body (CollectPatchNeib Body.cpp:822-892).  The walk
 // never leaves the body it starts on.
+// The patch data of the first step of the walk (the start patch, its two
+// neighbour segments and the start patch's normal) are loaded together with
+// the body's patch range, so that a walk that stays where it starts (most
+// cells, most moves) costs one round trip to the cache after its start
+// patch; the comparisons are the same and in the same order as the plain
+// walk, so are the results.
+__device__ __forceinline__ double patch_dist_at(const double *x, double px, double py, double pz) {
+#pragma clang fp contract(off)
+  const double dx = x[0] - px, dy = x[1] - py, dz = x[2] - pz;
+  return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
 __device__ void walk_body(const MoveArgs &a, const double *x, int &patch, double &dist,
                           double &sd) {
 #pragma clang fp contract(off)
-  const int b = a.pat_body[patch];
-  const int lo = a.body_p0[b], hi = a.body_p0[b + 1];
+  const int p0 = patch;
+  const int pm = p0 > 0 ? p0 - 1 : p0, pq = p0 + 1 < a.npatch ? p0 + 1 : p0;   // in range
+  const int2 lh = a.pat_lh[p0];
+  const double *c0 = a.pat_pc + (size_t)p0 * 3, *cm = a.pat_pc + (size_t)pm * 3;
+  const double *cq = a.pat_pc + (size_t)pq * 3, *n0 = a.pat_nv + (size_t)p0 * 3;
+  const double c0x = c0[0], c0y = c0[1], c0z = c0[2];
+  const double cmx = cm[0], cmy = cm[1], cmz = cm[2];
+  const double cqx = cq[0], cqy = cq[1], cqz = cq[2];
+  const double n0x = n0[0], n0y = n0[1], n0z = n0[2];
+  const int lo = lh.x, hi = lh.y;
   dist = 999.0;
-  patch_dist(a, x, patch, patch, dist);
-  int old = -1, ps0 = patch;
+  {
+    const double d = patch_dist_at(x, c0x, c0y, c0z);
+    if (d < dist) { dist = d; patch = p0; }
+  }
+  // first step of the walk from the preloaded data
+  if (p0 - 1 >= lo) {
+    const double d = patch_dist_at(x, cmx, cmy, cmz);
+    if (d < dist) { dist = d; patch = p0 - 1; }
+  }
+  if (p0 + 1 < hi) {
+    const double d = patch_dist_at(x, cqx, cqy, cqz);
+    if (d < dist) { dist = d; patch = p0 + 1; }
+  }
+  if (patch == p0) {
+    sd = n0x * (x[0] - c0x) + n0y * (x[1] - c0y) + n0z * (x[2] - c0z);
+    return;
+  }
+  // the walk moved: its remaining steps as in FindClosePatchtoCell
+  int old = p0, ps0 = patch;
   for (int guard = 0; old != ps0 && guard < 1 << 20; ++guard) {
     old = patch;
     if (ps0 - 1 >= lo) patch_dist(a, x, ps0 - 1, patch, dist);

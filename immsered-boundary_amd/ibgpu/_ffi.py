@@ -93,7 +93,7 @@ class Config(C.Structure):
         ("mode", C.c_int32), ("nstage", C.c_int32), ("alpha", C.c_double * 4),
         ("dt_var_step", C.c_int32), ("temporal_refine", C.c_int32),
         ("limiter", C.c_int32), ("viscous", C.c_int32), ("tile", C.c_int32),
-        ("turbulence", C.c_int32),
+        ("turbulence", C.c_int32), ("schedule", C.c_int32),
     ]
 
 

@@ -24,7 +24,7 @@ class IBGError(RuntimeError):
 
 
 def make_config(mode="fp64", nstage=3, dt_var_step=None, temporal_refine=True, limiter=0,
-                viscous=True, tile=0, nlevels=1, turbulence=False):
+                viscous=True, tile=0, nlevels=1, turbulence=False, schedule=0):
     cfg = _ffi.Config()
     cfg.mode = MODES[mode]
     cfg.nstage = nstage
@@ -37,17 +37,18 @@ def make_config(mode="fp64", nstage=3, dt_var_step=None, temporal_refine=True, l
     cfg.viscous = 1 if viscous else 0
     cfg.tile = tile
     cfg.turbulence = 1 if turbulence else 0
+    cfg.schedule = schedule
     return cfg
 
 
 class Engine:
     def __init__(self, mesh, mode="fp64", nstage=3, temporal_refine=True, limiter=0,
-                 viscous=True, tile=0, dt_var_step=None, turbulence=False):
+                 viscous=True, tile=0, dt_var_step=None, turbulence=False, schedule=0):
         self.lib = _ffi.gpu_lib()
         self.mesh = mesh
         self.turbulence = bool(turbulence)
         self.cfg = make_config(mode, nstage, dt_var_step, temporal_refine, limiter, viscous, tile,
-                               mesh.nlevels, turbulence)
+                               mesh.nlevels, turbulence, schedule)
         self.ctx = C.c_void_p()
         self._chk(self.lib.ibg_create(C.byref(self.cfg), C.byref(self.ctx)), "ibg_create")
         self._chk(self.lib.ibg_upload_mesh(self.ctx, mesh.soa_ptr, mesh.phys), "ibg_upload_mesh")
@@ -176,15 +177,19 @@ class Engine:
         return out
 
     def autotune_tile(self, candidates=(64, 128, 192, 256), reps=20):
-        """Pick the stage-kernel tile size by timing (resets the state)."""
+        """Pick the stage-kernel tile size and schedule by timing (resets the
+        state).  Returns (best tile, {(tile, schedule): ms}); the schedule
+        stays in the context (self.schedule)."""
         cand = np.ascontiguousarray(candidates, dtype=np.int32)
-        ms = np.zeros(cand.size)
+        ms = np.zeros(2 * cand.size)
         best = C.c_int32()
         self._chk(self.lib.ibg_autotune_tile(self.ctx, self.mesh.soa_ptr, self.mesh.phys,
                                              cand.ctypes.data_as(_ffi.i32p), cand.size, reps,
                                              C.byref(best), ms.ctypes.data_as(_ffi.f64p)),
                   "ibg_autotune_tile")
-        return best.value, dict(zip(cand.tolist(), ms.tolist()))
+        table = {(int(t), s): float(ms[2 * k + s]) for k, t in enumerate(cand.tolist()) for s in (0, 1)}
+        self.schedule = min(table, key=table.get)[1]
+        return best.value, table
 
     # learned closure ----------------------------------------------------------
     def closure_init(self, capacity, seed=20261202):

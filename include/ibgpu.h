@@ -171,6 +171,11 @@ typedef struct ibg_config {
   int32_t turbulence;          /* 1: Spalart-Allmaras closure of the
                                   -D TURBULENCE build (Turbulence.cpp,
                                   NS_Solver.cpp:632-840); 0: laminar          */
+  int32_t schedule;            /* stage-kernel schedule: 0 one workgroup per
+                                  tile; 1 persistent workgroups (as many as
+                                  the device holds at once) looping over the
+                                  tiles, the learned closure's weights staged
+                                  once per workgroup                          */
 } ibg_config;
 
 typedef struct ibg_ctx ibg_ctx;
@@ -367,11 +372,13 @@ ibg_status ibg_bench_stage(ibg_ctx *ctx, int32_t ts, int32_t subt, int32_t reps,
  * context stream; avg_ms per copy.  Calibrates HBM counters and bandwidth. */
 ibg_status ibg_bench_copy(ibg_ctx *ctx, int64_t nbytes, int32_t reps, double *avg_ms);
 
-/* Tile-size autotuner with no counterpart in the reference: for each
- * candidate (a multiple of 64 up to 256 cells per tile) rebuild the tiles,
- * time the stage kernel of a full-march step on the freestream, and keep
- * the fastest (ms[k]: the time of candidate k).  The mesh is re-uploaded
- * and the state reset: call it before ibg_set_state.                      */
+/* Tile and schedule autotuner with no counterpart in the reference: for
+ * each tile candidate (a multiple of 64 up to 256 cells per tile) rebuild
+ * the tiles and time the stage kernel of a full-march step on the
+ * freestream under both schedules (ibg_config.schedule), and keep the
+ * fastest pair (ms[2k+s]: candidate k under schedule s; best = its tile,
+ * the schedule is left in the context).  The mesh is re-uploaded and the
+ * state reset: call it before ibg_set_state.                              */
 ibg_status ibg_autotune_tile(ibg_ctx *ctx, const ibg_mesh_soa *mesh, const ibg_physics *phys,
                              const int32_t *candidates, int32_t ncand, int32_t reps,
                              int32_t *best, double *ms);

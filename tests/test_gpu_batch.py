@@ -47,19 +47,27 @@ def test_ensemble_equals_single_runs(mode):
         assert np.array_equal(es.get_nut(), nb_[lo:hi]), c
 
 
-def test_tile_size_does_not_change_bits():
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_tile_size_and_schedule_do_not_change_bits(mode):
+    """Tile size and schedule (one workgroup per tile, or persistent
+    workgroups looping over the tiles) change the launch shape only."""
     mesh = ibgpu.Mesh(ibgpu.tiny_case())
     fv = perturbed_state(mesh)
     out = []
     for tile in (64, 128, 256):
-        e = ibgpu.Engine(mesh, mode="fp64", nstage=3, tile=tile, turbulence=True)
-        e.set_state(fv)
-        e.set_nut(np.full(mesh.ncells, 5.0))
-        e.TimeMarching(0, 8, use_graph=False)
-        out.append((e.get_state(), e.get_nut()))
-    for a, b in out[1:]:
+        for schedule in (0, 1):
+            print(f"tile {tile}, schedule {schedule}, {mode}", flush=True)
+            e = ibgpu.Engine(mesh, mode=mode, nstage=3, tile=tile, turbulence=True, schedule=schedule)
+            e.set_state(fv)
+            e.set_nut(np.full(mesh.ncells, 5.0))
+            e.TimeMarching(0, 8, use_graph=(schedule == 1))
+            out.append((e.get_state(), e.get_nut(), e.ResinDomain()))
+            e.close()
+    for a, b, (linf, l2) in out[1:]:
         assert np.array_equal(a, out[0][0])
         assert np.array_equal(b, out[0][1])
+        assert np.array_equal(linf, out[0][2][0])
+        assert np.allclose(l2, out[0][2][1], rtol=1e-13, atol=0)
 
 
 def test_autotuner_picks_a_candidate():
@@ -67,8 +75,9 @@ def test_autotuner_picks_a_candidate():
     e = ibgpu.Engine(mesh, mode="mixed", nstage=4, turbulence=True)
     best, table = e.autotune_tile((64, 128, 256), reps=5)
     assert best in (64, 128, 256)
+    assert set(table) == {(t, s) for t in (64, 128, 256) for s in (0, 1)}
     assert all(v > 0 for v in table.values())
-    assert table[best] == min(table.values())
+    assert table[(best, e.schedule)] == min(table.values())
     e.set_state(mesh.init_state())
     e.set_nut(np.full(mesh.ncells, 5.0))
     e.TimeMarching(0, 8, use_graph=True)

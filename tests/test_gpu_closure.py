@@ -225,14 +225,14 @@ def range_params(x, z, seed=3, scale=0.3, q=(2.0, 98.0)):
     return np.concatenate([w, stats]).astype(np.float32)
 
 
-def network_stage_check(mesh, fv, nut, prm=None, sample=None, seed=7):
+def network_stage_check(mesh, fv, nut, prm=None, sample=None, seed=7, schedule=0):
     """One stage of the learned-closure kernel (k_stage_ml) against the CPU
     mirror, cell by cell: the SA source of every sampled SA cell is either
     the plain closure (cells outside the network's range) or the plain
     closure with its point-wise term replaced by network output x production
     scale; the network/fallback counts equal the oracle's range test.
     Returns (params, statistics)."""
-    e = ibgpu.Engine(mesh, mode="mixed", nstage=4, turbulence=True)
+    e = ibgpu.Engine(mesh, mode="mixed", nstage=4, turbulence=True, schedule=schedule)
     e.set_state(fv)
     e.set_nut(nut)
     ts = 0
@@ -302,9 +302,34 @@ def network_stage_check(mesh, fv, nut, prm=None, sample=None, seed=7):
     return prm, stats
 
 
-def test_network_in_stage_matches_cpu_mirror(tiny):
-    print("network inside the stage kernel: tiny case", flush=True)
+@pytest.mark.parametrize("schedule", [0, 1])
+def test_network_in_stage_matches_cpu_mirror(tiny, schedule):
+    print(f"network inside the stage kernel: tiny case, schedule {schedule}", flush=True)
     fv = perturbed_state(tiny)
     rng = np.random.default_rng(4)
     nut = tiny.phys[0].nut_inf * (1.0 + 0.5 * rng.uniform(-1.0, 1.0, tiny.ncells))
-    network_stage_check(tiny, fv, nut)
+    network_stage_check(tiny, fv, nut, schedule=schedule)
+
+
+def test_network_march_same_bits_either_schedule(tiny):
+    """The learned-closure march under the persistent schedule (weights
+    staged once per workgroup) gives the bits and the network / fallback
+    counts of the one-workgroup-per-tile schedule."""
+    prm = rand_prm(seed=5, scale=0.05)
+    prm[O.NPARAM + 16:O.NPARAM + 24] = -1e30
+    prm[O.NPARAM + 24:O.NPARAM + 32] = 1e30
+    prm[O.NPARAM + 24] = 0.0          # feature 0 above 0: plain closure
+    out = []
+    for schedule in (0, 1):
+        e = ibgpu.Engine(tiny, mode="mixed", nstage=4, turbulence=True, schedule=schedule)
+        e.set_state(perturbed_state(tiny))
+        e.set_nut(np.full(tiny.ncells, tiny.phys[0].nut_inf))
+        e.closure_init(1 << 10)
+        e.closure_set_params(prm)
+        e.closure_enable(True)
+        e.TimeMarching(0, 8, use_graph=True)
+        out.append((e.get_state(), e.get_nut(), e.closure_counts()))
+        e.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2] and out[0][2][0] > 0 and out[0][2][1] > 0
