@@ -40,6 +40,13 @@ class CaseSpec:
     span_patches: int = 1
     pivot_y: float = 0.0          # y of the quarter chord (x is 0)
     face_bc: tuple = (0, 0, 0, 0)  # BC codes of x-, x+, y-, y+ (0: -13, -12, -50, -50)
+    # SA working variable of the -13 inlet: None keeps the reference's (its
+    # facevar[0].viseddy is never set, so 0); "freestream" uses the
+    # freestream value.  With 0 the uniform inflow drives the first cells'
+    # working variable to exactly 0, where the SA destruction term is 0/0
+    # (r = nu / (S~ kappa^2 d^2) with nu = 0 and |Omega| = 0): NaN after
+    # ~1400 steps of the 2.6M-cell case.
+    inlet_nut: object = None
 
     def to_ctypes(self):
         c = _ffi.Case()
@@ -90,7 +97,7 @@ def config2(levels=6, base=256):
     bands = [(-0.3, 6.0), (-0.15, 4.0), (-0.08, 2.5), (-0.04, 1.6), (-0.02, 1.0)]
     return CaseSpec(name="naca0015_m03_re10000", dim=2, nx=base, ny=base, nz=1,
                     nlevels=levels, bands=bands[:levels - 1], thickness=0.15,
-                    npts=200, aoa=0.0, Ma=0.3, Re=10000.0)
+                    npts=200, aoa=0.0, Ma=0.3, Re=10000.0, inlet_nut="freestream")
 
 
 def tiny_case(levels=3, base=32, aoa=4.0, Ma=0.5, Re=2000.0, thickness=0.12):
@@ -118,7 +125,8 @@ def config3(levels=5, base=128, nz=8):
     bands = [(-0.3, 6.0), (-0.15, 4.0), (-0.08, 3.2), (-0.04, 2.8)]
     return CaseSpec(name="naca0012_3d_m06_re5e4", dim=3, nx=base, ny=base, nz=nz, span=0.1,
                     nlevels=levels, twod_from=3, bands=bands[:levels - 1], thickness=0.12,
-                    npts=100, aoa=4.0, Ma=0.6, Re=50000.0, span_patches=nz)
+                    npts=100, aoa=4.0, Ma=0.6, Re=50000.0, span_patches=nz,
+                    inlet_nut="freestream")
 
 
 def ensemble_cases(n=32, seed=20261202, base=256, levels=6):
@@ -136,7 +144,7 @@ def ensemble_cases(n=32, seed=20261202, base=256, levels=6):
         bands = [(-0.3, 6.0 * w), (-0.15, 4.0 * w), (-0.08, 2.5 * w), (-0.04, 1.6 * w), (-0.02, 1.0 * w)]
         out.append(CaseSpec(name=f"ens{i:02d}", dim=2, nx=base, ny=base, nz=1, nlevels=levels,
                             bands=bands[:levels - 1], thickness=t, npts=100, aoa=a, Ma=m,
-                            Re=10000.0))
+                            Re=10000.0, inlet_nut="freestream"))
     return out
 
 
@@ -162,6 +170,9 @@ class Mesh:
         self.phys = (_ffi.Physics * len(self.specs))()
         for i in range(len(self.specs)):
             host.ibm_physics(C.byref(arr[i]), C.byref(self.phys[i]))
+            nin = self.specs[i].inlet_nut
+            if nin is not None:
+                self.phys[i].nut_inlet = self.phys[i].nut_inf if nin == "freestream" else float(nin)
         m = self.soa
         self.dim, self.ncases, self.nlevels = m.dim, m.ncases, m.nlevels
         self.ncells, self.nfaces = m.ncells, m.nfaces

@@ -293,3 +293,30 @@ def test_sa_exchange_linear_and_mean():
                 continue
             f = mesh.pro_fi[i][q]
             assert nut[f] == pytest.approx(before[f], rel=1e-12, abs=1e-13)
+
+
+# ------------------------------------------------------------ the 0/0 of the SA source
+def test_source_is_undefined_at_zero_nu_in_uniform_flow():
+    """The reference's SA source (NS_Solver.cpp:700-767) with nu = 0 in a
+    uniform stream (|Omega| = 0): S~ = max(|Omega| + ..., 0.3 |Omega|) = 0,
+    r = nu / (S~ kappa^2 d^2 Re) = 0/0, and the destruction term is NaN
+    (multiplying (nu/d)^2 = 0 does not save it).  The reference's -13 inlet
+    carries nu = 0 (its facevar[0].viseddy is never set), so the uniform
+    inflow reaches this state; the measured configurations set the inlet
+    value to the freestream one (Mesh inlet_nut)."""
+    assert math.isnan(cell_src(u=0.3, nu=0.0, sd=5.0, Re=1e4))
+    # any nonzero nu, or any vorticity, keeps it finite
+    assert math.isfinite(cell_src(u=0.3, nu=1e-12, sd=5.0, Re=1e4))
+    assert math.isfinite(cell_src(u=0.3, nu=0.0, sd=5.0, gu=(0, 1e-3, 0), Re=1e4))
+
+
+def test_inlet_nut_option():
+    spec = ibgpu.tiny_case()
+    import dataclasses
+    m0 = ibgpu.Mesh(spec)
+    m1 = ibgpu.Mesh(dataclasses.replace(spec, inlet_nut="freestream"))
+    m2 = ibgpu.Mesh(dataclasses.replace(spec, inlet_nut=0.25))
+    assert m0.phys[0].nut_inlet == 0.0                  # the reference's value
+    assert m1.phys[0].nut_inlet == m1.phys[0].nut_inf > 0.0
+    assert m2.phys[0].nut_inlet == 0.25
+    assert ibgpu.config2().inlet_nut == "freestream"
