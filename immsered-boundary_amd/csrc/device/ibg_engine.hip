@@ -130,6 +130,9 @@ struct ibg_ctx {
     double *pt_off = nullptr, *pat_dz = nullptr;
     int *pat_pt0 = nullptr, *pat_body = nullptr, *body_p0 = nullptr, *pat_cc = nullptr;
     int2 *pat_lh = nullptr;    // [npatch] patch range of the patch's body
+    int4 *fbox = nullptr;      // [ncases] finest-level key box: ix0, iy0, nx, ny
+    long long *foff = nullptr; // [ncases] first entry of the case's key grid
+    int *fgrid = nullptr;      // finest-level cell of each key, -1 for none
     int8_t *pat_flip = nullptr;
     Pose *pose = nullptr;
     int *mk1 = nullptr, *mkib = nullptr;  // move stamps: layer 1, IB set
@@ -520,6 +523,7 @@ MoveArgs move_args(ibg_ctx *c, unsigned long long *upd, int lmin) {
   a.pt_off = v.pt_off; a.pat_pt0 = v.pat_pt0; a.pat_body = v.pat_body; a.pat_case = c->pat_case;
   a.pat_flip = v.pat_flip; a.pat_dz = v.pat_dz; a.body_p0 = v.body_p0; a.body_case = v.bcase_d;
   a.pat_lh = v.pat_lh;
+  a.fbox = v.fbox; a.foff = v.foff; a.fgrid = v.fgrid;
   a.pose = v.pose; a.use_law = v.law ? 1 : 0; a.law = v.lawd; a.t = v.t;
   a.mot = c->motion; a.motc = v.motc; a.case_center = c->case_center;
   a.pat_pc = c->pat_pc; a.pat_nv = c->pat_nv; a.pat_area = c->pat_area;
@@ -1705,6 +1709,32 @@ ibg_status ibg_upload_surface(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_surfa
       lh[p] = make_int2(sf->body_p0[b], sf->body_p0[b + 1]);
     }
     UP(v.pat_lh, lh);
+  }
+  {
+    // finest-level key grids: the cell (normal or ghost) of every in-plane
+    // key of the level's bounding box, the table the image searches read
+    // instead of walking the neighbour table one cell at a time
+    std::vector<int4> box(c->ncases, make_int4(0, 0, 0, 0));
+    std::vector<long long> foff(c->ncases + 1, 0);
+    for (int cs = 0; cs < c->ncases; ++cs) {
+      const int b = cs * L + L - 1;
+      int x0 = 1 << 30, y0 = 1 << 30, x1 = -(1 << 30), y1 = -(1 << 30);
+      for (int g = m->blk_start[b]; g < m->blk_start[b + 1]; ++g) {
+        x0 = std::min(x0, m->key[(size_t)g * 3]); x1 = std::max(x1, m->key[(size_t)g * 3]);
+        y0 = std::min(y0, m->key[(size_t)g * 3 + 1]); y1 = std::max(y1, m->key[(size_t)g * 3 + 1]);
+      }
+      if (x1 >= x0) box[cs] = make_int4(x0, y0, x1 - x0 + 1, y1 - y0 + 1);
+      foff[cs + 1] = foff[cs] + (long long)box[cs].z * box[cs].w;
+    }
+    std::vector<int> grid(std::max<long long>(foff[c->ncases], 1), -1);
+    for (int cs = 0; cs < c->ncases; ++cs) {
+      const int b = cs * L + L - 1;
+      for (int g = m->blk_start[b]; g < m->blk_start[b + 1]; ++g) {
+        const long long rx = m->key[(size_t)g * 3] - box[cs].x, ry = m->key[(size_t)g * 3 + 1] - box[cs].y;
+        grid[(size_t)(foff[cs] + ry * box[cs].z + rx)] = c->h2d[g];
+      }
+    }
+    UP(v.fbox, box); UP(v.foff, foff); UP(v.fgrid, grid);
   }
   {
     // ghosts by level, finest first: the ghosts of the levels >= l are a

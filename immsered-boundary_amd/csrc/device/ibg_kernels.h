@@ -114,11 +114,12 @@ struct ViscArgs {
 // LDS, then per cell Rv = sum_d (Fv_hi - Fv_lo).  With the SA closure the
 // face flux of the working variable (mutflux, :534-542) is reduced the same
 // way into sum_d (mutflux_hi - mutflux_lo) / V / Re (:791-800).
-// halt: an LDS flag, set by thread 0 before the call from the error word,
-// tested after the face phase's barrier: a halted workgroup writes nothing.
+// halt: an LDS flag that thread 0 sets from herr (the error word, loaded at
+// kernel start) just before the face phase's barrier and that every thread
+// tests after it: a halted workgroup writes nothing.
 template <typename R, int D>
 __device__ __forceinline__ void visc_prep_tile(const ViscArgs<R, D> &a, const TileInfo &ti,
-                                               unsigned char *smem_raw, const int *halt) {
+                                               unsigned char *smem_raw, int *halt, bool herr) {
   constexpr int NV = D + 2;
   using V = typename V4<R>::T;
   V *fl4 = reinterpret_cast<V *>(smem_raw);
@@ -161,6 +162,7 @@ __device__ __forceinline__ void visc_prep_tile(const ViscArgs<R, D> &a, const Ti
       if (a.turb) mfl[f] = mf;
     }
   }
+  if (threadIdx.x == 0) *halt = herr ? 1 : 0;
   __syncthreads();
   if (*halt) return;
   const int i = threadIdx.x;
@@ -206,9 +208,9 @@ template <typename R, int D>
 __global__ void __launch_bounds__(kTile) k_visc_prep(ViscArgs<R, D> a) {
   extern __shared__ __align__(32) unsigned char smem_raw[];
   __shared__ int halt;
-  if (threadIdx.x == 0) halt = halted(a.err) ? 1 : 0;
+  const bool herr = halted(a.err);
   const TileInfo ti = a.tiles[xcd_tile()];
-  visc_prep_tile<R, D>(a, ti, smem_raw, &halt);
+  visc_prep_tile<R, D>(a, ti, smem_raw, &halt, herr);
 }
 
 template <typename R, int D>
@@ -345,21 +347,20 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R, D> &a, int tile) {
   constexpr bool kMixed = std::is_same<R, float>::value;
   using V = typename V4<R>::T;
   extern __shared__ __align__(32) unsigned char smem_raw[];
-  // the error word is read while the face phase runs and tested after its
-  // barrier (the face phase writes only LDS)
+  // the error word: loaded here, its load in flight while the face phase
+  // runs, published by thread 0 just before the face phase's barrier and
+  // tested after it (the face phase writes only LDS)
   __shared__ int halt;
   __shared__ unsigned mlc[2];   // learned closure: network / fallback cells of this tile
-  if (threadIdx.x == 0) {
-    halt = halted(a.err) ? 1 : 0;
-    if (ML) { mlc[0] = 0u; mlc[1] = 0u; }
-  }
+  const bool herr = halted(a.err);
+  if (ML && threadIdx.x == 0) { mlc[0] = 0u; mlc[1] = 0u; }
   const TileInfo ti = a.tiles[tile];
   if constexpr (kFuseVisc<R, D>) {
     if (a.fuse_visc) {
       // Rv and mutf of this tile's cells from the step-start state (the
       // buffer this stage reads); each thread reads back only its own
       // cell's entries in phase 2, so no other workgroup is involved
-      visc_prep_tile<R, D>(a.va, ti, smem_raw, &halt);
+      visc_prep_tile<R, D>(a.va, ti, smem_raw, &halt, herr);
       if (halt) return;  // uniform: read after the barrier in visc_prep_tile
       __syncthreads();   // the face arrays are reused by phase 1
     }
@@ -399,6 +400,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R, D> &a, int tile) {
       fR[f] = st.z;
     }
   }
+  if (threadIdx.x == 0) halt = herr ? 1 : 0;
   __syncthreads();
   if (halt) return;   // uniform: the flag was written before the barrier
 

@@ -101,6 +101,9 @@ struct MoveArgs {
   const double *pat_dz;  // [npatch] z extent
   const int *body_p0;    // [nbody+1] patches of body b: [body_p0[b], body_p0[b+1])
   const int2 *pat_lh;    // [npatch] {body_p0[b], body_p0[b+1]} of the patch's body b
+  const int4 *fbox;      // [ncases] finest-level key box {ix0, iy0, nx, ny}
+  const long long *foff; // [ncases] first entry of the case's key grid
+  const int *fgrid;      // cell of each in-plane key of the finest level, -1 for none
   const int *body_case;  // [nbody]
   Pose *pose;            // [ncases]
   int use_law;
@@ -495,33 +498,59 @@ __global__ void __launch_bounds__(256) k_move_mark2(MoveArgs a) {
   }
 }
 
-// Cell of the finest level holding key ik, reached from `home` through the
-// neighbour table (one step per axis at a time); -1 when the path leaves the
-// level.
-__device__ int locate_dev(const MoveArgs &a, int home, const int *ik) {
-  int cur = home;
-  int k0 = a.key[(size_t)cur * 3], k1 = a.key[(size_t)cur * 3 + 1];
-  if (ik[2] != a.key[(size_t)cur * 3 + 2]) return -1;
-  for (int guard = 0; guard < 256; ++guard) {
-    if (k0 == ik[0] && k1 == ik[1]) return cur;
-    const int sx = (ik[0] > k0) - (ik[0] < k0), sy = (ik[1] > k1) - (ik[1] < k1);
-    const int nx = a.nb27[(size_t)cur * 27 + (1 + sx) * 9 + (1 + sy) * 3 + 1];
-    if (nx < 0) return -1;
-    cur = nx;
-    k0 += sx; k1 += sy;
+// Finest-level cell of case cs with in-plane key (ix, iy), -1 for none.
+__device__ __forceinline__ int fine_cell(const MoveArgs &a, int cs, int ix, int iy) {
+  const int4 bx = a.fbox[cs];
+  const int rx = ix - bx.x, ry = iy - bx.y;
+  if (rx < 0 || ry < 0 || rx >= bx.z || ry >= bx.w) return -1;
+  return a.fgrid[a.foff[cs] + (long long)ry * bx.z + rx];
+}
+
+// Cell of the finest level holding key ik, reached from `home` (key k0, k1,
+// kz) along the walk of the neighbour table the reference's search takes
+// (one step per axis toward the target at a time, at most 256 steps); -1
+// when a key on that path is not a cell of the level.  The keys of the path
+// are known in advance, so their cells are read from the key grid together
+// rather than hop by hop.
+__device__ int locate_dev(const MoveArgs &a, int cs, int home, int k0, int k1, int kz,
+                          const int *ik) {
+  if (ik[2] != kz) return -1;
+  const int d0 = ik[0] - k0, d1 = ik[1] - k1;
+  const int a0 = d0 < 0 ? -d0 : d0, a1 = d1 < 0 ? -d1 : d1;
+  const int s0 = (d0 > 0) - (d0 < 0), s1 = (d1 > 0) - (d1 < 0);
+  const int steps = a0 > a1 ? a0 : a1;
+  if (steps == 0) return home;
+  if (steps > 255) return -1;   // the walk gives up after 256 checks
+  constexpr int kNear = 4;
+  int c[kNear];
+#pragma unroll
+  for (int t = 1; t <= kNear; ++t)
+    c[t - 1] = (t <= steps) ? fine_cell(a, cs, k0 + s0 * (t < a0 ? t : a0), k1 + s1 * (t < a1 ? t : a1))
+                            : 0;
+#pragma unroll
+  for (int t = 1; t <= kNear; ++t)
+    if (t <= steps && c[t - 1] < 0) return -1;
+  if (steps <= kNear) return c[steps - 1];
+  int cur = c[kNear - 1];
+  for (int t = kNear + 1; t <= steps; ++t) {
+    cur = fine_cell(a, cs, k0 + s0 * (t < a0 ? t : a0), k1 + s1 * (t < a1 ? t : a1));
+    if (cur < 0) return -1;
   }
-  return -1;
+  return cur;
 }
 
 // Image-point search of the host mesh builder (Body.cpp:1044-1183,
 // 1527-1575): march from `start` along nv in steps of one cell in the
 // dominant direction until the 2x2 bracketing cells are neither solid nor
 // IB cells, then the weights of Mesh::ComptHgIntpCoef (Mesh.cpp:1403-1442).
-// `home` is a finest-level cell near start; on success it is the cell
-// holding the image point.
-__device__ bool search_image_dev(const MoveArgs &a, int ep, const double *start, const double *nv,
-                                 const double *h, const double *org, int &home, int *intp,
-                                 double *coef, double &hgdis_out) {
+// `home` is a finest-level cell near start (case cs); on success it is the
+// cell holding the image point.  Cells are found in the finest level's key
+// grid: the located cell's key is the target key, its centre follows from
+// it, and the bracketing cells are its in-plane neighbours by key (the z
+// neighbours of a one-layer 2D level are the cell itself).
+__device__ bool search_image_dev(const MoveArgs &a, int ep, int cs, const double *start,
+                                 const double *nv, const double *h, const double *org, int &home,
+                                 int *intp, double *coef, double &hgdis_out) {
 #pragma clang fp contract(off)
   double dk[3];
   for (int d = 0; d < 3; ++d) dk[d] = nv[d] / h[d];
@@ -531,6 +560,8 @@ __device__ bool search_image_dev(const MoveArgs &a, int ep, const double *start,
   const double steplen = sqrt(dxyz[0] * dxyz[0] + dxyz[1] * dxyz[1] + dxyz[2] * dxyz[2]);
   const double ca[3] = {0.0, 1.0, 0.0};
   const double cb[3] = {1.0, -1.0, 1.0};
+  int hk0 = a.key[(size_t)home * 3], hk1 = a.key[(size_t)home * 3 + 1];
+  int hkz = a.key[(size_t)home * 3 + 2];
   double hgdis = 0.0;
   for (int it = 0; it < 64; ++it) {
     hgdis += steplen;
@@ -538,23 +569,29 @@ __device__ bool search_image_dev(const MoveArgs &a, int ep, const double *start,
     for (int d = 0; d < 3; ++d) pt[d] = start[d] + nv[d] * hgdis;
     int ik[3];
     for (int d = 0; d < 3; ++d) ik[d] = (int)floor((pt[d] - org[d]) / h[d]);
-    const int cc = locate_dev(a, home, ik);
-    if (cc < 0) return false;
-    if (a.kind[cc] == IBG_DMGHOST) return false;
-    home = cc;
     double cen[3];
-    cell_center_dev(a, cc, cen);
+    for (int d = 0; d < 3; ++d) cen[d] = org[d] + ((double)ik[d] + 0.5) * h[d];
     double dkt[3];
     for (int d = 0; d < 3; ++d) dkt[d] = (pt[d] - cen[d]) / h[d];
     dkt[2] = 0.0;   // one z layer: the image point is in it
     int hgdir[3];
     for (int d = 0; d < 3; ++d) hgdir[d] = dkt[d] > 0.0 ? 1 : 0;
+    // the bracketing cells by key, read with the path (independent reads)
+    int br[2][2];
+    for (int pi = 0; pi < 2; ++pi)
+      for (int pj = 0; pj < 2; ++pj)
+        br[pi][pj] = fine_cell(a, cs, ik[0] + pi + hgdir[0] - 1, ik[1] + pj + hgdir[1] - 1);
+    const int cc = locate_dev(a, cs, home, hk0, hk1, hkz, ik);
+    if (cc < 0) return false;
+    if (a.kind[cc] == IBG_DMGHOST) return false;
+    home = cc;
+    hk0 = ik[0]; hk1 = ik[1]; hkz = ik[2];
     bool ok = true;
     for (int pi = 0; pi < 2; ++pi)
       for (int pj = 0; pj < 2; ++pj)
         for (int pk = 0; pk < 2; ++pk) {
           const int nx = pi + hgdir[0], ny = pj + hgdir[1], nz = pk + hgdir[2];
-          const int an = a.nb27[(size_t)cc * 27 + nx * 9 + ny * 3 + nz];
+          const int an = br[pi][pj];
           const int q = pi * 4 + pj * 2 + pk;
           intp[q] = an;
           coef[q] = 0.0;
@@ -591,7 +628,7 @@ __device__ void move_image(const MoveArgs &a, int j) {
     double cen[3];
     cell_center_dev(a, g, cen);
     int home = g;
-    if (!search_image_dev(a, ep, cen, nv, B.h, org, home, intp, coef, hgdis)) {
+    if (!search_image_dev(a, ep, B.case_id, cen, nv, B.h, org, home, intp, coef, hgdis)) {
       report_error(a.err, g, 1183);
       for (int q = 0; q < 8; ++q) { intp[q] = -1; coef[q] = 0.0; }
     }
@@ -618,7 +655,7 @@ __device__ void move_image(const MoveArgs &a, int j) {
                         a.pat_pc[(size_t)p * 3 + 2]};
   const double nv[3] = {a.pat_nv[(size_t)p * 3], a.pat_nv[(size_t)p * 3 + 1],
                         a.pat_nv[(size_t)p * 3 + 2]};
-  if (!search_image_dev(a, ep, pc, nv, B.h, org, home, intp, coef, hgdis)) {
+  if (!search_image_dev(a, ep, B.case_id, pc, nv, B.h, org, home, intp, coef, hgdis)) {
     report_error(a.err, a.pat_cc[p], 1575);
     for (int q = 0; q < 8; ++q) { intp[q] = -1; coef[q] = 0.0; }
   } else {
