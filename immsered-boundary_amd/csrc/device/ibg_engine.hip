@@ -40,6 +40,7 @@
 #include "ibg_kernels.h"
 #include "ibg_closure.h"
 #include "ibg_move.h"
+#include "ibg_reflux.h"
 
 using namespace ibg;
 
@@ -109,6 +110,12 @@ struct ibg_ctx {
   DmgWall *dmgw = nullptr;      // wall ghosts (BC -81), parallel to dmg; null without walls
   double *dist_dm = nullptr;    // [N] distance to the domain walls; null without walls
   int *res_ci = nullptr, *res_fi = nullptr, *pro_ci = nullptr, *pro_fi = nullptr;
+  // optional coarse/fine flux balance: pairs, tile face -> balanced-flux
+  // entry, the balanced inviscid / viscous fluxes [nrf][5][NV] (R-typed)
+  int nrf = 0;
+  RfPair *rfp = nullptr;
+  int *tmod = nullptr;
+  void *rbal = nullptr, *rbalv = nullptr;
   int *ib_cell = nullptr, *ib_intp = nullptr, *ib_body = nullptr, *ib_case = nullptr;
   double *ib_coef = nullptr, *ib_hgdis = nullptr, *ib_attach = nullptr, *ib_nv = nullptr,
          *signdis = nullptr;
@@ -270,6 +277,8 @@ void free_all(ibg_ctx *c) {
   for (void *p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
   c->have_mesh = false;
+  c->nrf = 0;
+  c->rfp = nullptr; c->tmod = nullptr; c->rbal = nullptr; c->rbalv = nullptr;
   c->mv = ibg_ctx::MoveState{};
 }
 
@@ -331,6 +340,8 @@ ViscArgs<R, D> visc_args(ibg_ctx *c) {
   a.viscous = c->cfg.viscous;
   a.turb = c->turb ? 1 : 0;
   a.err = c->err;
+  a.tmod = c->tmod;
+  a.rbalv = (c->cfg.reflux && c->nrf > 0) ? (const R *)c->rbalv : nullptr;
   return a;
 }
 
@@ -366,6 +377,17 @@ ibg_status launch_stage(ibg_ctx *c, int ntile, int ncell, int subt, bool last, d
   // the step's first stage reads the home buffer S0, the state k_visc_prep reads
   a.fuse_visc = (fuse && subt == 0 && kFuseVisc<R, D>) ? 1 : 0;
   a.va = visc_args<R, D>(c);
+  a.tmod = c->tmod;
+  a.rbal = nullptr;
+  if (c->cfg.reflux && last && c->nrf > 0) {
+    // BalanceFaceFlux on the last stage's inviscid fluxes (NS_Solver.H:686)
+    hipLaunchKernelGGL((k_balance_inv<R, D>), dim3(grid(c->nrf, 256)), dim3(256), 0, c->s, c->rfp,
+                       c->nrf, (const int4 *)c->tst, (const int8_t *)c->tdir, a.Win,
+                       (const BlkInfo *)c->blk, (const DevPhys *)c->phys, c->cfg.limiter,
+                       (R *)c->rbal, (const unsigned long long *)c->err);
+    HIPCHK(hipGetLastError());
+    a.rbal = (const R *)c->rbal;
+  }
   const size_t smem = tile_smem(c, sizeof(R), true);
   a.ntile = ntile;
   a.mlw_off = (unsigned)ml_resident_offset(c);
@@ -422,9 +444,28 @@ ibg_status run_stage(ibg_ctx *c, int ts, int subt, double *src, double *nusrc, b
 // whether the time march folds k_visc_prep into the first stage
 inline bool fused_visc(const ibg_ctx *c) { return c->D == 2 && !c->mixed; }
 
+// BalanceFaceFlux on the step's viscous fluxes (NS_Solver.H:672), before
+// the viscous residual that reads them
+template <typename R, int D>
+ibg_status launch_visc_balance(ibg_ctx *c) {
+  if (!c->cfg.reflux || !c->cfg.viscous || c->nrf <= 0) return IBG_OK;
+  hipLaunchKernelGGL((k_balance_visc<R, D>), dim3(grid(c->nrf, 256)), dim3(256), 0, c->s,
+                     visc_args<R, D>(c), (const RfPair *)c->rfp, c->nrf, (R *)c->rbalv);
+  HIPCHK(hipGetLastError());
+  return IBG_OK;
+}
+
+ibg_status run_visc_balance(ibg_ctx *c) {
+  if (c->D == 2)
+    return c->mixed ? launch_visc_balance<float, 2>(c) : launch_visc_balance<double, 2>(c);
+  return c->mixed ? launch_visc_balance<float, 3>(c) : launch_visc_balance<double, 3>(c);
+}
+
 template <typename R, int D>
 ibg_status launch_visc_prep(ibg_ctx *c, int ntile) {
   if (ntile <= 0) return IBG_OK;
+  ibg_status st;
+  if ((st = launch_visc_balance<R, D>(c))) return st;
   const ViscArgs<R, D> a = visc_args<R, D>(c);
   hipLaunchKernelGGL((k_visc_prep<R, D>), dim3(ntile), dim3(c->tile), tile_smem(c, sizeof(R), false), c->s, a);
   HIPCHK(hipGetLastError());
@@ -581,8 +622,12 @@ ibg_status run_step(ibg_ctx *c, int ts) {
     if ((st = run_dt(c))) return st;
   if ((st = run_bc(c, ts))) return st;
   const bool fuse = fused_visc(c);
-  if (!fuse)
+  if (!fuse) {
     if ((st = run_visc_prep(c, ts))) return st;
+  } else {
+    // the stage-0 launch computes the viscous residual: balance first
+    if ((st = run_visc_balance(c))) return st;
+  }
   for (int k = 0; k < c->cfg.nstage; ++k)
     if ((st = run_stage(c, ts, k, nullptr, nullptr, fuse))) return st;
   if (c->mv.law) {
@@ -708,6 +753,17 @@ ibg_status validate_mesh(const ibg_mesh_soa *m) {
       if (m->ibidx[a] >= 0) return fail("an image point interpolates from an IB cell");
     }
   }
+  if (m->nmdy < 0 || (m->nmdy > 0 && !m->mdy)) return fail("balance face list");
+  for (int i = 0; i < m->nmdy; ++i) {
+    const int32_t *r = m->mdy + (size_t)i * 6;
+    if (r[0] < 0 || r[0] > 2) return fail("balance face form");
+    const int nf = rf_nfine(r[0]);
+    if (r[1] < 0 || r[1] >= m->nfaces) return fail("balance coarse face out of range");
+    for (int k = 0; k < 4; ++k) {
+      const int f = r[2 + k];
+      if (k < nf ? (f < 0 || f >= m->nfaces) : f != -1) return fail("balance fine face out of range");
+    }
+  }
   for (int i = 0; i < m->npatch; ++i) {
     if (m->pat_case[i] < 0 || m->pat_case[i] >= m->ncases) return fail("patch case");
     for (int q = 0; q < 8; ++q) {
@@ -737,6 +793,9 @@ ibg_status ibg_create(const ibg_config *cfg, ibg_ctx **out) {
   if (cfg->turbulence != 0 && cfg->turbulence != 1) return IBG_ERR_ARG;
   if (cfg->turbulence && !cfg->viscous) return IBG_ERR_ARG;
   if (cfg->schedule != 0 && cfg->schedule != 1) return IBG_ERR_ARG;
+  // BalanceFaceFlux exists only without TEMPORAL_REFINE (Fluxmodification.cpp:293)
+  if (cfg->reflux != 0 && cfg->reflux != 1) return IBG_ERR_ARG;
+  if (cfg->reflux && cfg->temporal_refine) return IBG_ERR_ARG;
   const int tile = cfg->tile > 0 ? cfg->tile : kTile;
   if (tile % 64 != 0 || tile > kTile) return IBG_ERR_ARG;
   int ndev = 0;
@@ -990,6 +1049,8 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
   c->tile_end.assign(L, 0);
   std::vector<int> case_tiles((size_t)nc * L * 2, 0);   // [case][level] = {first, end}
   std::vector<int> slot_of(F, -1);
+  // per device face: its first tile face, that tile's block, its tile count
+  std::vector<int> face_slot0(F, -1), face_tblk(F, -1), face_ntile(F, 0);
   int maxf = 1;
   {
     int pos = 0;
@@ -1012,6 +1073,8 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
               if (slot_of[f] < 0) {
                 slot_of[f] = (int)used.size();
                 used.push_back(f);
+                if (face_slot0[f] < 0) { face_slot0[f] = (int)tst.size(); face_tblk[f] = b; }
+                ++face_ntile[f];
                 tst.push_back(face_st[f]);
                 tdir.push_back(face_dir[f]);
                 for (int k2 = 0; k2 < D - 1; ++k2) tvs.push_back(face_vst[(size_t)f * (D - 1) + k2]);
@@ -1033,6 +1096,36 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
   }
   c->ntiles = (int)tiles.size();
   c->max_tile_faces = maxf;
+  // ---- optional coarse/fine balance: the pairs (tile faces and blocks of
+  // the coarse face and its fine faces) and the tile face -> entry map
+  std::vector<RfPair> rfp;
+  std::vector<int> tmod(tst.size(), -1);
+  for (int i = 0; i < m->nmdy; ++i) {
+    const int32_t *r = m->mdy + (size_t)i * 6;
+    RfPair p{};
+    p.mode = r[0];
+    const int nf = rf_nfine(p.mode);
+    const int j = (int)rfp.size();
+    for (int k = 0; k < 4; ++k) p.fslot[k] = -1;
+    for (int k = 0; k <= nf; ++k) {
+      const int df = c->fh2d[r[1 + k]];
+      // each balanced face lies in exactly one tile (its other cell is a
+      // ghost) and in one pair
+      if (df < 0 || face_ntile[df] != 1 || tmod[face_slot0[df]] >= 0) return IBG_ERR_ARG;
+      const int sl = face_slot0[df];
+      tmod[sl] = j * 5 + k;
+      if (k == 0) {
+        p.cslot = sl;
+        p.cblk = face_tblk[df];
+      } else {
+        if (k > 1 && face_tblk[df] != p.fblk) return IBG_ERR_ARG;
+        p.fslot[k - 1] = sl;
+        p.fblk = face_tblk[df];
+      }
+    }
+    rfp.push_back(p);
+  }
+  c->nrf = (int)rfp.size();
   // ---- domain ghosts: edge entries first, then corners (whose refcell is an
   // edge ghost)
   std::vector<int4> dmg;
@@ -1176,6 +1269,10 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
   UP(c->tiles, tiles);
   UP(c->tst, tst);
   UP(c->tvs, tvs);
+  UP(c->rfp, rfp);
+  UP(c->tmod, tmod);
+  if ((st = dalloc_bytes(c, &c->rbal, (size_t)std::max(c->nrf, 1) * 5 * 5 * sizeof(double)))) return st;
+  if ((st = dalloc_bytes(c, &c->rbalv, (size_t)std::max(c->nrf, 1) * 5 * 5 * sizeof(double)))) return st;
   UP(c->tdir, tdir);
   UP(c->slots, slots);
   UP(c->face_st, face_st);

@@ -107,7 +107,48 @@ struct ViscArgs {
   R *mutf;                   // [Nn] frozen face-flux term (turb)
   int viscous, turb;
   const unsigned long long *err;
+  // optional coarse/fine balance: tile face -> entry of rbalv (-1: none);
+  // rbalv NULL when the balance is off
+  const int *tmod;
+  const R *rbalv;
 };
+
+template <typename R, int D>
+__device__ __forceinline__ ViscPhys<R, D> visc_phys(const BlkInfo &bi, const DevPhys &P) {
+  ViscPhys<R, D> vp;
+  vp.Rg = (R)P.Rg; vp.Re = (R)P.Re; vp.qk = (R)P.qk; vp.SoT = (R)P.SoT;
+  vp.ih0 = (R)bi.invh[0]; vp.ih1 = (R)bi.invh[1]; vp.ih2 = (R)bi.invh[2];
+  vp.ar0 = (R)bi.area[0]; vp.ar1 = (R)bi.area[1]; vp.ar2 = (R)bi.area[2];
+  vp.qkt = (R)P.qkt_den;
+  return vp;
+}
+
+// The viscous flux (and, with the SA closure, the working variable's face
+// flux mf) of tile face k (an index into tst / tvs / tdir).
+template <typename R, int D>
+__device__ __forceinline__ void visc_face(const ViscArgs<R, D> &a, const ViscPhys<R, D> &vp, int k,
+                                          R *out, R &mf) {
+  const int4 st = a.tst[k];
+  const int dir = a.tdir[k];
+  const Prim<R, D> L = pload<R, D>(a.W, st.y);
+  const Prim<R, D> Rr = pload<R, D>(a.W, st.z);
+  Prim<R, D> t[(D == 3) ? 8 : 4];
+  const int4 v0 = a.tvs[(size_t)k * (D - 1)];
+  t[0] = pload<R, D>(a.W, v0.x); t[1] = pload<R, D>(a.W, v0.y);
+  t[2] = pload<R, D>(a.W, v0.z); t[3] = pload<R, D>(a.W, v0.w);
+  if (D == 3) {
+    const int4 v1 = a.tvs[(size_t)k * (D - 1) + 1];
+    t[4 % ((D == 3) ? 8 : 4)] = pload<R, D>(a.W, v1.x);
+    t[5 % ((D == 3) ? 8 : 4)] = pload<R, D>(a.W, v1.y);
+    t[6 % ((D == 3) ? 8 : 4)] = pload<R, D>(a.W, v1.z);
+    t[7 % ((D == 3) ? 8 : 4)] = pload<R, D>(a.W, v1.w);
+  }
+  mf = R(0);
+  if (a.turb)
+    visc_flux<R, D>(L, Rr, t, dir, vp, out, true, a.nu[st.y], a.nu[st.z], &mf);
+  else
+    visc_flux<R, D>(L, Rr, t, dir, vp, out);
+}
 
 // Once per step (ComptViscousStress NS_Solver.cpp:365-629 is frozen across
 // the stages, NS_Solver.H:669-699): viscous fluxes of the tile faces into
@@ -128,33 +169,17 @@ __device__ __forceinline__ void visc_prep_tile(const ViscArgs<R, D> &a, const Ti
   const BlkInfo &bi = a.blk[ti.blk];
   const DevPhys &P = a.phys[bi.case_id];
   if (a.viscous) {
-    ViscPhys<R, D> vp;
-    vp.Rg = (R)P.Rg; vp.Re = (R)P.Re; vp.qk = (R)P.qk; vp.SoT = (R)P.SoT;
-    vp.ih0 = (R)bi.invh[0]; vp.ih1 = (R)bi.invh[1]; vp.ih2 = (R)bi.invh[2];
-    vp.ar0 = (R)bi.area[0]; vp.ar1 = (R)bi.area[1]; vp.ar2 = (R)bi.area[2];
-    vp.qkt = (R)P.qkt_den;
+    const ViscPhys<R, D> vp = visc_phys<R, D>(bi, P);
     for (int f = threadIdx.x; f < ti.nface; f += blockDim.x) {
-      const int4 st = a.tst[ti.face0 + f];
-      const int dir = a.tdir[ti.face0 + f];
-      const Prim<R, D> L = pload<R, D>(a.W, st.y);
-      const Prim<R, D> Rr = pload<R, D>(a.W, st.z);
-      Prim<R, D> t[(D == 3) ? 8 : 4];
-      const int4 v0 = a.tvs[(size_t)(ti.face0 + f) * (D - 1)];
-      t[0] = pload<R, D>(a.W, v0.x); t[1] = pload<R, D>(a.W, v0.y);
-      t[2] = pload<R, D>(a.W, v0.z); t[3] = pload<R, D>(a.W, v0.w);
-      if (D == 3) {
-        const int4 v1 = a.tvs[(size_t)(ti.face0 + f) * (D - 1) + 1];
-        t[4 % ((D == 3) ? 8 : 4)] = pload<R, D>(a.W, v1.x);
-        t[5 % ((D == 3) ? 8 : 4)] = pload<R, D>(a.W, v1.y);
-        t[6 % ((D == 3) ? 8 : 4)] = pload<R, D>(a.W, v1.z);
-        t[7 % ((D == 3) ? 8 : 4)] = pload<R, D>(a.W, v1.w);
-      }
       R out[NV];
-      R mf = R(0);
-      if (a.turb)
-        visc_flux<R, D>(L, Rr, t, dir, vp, out, true, a.nu[st.y], a.nu[st.z], &mf);
-      else
-        visc_flux<R, D>(L, Rr, t, dir, vp, out);
+      R mf;
+      visc_face<R, D>(a, vp, ti.face0 + f, out, mf);
+      if (a.rbalv) {
+        // a coarse/fine face of the optional balance: its balanced flux
+        const int q = a.tmod[ti.face0 + f];
+        if (q >= 0)
+          for (int v = 0; v < NV; ++v) out[v] = a.rbalv[(size_t)q * NV + v];
+      }
       V o;
       o.x = out[0]; o.y = out[1]; o.z = out[2]; o.w = out[3];
       fl4[f] = o;
@@ -255,6 +280,10 @@ struct StageArgs {
   // the dynamic LDS
   int ntile;
   unsigned mlw_off;
+  // optional coarse/fine balance (last stage): tile face -> entry of rbal
+  // (-1: none); rbal NULL when the balance is off or not this stage's
+  const int *tmod;
+  const R *rbal;
 };
 
 // The fused first stage for the 2D FP64 variant only.  There the viscous
@@ -391,6 +420,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R, D> &a, int tile) {
       roe_flux_fast<D>(q0, q1, q2, q3, dir, areas[dir], ph, a.limiter, flux);
     else
       roe_flux<R, D>(q0, q1, q2, q3, dir, areas[dir], ph, a.limiter, flux);
+    if (a.rbal) {
+      // a coarse/fine face of the optional balance: its balanced flux
+      const int q = a.tmod[ti.face0 + f];
+      if (q >= 0)
+        for (int v = 0; v < NV; ++v) flux[v] = a.rbal[(size_t)q * NV + v];
+    }
     V o;
     o.x = flux[0]; o.y = flux[1]; o.z = flux[2]; o.w = flux[3];
     fl4[f] = o;

@@ -351,6 +351,7 @@ struct ibm_handle {
   std::vector<double> dmg_nmv, dist_dm;
   bool walls = false;
   std::vector<int32_t> res_ci, res_fi, res_blk, pro_ci, pro_fi, pro_blk;
+  std::vector<int32_t> mdy;   // [nmdy][6] flux-balance faces
   std::vector<int32_t> ib_cell, ib_intp, ib_body;
   std::vector<double> ib_coef, ib_hgdis, ib_nv, ib_attach;
   std::vector<int32_t> pat_intp, pat_case, pat_close;
@@ -621,6 +622,45 @@ void build_all(ibm_handle &H, std::vector<CaseMesh> &cms) {
           }
         }
     }
+  // coarse/fine flux-balance faces (AMR::FindModifyFace Fluxmodification.cpp:
+  // 5-289): for every prolongation pair -- a normal coarse cell whose sons
+  // are prolongation ghosts -- each face towards a first-layer restriction
+  // ghost (a coarse ghost whose sons are normal cells), with the faces of the
+  // coarse cell's sons on that side.  The sons follow the reference's
+  // fface[nx][ny]: first transverse axis outer, second inner.
+  for (size_t i = 0; i < H.pro_ci.size(); ++i) {
+    const int C = H.pro_ci[i];
+    const int bf = H.pro_blk[i] + 1;   // the fine level of the same case
+    const bool twod = H.blk_twod[bf] != 0;
+    const int32_t *fi = &H.pro_fi[i * 8];
+    for (int d = 0; d < dim; ++d)
+      for (int s2 = 0; s2 < 2; ++s2) {
+        int o[3] = {0, 0, 0};
+        o[d] = s2 ? 1 : -1;
+        const int P = H.nb[(size_t)C * 27 + (o[0] + 1) * 9 + (o[1] + 1) * 3 + (o[2] + 1)];
+        if (P < 0 || H.kind[P] != IBG_RES1) continue;
+        const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+        const int n1 = 2, n2 = (twod && d != 2) ? 1 : (dim == 3 ? 2 : 1);
+        const int mode = !twod ? 0 : (d == 2 ? 2 : 1);
+        H.mdy.push_back(mode);
+        H.mdy.push_back(H.cell_faces[(size_t)C * 6 + d * 2 + s2]);
+        int nf = 0;
+        for (int u = 0; u < n1; ++u)
+          for (int v = 0; v < n2; ++v) {
+            int idx[3];
+            idx[d] = (twod && d == 2) ? 0 : s2;   // a twod son spans the coarse z extent
+            idx[t1] = u;
+            idx[t2] = v;
+            const int son = fi[idx[0] * 4 + idx[1] * 2 + idx[2]];
+            if (son < 0) throw std::runtime_error("a prolongation pair without the son of a balance face");
+            const int ff = H.cell_faces[(size_t)son * 6 + d * 2 + s2];
+            if (ff < 0) throw std::runtime_error("a balance face's son without that face");
+            H.mdy.push_back(ff);
+            ++nf;
+          }
+        for (; nf < 4; ++nf) H.mdy.push_back(-1);
+      }
+  }
   // immersed boundary on the finest level of each case
   int patch_base = 0;
   for (int c = 0; c < nc; ++c) {
@@ -808,6 +848,8 @@ void fill_soa(ibm_handle &H) {
   if (H.walls) {
     m.dmg_nbr = H.dmg_nbr.data(); m.dmg_nmv = H.dmg_nmv.data(); m.dist_dm = H.dist_dm.data();
   }
+  m.nmdy = (int32_t)(H.mdy.size() / 6);
+  m.mdy = m.nmdy ? H.mdy.data() : nullptr;
 }
 
 }  // namespace
@@ -829,6 +871,15 @@ ibm_handle *ibm_build(const ibm_case *cases, int32_t ncases) {
       build_segments(cms[c]);
       build_patches(cms[c]);
       build_levels(cms[c]);
+      if (cs.refine_only) {
+        // the bodies have placed the refinement bands; the flow sees none of
+        // them (no solid or immersed-boundary cells, no wall patches)
+        CaseMesh &cm = cms[c];
+        cm.cs.nbody = 0;
+        cm.patches.clear(); cm.segxy.clear(); cm.seg_body.clear(); cm.seg_first.clear();
+        cm.seg_cx.clear(); cm.seg_cy.clear(); cm.seg_nx.clear(); cm.seg_ny.clear();
+        cm.seg_len.clear();
+      }
     }
     ibm_handle *H = new ibm_handle();
     H->cases.assign(cases, cases + ncases);

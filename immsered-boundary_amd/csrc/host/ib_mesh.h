@@ -52,6 +52,10 @@ typedef struct ibm_case {
      -13 inlet, -12 outlet, -50, -50 (Userfunction_cy_naca15.cpp:283-346);
      -81 is the wall of the TURBULENCE build (BCValues.H:89-125)          */
   int32_t face_bc[4];
+  /* 1: the bodies steer the refinement bands only -- no solid or immersed-
+     boundary cells, no wall patches, signdis 9999 (a body-free flow on the
+     refined level set, e.g. for conservation checks); 0: immersed bodies   */
+  int32_t refine_only;
 } ibm_case;
 
 typedef struct ibm_handle ibm_handle;

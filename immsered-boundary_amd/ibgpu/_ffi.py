@@ -48,6 +48,7 @@ class MeshSoA(C.Structure):
         ("pat_area", f64p), ("pat_case", i32p),
         ("nbody", C.c_int32), ("body_case", i32p), ("body_center", f64p),
         ("dmg_nbr", i32p), ("dmg_nmv", f64p), ("dist_dm", f64p),
+        ("nmdy", C.c_int32), ("mdy", i32p),
     ]
 
 
@@ -93,7 +94,7 @@ class Config(C.Structure):
         ("mode", C.c_int32), ("nstage", C.c_int32), ("alpha", C.c_double * 4),
         ("dt_var_step", C.c_int32), ("temporal_refine", C.c_int32),
         ("limiter", C.c_int32), ("viscous", C.c_int32), ("tile", C.c_int32),
-        ("turbulence", C.c_int32), ("schedule", C.c_int32),
+        ("turbulence", C.c_int32), ("schedule", C.c_int32), ("reflux", C.c_int32),
     ]
 
 
@@ -109,7 +110,7 @@ class Case(C.Structure):
         ("span", C.c_double), ("span_patches", C.c_int32),
         ("gama", C.c_double), ("Ma", C.c_double), ("Re", C.c_double),
         ("Pr", C.c_double), ("cfl", C.c_double), ("backpressure", C.c_double),
-        ("dt_init", C.c_double), ("face_bc", C.c_int32 * 4),
+        ("dt_init", C.c_double), ("face_bc", C.c_int32 * 4), ("refine_only", C.c_int32),
     ]
 
 

@@ -24,7 +24,7 @@ class IBGError(RuntimeError):
 
 
 def make_config(mode="fp64", nstage=3, dt_var_step=None, temporal_refine=True, limiter=0,
-                viscous=True, tile=0, nlevels=1, turbulence=False, schedule=0):
+                viscous=True, tile=0, nlevels=1, turbulence=False, schedule=0, reflux=False):
     cfg = _ffi.Config()
     cfg.mode = MODES[mode]
     cfg.nstage = nstage
@@ -38,17 +38,22 @@ def make_config(mode="fp64", nstage=3, dt_var_step=None, temporal_refine=True, l
     cfg.tile = tile
     cfg.turbulence = 1 if turbulence else 0
     cfg.schedule = schedule
+    cfg.reflux = 1 if reflux else 0
     return cfg
 
 
 class Engine:
     def __init__(self, mesh, mode="fp64", nstage=3, temporal_refine=True, limiter=0,
-                 viscous=True, tile=0, dt_var_step=None, turbulence=False, schedule=0):
+                 viscous=True, tile=0, dt_var_step=None, turbulence=False, schedule=0,
+                 reflux=False):
+        """reflux: the optional coarse/fine flux balance (BalanceFaceFlux
+        Fluxmodification.cpp:291-358, inactive in the reference), which
+        needs temporal_refine=False."""
         self.lib = _ffi.gpu_lib()
         self.mesh = mesh
         self.turbulence = bool(turbulence)
         self.cfg = make_config(mode, nstage, dt_var_step, temporal_refine, limiter, viscous, tile,
-                               mesh.nlevels, turbulence, schedule)
+                               mesh.nlevels, turbulence, schedule, reflux)
         self.ctx = C.c_void_p()
         self._chk(self.lib.ibg_create(C.byref(self.cfg), C.byref(self.ctx)), "ibg_create")
         self._chk(self.lib.ibg_upload_mesh(self.ctx, mesh.soa_ptr, mesh.phys), "ibg_upload_mesh")

@@ -47,6 +47,9 @@ class CaseSpec:
     # (r = nu / (S~ kappa^2 d^2) with nu = 0 and |Omega| = 0): NaN after
     # ~1400 steps of the 2.6M-cell case.
     inlet_nut: object = None
+    # True: the section steers the refinement bands only (no solid or IB
+    # cells, no wall patches): a body-free flow on the refined level set
+    refine_only: bool = False
 
     def to_ctypes(self):
         c = _ffi.Case()
@@ -78,6 +81,7 @@ class CaseSpec:
         c.cfl = self.cfl
         c.backpressure = self.backpressure
         c.dt_init = 1.0e30
+        c.refine_only = 1 if self.refine_only else 0
         for j in range(4):
             c.face_bc[j] = int(self.face_bc[j])
         return c
@@ -203,6 +207,9 @@ class Mesh:
         self.ib_hgdis = v(m.ib_hgdis, (m.nib,)) if m.nib else np.zeros(0)
         self.pat_intp = v(m.pat_intp, (m.npatch, 8)) if m.npatch else np.zeros((0, 8), np.int32)
         self.pat_coef = v(m.pat_coef, (m.npatch, 8)) if m.npatch else np.zeros((0, 8))
+        # coarse/fine flux-balance faces: form, coarse face, fine faces
+        self.nmdy = m.nmdy
+        self.mdy = v(m.mdy, (m.nmdy, 6)) if m.nmdy else np.zeros((0, 6), np.int32)
         self.centers = np.zeros((n, 3))
         host.ibm_cell_centers(self._h, self.centers.ctypes.data_as(_ffi.f64p))
 

@@ -128,6 +128,17 @@ typedef struct ibg_mesh_soa {
   const double  *dmg_nmv;      /* [ndmg][3] Domainghost::nmv, wall normal     */
   const double  *dist_dm;      /* [ncells] BoxtoWall::distance_to_dm
                                   (Mesh.cpp:1638-1757), 9999 away from walls  */
+  /* coarse/fine flux-balance faces: Facepair{cface, fface[2][2]}
+     GhostPair.H:67-84 as AMR::FindModifyFace Fluxmodification.cpp:5-289
+     builds them, one per face between a normal cell and a coarse ghost
+     whose sons are normal cells.  mdy [nmdy][6]: the balance form (0: fine
+     level refined in x, y and z, four fine faces; 1: fine level refined in
+     x and y only, x or y face, two fine faces; 2: the same, z face, four
+     fine faces), the coarse face, then the fine faces (faces of the coarse
+     cell's sons on that side, the first transverse axis outer; -1 past the
+     count).  Used only by the optional balance (ibg_config.reflux).        */
+  int32_t nmdy;
+  const int32_t *mdy;
 } ibg_mesh_soa;
 
 /* Nondimensional constants of one case, FlowConstants.cpp:5-44 and the case
@@ -176,6 +187,14 @@ typedef struct ibg_config {
                                   the device holds at once) looping over the
                                   tiles, the learned closure's weights staged
                                   once per workgroup                          */
+  int32_t reflux;              /* 1: coarse/fine flux balance, BalanceFaceFlux
+                                  Fluxmodification.cpp:291-358, on the viscous
+                                  fluxes once per step and on the inviscid
+                                  fluxes of the last stage (the call sites the
+                                  reference leaves commented, NS_Solver.H:672,
+                                  686).  The reference compiles it only
+                                  without TEMPORAL_REFINE: requires
+                                  temporal_refine = 0.  Default 0 (off).      */
 } ibg_config;
 
 typedef struct ibg_ctx ibg_ctx;

@@ -300,6 +300,37 @@ void orc_face_flux(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *
     }
 }
 
+/* BalanceFaceFlux Fluxmodification.cpp:291-358.  Per Facepair (mdy row:
+ * form, coarse face, fine faces), FlowVec arithmetic in the reference's
+ * order:
+ *   form 0 (fine level refined in x,y,z, :299-318): c_old = F_c*0.25;
+ *     F_c *= 0.5; per fine face F_f *= 0.5, F_c += F_f, F_f += c_old*0.5
+ *   form 1 (x,y-refined fine level, x/y face, :324-334): c_old = F_c;
+ *     F_c *= 0.5; per fine face (two) F_f *= 0.5, F_c += F_f,
+ *     F_f += c_old*0.25
+ *   form 2 (x,y-refined fine level, z face, :336-348): as form 1 over four
+ *     fine faces with F_f += c_old*0.125.                                  */
+void orc_balance_flux(const ibg_mesh_soa *m, double *flux5) {
+  for (int i = 0; i < m->nmdy; ++i) {
+    const int32_t *r = m->mdy + (size_t)i * 6;
+    const int form = r[0];
+    const int nf = form == 1 ? 2 : 4;
+    const double w = form == 0 ? 0.5 : (form == 1 ? 0.25 : 0.125);
+    double *fc = &flux5[(size_t)r[1] * 5];
+    double cold[5];
+    for (int v = 0; v < 5; ++v) cold[v] = form == 0 ? fc[v] * 0.25 : fc[v];
+    for (int v = 0; v < 5; ++v) fc[v] *= 0.5;
+    for (int k = 0; k < nf; ++k) {
+      double *ff = &flux5[(size_t)r[2 + k] * 5];
+      for (int v = 0; v < 5; ++v) {
+        ff[v] *= 0.5;
+        fc[v] += ff[v];
+        ff[v] += cold[v] * w;
+      }
+    }
+  }
+}
+
 /* ComptViscousStress NS_Solver.cpp:365-629 (laminar) with VarReflect /
  * pfpkesai0 / pfpkesai1 of NS_Solver.H:115-154.  The cell metric keisa is
  * diag(1/dh) (ComptCellParams MeshOp.H:10-12); its zero off-diagonal terms
@@ -854,9 +885,13 @@ int32_t orc_step(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg *cf
     orc_old_cons(m, cfg, fv, ts, cvo);
     code = orc_bc_fill(m, ph, cfg, fv, ts);
   }
-  if (!code && cfg->viscous) orc_viscous_flux(m, ph, cfg, fv, ts, vs);
+  if (!code && cfg->viscous) {
+    orc_viscous_flux(m, ph, cfg, fv, ts, vs);
+    if (cfg->reflux) orc_balance_flux(m, vs);   /* NS_Solver.H:672 */
+  }
   for (int st = 0; st < cfg->nstage && !code; ++st) {
     orc_face_flux(m, ph, cfg, fv, ts, ff);
+    if (cfg->reflux && st == cfg->nstage - 1) orc_balance_flux(m, ff);   /* NS_Solver.H:686 */
     orc_source(m, cfg, ff, vs, ts, src);
     code = orc_update(m, ph, cfg, cvo, src, dt, ts, st, cvn, fv, bad_cell);
   }
@@ -1218,9 +1253,13 @@ int32_t orc_step_sa(const ibg_mesh_soa *m, const ibg_physics *ph, const orc_cfg 
     memcpy(nuo, nut, sizeof(double) * N);   /* c_vis_old */
     code = bc_fill(m, ph, cfg, fv, nut, ts);
   }
-  if (!code) orc_viscous_flux_sa(m, ph, cfg, fv, nut, ts, vs, mf);
+  if (!code) {
+    orc_viscous_flux_sa(m, ph, cfg, fv, nut, ts, vs, mf);
+    if (cfg->reflux) orc_balance_flux(m, vs);   /* NS_Solver.H:672 */
+  }
   for (int st = 0; st < cfg->nstage && !code; ++st) {
     orc_face_flux(m, ph, cfg, fv, ts, ff);
+    if (cfg->reflux && st == cfg->nstage - 1) orc_balance_flux(m, ff);   /* NS_Solver.H:686 */
     orc_source(m, cfg, ff, vs, ts, src);
     orc_mut_source(m, ph, cfg, fv, nut, mf, ts, msrc);
     code = orc_update(m, ph, cfg, cvo, src, dt, ts, st, cvn, fv, bad_cell);

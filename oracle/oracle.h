@@ -27,6 +27,9 @@ typedef struct orc_cfg {
   int32_t temporal_refine;   /* TEMPORAL_REFINE                            */
   int32_t limiter;           /* 0 van Leer, 1 van Albada                   */
   int32_t viscous;           /* VISCOUSITY                                 */
+  int32_t reflux;            /* optional coarse/fine balance (off in the
+                                reference: its call sites are commented);
+                                meaningful only without TEMPORAL_REFINE    */
 } orc_cfg;
 
 /* schedule (mpiparams.cpp:186-212) */
@@ -42,6 +45,10 @@ double orc_muscl_right(double q_i, double qi, double qi1, int32_t limiter);
 void orc_roe_flux(const double *fLL, const double *fL, const double *fR,
                   const double *fRR, int32_t dir, double area,
                   const ibg_physics *ph, int32_t limiter, double *flux5);
+
+/* BalanceFaceFlux Fluxmodification.cpp:291-358 on a face-flux array
+   [nfaces][5] over the mesh's flux-balance faces (mdy)                    */
+void orc_balance_flux(const ibg_mesh_soa *m, double *flux5);
 
 /* per-phase functions over the whole mesh; only marched blocks are touched */
 void orc_face_flux(const ibg_mesh_soa *m, const ibg_physics *ph,

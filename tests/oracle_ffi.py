@@ -22,7 +22,7 @@ f64p, i32p, f32p = _ffi.f64p, _ffi.i32p, _ffi.f32p
 class OrcCfg(C.Structure):
     _fields_ = [("nstage", C.c_int32), ("alpha", C.c_double * 4),
                 ("dt_var_step", C.c_int32), ("temporal_refine", C.c_int32),
-                ("limiter", C.c_int32), ("viscous", C.c_int32)]
+                ("limiter", C.c_int32), ("viscous", C.c_int32), ("reflux", C.c_int32)]
 
 
 class OrcSurface(C.Structure):
@@ -92,6 +92,7 @@ def lib():
         "orc_residual_nut": (None, [M, G, f64p, C.c_int32, f64p, f64p]),
         "orc_step_sa": (C.c_int32, [M, P, G, f64p, f64p, f64p, C.c_int32, f64p, f64p, f64p,
                                     f64p, f64p, f64p, C.POINTER(C.c_int64)]),
+        "orc_balance_flux": (None, [M, f64p]),
         "orc_bf16": (C.c_float, [C.c_float]),
         "orc_mlp_forward": (None, [f32p, f32p, C.c_int64, f32p]),
         "orc_mlp_grad": (None, [f32p, f32p, f32p, C.c_int64, f64p, f64p]),
@@ -303,7 +304,7 @@ class PatchedMesh:
         return getattr(self._m, k)
 
 
-def cfg(nstage=3, alpha=None, dt_var_step=1, temporal_refine=1, limiter=0, viscous=1):
+def cfg(nstage=3, alpha=None, dt_var_step=1, temporal_refine=1, limiter=0, viscous=1, reflux=0):
     c = OrcCfg()
     c.nstage = nstage
     if alpha is None:
@@ -314,6 +315,7 @@ def cfg(nstage=3, alpha=None, dt_var_step=1, temporal_refine=1, limiter=0, visco
     c.temporal_refine = temporal_refine
     c.limiter = limiter
     c.viscous = viscous
+    c.reflux = reflux
     return c
 
 

@@ -53,6 +53,11 @@ def test_bad_arguments_rejected():
     cfg.mode, cfg.nstage = 0, 2          # only 1, 3 or 4 stages exist
     ctx = C.c_void_p()
     assert lib.ibg_create(C.byref(cfg), C.byref(ctx)) == 1
+    # the coarse/fine balance exists only without temporal refinement
+    cfg.nstage, cfg.reflux, cfg.temporal_refine = 4, 1, 1
+    assert lib.ibg_create(C.byref(cfg), C.byref(ctx)) == 1
+    cfg.reflux, cfg.temporal_refine = 2, 0
+    assert lib.ibg_create(C.byref(cfg), C.byref(ctx)) == 1
     assert lib.ibg_destroy(None) == 1
     assert lib.ibg_check_mesh(None, None) == 1
 
@@ -86,6 +91,7 @@ def test_clean_mesh_passes(tiny_mesh):
     ("ib_intp", "IB interpolation cell"),
     ("ib_into_ib", "image point interpolates from an IB cell"),
     ("pat_intp", "patch interpolation cell missing"),
+    ("mdy", "balance fine face out of range"),
     ("blk_start", "blk_start"),
 ])
 def test_corrupted_mesh_rejected(tiny_mesh, what, expect):
@@ -112,6 +118,9 @@ def test_corrupted_mesh_rejected(tiny_mesh, what, expect):
         arr, idx, bad = m.ib_intp, (0, 1), int(m.ib_cell[1])
     elif what == "pat_intp":
         arr, idx, bad = m.pat_intp, (0, 1), m.ncells + 1
+    elif what == "mdy":
+        # a two-face (x,y-refined) balance row with a third fine face
+        arr, idx, bad = m.mdy, (0, 4), 5
     else:
         arr = v(m.soa.blk_start, (m.ncases * m.nlevels + 1,))
         idx, bad = (1,), -5
