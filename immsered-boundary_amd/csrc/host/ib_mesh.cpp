@@ -15,7 +15,11 @@
 #include <functional>
 #include <stdexcept>
 #include <string>
+#include <climits>
+#include <memory>
+#include <parallel/algorithm>
 #include <unordered_map>
+#include <utility>
 #include <vector>
 
 namespace {
@@ -25,10 +29,59 @@ std::string g_last_error;
 enum { NORMAL = IBG_NORMAL, DMGHOST = IBG_DMGHOST, RES1 = IBG_RES1,
        RES2 = IBG_RES2, PRO = IBG_PRO };
 
+// An allocator that leaves new elements uninitialised: for the per-cell
+// arrays that the flattening loop writes in full (in parallel, so the pages
+// are first touched by the threads that fill them).
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+  template <class U> struct rebind { using other = NoInitAlloc<U>; };
+  NoInitAlloc() = default;
+  template <class U> NoInitAlloc(const NoInitAlloc<U> &) {}
+  template <class U> void construct(U *p) noexcept { ::new ((void *)p) U; }
+  template <class U, class... A> void construct(U *p, A &&...a) {
+    ::new ((void *)p) U(std::forward<A>(a)...);
+  }
+};
+template <class T> using fvec = std::vector<T, NoInitAlloc<T>>;
+
 inline int64_t pack_key(int ix, int iy, int iz) {
   // 21 bits per coordinate, offset so that ghosts (>= -4) stay positive
   return ((int64_t)(ix + 8) << 42) | ((int64_t)(iy + 8) << 21) | (int64_t)iz;
 }
+
+// Index of the cells of one level by integer key: a dense table over the
+// bounding box of the keys (-1 where no cell is).  A refined level covers a
+// band around the bodies, so its box holds a small multiple of its cells;
+// a lookup is one load where a hash map costs a probe chain.
+struct KeyGrid {
+  int x0 = 0, y0 = 0, z0 = 0, wx = 0, wy = 0, wz = 0;
+  std::vector<int32_t> idx;
+  template <class KeyOf>
+  void build(int n, KeyOf key) {
+    idx.clear();
+    wx = wy = wz = 0;
+    if (n <= 0) return;
+    int lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    for (int i = 0; i < n; ++i) {
+      int k[3];
+      key(i, k);
+      for (int d = 0; d < 3; ++d) { lo[d] = std::min(lo[d], k[d]); hi[d] = std::max(hi[d], k[d]); }
+    }
+    x0 = lo[0]; y0 = lo[1]; z0 = lo[2];
+    wx = hi[0] - lo[0] + 1; wy = hi[1] - lo[1] + 1; wz = hi[2] - lo[2] + 1;
+    idx.assign((size_t)wx * wy * wz, -1);
+    for (int i = 0; i < n; ++i) {
+      int k[3];
+      key(i, k);
+      idx[((size_t)(k[2] - z0) * wy + (k[1] - y0)) * wx + (k[0] - x0)] = i;
+    }
+  }
+  int find(int ix, int iy, int iz) const {
+    const unsigned x = (unsigned)(ix - x0), y = (unsigned)(iy - y0), z = (unsigned)(iz - z0);
+    if (x >= (unsigned)wx || y >= (unsigned)wy || z >= (unsigned)wz) return -1;
+    return idx[((size_t)z * wy + y) * wx + x];
+  }
+};
 
 inline uint64_t spread2(uint32_t v) {
   uint64_t x = v & 0xffffffffu;
@@ -61,20 +114,18 @@ struct LCell {
 
 struct Level {
   std::vector<LCell> cells;
-  std::unordered_map<int64_t, int> map;
+  KeyGrid map;
   int nzl = 1;          // z cells on this level (periodic)
   double h[3];
   bool twod = true;     // refined from the coarser level in x,y only
   void rebuild_map() {
-    map.clear();
-    map.reserve(cells.size() * 2);
-    for (int i = 0; i < (int)cells.size(); ++i)
-      map[pack_key(cells[i].ix, cells[i].iy, cells[i].iz)] = i;
+    map.build((int)cells.size(), [&](int i, int *k) {
+      k[0] = cells[i].ix; k[1] = cells[i].iy; k[2] = cells[i].iz;
+    });
   }
   int find(int ix, int iy, int iz) const {
     int zz = ((iz % nzl) + nzl) % nzl;   // z is periodic (Userfunction_cy_naca15.cpp:176)
-    auto it = map.find(pack_key(ix, iy, zz));
-    return it == map.end() ? -1 : it->second;
+    return map.find(ix, iy, zz);
   }
 };
 
@@ -98,7 +149,25 @@ struct CaseMesh {
   std::vector<std::vector<double>> segxy;  // per body: points
   std::vector<int> seg_body, seg_first;    // segment -> body, point index
   std::vector<double> seg_cx, seg_cy, seg_nx, seg_ny, seg_len;
+  // runs of kSegRun consecutive segment centers with their bounding boxes
+  std::vector<double> run_x0, run_x1, run_y0, run_y1;
 };
+
+constexpr int kSegRun = 8;
+
+void build_segment_runs(CaseMesh &m) {
+  const int ns = (int)m.seg_cx.size();
+  m.run_x0.clear(); m.run_x1.clear(); m.run_y0.clear(); m.run_y1.clear();
+  for (int s0 = 0; s0 < ns; s0 += kSegRun) {
+    double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+    for (int s = s0; s < std::min(ns, s0 + kSegRun); ++s) {
+      x0 = std::min(x0, m.seg_cx[s]); x1 = std::max(x1, m.seg_cx[s]);
+      y0 = std::min(y0, m.seg_cy[s]); y1 = std::max(y1, m.seg_cy[s]);
+    }
+    m.run_x0.push_back(x0); m.run_x1.push_back(x1);
+    m.run_y0.push_back(y0); m.run_y1.push_back(y1);
+  }
+}
 
 void build_segments(CaseMesh &m) {
   const ibm_case &c = m.cs;
@@ -127,18 +196,40 @@ void build_segments(CaseMesh &m) {
   }
 }
 
-// nearest segment center to (x,y).  A section has a few hundred segments, so
-// a straight scan (vectorised, threads over cells) beats any search structure.
+// Nearest segment center to (x,y): the smallest dx^2 + dy^2, the lowest
+// index among equal ones (what a scan with a strict < in index order keeps).
+// The runs' boxes bound the distance of their centers from below (the same
+// rounded differences, monotone), so a run whose bound exceeds the best
+// distance so far cannot hold the answer and is skipped; the run with the
+// smallest bound is scanned first.
 int nearest_segment(const CaseMesh &m, double x, double y) {
   const int ns = (int)m.seg_cx.size();
+  const int nr = (int)m.run_x0.size();
+  if (ns == 0) return -1;
+  const double *cx = m.seg_cx.data(), *cy = m.seg_cy.data();
   int best = -1;
   double bd = 1e300;
-  const double *cx = m.seg_cx.data(), *cy = m.seg_cy.data();
-  for (int s = 0; s < ns; ++s) {
-    double dx = x - cx[s], dy = y - cy[s];
-    double d = dx * dx + dy * dy;
-    if (d < bd) { bd = d; best = s; }
+  auto scan = [&](int r) {
+    const int s1 = std::min(ns, (r + 1) * kSegRun);
+    for (int s = r * kSegRun; s < s1; ++s) {
+      const double dx = x - cx[s], dy = y - cy[s];
+      const double d = dx * dx + dy * dy;
+      if (d < bd || (d == bd && s < best)) { bd = d; best = s; }
+    }
+  };
+  double lbs[4096];
+  double *lb = nr <= 4096 ? lbs : new double[nr];
+  int r0 = 0;
+  for (int r = 0; r < nr; ++r) {
+    const double ex = std::max(0.0, std::max(m.run_x0[r] - x, x - m.run_x1[r]));
+    const double ey = std::max(0.0, std::max(m.run_y0[r] - y, y - m.run_y1[r]));
+    lb[r] = ex * ex + ey * ey;
+    if (lb[r] < lb[r0]) r0 = r;
   }
+  scan(r0);
+  for (int r = 0; r < nr; ++r)
+    if (r != r0 && lb[r] <= bd) scan(r);
+  if (lb != lbs) delete[] lb;
   return best;
 }
 
@@ -223,6 +314,7 @@ void build_levels(CaseMesh &m) {
     }
     // tag = 0: band rule of TagtheBox (Userfunction_cy_naca15.cpp:589-600)
     std::vector<int8_t> tag(n, -1);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) {
       if (lc.cells[i].kind != NORMAL) continue;
       if (sd[i] > c.band_lo[l] && sd[i] < c.band_hi[l]) tag[i] = 0;
@@ -231,6 +323,7 @@ void build_levels(CaseMesh &m) {
     // two cells of it (x,y) is a normal cell of this level, so that the new
     // prolongation ring and its gradient stencil lie on normal cells.
     std::vector<int8_t> keep(n, 0);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) {
       if (tag[i] != 0) continue;
       const LCell &cc = lc.cells[i];
@@ -246,22 +339,31 @@ void build_levels(CaseMesh &m) {
     // TagRefineGhost AMRGhost_refine.cpp:30-98: tag 1 for tagged cells with
     // an untagged 26-neighbour (which become prolongation cells, tag -2),
     // tag 2 for tagged cells with an untagged cell two cells away on an axis.
+    // Each cell's outcome depends on the tags before the pass only (the
+    // pass's own writes never turn a cell into or out of the "untagged"
+    // class), so each cell gathers from its neighbours.
     const int zr = d3 ? 1 : 0;
+    std::vector<int8_t> t2(tag);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) {
-      if (tag[i] < 0 || tag[i] == -2) continue;
-      if (tag[i] != 0 && tag[i] != 1) continue;
       const LCell &cc = lc.cells[i];
+      const bool tagged = tag[i] == 0;
+      bool hit = false, ring = false;
       for (int dk = -zr; dk <= zr; ++dk)
         for (int dj = -1; dj <= 1; ++dj)
           for (int di = -1; di <= 1; ++di) {
             int a = lc.find(cc.ix + di, cc.iy + dj, cc.iz + dk);
             if (a < 0) continue;
-            if (tag[a] == -1 || tag[a] == -2) {
-              tag[a] = -2;
-              if (lc.cells[a].kind != DMGHOST) tag[i] = 1;
-            }
+            if (tagged && tag[a] == -1 && lc.cells[a].kind != DMGHOST) hit = true;
+            // the neighbour relation is symmetric (offsets and periodic z)
+            if (!tagged && tag[a] == 0) ring = true;
           }
+      if (tagged && hit) t2[i] = 1;
+      if (!tagged && ring) t2[i] = -2;
     }
+    tag.swap(t2);
+    t2.assign(tag.begin(), tag.end());
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) {
       if (tag[i] != 0) continue;
       const LCell &cc = lc.cells[i];
@@ -271,9 +373,10 @@ void build_levels(CaseMesh &m) {
         if (a < 0) continue;
         int a2 = lc.find(cc.ix + 2 * dd[q][0], cc.iy + 2 * dd[q][1], cc.iz + 2 * dd[q][2]);
         if (a2 < 0) continue;
-        if ((tag[a2] == -1 || tag[a2] == -2) && lc.cells[a2].kind != DMGHOST) tag[i] = 2;
+        if ((tag[a2] == -1 || tag[a2] == -2) && lc.cells[a2].kind != DMGHOST) t2[i] = 2;
       }
     }
+    tag.swap(t2);
     // the prolongation ring must be made of normal cells
     for (int i = 0; i < n; ++i)
       if (tag[i] == -2 && lc.cells[i].kind != NORMAL)
@@ -342,11 +445,13 @@ struct ibm_handle {
   int ncases = 0, L = 0, dim = 2;
   std::vector<int32_t> blk_start, blk_nnormal, blk_twod;
   std::vector<double> blk_h;
-  std::vector<int32_t> key, nb, cell_faces, ibidx;
-  std::vector<int8_t> kind;
-  std::vector<double> signdis, center;
-  std::vector<int32_t> blk_face_start, face_cells;
-  std::vector<int8_t> face_dir;
+  fvec<int32_t> key, nb, cell_faces;
+  std::vector<int32_t> ibidx;
+  fvec<int8_t> kind;
+  fvec<double> signdis, center;
+  std::vector<int32_t> blk_face_start;
+  fvec<int32_t> face_cells;
+  fvec<int8_t> face_dir;
   std::vector<int32_t> dmg_cell, dmg_ref, dmg_type, dmg_nbr;
   std::vector<double> dmg_nmv, dist_dm;
   bool walls = false;
@@ -388,7 +493,7 @@ struct Finder {
 
 bool search_image(const Finder &F, const ibm_handle &H, const double *h,
                   const double *origin, const double *start, const double *nv,
-                  const std::vector<int8_t> &solid_g, const std::vector<int32_t> &ibidx_g,
+                  const fvec<int8_t> &solid_g, const std::vector<int32_t> &ibidx_g,
                   int32_t *intp, double *coef, double *hgdis_out, int *closecell) {
   double dk[3];
   for (int d = 0; d < 3; ++d) dk[d] = nv[d] / h[d];
@@ -457,7 +562,8 @@ void build_all(ibm_handle &H, std::vector<CaseMesh> &cms) {
       for (int i = 0; i < n; ++i) o[i] = i;
       std::vector<uint64_t> mk(n);
       for (int i = 0; i < n; ++i) mk[i] = morton(dim, lv.cells[i].ix, lv.cells[i].iy, lv.cells[i].iz);
-      std::sort(o.begin(), o.end(), [&](int a, int b) {
+      // keys are unique, so the order is too, whatever the sort
+      __gnu_parallel::sort(o.begin(), o.end(), [&](int a, int b) {
         int ka = lv.cells[a].kind == NORMAL ? 0 : lv.cells[a].kind;
         int kb = lv.cells[b].kind == NORMAL ? 0 : lv.cells[b].kind;
         if (ka != kb) return ka < kb;
@@ -478,15 +584,18 @@ void build_all(ibm_handle &H, std::vector<CaseMesh> &cms) {
   if (total > 2000000000LL) throw std::runtime_error("mesh exceeds int32 cell indices");
   H.blk_start[nblk] = (int32_t)total;
   const size_t N = (size_t)total;
-  H.key.assign(N * 3, 0);
-  H.kind.assign(N, 0);
-  H.nb.assign(N * 27, -1);
-  H.cell_faces.assign(N * 6, -1);
-  H.signdis.assign(N, 9999.0);
+  // written in full by the loop below (every cell of every block)
+  H.key.resize(N * 3);
+  H.kind.resize(N);
+  H.nb.resize(N * 27);
+  H.signdis.resize(N);
+  H.center.resize(N * 3);
+  fvec<int32_t> cpatch(N);
+  fvec<int8_t> solid(N);
+  H.cell_faces.resize(N * 6);
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)N * 6; ++i) H.cell_faces[i] = -1;
   H.ibidx.assign(N, -1);
-  H.center.assign(N * 3, 0.0);
-  std::vector<int32_t> cpatch(N, -1);
-  std::vector<int8_t> solid(N, 0);
   const int dmax = (dim == 3) ? 3 : 2;
 
   for (int c = 0; c < nc; ++c) {
@@ -523,32 +632,63 @@ void build_all(ibm_handle &H, std::vector<CaseMesh> &cms) {
   // 2*dim faces; a face is shared with the neighbour on that side, bi[0] is
   // the low side.  2D builds no z faces (ComputeFaceFlux skips fnv==2,
   // NS_Solver.H:728-730).
+  // Face numbering: per block, per direction, cells in index order, low side
+  // then high side; a face between two normal cells of the block is numbered
+  // at the lower-indexed of them.  Counted, scanned, then filled in parallel.
   H.blk_face_start.assign(nblk + 1, 0);
-  for (int c = 0; c < nc; ++c)
-    for (int l = 0; l < L; ++l) {
-      int b = c * L + l;
-      H.blk_face_start[b] = (int32_t)(H.face_cells.size() / 2);
-      int s = H.blk_start[b], e = s + H.blk_nnormal[b];
-      for (int d = 0; d < dmax; ++d) {
-        for (int g = s; g < e; ++g) {
-          for (int sd = 0; sd < 2; ++sd) {
-            int off[3] = {1, 1, 1};
-            off[d] = sd == 0 ? 0 : 2;
-            int a = H.nb[(size_t)g * 27 + off[0] * 9 + off[1] * 3 + off[2]];
-            if (a < 0) throw std::runtime_error("normal cell without a face neighbour");
-            if (H.cell_faces[(size_t)g * 6 + d * 2 + sd] >= 0) continue;
-            int f = (int)(H.face_cells.size() / 2);
-            int lo = sd == 0 ? a : g, hi = sd == 0 ? g : a;
-            H.face_cells.push_back(lo);
-            H.face_cells.push_back(hi);
-            H.face_dir.push_back((int8_t)d);
-            H.cell_faces[(size_t)g * 6 + d * 2 + sd] = f;
-            H.cell_faces[(size_t)a * 6 + d * 2 + (1 - sd)] = f;
+  {
+    size_t nf = 0;
+    std::vector<int32_t> cnt;
+    for (int c = 0; c < nc; ++c)
+      for (int l = 0; l < L; ++l) {
+        const int b = c * L + l;
+        H.blk_face_start[b] = (int32_t)nf;
+        const int s = H.blk_start[b], e = s + H.blk_nnormal[b], n = e - s;
+        for (int d = 0; d < dmax; ++d) {
+          int o0[3] = {1, 1, 1}, o1[3] = {1, 1, 1};
+          o0[d] = 0; o1[d] = 2;
+          const int q0 = o0[0] * 9 + o0[1] * 3 + o0[2], q1 = o1[0] * 9 + o1[1] * 3 + o1[2];
+          auto nbr = [&](int g, int sd) { return H.nb[(size_t)g * 27 + (sd ? q1 : q0)]; };
+          auto owns = [&](int g, int a) { return !(a >= s && a < e && a < g); };
+          cnt.assign((size_t)n + 1, 0);
+          int missing = 0;
+#pragma omp parallel for schedule(static) reduction(+ : missing)
+          for (int i = 0; i < n; ++i) {
+            const int g = s + i;
+            int k = 0;
+            for (int sd = 0; sd < 2; ++sd) {
+              const int a = nbr(g, sd);
+              if (a < 0) { ++missing; continue; }
+              k += owns(g, a) ? 1 : 0;
+            }
+            cnt[i] = k;
           }
+          if (missing) throw std::runtime_error("normal cell without a face neighbour");
+          int32_t run = 0;
+          for (int i = 0; i < n; ++i) { const int32_t k = cnt[i]; cnt[i] = run; run += k; }
+          if (nf + (size_t)run > (size_t)INT32_MAX) throw std::runtime_error("mesh exceeds int32 face indices");
+          H.face_cells.resize((nf + run) * 2);
+          H.face_dir.resize(nf + run);
+#pragma omp parallel for schedule(static)
+          for (int i = 0; i < n; ++i) {
+            const int g = s + i;
+            int f = (int)nf + cnt[i];
+            for (int sd = 0; sd < 2; ++sd) {
+              const int a = nbr(g, sd);
+              if (!owns(g, a)) continue;
+              H.face_cells[(size_t)f * 2] = sd == 0 ? a : g;
+              H.face_cells[(size_t)f * 2 + 1] = sd == 0 ? g : a;
+              H.face_dir[f] = (int8_t)d;
+              H.cell_faces[(size_t)g * 6 + d * 2 + sd] = f;
+              H.cell_faces[(size_t)a * 6 + d * 2 + (1 - sd)] = f;
+              ++f;
+            }
+          }
+          nf += (size_t)run;
         }
       }
-    }
-  H.blk_face_start[nblk] = (int32_t)(H.face_cells.size() / 2);
+    H.blk_face_start[nblk] = (int32_t)nf;
+  }
 
   // domain ghosts (Domain::FaceProperty Domain.H:372-446): the first matching
   // domain face in the order x-, x+, y-, y+ (Userfunction_cy_naca15.cpp:
@@ -869,6 +1009,7 @@ ibm_handle *ibm_build(const ibm_case *cases, int32_t ncases) {
       if (cs.nbody > IBM_MAX_BODIES) throw std::runtime_error("too many bodies");
       cms[c].cs = cs;
       build_segments(cms[c]);
+      build_segment_runs(cms[c]);
       build_patches(cms[c]);
       build_levels(cms[c]);
       if (cs.refine_only) {
@@ -879,6 +1020,7 @@ ibm_handle *ibm_build(const ibm_case *cases, int32_t ncases) {
         cm.patches.clear(); cm.segxy.clear(); cm.seg_body.clear(); cm.seg_first.clear();
         cm.seg_cx.clear(); cm.seg_cy.clear(); cm.seg_nx.clear(); cm.seg_ny.clear();
         cm.seg_len.clear();
+        cm.run_x0.clear(); cm.run_x1.clear(); cm.run_y0.clear(); cm.run_y1.clear();
       }
     }
     ibm_handle *H = new ibm_handle();
@@ -920,18 +1062,19 @@ int32_t ibm_transfer(const ibm_handle *oh, const ibg_physics *ph, const double *
     const int L = oh->L, nc = oh->ncases, dim = oh->dim;
     const int NF = IBG_NFV;
     if (oh->blk_twod != nh->blk_twod) throw std::runtime_error("level sets refine differently");
-    std::vector<std::unordered_map<int64_t, int>> omap((size_t)nc * L);
+    std::vector<KeyGrid> omap((size_t)nc * L);
+#pragma omp parallel for schedule(dynamic, 1)
     for (int b = 0; b < nc * L; ++b) {
-      auto &mp = omap[b];
-      mp.reserve((size_t)(oh->blk_start[b + 1] - oh->blk_start[b]) * 2);
-      for (int g = oh->blk_start[b]; g < oh->blk_start[b + 1]; ++g)
-        mp[pack_key(oh->key[(size_t)g * 3], oh->key[(size_t)g * 3 + 1], oh->key[(size_t)g * 3 + 2])] = g;
+      const int s0 = oh->blk_start[b];
+      omap[b].build(oh->blk_start[b + 1] - s0, [&](int i, int *k) {
+        for (int d = 0; d < 3; ++d) k[d] = oh->key[(size_t)(s0 + i) * 3 + d];
+      });
     }
     auto old_at = [&](int c, int l, int ix, int iy, int iz) -> int {
       if (l < 0 || l >= L) return -1;
-      const auto &mp = omap[(size_t)c * L + l];
-      auto it = mp.find(pack_key(ix, iy, iz));
-      return it == mp.end() ? -1 : it->second;
+      const int b = c * L + l;
+      const int i = omap[(size_t)b].find(ix, iy, iz);
+      return i < 0 ? -1 : oh->blk_start[b] + i;
     };
     auto fdiv2 = [](int k) { return k >= 0 ? k / 2 : -((-k + 1) / 2); };
     auto put = [&](double *f, const ibg_physics &P, double r, double u, double v, double w, double pr) {
