@@ -192,6 +192,7 @@ def regrid_cost(eng, spec, every=20):
     eng.synchronize()
     t2 = time.perf_counter()
     return {"every_steps": every, "seconds": t2 - t0, "build_s": t1 - t0, "transfer_upload_s": t2 - t1,
+            "parts_s": getattr(eng, "last_regrid_timing", None),
             "cells_before": int(n0), "cells_after": int(new.ncells), "t": t,
             "note": "host rebuild of the level set at the pose (not incremental); not in value"}
 

@@ -133,7 +133,7 @@ struct ibg_ctx {
     int *key = nullptr, *cpatch = nullptr, *cp1 = nullptr, *nb8 = nullptr;
     int8_t *kind = nullptr, *fin = nullptr;
     int16_t *cblk = nullptr;
-    double *origin = nullptr, *sd1 = nullptr;
+    double *origin = nullptr, *borg = nullptr, *sd1 = nullptr;
     double *pt_off = nullptr, *pat_dz = nullptr;
     int *pat_pt0 = nullptr, *pat_body = nullptr, *body_p0 = nullptr, *pat_cc = nullptr;
     int2 *pat_lh = nullptr;    // [npatch] patch range of the patch's body
@@ -147,7 +147,8 @@ struct ibg_ctx {
     int *chg = nullptr;                   // cells whose body changed
     int cap_chg = 0;
     int *epoch = nullptr;                 // [1] move counter
-    int *cnt = nullptr;                   // [5] solid, layer-1, in->out, IB, changed
+    int *cnt = nullptr;                   // [6] solid, layer-1, in->out, IB, changed, mixed
+    int *mixlist = nullptr;               // [N] cells deferred to k_move_mixed
     int *slist = nullptr, *l1list = nullptr, *iolist = nullptr;
     int *gidx = nullptr;          // ghosts sorted by level, finest first
     std::vector<int> gend;        // [l]: ghosts of levels >= l
@@ -554,10 +555,11 @@ MoveArgs move_args(ibg_ctx *c, unsigned long long *upd, int lmin) {
   a.nw = c->cell_end[lmin]; a.ngw = v.gend[lmin]; a.nflag = c->cell_end[lmin];
   a.gidx = v.gidx;
   a.key = v.key; a.kind = v.kind; a.fin = v.fin; a.cblk = v.cblk; a.blk = c->blk;
-  a.origin = v.origin; a.nb8 = v.nb8; a.nb27 = c->nb27; a.dist_dm = c->dist_dm;
+  a.origin = v.origin; a.borg = v.borg; a.nb8 = v.nb8; a.nb27 = c->nb27; a.dist_dm = c->dist_dm;
   a.signdis = c->signdis; a.sd32 = c->sd32; a.cpatch = v.cpatch; a.cp1 = v.cp1; a.sd1 = v.sd1;
   a.mk1 = v.mk1; a.mkib = v.mkib; a.epoch = v.epoch; a.cnt = v.cnt;
   a.cmix = v.cmix; a.chg = v.chg; a.cap_chg = v.cap_chg;
+  a.mixlist = v.mixlist; a.cap_mix = c->N;
   a.cap_s = v.cap_s; a.cap_l1 = v.cap_l1; a.cap_io = v.cap_io; a.ib_cap = v.cap;
   a.slist = v.slist; a.l1list = v.l1list; a.iolist = v.iolist;
   a.flags = c->flags;
@@ -584,6 +586,7 @@ ibg_status run_move(ibg_ctx *c, unsigned long long *upd, int lmin) {
   const int nw = a.nw + a.ngw;
   hipLaunchKernelGGL(k_move_front, dim3(1), dim3(256), 0, c->s, a);
   hipLaunchKernelGGL(k_move_dist, dim3(grid(nw, 256)), dim3(256), 0, c->s, a);
+  hipLaunchKernelGGL(k_move_mixed, dim3(256), dim3(256), 0, c->s, a);
   hipLaunchKernelGGL(k_move_mark1, dim3(grid(a.cap_s + a.cap_chg, 256)), dim3(256), 0, c->s, a);
   hipLaunchKernelGGL(k_move_mark2, dim3(grid(a.cap_l1 + a.cap_io, 256)), dim3(256), 0, c->s, a);
   const int ncommit = (int)std::min(grid(nw, 256), 2048u);
@@ -1796,7 +1799,13 @@ ibg_status ibg_upload_surface(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_surfa
 #define UP(dst, vec) if ((st = dupload(c, &dst, vec))) return st
 #define AL(dst, n) if ((st = dalloc(c, &dst, n))) return st
   UP(v.key, key); UP(v.kind, kind); UP(v.fin, fin); UP(v.cblk, cblk); UP(v.cpatch, cp);
-  UP(v.origin, org); UP(v.pt_off, off); UP(v.pat_pt0, pt0); UP(v.pat_body, pb);
+  UP(v.origin, org); UP(v.pt_off, off);
+  {
+    std::vector<double> borg((size_t)c->nblk * 3);
+    for (int b = 0; b < c->nblk; ++b)
+      for (int d = 0; d < 3; ++d) borg[(size_t)b * 3 + d] = org[(size_t)(b / c->L) * 3 + d];
+    UP(v.borg, borg);
+  } UP(v.pat_pt0, pt0); UP(v.pat_body, pb);
   UP(v.pat_flip, flip); UP(v.pat_dz, dz); UP(v.body_p0, bp0); UP(v.pat_cc, cc);
   UP(v.bcase_d, bcase);
   {
@@ -1908,8 +1917,9 @@ ibg_status ibg_upload_surface(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_surfa
   v.cap_l1 = cap;   // layer 1 is part of the IB set
   {
     ibg_status st2;
-    std::vector<int> cnt0 = {0, 0, 0, c->nib, 0};
+    std::vector<int> cnt0 = {0, 0, 0, c->nib, 0, 0};
     if ((st2 = dupload(c, &v.cnt, cnt0))) return st2;
+    if ((st2 = dalloc(c, &v.mixlist, (size_t)N))) return st2;
     if ((st2 = dalloc(c, &v.chg, (size_t)v.cap_chg))) return st2;
     if ((st2 = dalloc(c, &v.slist, (size_t)v.cap_s))) return st2;
     if ((st2 = dalloc(c, &v.l1list, (size_t)v.cap_l1))) return st2;

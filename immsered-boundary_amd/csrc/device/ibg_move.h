@@ -1,6 +1,6 @@
 // ibg_move.h — per-step immersed-boundary re-tagging of moving bodies on a
 // fixed level set (the part of Body::MovingBody_Image, Bodymove.cpp:37-125,
-// that runs between regrids), 2D sections.  Five launches per move:
+// that runs between regrids), 2D sections.  Six launches per move:
 //
 //   k_move_front   pose of every case (given, or from the pitch/heave law at
 //                  the case's time, with the wall motion of the IB fill and
@@ -10,8 +10,10 @@
 //   k_move_dist    closest patch and signed distance of every cell of the
 //                  marched levels: the local walk from the previous patch
 //                  (RenewLevelDistance :136-155, FindClosePatchtoCell
-//                  Body.H:505-542) and the cross-body cycle (:156-223);
-//                  finest-level solid and in->out cells go to lists
+//                  Body.H:505-542); finest-level solid and in->out cells go
+//                  to lists, cells with a neighbour of another body to
+//   k_move_mixed   the same with the cross-body cycle (:156-223), eight
+//                  lanes per cell
 //   k_move_mark1   layer 1 of the IB set: non-solid neighbours of solid cells;
 //                  the mixed-body neighbourhood flags around changed cells
 //   k_move_mark2   the IB set: layer-1 cells and their non-solid neighbours
@@ -75,6 +77,7 @@ struct MoveArgs {
   const int16_t *cblk;   // [N] block (case, level)
   const BlkInfo *blk;
   const double *origin;  // [ncases][3]
+  const double *borg;    // [nblk][3] the origin of each block's case
   const int *nb8;        // [8][N] in-plane neighbours, SoA
   const int *nb27;       // [N][27]
   const double *dist_dm; // [N] distance to the domain walls, or null
@@ -87,8 +90,10 @@ struct MoveArgs {
   int8_t *cmix;          // [N] an in-plane neighbour's closest body differs
   int *chg;              // cells whose closest body changed in this move
   int cap_chg;
+  int *mixlist;          // cells left to k_move_mixed (their neighbours' bodies differ)
+  int cap_mix;
   int *epoch;            // [1] move counter
-  int *cnt;              // [5] solid, layer-1, in->out, IB, changed list lengths
+  int *cnt;              // [6] solid, layer-1, in->out, IB, changed, mixed list lengths
   int cap_s, cap_l1, cap_io, ib_cap;
   int *slist, *l1list, *iolist;
   uint8_t *flags;        // [Nn]
@@ -211,35 +216,47 @@ __device__ __forceinline__ double patch_dist_at(const double *x, double px, doub
   return sqrt(dx * dx + dy * dy + dz * dz);
 }
 
-__device__ void walk_body(const MoveArgs &a, const double *x, int &patch, double &dist,
-                          double &sd) {
-#pragma clang fp contract(off)
-  const int p0 = patch;
+// The patch data of the first step of a walk from patch p0 (its two
+// neighbour segments, clamped to the patch range, and its normal) and the
+// body's patch range.
+struct WalkStart {
+  int2 lh;
+  double c0x, c0y, c0z, cmx, cmy, cmz, cqx, cqy, cqz, n0x, n0y, n0z;
+};
+
+__device__ __forceinline__ void walk_start(const MoveArgs &a, int p0, WalkStart &w) {
   const int pm = p0 > 0 ? p0 - 1 : p0, pq = p0 + 1 < a.npatch ? p0 + 1 : p0;   // in range
-  const int2 lh = a.pat_lh[p0];
+  w.lh = a.pat_lh[p0];
   const double *c0 = a.pat_pc + (size_t)p0 * 3, *cm = a.pat_pc + (size_t)pm * 3;
   const double *cq = a.pat_pc + (size_t)pq * 3, *n0 = a.pat_nv + (size_t)p0 * 3;
-  const double c0x = c0[0], c0y = c0[1], c0z = c0[2];
-  const double cmx = cm[0], cmy = cm[1], cmz = cm[2];
-  const double cqx = cq[0], cqy = cq[1], cqz = cq[2];
-  const double n0x = n0[0], n0y = n0[1], n0z = n0[2];
-  const int lo = lh.x, hi = lh.y;
+  w.c0x = c0[0]; w.c0y = c0[1]; w.c0z = c0[2];
+  w.cmx = cm[0]; w.cmy = cm[1]; w.cmz = cm[2];
+  w.cqx = cq[0]; w.cqy = cq[1]; w.cqz = cq[2];
+  w.n0x = n0[0]; w.n0y = n0[1]; w.n0z = n0[2];
+}
+
+// The walk from `patch` given its preloaded start w.
+__device__ __forceinline__ void walk_from(const MoveArgs &a, const double *x, int &patch,
+                                          double &dist, double &sd, const WalkStart &w) {
+#pragma clang fp contract(off)
+  const int p0 = patch;
+  const int lo = w.lh.x, hi = w.lh.y;
   dist = 999.0;
   {
-    const double d = patch_dist_at(x, c0x, c0y, c0z);
+    const double d = patch_dist_at(x, w.c0x, w.c0y, w.c0z);
     if (d < dist) { dist = d; patch = p0; }
   }
   // first step of the walk from the preloaded data
   if (p0 - 1 >= lo) {
-    const double d = patch_dist_at(x, cmx, cmy, cmz);
+    const double d = patch_dist_at(x, w.cmx, w.cmy, w.cmz);
     if (d < dist) { dist = d; patch = p0 - 1; }
   }
   if (p0 + 1 < hi) {
-    const double d = patch_dist_at(x, cqx, cqy, cqz);
+    const double d = patch_dist_at(x, w.cqx, w.cqy, w.cqz);
     if (d < dist) { dist = d; patch = p0 + 1; }
   }
   if (patch == p0) {
-    sd = n0x * (x[0] - c0x) + n0y * (x[1] - c0y) + n0z * (x[2] - c0z);
+    sd = w.n0x * (x[0] - w.c0x) + w.n0y * (x[1] - w.c0y) + w.n0z * (x[2] - w.c0z);
     return;
   }
   // the walk moved: its remaining steps as in FindClosePatchtoCell
@@ -253,6 +270,13 @@ __device__ void walk_body(const MoveArgs &a, const double *x, int &patch, double
   const double *pc = a.pat_pc + (size_t)patch * 3;
   const double *nv = a.pat_nv + (size_t)patch * 3;
   sd = nv[0] * (x[0] - pc[0]) + nv[1] * (x[1] - pc[1]) + nv[2] * (x[2] - pc[2]);
+}
+
+__device__ void walk_body(const MoveArgs &a, const double *x, int &patch, double &dist,
+                          double &sd) {
+  WalkStart w;
+  walk_start(a, patch, w);
+  walk_from(a, x, patch, dist, sd, w);
 }
 
 __device__ __forceinline__ Motion law_motion(const PitchHeave &Lw, double t, double zc) {
@@ -275,7 +299,7 @@ __global__ void __launch_bounds__(256) k_move_front(MoveArgs a) {
   if (wg_halted(a.err)) return;
   if (threadIdx.x == 0) {
     a.epoch[0] += 1;
-    for (int k = 0; k < 5; ++k) a.cnt[k] = 0;
+    for (int k = 0; k < 6; ++k) a.cnt[k] = 0;
   }
   for (int k = threadIdx.x; k < a.ncases; k += blockDim.x) {
     Pose q;
@@ -331,74 +355,168 @@ __global__ void __launch_bounds__(256) k_move_front(MoveArgs a) {
 // level: solid cells to
 // the solid list, in->out cells to their list, and an out->in cell that was
 // not an IB cell is the reference's abort 263 (Bodymove.cpp:494-544).
-__global__ void __launch_bounds__(256) k_move_dist(MoveArgs a) {
+// Held to 64 registers (eight waves per SIMD): the latency of the two
+// round trips is hidden by the other waves, not by more registers.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_move_dist(MoveArgs a) {
 #pragma clang fp contract(off)
   if (halted(a.err)) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const bool in = t < a.nw + a.ngw;
   const int i = in ? set_cell(a, t, a.nw) : 0;
-  bool solid = false, io = false, chg = false;
+  bool solid = false, io = false, chg = false, defer = false;
   if (in) {
-    // every per-cell operand up front: the walk below is a chain of
-    // dependent loads, and loads placed after it would add one more
-    // memory round trip to every wave
+    // Two memory round trips for most cells: every per-cell operand first,
+    // unconditionally (a load placed under a condition or after another's
+    // use waits for its own round trip), then the block, the case origin
+    // and the start of the walk, which depend only on those.
+#pragma clang fp contract(off)
     const int p0 = a.cpatch[i];
     const double s0 = a.signdis[i];
     const double dm = a.dist_dm ? a.dist_dm[i] : 0.0;
     const bool mixed = a.cmix[i] != 0;
-    const bool fin = a.fin[i] && a.kind[i] != IBG_DMGHOST;
+    const int8_t fn = a.fin[i], kd = a.kind[i];
     const int mk = a.mkib[i];
-    const int b0 = p0 >= 0 ? a.pat_body[p0] : -1;
+    const int cb = a.cblk[i];
+    const int k0 = a.key[(size_t)i * 3], k1 = a.key[(size_t)i * 3 + 1], k2 = a.key[(size_t)i * 3 + 2];
+    const bool fin = (fn != 0) & (kd != IBG_DMGHOST);
+    const int ps = p0 >= 0 ? p0 : 0;
+    WalkStart w;
+    walk_start(a, ps, w);
+    const int bs = a.pat_body[ps];
+    const double *bh = a.blk[cb].h;
+    const double *o = a.borg + (size_t)cb * 3;
+    // the cell centre as cell_center_dev forms it
+    double x[3];
+    x[0] = o[0] + ((double)k0 + 0.5) * bh[0];
+    x[1] = o[1] + ((double)k1 + 0.5) * bh[1];
+    x[2] = o[2] + ((double)k2 + 0.5) * bh[2];
+    const int b0 = p0 >= 0 ? bs : -1;
     int p = p0;
     double sd = s0;
-    if (p >= 0 && s0 < kCriticDis) {
-      double x[3], dist;
-      cell_center_dev(a, i, x);
-      walk_body(a, x, p, dist, sd);
-      // cells whose neighbours all share their body have no cycle entries
-      if (sd < kCriticDis && mixed) {
-        const int myb0 = a.pat_body[p];
-        int myb = myb0;
-        for (int k = 0; k < kNb8; ++k) {
-          const int an = a.nb8[(size_t)k * a.N + i];
-          if (an < 0) continue;
-          const int q0 = a.cpatch[an];
-          if (q0 < 0) continue;
-          const int nbb = a.pat_body[q0];
-          if (nbb == myb0) continue;      // pushed only when it differs (:172-179)
-          if (nbb == myb) continue;       // processed only when it still differs (:193)
-          int q1 = q0;
-          if (a.signdis[an] < kCriticDis) {
-            double xa[3], da, sa;
-            cell_center_dev(a, an, xa);
-            walk_body(a, xa, q1, da, sa);
-          }
-          double dn, sn;
-          walk_body(a, x, q1, dn, sn);
-          bool change = false;
-          if (sn > 0.0 && sd < 0.0) change = true;
-          else if (dn < dist) {
-            if (sd * sn > 0.0) change = true;
-            else if (sn > 0.0) change = true;
-          }
-          if (change) { myb = nbb; p = q1; dist = dn; sd = sn; }
-        }
+    if (mixed && p0 >= 0 && s0 < kCriticDis) {
+      // a neighbour's closest body differs: the cycle list's chains of
+      // dependent walks go to k_move_mixed, which runs them side by side
+      defer = true;
+    } else {
+      if (p >= 0 && s0 < kCriticDis) {
+        double dist;
+        walk_from(a, x, p, dist, sd, w);
+        // solid loop (:224-249): signdis = min(signdis, distance_to_dm)
+        if (a.dist_dm && sd < kCriticDis) sd = (dm < sd) ? dm : sd;
       }
-      // solid loop (:224-249): signdis = min(signdis, distance_to_dm)
-      if (a.dist_dm && sd < kCriticDis) sd = (dm < sd) ? dm : sd;
-    }
-    a.cp1[i] = p;
-    a.sd1[i] = sd;
-    chg = p0 >= 0 && a.pat_body[p] != b0;
-    if (fin) {
-      solid = sd < 0.0;
-      io = s0 < 0.0 && !(sd < 0.0);
-      if (!(s0 < 0.0) && sd < 0.0 && mk != a.epoch[0] - 1) report_error(a.err, i, 263);
+      a.cp1[i] = p;
+      a.sd1[i] = sd;
+      chg = p0 >= 0 && a.pat_body[p] != b0;
+      if (fin) {
+        solid = sd < 0.0;
+        io = s0 < 0.0 && !(sd < 0.0);
+        if (!(s0 < 0.0) && sd < 0.0 && mk != a.epoch[0] - 1) report_error(a.err, i, 263);
+      }
     }
   }
   wave_append(a.cnt + 0, solid, i, a.slist, a.cap_s, a.err);
   wave_append(a.cnt + 2, io, i, a.iolist, a.cap_io, a.err);
   wave_append(a.cnt + 4, chg, i, a.chg, a.cap_chg, a.err);
+  wave_append(a.cnt + 5, defer, i, a.mixlist, a.cap_mix, a.err);
+}
+
+// The cells k_move_dist deferred: RenewLevelDistance with the cycle list
+// (Bodymove.cpp:136-249).  Eight lanes per cell, one per in-plane
+// neighbour: each lane walks its neighbour from that neighbour's patch and
+// the cell from the result (the two walks of the list entry, independent of
+// the list's state), then every lane applies the list in neighbour order
+// to the gathered results -- the reference's sequence and change rule.  A
+// fixed grid loops over the list (its length is known on the device only).
+__global__ void __launch_bounds__(256) k_move_mixed(MoveArgs a) {
+#pragma clang fp contract(off)
+  if (halted(a.err)) return;
+  const int n = min(a.cnt[5], a.cap_mix);
+  const int k = threadIdx.x & 7;                  // this lane's neighbour
+  const int g0 = (threadIdx.x & 63) & ~7;         // first lane of the cell's group
+  const int per = (int)blockDim.x / 8;
+  for (int base = blockIdx.x * per; base < n; base += gridDim.x * per) {
+    const int j = base + (int)(threadIdx.x >> 3);
+    const bool in = j < n;
+    const int i = in ? a.mixlist[j] : 0;
+    bool solid = false, io = false, chg = false;
+    // the cell's own walk (every lane of the group, the same result)
+    int p0 = -1, p = -1, myb0 = -1, mk = 0;
+    double s0 = 0.0, sd = 0.0, dist = 0.0, dm = 0.0, x[3] = {0.0, 0.0, 0.0};
+    bool fin = false, cyc = false;
+    if (in) {
+      p0 = a.cpatch[i];
+      s0 = a.signdis[i];
+      dm = a.dist_dm ? a.dist_dm[i] : 0.0;
+      fin = a.fin[i] && a.kind[i] != IBG_DMGHOST;
+      mk = a.mkib[i];
+      p = p0;
+      sd = s0;
+      cell_center_dev(a, i, x);
+      walk_body(a, x, p, dist, sd);   // deferred cells have p0 >= 0, s0 < critic_dis
+      cyc = sd < kCriticDis;
+      if (cyc) myb0 = a.pat_body[p];
+    }
+    // this lane's list entry
+    bool ok = false;
+    int nbb = -1, q1 = -1;
+    double dn = 0.0, sn = 0.0;
+    if (in && cyc) {
+      const int an = a.nb8[(size_t)k * a.N + i];
+      const int q0 = an >= 0 ? a.cpatch[an] : -1;
+      if (q0 >= 0) {
+        ok = true;
+        nbb = a.pat_body[q0];
+        if (nbb != myb0) {
+          q1 = q0;
+          if (a.signdis[an] < kCriticDis) {
+            double xa[3], da, sa;
+            cell_center_dev(a, an, xa);
+            walk_body(a, xa, q1, da, sa);
+          }
+          walk_body(a, x, q1, dn, sn);
+        }
+      }
+    }
+    // the list in neighbour order (every lane of the wave takes part in
+    // the exchanges)
+    int myb = myb0;
+#pragma unroll
+    for (int e = 0; e < kNb8; ++e) {
+      const int src = g0 + e;
+      const bool ok_e = __shfl(ok ? 1 : 0, src) != 0;
+      const int nbb_e = __shfl(nbb, src);
+      const int q1_e = __shfl(q1, src);
+      const double dn_e = __shfl(dn, src);
+      const double sn_e = __shfl(sn, src);
+      if (!cyc || !ok_e) continue;
+      if (nbb_e == myb0) continue;      // pushed only when it differs (:172-179)
+      if (nbb_e == myb) continue;       // processed only when it still differs (:193)
+      bool change = false;
+      if (sn_e > 0.0 && sd < 0.0) change = true;
+      else if (dn_e < dist) {
+        if (sd * sn_e > 0.0) change = true;
+        else if (sn_e > 0.0) change = true;
+      }
+      if (change) { myb = nbb_e; p = q1_e; dist = dn_e; sd = sn_e; }
+    }
+    const bool lead = in && k == 0;
+    if (lead) {
+      // solid loop (:224-249): signdis = min(signdis, distance_to_dm)
+      if (a.dist_dm && sd < kCriticDis) sd = (dm < sd) ? dm : sd;
+      a.cp1[i] = p;
+      a.sd1[i] = sd;
+      chg = a.pat_body[p] != a.pat_body[p0];
+      if (fin) {
+        solid = sd < 0.0;
+        io = s0 < 0.0 && !(sd < 0.0);
+        if (!(s0 < 0.0) && sd < 0.0 && mk != a.epoch[0] - 1) report_error(a.err, i, 263);
+      }
+    }
+    wave_append(a.cnt + 0, solid, i, a.slist, a.cap_s, a.err);
+    wave_append(a.cnt + 2, io, i, a.iolist, a.cap_io, a.err);
+    wave_append(a.cnt + 4, chg, i, a.chg, a.cap_chg, a.err);
+  }
 }
 
 // cmix of cell x from the new closest patches of x and its neighbours

@@ -329,26 +329,42 @@ class Engine:
         of the same cases (for a moving body: at its current pose): the state
         moves over by canonical keys (Mesh.transfer_to) and replaces the
         uploaded mesh; dt, the time and a prescribed motion carry on."""
+        import time
+        tick = [time.perf_counter()]
+        timing = {}
+
+        def lap(name):
+            now = time.perf_counter()
+            timing[name] = now - tick[0]
+            tick[0] = now
+
         fv = self.get_state()
         nut = self.get_nut() if self.turbulence else None
         dt = self.get_dt()
         moving = getattr(self, "_surface", None) is not None
         t = self.get_time() if moving else None
+        lap("state_out_s")
         fv2, nut2 = self.mesh.transfer_to(new_mesh, fv, nut)
+        lap("transfer_s")
         self._chk(self.lib.ibg_upload_mesh(self.ctx, new_mesh.soa_ptr, new_mesh.phys),
                   "ibg_upload_mesh")
+        lap("upload_mesh_s")
         self.mesh = new_mesh
         self._surface = None
         self.set_state(fv2)
         if nut2 is not None:
             self.set_nut(nut2)
         self.set_dt(dt)
+        lap("state_in_s")
         if moving:
             self.upload_surface()
             if getattr(self, "_law", None) is not None:
                 self.set_motion(self._law)
                 self._chk(self.lib.ibg_set_time(self.ctx, np.ascontiguousarray(t).ctypes.data_as(_ffi.f64p)),
                           "ibg_set_time")
+            lap("surface_s")
+        # wall-clock seconds of each part of the last regrid
+        self.last_regrid_timing = timing
 
     def write_solution(self, path, ts):
         """Tecplot solution of the current state (Pltoutput::Output_Soln) with
