@@ -25,14 +25,18 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "ibgpu.h"
@@ -243,6 +247,44 @@ size_t ml_resident_offset(const ibg_ctx *c) {
     }                                                                    \
   } while (0)
 
+// Host loops over cells or faces (mesh upload, the state boundary): the
+// range [0, n) split into contiguous chunks over up to 16 threads; f(lo, hi)
+// must touch only its own entries.
+template <typename F>
+void host_parallel(int n, F f) {
+  const int nt = std::max(1, std::min<int>(16, (int)std::thread::hardware_concurrency()));
+  if (n < (1 << 16) || nt == 1) { f(0, n); return; }
+  std::vector<std::thread> th;
+  const int chunk = (n + nt - 1) / nt;
+  for (int t = 0; t < nt; ++t) {
+    const int a = t * chunk, b = std::min(n, a + chunk);
+    if (a < b) th.emplace_back([=] { f(a, b); });
+  }
+  for (auto &x : th) x.join();
+}
+
+// The first index in [0, n) at which check(i) reports a failure (a non-null
+// message), searched in parallel chunks: the same index and message as a
+// serial scan.  Returns -1 when every index passes.
+template <typename Check>
+int first_failure(int n, Check check, const char **msg) {
+  std::mutex mu;
+  int best = -1;
+  const char *bmsg = nullptr;
+  host_parallel(n, [&](int lo, int hi) {
+    for (int i = lo; i < hi; ++i) {
+      const char *e = check(i);
+      if (e) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (best < 0 || i < best) { best = i; bmsg = e; }
+        return;
+      }
+    }
+  });
+  if (msg) *msg = bmsg;
+  return best;
+}
+
 ibg_status dalloc_bytes(ibg_ctx *c, void **p, size_t bytes) {
   void *q = nullptr;
   HIPCHK(hipMalloc(&q, std::max<size_t>(bytes, 64)));
@@ -260,8 +302,22 @@ ibg_status dalloc(ibg_ctx *c, T **p, size_t n) {
   return st;
 }
 
-template <typename T>
-ibg_status dupload(ibg_ctx *c, T **p, const std::vector<T> &v) {
+// An allocator that leaves new elements uninitialised, for the large host
+// staging arrays of an upload that a parallel loop fills in full.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+  template <class U> struct rebind { using other = NoInitAlloc<U>; };
+  NoInitAlloc() = default;
+  template <class U> NoInitAlloc(const NoInitAlloc<U> &) {}
+  template <class U> void construct(U *p) noexcept { ::new ((void *)p) U; }
+  template <class U, class... A> void construct(U *p, A &&...a) {
+    ::new ((void *)p) U(std::forward<A>(a)...);
+  }
+};
+template <class T> using hvec = std::vector<T, NoInitAlloc<T>>;
+
+template <typename T, typename A>
+ibg_status dupload(ibg_ctx *c, T **p, const std::vector<T, A> &v) {
   ibg_status st = dalloc(c, p, v.size());
   if (st) return st;
   if (!v.empty()) HIPCHK(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
@@ -683,36 +739,49 @@ ibg_status validate_mesh(const ibg_mesh_soa *m) {
   }
   auto in = [N](int a) { return a >= 0 && a < N; };
   auto nb = [m](int c, int i, int j, int k) { return m->nb[(size_t)c * 27 + i * 9 + j * 3 + k]; };
+  // the cells of the normal ranges are the first-ranked of their blocks
+  // (checked above), so cell g is normal exactly when it lies below its
+  // block's normal count
+  std::vector<int> gblk(N);
   for (int b = 0; b < nblk; ++b)
-    for (int g = m->blk_start[b]; g < m->blk_start[b] + m->blk_nnormal[b]; ++g)
-      for (int d = 0; d < D; ++d)
-        for (int s2 = 0; s2 < 2; ++s2) {
-          const int f = m->cell_faces[(size_t)g * 6 + d * 2 + s2];
-          if (f < 0 || f >= m->nfaces) return fail("normal cell without face");
-          if (m->face_cells[2 * f + (1 - s2)] != g) return fail("cell_faces and face_cells disagree");
-        }
-  for (int f = 0; f < m->nfaces; ++f) {
-    const int lo = m->face_cells[2 * f], hi = m->face_cells[2 * f + 1], d = m->face_dir[f];
-    if (!in(lo) || !in(hi) || d < 0 || d >= D) return fail("face out of range");
-    int o[3] = {1, 1, 1};
-    o[d] = 0;
-    int LL = nb(lo, o[0], o[1], o[2]);
-    o[d] = 2;
-    int RR = nb(hi, o[0], o[1], o[2]);
-    int LLf = LL, RRf = RR;
-    if (LL < 0) LLf = RR;
-    if (RR < 0) RRf = LL;
-    if (!in(LLf) || !in(RRf)) return fail("face stencil without LL/RR");
-    // viscous stencil of the low cell (NS_Solver.cpp:413-425)
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j)
-        for (int k = 0; k < 3; ++k) {
-          int idx[3] = {i, j, k};
-          if (idx[d] == 0) continue;
-          if (D == 2 && k != 1) continue;
-          if (!in(nb(lo, i, j, k))) return fail("viscous stencil neighbour missing");
-        }
-  }
+    for (int g = m->blk_start[b]; g < m->blk_start[b + 1]; ++g) gblk[g] = b;
+  const char *msg = nullptr;
+  if (first_failure(N, [&](int g) -> const char * {
+        const int b = gblk[g];
+        if (g >= m->blk_start[b] + m->blk_nnormal[b]) return nullptr;
+        for (int d = 0; d < D; ++d)
+          for (int s2 = 0; s2 < 2; ++s2) {
+            const int f = m->cell_faces[(size_t)g * 6 + d * 2 + s2];
+            if (f < 0 || f >= m->nfaces) return "normal cell without face";
+            if (m->face_cells[2 * f + (1 - s2)] != g) return "cell_faces and face_cells disagree";
+          }
+        return nullptr;
+      }, &msg) >= 0)
+    return fail(msg);
+  if (first_failure(m->nfaces, [&](int f) -> const char * {
+        const int lo = m->face_cells[2 * f], hi = m->face_cells[2 * f + 1], d = m->face_dir[f];
+        if (!in(lo) || !in(hi) || d < 0 || d >= D) return "face out of range";
+        int o[3] = {1, 1, 1};
+        o[d] = 0;
+        int LL = nb(lo, o[0], o[1], o[2]);
+        o[d] = 2;
+        int RR = nb(hi, o[0], o[1], o[2]);
+        int LLf = LL, RRf = RR;
+        if (LL < 0) LLf = RR;
+        if (RR < 0) RRf = LL;
+        if (!in(LLf) || !in(RRf)) return "face stencil without LL/RR";
+        // viscous stencil of the low cell (NS_Solver.cpp:413-425)
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k) {
+              int idx[3] = {i, j, k};
+              if (idx[d] == 0) continue;
+              if (D == 2 && k != 1) continue;
+              if (!in(nb(lo, i, j, k))) return "viscous stencil neighbour missing";
+            }
+        return nullptr;
+      }, &msg) >= 0)
+    return fail(msg);
   for (int i = 0; i < m->nres; ++i) {
     if (!in(m->res_ci[i])) return fail("restriction coarse cell");
     const int b = m->res_blk[i];
@@ -902,7 +971,7 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
   c->N = N;
   std::vector<int> hblk(N);
   for (int b = 0; b < nblk; ++b)
-    for (int g = m->blk_start[b]; g < m->blk_start[b + 1]; ++g) hblk[g] = b;
+    std::fill(hblk.begin() + m->blk_start[b], hblk.begin() + m->blk_start[b + 1], b);
   // ---- device cell order: normal cells by (level desc, case, Morton), then ghosts
   std::vector<int> order;
   order.reserve(N);
@@ -920,7 +989,9 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
     for (int g = m->blk_start[b] + m->blk_nnormal[b]; g < m->blk_start[b + 1]; ++g) order.push_back(g);
   c->d2h = order;
   c->h2d.assign(N, -1);
-  for (int i = 0; i < N; ++i) c->h2d[order[i]] = i;
+  host_parallel(N, [&](int lo, int hi) {
+    for (int i = lo; i < hi; ++i) c->h2d[order[i]] = i;
+  });
   const std::vector<int> &h2d = c->h2d;
   // ---- faces: device order by (level desc, case)
   const int F = m->nfaces;
@@ -971,20 +1042,24 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
   }
   // ---- per-cell arrays (device order)
   std::vector<int16_t> cellblk(N);
-  std::vector<int> cellcase(N), nb27((size_t)N * 27);
+  std::vector<int> cellcase(N);
+  hvec<int> nb27((size_t)N * 27);
   std::vector<double> sd(N);
-  for (int i = 0; i < N; ++i) {
-    const int g = order[i];
-    cellblk[i] = (int16_t)hblk[g];
-    cellcase[i] = hblk[g] / L;
-    sd[i] = m->signdis[g];
-    for (int q = 0; q < 27; ++q) {
-      const int a = m->nb[(size_t)g * 27 + q];
-      nb27[(size_t)i * 27 + q] = a < 0 ? -1 : h2d[a];
+  host_parallel(N, [&](int lo, int hi) {
+    for (int i = lo; i < hi; ++i) {
+      const int g = order[i];
+      cellblk[i] = (int16_t)hblk[g];
+      cellcase[i] = hblk[g] / L;
+      sd[i] = m->signdis[g];
+      for (int q = 0; q < 27; ++q) {
+        const int a = m->nb[(size_t)g * 27 + q];
+        nb27[(size_t)i * 27 + q] = a < 0 ? -1 : h2d[a];
+      }
     }
-  }
+  });
   std::vector<uint8_t> flags(Nn, 0);
-  for (int i = 0; i < Nn; ++i) {
+  host_parallel(Nn, [&](int lo, int hi) {
+  for (int i = lo; i < hi; ++i) {
     const int g = order[i];
     const BlkInfo &B = hb[hblk[g]];
     const bool ib = m->ibidx[g] >= 0;
@@ -1003,21 +1078,25 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
     if (m->signdis[g] >= 0.0) f |= F_SA;
     flags[i] = f;
   }
-  std::vector<int> cfaces((size_t)Nn * 2 * D);
-  for (int i = 0; i < Nn; ++i) {
-    const int g = order[i];
-    for (int d = 0; d < D; ++d)
-      for (int s2 = 0; s2 < 2; ++s2)
-        cfaces[(size_t)i * 2 * D + 2 * d + s2] = c->fh2d[m->cell_faces[(size_t)g * 6 + d * 2 + s2]];
-  }
+  });
+  hvec<int> cfaces((size_t)Nn * 2 * D);
+  host_parallel(Nn, [&](int lo, int hi) {
+    for (int i = lo; i < hi; ++i) {
+      const int g = order[i];
+      for (int d = 0; d < D; ++d)
+        for (int s2 = 0; s2 < 2; ++s2)
+          cfaces[(size_t)i * 2 * D + 2 * d + s2] = c->fh2d[m->cell_faces[(size_t)g * 6 + d * 2 + s2]];
+    }
+  });
   // ---- faces: stencils
   auto nbo = [&](int dc, int dx, int dy, int dz) {
     return nb27[(size_t)dc * 27 + (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)];
   };
-  std::vector<int4> face_st(F), face_vst((size_t)F * (D - 1));
+  hvec<int4> face_st(F), face_vst((size_t)F * (D - 1));
   std::vector<int8_t> face_dir(F);
   std::vector<int16_t> face_blk(F);
-  for (int i = 0; i < F; ++i) {
+  host_parallel(F, [&](int ilo, int ihi) {
+  for (int i = ilo; i < ihi; ++i) {
     const int f = forder[i];
     const int lo = h2d[m->face_cells[2 * f]], hi = h2d[m->face_cells[2 * f + 1]];
     const int d = m->face_dir[f];
@@ -1044,17 +1123,14 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
                     nbo(lo, -e[0], -e[1], -e[2]), nbo(hi, -e[0], -e[1], -e[2]));
     }
   }
-  // ---- tiles over each (level, case) run of normal cells
+  });
+  // ---- tiles over each (level, case) run of normal cells.  A tile's face
+  // list is the faces of its cells in first-occurrence order (cells in
+  // order, each cell's faces low/high per direction); the lists are formed
+  // per tile in parallel, placed by a prefix sum, then filled in parallel.
   std::vector<TileInfo> tiles;
-  std::vector<int4> tst, tvs;
-  std::vector<int8_t> tdir;
-  std::vector<uint16_t> slots((size_t)Nn * 2 * D, 0);
-  c->tile_end.assign(L, 0);
   std::vector<int> case_tiles((size_t)nc * L * 2, 0);   // [case][level] = {first, end}
-  std::vector<int> slot_of(F, -1);
-  // per device face: its first tile face, that tile's block, its tile count
-  std::vector<int> face_slot0(F, -1), face_tblk(F, -1), face_ntile(F, 0);
-  int maxf = 1;
+  c->tile_end.assign(L, 0);
   {
     int pos = 0;
     for (int l = L - 1; l >= 0; --l) {
@@ -1066,29 +1142,7 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
           TileInfo ti{};
           ti.cell0 = pos + t0;
           ti.ncell = std::min(c->tile, n - t0);
-          ti.face0 = (int)tst.size();
           ti.blk = b;
-          std::vector<int> used;
-          for (int k = 0; k < ti.ncell; ++k) {
-            const int dc = ti.cell0 + k;
-            for (int q = 0; q < 2 * D; ++q) {
-              const int f = cfaces[(size_t)dc * 2 * D + q];
-              if (slot_of[f] < 0) {
-                slot_of[f] = (int)used.size();
-                used.push_back(f);
-                if (face_slot0[f] < 0) { face_slot0[f] = (int)tst.size(); face_tblk[f] = b; }
-                ++face_ntile[f];
-                tst.push_back(face_st[f]);
-                tdir.push_back(face_dir[f]);
-                for (int k2 = 0; k2 < D - 1; ++k2) tvs.push_back(face_vst[(size_t)f * (D - 1) + k2]);
-              }
-              slots[(size_t)dc * 2 * D + q] = (uint16_t)slot_of[f];
-            }
-          }
-          for (int f : used) slot_of[f] = -1;
-          ti.nface = (int)used.size();
-          if (ti.nface > 65535) return IBG_ERR_ARG;
-          maxf = std::max(maxf, ti.nface);
           tiles.push_back(ti);
         }
         case_tiles[(size_t)b * 2 + 1] = (int)tiles.size();
@@ -1097,6 +1151,94 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
       c->tile_end[l] = (int)tiles.size();
     }
   }
+  const int ntl = (int)tiles.size();
+  const int q2 = 2 * D;
+  hvec<uint16_t> slots((size_t)Nn * q2);   // every entry set below
+  // per tile: its faces in first-occurrence order (a small open-addressed
+  // table per thread removes the repeats) and each cell face's slot
+  std::vector<std::vector<int>> tface(ntl);
+  host_parallel(ntl, [&](int lo, int hi) {
+    constexpr int kH = 4096;   // > the 6 x 256 faces a tile can name
+    std::vector<int> hkey(kH, -1), hval(kH, 0);
+    for (int t = lo; t < hi; ++t) {
+      const TileInfo &ti = tiles[t];
+      std::vector<int> &used = tface[t];
+      used.clear();
+      std::vector<int> touched;
+      for (int k = 0; k < ti.ncell; ++k) {
+        const int dc = ti.cell0 + k;
+        for (int q = 0; q < q2; ++q) {
+          const int f = cfaces[(size_t)dc * q2 + q];
+          unsigned h = ((unsigned)f * 2654435761u) >> 20;   // 12 bits
+          while (hkey[h] >= 0 && hkey[h] != f) h = (h + 1) & (kH - 1);
+          if (hkey[h] < 0) {
+            hkey[h] = f;
+            hval[h] = (int)used.size();
+            used.push_back(f);
+            touched.push_back((int)h);
+          }
+          slots[(size_t)dc * q2 + q] = (uint16_t)hval[h];
+        }
+      }
+      for (int h : touched) hkey[h] = -1;
+    }
+  });
+  int maxf = 1;
+  {
+    int off = 0;
+    for (int t = 0; t < ntl; ++t) {
+      tiles[t].face0 = off;
+      tiles[t].nface = (int)tface[t].size();
+      if (tiles[t].nface > 65535) return IBG_ERR_ARG;
+      maxf = std::max(maxf, tiles[t].nface);
+      off += tiles[t].nface;
+    }
+  }
+  const size_t nts = ntl ? (size_t)tiles[ntl - 1].face0 + tiles[ntl - 1].nface : 0;
+  hvec<int4> tst(nts), tvs(nts * (D - 1));
+  hvec<int8_t> tdir(nts);
+  // per device face: its first tile face (the lowest slot), the number of
+  // tiles naming it; set with atomics while the tile faces are filled
+  std::vector<std::atomic<int>> fslot(F), fcount(F);
+  host_parallel(F, [&](int lo, int hi) {
+    for (int f = lo; f < hi; ++f) {
+      fslot[f].store(INT_MAX, std::memory_order_relaxed);
+      fcount[f].store(0, std::memory_order_relaxed);
+    }
+  });
+  hvec<int> slot_tile(nts);
+  host_parallel(ntl, [&](int lo, int hi) {
+    for (int t = lo; t < hi; ++t) {
+      const size_t f0 = (size_t)tiles[t].face0;
+      const std::vector<int> &used = tface[t];
+      for (size_t k = 0; k < used.size(); ++k) {
+        const int f = used[k];
+        tst[f0 + k] = face_st[f];
+        tdir[f0 + k] = face_dir[f];
+        for (int k2 = 0; k2 < D - 1; ++k2) tvs[(f0 + k) * (D - 1) + k2] = face_vst[(size_t)f * (D - 1) + k2];
+        slot_tile[f0 + k] = t;
+        const int sl = (int)(f0 + k);
+        int cur = fslot[f].load(std::memory_order_relaxed);
+        while (sl < cur && !fslot[f].compare_exchange_weak(cur, sl, std::memory_order_relaxed)) {
+        }
+        fcount[f].fetch_add(1, std::memory_order_relaxed);
+      }
+    }
+  });
+  tface.clear();
+  tface.shrink_to_fit();
+  // that first tile face, its tile's block, the tile count (-1 / 0 for a
+  // face no tile names)
+  std::vector<int> face_slot0(F), face_tblk(F), face_ntile(F);
+  host_parallel(F, [&](int lo, int hi) {
+    for (int f = lo; f < hi; ++f) {
+      const int sl = fslot[f].load(std::memory_order_relaxed);
+      const bool named = sl != INT_MAX;
+      face_slot0[f] = named ? sl : -1;
+      face_tblk[f] = named ? tiles[slot_tile[sl]].blk : -1;
+      face_ntile[f] = fcount[f].load(std::memory_order_relaxed);
+    }
+  });
   c->ntiles = (int)tiles.size();
   c->max_tile_faces = maxf;
   // ---- optional coarse/fine balance: the pairs (tile faces and blocks of
@@ -1338,20 +1480,6 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
 }  // extern "C"
 
 namespace {
-// host loops of the state boundary: split over up to 16 threads
-template <typename F>
-void host_parallel(int n, F f) {
-  const int nt = std::max(1, std::min<int>(16, (int)std::thread::hardware_concurrency()));
-  if (n < (1 << 16) || nt == 1) { f(0, n); return; }
-  std::vector<std::thread> th;
-  const int chunk = (n + nt - 1) / nt;
-  for (int t = 0; t < nt; ++t) {
-    const int a = t * chunk, b = std::min(n, a + chunk);
-    if (a < b) th.emplace_back([=] { f(a, b); });
-  }
-  for (auto &x : th) x.join();
-}
-
 // pinned staging for the state boundary (grown on demand, kept until the
 // context is destroyed): page-locked copies run at the link rate
 void *host_stage(ibg_ctx *c, size_t bytes) {
@@ -1763,10 +1891,15 @@ ibg_status ibg_upload_surface(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_surfa
     const int b = c->hcellblk[c->h2d[g]];
     return (b % L) == L - 1;
   };
-  for (int g = 0; g < N; ++g) {
-    const int p = sf->cell_patch[g];
-    if (p < -1 || p >= np) return fail("cell patch out of range");
-    if (p >= 0 && pcase[p] != (int)c->hcellblk[c->h2d[g]] / L) return fail("cell patch of another case");
+  {
+    const char *msg = nullptr;
+    if (first_failure(N, [&](int g) -> const char * {
+          const int p = sf->cell_patch[g];
+          if (p < -1 || p >= np) return "cell patch out of range";
+          if (p >= 0 && pcase[p] != (int)c->hcellblk[c->h2d[g]] / L) return "cell patch of another case";
+          return nullptr;
+        }, &msg) >= 0)
+      return fail(msg);
   }
   for (int p = 0; p < np; ++p) {
     const int g = sf->pat_close[p];
@@ -1776,14 +1909,16 @@ ibg_status ibg_upload_surface(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_surfa
   // device-order per-cell arrays
   std::vector<int> key((size_t)N * 3), cp(N);
   std::vector<int8_t> kind(N), fin(N);
-  for (int i = 0; i < N; ++i) {
-    const int g = c->d2h[i];
-    for (int d = 0; d < 3; ++d) key[(size_t)i * 3 + d] = m->key[(size_t)g * 3 + d];
-    kind[i] = m->kind[g];
-    const int b = c->hcellblk[i];
-    fin[i] = ((b % L) == L - 1 && nbody > 0) ? 1 : 0;
-    cp[i] = sf->cell_patch[g];
-  }
+  host_parallel(N, [&](int lo, int hi) {
+    for (int i = lo; i < hi; ++i) {
+      const int g = c->d2h[i];
+      for (int d = 0; d < 3; ++d) key[(size_t)i * 3 + d] = m->key[(size_t)g * 3 + d];
+      kind[i] = m->kind[g];
+      const int b = c->hcellblk[i];
+      fin[i] = ((b % L) == L - 1 && nbody > 0) ? 1 : 0;
+      cp[i] = sf->cell_patch[g];
+    }
+  });
   std::vector<int> pt0(sf->pat_pt0, sf->pat_pt0 + np), pb(sf->pat_body, sf->pat_body + np);
   std::vector<int8_t> flip(sf->pat_flip, sf->pat_flip + np);
   std::vector<double> dz(sf->pat_dz, sf->pat_dz + np);
@@ -1805,7 +1940,8 @@ ibg_status ibg_upload_surface(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_surfa
     for (int b = 0; b < c->nblk; ++b)
       for (int d = 0; d < 3; ++d) borg[(size_t)b * 3 + d] = org[(size_t)(b / c->L) * 3 + d];
     UP(v.borg, borg);
-  } UP(v.pat_pt0, pt0); UP(v.pat_body, pb);
+  }
+  UP(v.pat_pt0, pt0); UP(v.pat_body, pb);
   UP(v.pat_flip, flip); UP(v.pat_dz, dz); UP(v.body_p0, bp0); UP(v.pat_cc, cc);
   UP(v.bcase_d, bcase);
   {
@@ -1845,11 +1981,12 @@ ibg_status ibg_upload_surface(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_surfa
   {
     // ghosts by level, finest first: the ghosts of the levels >= l are a
     // prefix of this list
+    std::vector<std::vector<int>> byl(L);
+    for (int i = c->Nn; i < N; ++i) byl[c->hcellblk[i] % L].push_back(i);
     std::vector<int> gl;
     v.gend.assign(L, 0);
     for (int l = L - 1; l >= 0; --l) {
-      for (int i = c->Nn; i < N; ++i)
-        if (c->hcellblk[i] % L == l) gl.push_back(i);
+      gl.insert(gl.end(), byl[l].begin(), byl[l].end());
       v.gend[l] = (int)gl.size();
     }
     if (gl.empty()) gl.push_back(0);
@@ -1858,22 +1995,30 @@ ibg_status ibg_upload_surface(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_surfa
   {
     // in-plane neighbours in SoA (Point_iterator_2d order without the cell)
     static const int q8[kNb8] = {1, 4, 7, 10, 16, 19, 22, 25};
+    // (from the host mesh, in device order)
     std::vector<int> nb8((size_t)kNb8 * N);
-    std::vector<int> nbh((size_t)N * 27);
-    HIPCHK(hipMemcpy(nbh.data(), c->nb27, sizeof(int) * nbh.size(), hipMemcpyDeviceToHost));
-    for (int k = 0; k < kNb8; ++k)
-      for (int i = 0; i < N; ++i) nb8[(size_t)k * N + i] = nbh[(size_t)i * 27 + q8[k]];
+    host_parallel(N, [&](int lo, int hi) {
+      for (int i = lo; i < hi; ++i) {
+        const int g = c->d2h[i];
+        for (int k = 0; k < kNb8; ++k) {
+          const int a = m->nb[(size_t)g * 27 + q8[k]];
+          nb8[(size_t)k * N + i] = a < 0 ? -1 : c->h2d[a];
+        }
+      }
+    });
     UP(v.nb8, nb8);
     // mixed-body neighbourhoods of the uploaded closest patches
     std::vector<int8_t> cmix(N, 0);
-    for (int i = 0; i < N; ++i) {
-      if (cp[i] < 0) continue;
-      const int bi = sf->pat_body[cp[i]];
-      for (int k = 0; k < kNb8; ++k) {
-        const int y = nb8[(size_t)k * N + i];
-        if (y >= 0 && cp[y] >= 0 && sf->pat_body[cp[y]] != bi) { cmix[i] = 1; break; }
+    host_parallel(N, [&](int lo, int hi) {
+      for (int i = lo; i < hi; ++i) {
+        if (cp[i] < 0) continue;
+        const int bi = sf->pat_body[cp[i]];
+        for (int k = 0; k < kNb8; ++k) {
+          const int y = nb8[(size_t)k * N + i];
+          if (y >= 0 && cp[y] >= 0 && sf->pat_body[cp[y]] != bi) { cmix[i] = 1; break; }
+        }
       }
-    }
+    });
     UP(v.cmix, cmix);
     // move stamps: the uploaded IB set is move 1
     std::vector<int> mkib(N, 0), one(1, 1);
