@@ -47,9 +47,9 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 # every dispatch of k_stage_ml<2> in this script's profiled run,
 # scaled by what the counters read for a plain 1 GiB copy (FETCH_SIZE x2.0,
 # WRITE_SIZE x1.0), divided by the cells of the launch (the launch mix of the
-# timed steps).  Recorded in profiles/k_stage_naca2d_sa_traffic.md; update it
+# timed steps), as recorded with the traffic profile of this workload; update it
 # with the kernel.
-TRAFFIC_BYTES_PER_CELL = 162.6
+TRAFFIC_BYTES_PER_CELL = 164.2
 
 
 def stage_bytes(mode, dim, turb, cells, faces, last=False, first=False):
