@@ -266,7 +266,11 @@ def secondary(args):
             best, ms = eng.autotune_tile()
             eng.set_state(mesh.init_state())
             eng.set_nut(np.full(mesh.ncells, mesh.phys[0].nut_inf))
-            extra[f"tile_{mode}"] = {"chosen": best, "stage_ms_by_tile": ms}
+            extra[f"tile_{mode}"] = {
+                "chosen": best, "schedule": eng.schedule,
+                "stage_ms_by_tile_and_schedule": {f"tile {t}, "
+                                                  f"{'persistent' if sch else 'one workgroup per tile'}": v
+                                                  for (t, sch), v in ms.items()}}
         r = timed_run(eng, mesh, mode, args.warmup, args.steps, args.reps)
         if args.workload == "ensemble" and mode == "fp64":
             eng.closure_init(1 << 23)
