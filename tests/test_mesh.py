@@ -215,3 +215,35 @@ def test_level_counts_and_nesting():
         for f in fine[::5]:
             parent = (m.key[f, 0] // 2, m.key[f, 1] // 2)
             assert parent not in normal_keys
+
+
+# The builder runs its loops on threads (OpenMP): the level set, its order
+# and every table must not depend on the thread count.
+_DUMP = r"""
+import sys, hashlib
+import numpy as np
+import ibgpu
+m = ibgpu.Mesh(ibgpu.tiny_case(levels=4))
+h = hashlib.sha256()
+for k in ("blk_start", "blk_nnormal", "key", "kind", "nb", "cell_faces", "signdis", "ibidx",
+          "face_cells", "face_dir", "res_ci", "res_fi", "pro_ci", "pro_fi", "ib_cell", "ib_intp",
+          "ib_coef", "pat_intp", "pat_coef", "mdy"):
+    h.update(np.ascontiguousarray(getattr(m, k)).tobytes())
+print(h.hexdigest())
+"""
+
+
+def test_build_independent_of_thread_count():
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    outs = []
+    for nt in ("1", "3", "8"):
+        env = dict(os.environ, OMP_NUM_THREADS=nt,
+                   PYTHONPATH=os.path.join(root, "immsered-boundary_amd"))
+        r = subprocess.run([sys.executable, "-c", _DUMP], env=env, capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stderr[-2000:]
+        outs.append(r.stdout.strip().splitlines()[-1])
+    assert outs[0] == outs[1] == outs[2]
