@@ -48,8 +48,9 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 # scaled by what the counters read for a plain 1 GiB copy (FETCH_SIZE x2.0,
 # WRITE_SIZE x1.0), divided by the cells of the launch (the launch mix of the
 # timed steps), as recorded with the traffic profile of this workload; update it
-# with the kernel.
-TRAFFIC_BYTES_PER_CELL = 164.2
+# with the kernel.  FETCH_SIZE counts what the L2s fetch from the fabric,
+# last-level-cache (MALL) hits included, so it bounds the HBM bytes from above.
+TRAFFIC_BYTES_PER_CELL = 198.3
 
 
 def stage_bytes(mode, dim, turb, cells, faces, last=False, first=False):
@@ -503,7 +504,8 @@ def main():
                      "traffic_source": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of this kernel in a "
                                        "profiled run of this script (profiles/k_stage_naca2d_sa_"
                                        "traffic.md), bytes per cell x this run's cells per launch; "
-                                       "not counted in this run",
+                                       "not counted in this run; L2 fetches including hits in "
+                                       "the shared last-level cache, an upper bound on HBM bytes",
                      "kernel": "k_stage_ml<2>", "bytes_per_launch": ml_bytes,
                      "avg_launch_ms": ml["stage_ms"], "cells_per_launch": ml["stage_cells"],
                      "copy_probe_GBs": copy_gbs},

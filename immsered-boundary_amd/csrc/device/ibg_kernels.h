@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(kTile) k_visc_prep(ViscArgs<R, D> a) {
   extern __shared__ __align__(32) unsigned char smem_raw[];
   __shared__ int halt;
   const bool herr = halted(a.err);
-  const TileInfo ti = a.tiles[xcd_tile()];
+  const TileInfo ti = a.tiles[dispatch_tile()];
   visc_prep_tile<R, D>(a, ti, smem_raw, &halt, herr);
 }
 
@@ -738,7 +738,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R, D> &a, int tile) {
 template <typename R, int D, bool ML, bool P>
 __device__ __forceinline__ void stage_launch(const StageArgs<R, D> &a) {
   if constexpr (!P) {
-    stage_tile<R, D, ML, false>(a, xcd_tile());
+    stage_tile<R, D, ML, false>(a, dispatch_tile());
     return;
   }
   extern __shared__ __align__(32) unsigned char smem_raw[];
