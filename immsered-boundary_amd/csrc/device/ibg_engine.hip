@@ -133,7 +133,7 @@ struct ibg_ctx {
   // allocs with the mesh
   struct MoveState {
     bool on = false, law = false;
-    int npoint = 0, cap = 0, cap_s = 0, cap_l1 = 0, cap_io = 0;
+    int npoint = 0, cap = 0, cap_s = 0, cap_io = 0;
     int *key = nullptr, *cpatch = nullptr, *cp1 = nullptr, *nb8 = nullptr;
     int8_t *kind = nullptr, *fin = nullptr;
     int16_t *cblk = nullptr;
@@ -151,9 +151,9 @@ struct ibg_ctx {
     int *chg = nullptr;                   // cells whose body changed
     int cap_chg = 0;
     int *epoch = nullptr;                 // [1] move counter
-    int *cnt = nullptr;                   // [6] solid, layer-1, in->out, IB, changed, mixed
+    int *cnt = nullptr;                   // [6] solid, (spare), in->out, IB, changed, mixed
     int *mixlist = nullptr;               // [N] cells deferred to k_move_mixed
-    int *slist = nullptr, *l1list = nullptr, *iolist = nullptr;
+    int *slist = nullptr, *iolist = nullptr;
     int *gidx = nullptr;          // ghosts sorted by level, finest first
     std::vector<int> gend;        // [l]: ghosts of levels >= l
     unsigned long long *upd = nullptr, *lvlcnt = nullptr;
@@ -616,8 +616,8 @@ MoveArgs move_args(ibg_ctx *c, unsigned long long *upd, int lmin) {
   a.mk1 = v.mk1; a.mkib = v.mkib; a.epoch = v.epoch; a.cnt = v.cnt;
   a.cmix = v.cmix; a.chg = v.chg; a.cap_chg = v.cap_chg;
   a.mixlist = v.mixlist; a.cap_mix = c->N;
-  a.cap_s = v.cap_s; a.cap_l1 = v.cap_l1; a.cap_io = v.cap_io; a.ib_cap = v.cap;
-  a.slist = v.slist; a.l1list = v.l1list; a.iolist = v.iolist;
+  a.cap_s = v.cap_s; a.cap_io = v.cap_io; a.ib_cap = v.cap;
+  a.slist = v.slist; a.iolist = v.iolist;
   a.flags = c->flags;
   a.pt_off = v.pt_off; a.pat_pt0 = v.pat_pt0; a.pat_body = v.pat_body; a.pat_case = c->pat_case;
   a.pat_flip = v.pat_flip; a.pat_dz = v.pat_dz; a.body_p0 = v.body_p0; a.body_case = v.bcase_d;
@@ -643,8 +643,7 @@ ibg_status run_move(ibg_ctx *c, unsigned long long *upd, int lmin) {
   hipLaunchKernelGGL(k_move_front, dim3(1), dim3(256), 0, c->s, a);
   hipLaunchKernelGGL(k_move_dist, dim3(grid(nw, 256)), dim3(256), 0, c->s, a);
   hipLaunchKernelGGL(k_move_mixed, dim3(256), dim3(256), 0, c->s, a);
-  hipLaunchKernelGGL(k_move_mark1, dim3(grid(a.cap_s + a.cap_chg, 256)), dim3(256), 0, c->s, a);
-  hipLaunchKernelGGL(k_move_mark2, dim3(grid(a.cap_l1 + a.cap_io, 256)), dim3(256), 0, c->s, a);
+  hipLaunchKernelGGL(k_move_mark, dim3(grid(kNb8 * a.cap_s + a.cap_chg, 256)), dim3(256), 0, c->s, a);
   const int ncommit = (int)std::min(grid(nw, 256), 2048u);
   hipLaunchKernelGGL(k_move_finish, dim3(ncommit + grid(a.ib_cap + c->npatch, 256)), dim3(256), 0,
                      c->s, a, ncommit);
@@ -2059,7 +2058,6 @@ ibg_status ibg_upload_surface(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_surfa
   c->ib_cell = nc_; c->ib_intp = ni; c->ib_body = nbd; c->ib_case = ncs;
   c->ib_coef = nco; c->ib_hgdis = nh; c->ib_attach = nat; c->ib_nv = nnv;
   v.cap = cap;
-  v.cap_l1 = cap;   // layer 1 is part of the IB set
   {
     ibg_status st2;
     std::vector<int> cnt0 = {0, 0, 0, c->nib, 0, 0};
@@ -2067,7 +2065,6 @@ ibg_status ibg_upload_surface(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_surfa
     if ((st2 = dalloc(c, &v.mixlist, (size_t)N))) return st2;
     if ((st2 = dalloc(c, &v.chg, (size_t)v.cap_chg))) return st2;
     if ((st2 = dalloc(c, &v.slist, (size_t)v.cap_s))) return st2;
-    if ((st2 = dalloc(c, &v.l1list, (size_t)v.cap_l1))) return st2;
     if ((st2 = dalloc(c, &v.iolist, (size_t)v.cap_io))) return st2;
   }
   v.npoint = sf->npoint;

@@ -1,6 +1,6 @@
 // ibg_move.h — per-step immersed-boundary re-tagging of moving bodies on a
 // fixed level set (the part of Body::MovingBody_Image, Bodymove.cpp:37-125,
-// that runs between regrids), 2D sections.  Six launches per move:
+// that runs between regrids), 2D sections.  Five launches per move:
 //
 //   k_move_front   pose of every case (given, or from the pitch/heave law at
 //                  the case's time, with the wall motion of the IB fill and
@@ -14,13 +14,13 @@
 //                  to lists, cells with a neighbour of another body to
 //   k_move_mixed   the same with the cross-body cycle (:156-223), eight
 //                  lanes per cell
-//   k_move_mark1   layer 1 of the IB set: non-solid neighbours of solid cells;
-//                  the mixed-body neighbourhood flags around changed cells
-//   k_move_mark2   the IB set: layer-1 cells and their non-solid neighbours
-//                  (RemoveIBCells / InfectNewIBCells :345-767 keep, after
-//                  every move, the set of the initial rule of Body.cpp:
-//                  1016-1082); the in->out check (abort 314)
-//   k_move_finish  image points and weights of every IB cell (FindImageCell
+//   k_move_mark    the IB set: the non-solid neighbours of solid cells
+//                  (layer 1) and their non-solid neighbours (RemoveIBCells /
+//                  InfectNewIBCells :345-767 keep, after every move, the set
+//                  of the initial rule of Body.cpp:1016-1082); the
+//                  mixed-body neighbourhood flags around changed cells
+//   k_move_finish  the in->out check (abort 314); image points and weights
+//                  of every IB cell (FindImageCell
 //                  Body.cpp:999-1190, Mesh.cpp:1403-1442) and of the
 //                  wall-force patches (Body.cpp:1506-1575); new distances in
 //                  place, the stage masks of the marched cells
@@ -93,9 +93,9 @@ struct MoveArgs {
   int *mixlist;          // cells left to k_move_mixed (their neighbours' bodies differ)
   int cap_mix;
   int *epoch;            // [1] move counter
-  int *cnt;              // [6] solid, layer-1, in->out, IB, changed, mixed list lengths
-  int cap_s, cap_l1, cap_io, ib_cap;
-  int *slist, *l1list, *iolist;
+  int *cnt;              // [6] list lengths: solid, (spare), in->out, IB, changed, mixed
+  int cap_s, cap_io, ib_cap;
+  int *slist, *iolist;
   uint8_t *flags;        // [Nn]
   // surface
   const double *pt_off;  // [npoint][2]
@@ -535,14 +535,22 @@ __device__ __forceinline__ void refresh_cmix(const MoveArgs &a, int x) {
   a.cmix[x] = v;
 }
 
-// Layer 1: the non-solid, non-domain-ghost neighbours of each solid cell of
-// the finest level (2D: the in-plane 3x3).
-__global__ void __launch_bounds__(256) k_move_mark1(MoveArgs a) {
+// The IB set of the move in one pass: eight lanes per solid cell of the
+// finest level, one per in-plane neighbour x.  An eligible x (not a domain
+// ghost, not solid) is a layer-1 cell -- stamped for the in->out check of
+// k_move_finish -- and an IB cell, and so is every eligible in-plane
+// neighbour of x: the set of the initial rule (Body.cpp:1016-1082), which
+// RemoveIBCells / InfectNewIBCells (Bodymove.cpp:345-767) keep after every
+// move.  Stamps by atomicMax of the move counter; the first IB stamp of a
+// cell appends it to the IB list (one atomic per wave).  Threads past the
+// solid lanes refresh the mixed-body flags around cells whose body changed.
+__global__ void __launch_bounds__(256) k_move_mark(MoveArgs a) {
   if (halted(a.err)) return;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.cap_s) {
+  const int nsl = kNb8 * a.cap_s;   // lanes of the solid list
+  if (j >= nsl) {
     // the neighbourhood flags around cells whose body changed
-    const int r = j - a.cap_s;
+    const int r = j - nsl;
     if (r >= min(a.cnt[4], a.cap_chg)) return;
     const int c = a.chg[r];
     refresh_cmix(a, c);
@@ -552,68 +560,36 @@ __global__ void __launch_bounds__(256) k_move_mark1(MoveArgs a) {
     }
     return;
   }
-  // (the lanes past the solid list's capacity have left: those that reach
-  // the append are the low lanes of the wave)
-  const int ns = min(a.cnt[0], a.cap_s);
+  // (the lanes past the solid lanes have left: those that reach the append
+  // are the low lanes of the wave)
   const int ep = a.epoch[0];
-  int nb[kNb8];
+  const int js = j / kNb8, k = j % kNb8;
+  int c[kNb8 + 1];
   unsigned fresh = 0;
-  if (j < ns) {
-    const int s = a.slist[j];
-    bool ok[kNb8];
-#pragma unroll
-    for (int k = 0; k < kNb8; ++k) {
-      const int n = a.nb8[(size_t)k * a.N + s];
-      nb[k] = n;
-      ok[k] = n >= 0 && a.kind[n] != IBG_DMGHOST && !(a.sd1[n] < 0.0);
-    }
-    // the eight stamps are independent: issued together, not one round
-    // trip after another
-#pragma unroll
-    for (int k = 0; k < kNb8; ++k)
-      if (ok[k] && atomicMax(&a.mk1[nb[k]], ep) < ep) fresh |= 1u << k;
-  }
-  wave_append_set<kNb8>(a.cnt + 1, fresh, nb, a.l1list, a.cap_l1, a.err);
-}
-
-// The IB set: every layer-1 cell and its non-solid, non-domain-ghost
-// neighbours.  Threads past the layer-1 capacity check the in->out cells:
-// each must be a layer-1 cell (InfectNewIBCells Bodymove.cpp:575-650,
-// abort 314 -- the move step is too large for the incremental update).
-__global__ void __launch_bounds__(256) k_move_mark2(MoveArgs a) {
-  if (halted(a.err)) return;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ep = a.epoch[0];
-  if (j < a.cap_l1) {
-    // a layer-1 cell and its eligible neighbours: nine independent stamps,
-    // then one append per wave (the lanes that reach it are the low lanes)
-    int c[kNb8 + 1];
-    unsigned fresh = 0;
-    if (j < min(a.cnt[1], a.cap_l1)) {
-      const int n = a.l1list[j];
+  if (js < min(a.cnt[0], a.cap_s)) {
+    const int sc = a.slist[js];
+    const int x = a.nb8[(size_t)k * a.N + sc];
+    if (x >= 0 && a.kind[x] != IBG_DMGHOST && !(a.sd1[x] < 0.0)) {
       bool ok[kNb8 + 1];
-      c[0] = n;
+      c[0] = x;
       ok[0] = true;
 #pragma unroll
-      for (int k = 0; k < kNb8; ++k) {
-        const int m = a.nb8[(size_t)k * a.N + n];
-        c[k + 1] = m;
-        ok[k + 1] = m >= 0 && a.kind[m] != IBG_DMGHOST && !(a.sd1[m] < 0.0);
+      for (int kk = 0; kk < kNb8; ++kk) {
+        const int m = a.nb8[(size_t)kk * a.N + x];
+        c[kk + 1] = m;
+        ok[kk + 1] = m >= 0 && a.kind[m] != IBG_DMGHOST && !(a.sd1[m] < 0.0);
       }
+      atomicMax(&a.mk1[x], ep);
+      // the nine stamps are independent: issued together
 #pragma unroll
-      for (int k = 0; k <= kNb8; ++k)
-        if (ok[k] && atomicMax(&a.mkib[c[k]], ep) < ep) {
-          fresh |= 1u << k;
-          if (a.kind[c[k]] != IBG_NORMAL) report_error(a.err, c[k], 1082);
+      for (int kk = 0; kk <= kNb8; ++kk)
+        if (ok[kk] && atomicMax(&a.mkib[c[kk]], ep) < ep) {
+          fresh |= 1u << kk;
+          if (a.kind[c[kk]] != IBG_NORMAL) report_error(a.err, c[kk], 1082);
         }
     }
-    wave_append_set<kNb8 + 1>(a.cnt + 3, fresh, c, a.ib_cell, a.ib_cap, a.err);
-  } else {
-    const int r = j - a.cap_l1;
-    if (r >= min(a.cnt[2], a.cap_io)) return;
-    const int c = a.iolist[r];
-    if (a.mk1[c] != ep) report_error(a.err, c, 314);
   }
+  wave_append_set<kNb8 + 1>(a.cnt + 3, fresh, c, a.ib_cell, a.ib_cap, a.err);
 }
 
 // Finest-level cell of case cs with in-plane key (ix, iy), -1 for none.
@@ -794,6 +770,14 @@ __device__ void move_image(const MoveArgs &a, int j) {
 __device__ void move_commit(const MoveArgs &a, int nblk) {
   const int ep = a.epoch[0];
   const int n = a.nw + a.ngw;
+  // every in->out cell must be a layer-1 cell (InfectNewIBCells
+  // Bodymove.cpp:575-650, abort 314: the move step is too large for the
+  // incremental update)
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < min(a.cnt[2], a.cap_io);
+       r += nblk * blockDim.x) {
+    const int c = a.iolist[r];
+    if (a.mk1[c] != ep) report_error(a.err, c, 314);
+  }
   int cnt = 0;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += nblk * blockDim.x) {
     const int i = set_cell(a, t, a.nw);
