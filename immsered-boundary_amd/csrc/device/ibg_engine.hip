@@ -645,8 +645,8 @@ ibg_status run_move(ibg_ctx *c, unsigned long long *upd, int lmin) {
   hipLaunchKernelGGL(k_move_mixed, dim3(256), dim3(256), 0, c->s, a);
   hipLaunchKernelGGL(k_move_mark, dim3(grid(kNb8 * a.cap_s + a.cap_chg, 256)), dim3(256), 0, c->s, a);
   const int ncommit = (int)std::min(grid(nw, 256), 2048u);
-  hipLaunchKernelGGL(k_move_finish, dim3(ncommit + grid(a.ib_cap + c->npatch, 256)), dim3(256), 0,
-                     c->s, a, ncommit);
+  const int nimage = (int)grid(a.ib_cap + c->npatch, 256);
+  hipLaunchKernelGGL(k_move_finish, dim3(nimage + ncommit), dim3(256), 0, c->s, a, nimage);
   HIPCHK(hipGetLastError());
   return IBG_OK;
 }

@@ -366,6 +366,43 @@ __device__ __forceinline__ void ml_fill_weights(const MlpDev &M, unsigned char *
   }
 }
 
+// Reduction of 16 per-lane slots over a wave: slots 0..7 by maximum, 8..15
+// by sum.  A butterfly that halves the slots a lane keeps at each exchange
+// (offsets 32, 16, 8, 4: 8 + 4 + 2 + 1 exchanges; the lane with the offset's
+// bit set keeps the upper half, so after the first exchange lanes 0..31
+// hold maxima and lanes 32..63 sums), then two exchanges of the one slot
+// left: lane l returns slot l / 4 of the whole wave.  17 exchanges instead
+// of 6 per slot.  The partner of an exchange has the same slot set (their
+// lanes differ only in bits not yet used), so each slot combines the same
+// 64 values as a per-slot reduction, in another order.
+__device__ __forceinline__ double wave_slots16(double *v) {
+  const int lane = threadIdx.x & 63;
+  const bool sum = (lane & 32) != 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const double keep = sum ? v[j + 8] : v[j], send = sum ? v[j] : v[j + 8];
+    const double o = __shfl_xor(send, 32, 64);
+    v[j] = sum ? keep + o : fmax(keep, o);
+  }
+#pragma unroll
+  for (int m = 4, off = 16; m >= 1; m >>= 1, off >>= 1) {
+    const bool up = (lane & off) != 0;
+#pragma unroll
+    for (int j = 0; j < m; ++j) {
+      const double keep = up ? v[j + m] : v[j], send = up ? v[j] : v[j + m];
+      const double o = __shfl_xor(send, off, 64);
+      v[j] = sum ? keep + o : fmax(keep, o);
+    }
+  }
+  double x = v[0];
+#pragma unroll
+  for (int off = 2; off >= 1; off >>= 1) {
+    const double o = __shfl_xor(x, off, 64);
+    x = sum ? x + o : fmax(x, o);
+  }
+  return x;
+}
+
 // One tile.  WRES: the network's weights are resident at a.mlw_off
 // (persistent schedule); otherwise the ML instantiation stages them after
 // the face fluxes, over the face-cell indices (dead by then).
@@ -685,23 +722,21 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R, D> &a, int tile) {
     }
   }
   if (a.last) {
+    // the wave's partials as 16 slots: maxima (|s_k|, then |snu|) in 0..7,
+    // sums of squares in the same order in 8..15, zeros in the rest
+    double v[16];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      for (int off = 32; off > 0; off >>= 1) {
-        mx[k] = fmax(mx[k], __shfl_xor(mx[k], off, 64));
-        sq[k] += __shfl_xor(sq[k], off, 64);
-      }
+    for (int j = 0; j < 8; ++j) {
+      v[j] = j < NV ? mx[j < NV ? j : 0] : (j == NV ? nmx : 0.0);
+      v[8 + j] = j < NV ? sq[j < NV ? j : 0] : (j == NV ? nsq : 0.0);
     }
-    for (int off = 32; off > 0; off >>= 1) {
-      nmx = fmax(nmx, __shfl_xor(nmx, off, 64));
-      nsq += __shfl_xor(nsq, off, 64);
-    }
+    const double r = wave_slots16(v);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) {
-#pragma unroll
-      for (int k = 0; k < NV; ++k) { red[k][wid] = mx[k]; red[5 + k][wid] = sq[k]; }
-      red[10][wid] = nmx;
-      red[11][wid] = nsq;
+    if ((lane & 3) == 0) {
+      // lane 4 s holds slot s of the whole wave
+      const int sl = lane >> 2, j = sl & 7;
+      if (j < NV) red[sl < 8 ? j : 5 + j][wid] = r;
+      else if (j == NV) red[sl < 8 ? 10 : 11][wid] = r;
     }
     __syncthreads();
     if (threadIdx.x == 0) {

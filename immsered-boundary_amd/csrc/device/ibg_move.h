@@ -766,20 +766,20 @@ __device__ void move_image(const MoveArgs &a, int j) {
 // New distances and patches in place (the FP32 copy for the mixed SA
 // source), the stage masks of the marched normal cells as at upload; with
 // `upd` set, the stage updates of this step are accumulated (one atomic
-// per workgroup, grid-stride).
-__device__ void move_commit(const MoveArgs &a, int nblk) {
+// per workgroup, grid-stride).  Workgroup cb of the nblk committing ones.
+__device__ void move_commit(const MoveArgs &a, int cb, int nblk) {
   const int ep = a.epoch[0];
   const int n = a.nw + a.ngw;
   // every in->out cell must be a layer-1 cell (InfectNewIBCells
   // Bodymove.cpp:575-650, abort 314: the move step is too large for the
   // incremental update)
-  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < min(a.cnt[2], a.cap_io);
+  for (int r = cb * blockDim.x + threadIdx.x; r < min(a.cnt[2], a.cap_io);
        r += nblk * blockDim.x) {
     const int c = a.iolist[r];
     if (a.mk1[c] != ep) report_error(a.err, c, 314);
   }
   int cnt = 0;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += nblk * blockDim.x) {
+  for (int t = cb * blockDim.x + threadIdx.x; t < n; t += nblk * blockDim.x) {
     const int i = set_cell(a, t, a.nw);
     const double sd = a.sd1[i];
     a.cpatch[i] = a.cp1[i];
@@ -811,18 +811,22 @@ __device__ void move_commit(const MoveArgs &a, int nblk) {
     if (threadIdx.x == 0) {
       const int s = part[0] + part[1] + part[2] + part[3];
       if (s)
-        atomicAdd(a.upd + (size_t)(blockIdx.x % kUpdSlots) * kUpdSlotStride,
+        atomicAdd(a.upd + (size_t)(cb % kUpdSlots) * kUpdSlotStride,
                   (unsigned long long)s * a.nstage);
     }
   }
 }
 
-// Commit and image search in one launch: workgroups [0, ncommit) commit,
-// the others search (they read only the new arrays, not the committed ones).
-__global__ void __launch_bounds__(256) k_move_finish(MoveArgs a, int ncommit) {
+// Image search and commit in one launch: workgroups [0, nimage) search
+// (they read only the new arrays, not the committed ones), the others
+// commit.  The searches are a few thousand dependent chains of loads, the
+// commit a stream over every marched cell: dispatched first, the searches
+// run beside the stream instead of after it.
+__global__ void __launch_bounds__(256) k_move_finish(MoveArgs a, int nimage) {
   if (wg_halted(a.err)) return;
-  if ((int)blockIdx.x < ncommit) move_commit(a, ncommit);
-  else move_image(a, ((int)blockIdx.x - ncommit) * blockDim.x + threadIdx.x);
+  const int b = (int)blockIdx.x;
+  if (b < nimage) move_image(a, b * blockDim.x + threadIdx.x);
+  else move_commit(a, b - nimage, (int)gridDim.x - nimage);
 }
 
 // Per-level count of the cells the stage update touches.
