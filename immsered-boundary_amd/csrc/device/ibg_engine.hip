@@ -164,12 +164,17 @@ struct ibg_ctx {
   } mv;
   double *dt = nullptr, *cflpart = nullptr, *rpart = nullptr, *resid = nullptr, *probe = nullptr;
   unsigned long long *err = nullptr;
+  unsigned long long *err2 = nullptr;   // error word of the next step's move, run ahead
   int last_ntiles = 0;
   // captured step ranges of the time march, keyed by graph_key(): the
   // launch sequence of a range depends only on its first step's phase in
   // the level schedule, its length and whether it starts with a dt update
   std::map<long long, hipGraphExec_t> gexec;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // the next step's move runs ahead on s2, forked after the IB fill and dt
+  // (evf) and joined at the next step's start (evj)
+  hipStream_t s2 = nullptr;
+  hipEvent_t evf = nullptr, evj = nullptr;
   double last_total_ms = 0.0, last_stage_ms = 0.0;
   int64_t last_stage_launches = 0, last_updates = 0;
   std::vector<void *> allocs;
@@ -604,7 +609,7 @@ ibg_status launch_ib(ibg_ctx *c) {
 
 ibg_status run_ib(ibg_ctx *c) { return c->D == 2 ? launch_ib<2>(c) : launch_ib<3>(c); }
 
-MoveArgs move_args(ibg_ctx *c, unsigned long long *upd, int lmin) {
+MoveArgs move_args(ibg_ctx *c, unsigned long long *upd, int lmin, bool ahead = false) {
   MoveArgs a{};
   ibg_ctx::MoveState &v = c->mv;
   a.N = c->N; a.Nn = c->Nn; a.npatch = c->npatch; a.ncases = c->ncases; a.nbody = c->nbody;
@@ -623,14 +628,30 @@ MoveArgs move_args(ibg_ctx *c, unsigned long long *upd, int lmin) {
   a.pat_flip = v.pat_flip; a.pat_dz = v.pat_dz; a.body_p0 = v.body_p0; a.body_case = v.bcase_d;
   a.pat_lh = v.pat_lh;
   a.fbox = v.fbox; a.foff = v.foff; a.fgrid = v.fgrid;
-  a.pose = v.pose; a.use_law = v.law ? 1 : 0; a.law = v.lawd; a.t = v.t;
+  a.pose = v.pose; a.use_law = v.law ? 1 : 0; a.law = v.lawd; a.t = v.t; a.dt = c->dt;
+  a.ahead = ahead ? 1 : 0;
   a.mot = c->motion; a.motc = v.motc; a.case_center = c->case_center;
   a.pat_pc = c->pat_pc; a.pat_nv = c->pat_nv; a.pat_area = c->pat_area;
   a.pat_intp = c->pat_intp; a.pat_cc = v.pat_cc; a.pat_coef = c->pat_coef; a.pat_hgdis = c->pat_hgdis;
   a.ib_cell = c->ib_cell; a.ib_intp = c->ib_intp; a.ib_body = c->ib_body; a.ib_case = c->ib_case;
   a.ib_coef = c->ib_coef; a.ib_hgdis = c->ib_hgdis; a.ib_attach = c->ib_attach; a.ib_nv = c->ib_nv;
-  a.err = c->err; a.upd = upd; a.lvlcnt = v.lvlcnt; a.nstage = c->cfg.nstage;
+  a.err = ahead ? c->err2 : c->err; a.upd = upd; a.lvlcnt = v.lvlcnt; a.nstage = c->cfg.nstage;
   return a;
+}
+
+// The launches of one move up to the commit, on stream st.
+void launch_move(ibg_ctx *c, const MoveArgs &a, hipStream_t st, bool commit) {
+  const int nw = a.nw + a.ngw;
+  hipLaunchKernelGGL(k_move_front, dim3(1), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_move_dist, dim3(grid(nw, 256)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_move_mixed, dim3(256), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_move_mark, dim3(grid(kNb8 * a.cap_s + a.cap_chg, 256)), dim3(256), 0, st, a);
+  const int nimage = (int)grid(a.ib_cap + c->npatch, 256);
+  // with the commit: its workgroups also check; without: a few check
+  const int nrest = commit ? (int)std::min(grid(nw, 256), 2048u)
+                           : (int)std::min(grid(a.cap_io, 256), 64u);
+  hipLaunchKernelGGL(k_move_finish, dim3(nimage + std::max(nrest, 1)), dim3(256), 0, st, a, nimage,
+                     commit ? 1 : 0);
 }
 
 // One move of the bodies (to the poses in mv.pose, or by the law at each
@@ -638,15 +659,34 @@ MoveArgs move_args(ibg_ctx *c, unsigned long long *upd, int lmin) {
 // masks of the levels marched from lmin.  Stream-ordered, capturable.
 ibg_status run_move(ibg_ctx *c, unsigned long long *upd, int lmin) {
   if (c->D != 2) return IBG_ERR_STATE;
-  const MoveArgs a = move_args(c, upd, lmin);
-  const int nw = a.nw + a.ngw;
-  hipLaunchKernelGGL(k_move_front, dim3(1), dim3(256), 0, c->s, a);
-  hipLaunchKernelGGL(k_move_dist, dim3(grid(nw, 256)), dim3(256), 0, c->s, a);
-  hipLaunchKernelGGL(k_move_mixed, dim3(256), dim3(256), 0, c->s, a);
-  hipLaunchKernelGGL(k_move_mark, dim3(grid(kNb8 * a.cap_s + a.cap_chg, 256)), dim3(256), 0, c->s, a);
-  const int ncommit = (int)std::min(grid(nw, 256), 2048u);
-  const int nimage = (int)grid(a.ib_cap + c->npatch, 256);
-  hipLaunchKernelGGL(k_move_finish, dim3(nimage + ncommit), dim3(256), 0, c->s, a, nimage);
+  launch_move(c, move_args(c, upd, lmin), c->s, true);
+  HIPCHK(hipGetLastError());
+  return IBG_OK;
+}
+
+// The move of step ts (by the law, at t + dt) ahead of the time advance of
+// the step before, on s2 beside that step's remaining launches: forked
+// after the step's IB fill and dt, joined by run_move_join at step ts.
+// Nothing the step launches after the fork reads what the move writes
+// before its commit, or writes what the move reads.
+ibg_status run_move_ahead(ibg_ctx *c, int ts) {
+  if (c->D != 2 || !c->mv.law) return IBG_ERR_STATE;
+  HIPCHK(hipEventRecord(c->evf, c->s));
+  HIPCHK(hipStreamWaitEvent(c->s2, c->evf, 0));
+  launch_move(c, move_args(c, c->mv.upd, lmin_of(c, ts), true), c->s2, false);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c->evj, c->s2));
+  return IBG_OK;
+}
+
+// The start of step ts after run_move_ahead(ts): the time advance of the
+// step before, then the commit of the move (k_move_join).
+ibg_status run_move_join(ibg_ctx *c, int ts) {
+  HIPCHK(hipStreamWaitEvent(c->s, c->evj, 0));
+  const MoveArgs a = move_args(c, c->mv.upd, lmin_of(c, ts));
+  const int ncommit = (int)std::min(grid(a.nw + a.ngw, 256), 2048u);
+  hipLaunchKernelGGL(k_move_join, dim3(std::max(ncommit, 1)), dim3(256), 0, c->s, a,
+                     (const unsigned long long *)c->err2, c->mv.t);
   HIPCHK(hipGetLastError());
   return IBG_OK;
 }
@@ -667,17 +707,23 @@ ibg_status run_dt(ibg_ctx *c) {
   return IBG_OK;
 }
 
-// one full step: main.cpp:87-97
-ibg_status run_step(ibg_ctx *c, int ts) {
+// one full step: main.cpp:87-97.  With the prescribed law, `joined`: the
+// step's move ran ahead during the step before (run_move_ahead), `ahead`:
+// the next step's move runs ahead during this one.
+ibg_status run_step(ibg_ctx *c, int ts, bool joined, bool ahead) {
   ibg_status st;
   if ((st = run_exchange(c, ts))) return st;
   if (c->mv.law) {
     // MovingBody_Image at the start of IBConditions_Image (Body.cpp:1353-1357)
-    if ((st = run_move(c, c->mv.upd, lmin_of(c, ts)))) return st;
+    if (joined) st = run_move_join(c, ts);
+    else st = run_move(c, c->mv.upd, lmin_of(c, ts));
+    if (st) return st;
   }
   if ((st = run_ib(c))) return st;
   if (c->cfg.dt_var_step > 0 && ts % c->cfg.dt_var_step == 0)
     if ((st = run_dt(c))) return st;
+  if (c->mv.law && ahead)
+    if ((st = run_move_ahead(c, ts + 1))) return st;
   if ((st = run_bc(c, ts))) return st;
   const bool fuse = fused_visc(c);
   if (!fuse) {
@@ -688,7 +734,8 @@ ibg_status run_step(ibg_ctx *c, int ts) {
   }
   for (int k = 0; k < c->cfg.nstage; ++k)
     if ((st = run_stage(c, ts, k, nullptr, nullptr, fuse))) return st;
-  if (c->mv.law) {
+  if (c->mv.law && !ahead) {
+    // (with the next move ahead, its join advances the time)
     hipLaunchKernelGGL(k_motion_advance, dim3(grid(c->ncases, 64)), dim3(64), 0, c->s, c->ncases,
                        c->mv.t, (const double *)c->dt, (const unsigned long long *)c->err);
     HIPCHK(hipGetLastError());
@@ -698,6 +745,7 @@ ibg_status run_step(ibg_ctx *c, int ts) {
 
 ibg_status reset_err(ibg_ctx *c) {
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->s, c->err, 1, ~0ull);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->s, c->err2, 1, ~0ull);
   HIPCHK(hipGetLastError());
   return IBG_OK;
 }
@@ -901,6 +949,16 @@ ibg_status ibg_create(const ibg_config *cfg, ibg_ctx **out) {
   c->tile = tile;
   // default stream priority (no priority streams)
   if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) { delete c; return IBG_ERR_HIP; }
+  if (hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->evf, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->evj, hipEventDisableTiming) != hipSuccess) {
+    if (c->evf) (void)hipEventDestroy(c->evf);
+    if (c->evj) (void)hipEventDestroy(c->evj);
+    if (c->s2) (void)hipStreamDestroy(c->s2);
+    (void)hipStreamDestroy(c->s);
+    delete c;
+    return IBG_ERR_HIP;
+  }
   (void)hipEventCreate(&c->ev0);
   (void)hipEventCreate(&c->ev1);
   *out = c;
@@ -915,6 +973,9 @@ ibg_status ibg_destroy(ibg_ctx *c) {
   c->cl.allocs.clear();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->evf) (void)hipEventDestroy(c->evf);
+  if (c->evj) (void)hipEventDestroy(c->evj);
+  if (c->s2) (void)hipStreamDestroy(c->s2);
   if (c->s) (void)hipStreamDestroy(c->s);
   if (c->hstage) (void)hipHostFree(c->hstage);
   delete c;
@@ -1452,6 +1513,7 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
   AL(c->cflpart, (size_t)std::max(c->ntiles, 1));
   UP(c->case_tiles, case_tiles);
   AL(c->err, 1);
+  AL(c->err2, 1);
   AL(c->rpart, (size_t)std::max(c->ntiles, 1) * kRpart);
   AL(c->resid, (size_t)nc * kRpart);
   AL(c->probe, (size_t)std::max(N, F) * 5);
@@ -1698,7 +1760,7 @@ ibg_status range_graph(ibg_ctx *c, const MarchRange &r, bool launch) {
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < r.n; ++k) {
-      const ibg_status st = run_step(c, r.ts + k);
+      const ibg_status st = run_step(c, r.ts + k, k > 0, k + 1 < r.n);
       if (st) { hipGraph_t tmp; (void)hipStreamEndCapture(c->s, &tmp); if (tmp) (void)hipGraphDestroy(tmp); return st; }
     }
     HIPCHK(hipStreamEndCapture(c->s, &g));
@@ -1765,7 +1827,7 @@ ibg_status ibg_time_march(ibg_ctx *c, int32_t ts0, int32_t nsteps, int32_t use_g
       if ((st = range_graph(c, r, true))) return st;
     } else {
       for (int k = 0; k < r.n; ++k)
-        if ((st = run_step(c, ts + k))) return st;
+        if ((st = run_step(c, ts + k, k > 0, k + 1 < r.n))) return st;
     }
     ts += r.n;
   }

@@ -26,6 +26,14 @@
 //                  place, the stage masks of the marched cells
 //                  (NS_Solver.cpp:856-983, 19-179), update count
 //
+// A move of the prescribed law does not depend on the flow: the time march
+// runs the next step's move ahead, beside the current step's stages, on a
+// second stream and with its own error word (`ahead`: the front takes the
+// law at t + dt, k_move_finish only searches and checks), and the next step
+// begins with k_move_join: the time advance of the step before, the move's
+// error (if any) made the march's, the commit.  The results are those of
+// the move run in place.
+//
 // The IB set is stamped with a move counter (a cell is an IB cell when its
 // stamp equals the current move), so marking grows from the solid cells of
 // the finest level and costs what the wall region holds, not the grid.
@@ -114,6 +122,8 @@ struct MoveArgs {
   int use_law;
   const PitchHeave *law; // [ncases]
   const double *t;       // [ncases]
+  const double *dt;      // [ncases] (ahead: the law at t + dt)
+  int ahead;             // the next step's move, run before the time advance
   Motion *mot, *motc;    // per body (IB fill), per case (wall force)
   double *case_center;   // [ncases][3]
   double *pat_pc, *pat_nv, *pat_area;
@@ -293,6 +303,12 @@ __device__ __forceinline__ Motion law_motion(const PitchHeave &Lw, double t, dou
 // law, the rigid-body velocity of Body::Surface_Point_Velocity Body.H:
 // 709-721 about the pivot), the reference point of each case (the pivot),
 // then the patches.
+// The time of case k the move takes the law at: t, or t + dt (the sum the
+// time advance forms) for the next step's move run ahead of the advance.
+__device__ __forceinline__ double case_time(const MoveArgs &a, int k) {
+  return a.ahead ? a.t[k] + a.dt[k] : a.t[k];
+}
+
 __global__ void __launch_bounds__(256) k_move_front(MoveArgs a) {
 #pragma clang fp contract(off)
   __shared__ Pose P[IBG_MAX_CASES];
@@ -305,14 +321,15 @@ __global__ void __launch_bounds__(256) k_move_front(MoveArgs a) {
     Pose q;
     if (a.use_law) {
       const PitchHeave Lw = a.law[k];
-      const double sw = sin(Lw.omega * a.t[k]);
+      const double tk = case_time(a, k);
+      const double sw = sin(Lw.omega * tk);
       const double ang = -(Lw.aoa0 + Lw.amp * sw);
       q.px = Lw.x0;
       q.py = Lw.y0 + Lw.heave * sw;
       q.c = cos(ang);
       q.s = sin(ang);
       a.pose[k] = q;
-      a.motc[k] = law_motion(Lw, a.t[k], a.case_center[(size_t)k * 3 + 2]);
+      a.motc[k] = law_motion(Lw, tk, a.case_center[(size_t)k * 3 + 2]);
     } else {
       q = a.pose[k];
     }
@@ -323,7 +340,7 @@ __global__ void __launch_bounds__(256) k_move_front(MoveArgs a) {
   if (a.use_law)
     for (int b = threadIdx.x; b < a.nbody; b += blockDim.x) {
       const int cs = a.body_case[b];
-      a.mot[b] = law_motion(a.law[cs], a.t[cs], a.case_center[(size_t)cs * 3 + 2]);
+      a.mot[b] = law_motion(a.law[cs], case_time(a, cs), a.case_center[(size_t)cs * 3 + 2]);
     }
   __syncthreads();
   for (int p = threadIdx.x; p < a.npatch; p += blockDim.x) {
@@ -763,6 +780,18 @@ __device__ void move_image(const MoveArgs &a, int j) {
   a.pat_hgdis[p] = hgdis;
 }
 
+// Every in->out cell must be a layer-1 cell (InfectNewIBCells
+// Bodymove.cpp:575-650, abort 314: the move step is too large for the
+// incremental update).  Workgroup cb of nblk, grid-stride.
+__device__ void move_io_check(const MoveArgs &a, int cb, int nblk) {
+  const int ep = a.epoch[0];
+  for (int r = cb * blockDim.x + threadIdx.x; r < min(a.cnt[2], a.cap_io);
+       r += nblk * blockDim.x) {
+    const int c = a.iolist[r];
+    if (a.mk1[c] != ep) report_error(a.err, c, 314);
+  }
+}
+
 // New distances and patches in place (the FP32 copy for the mixed SA
 // source), the stage masks of the marched normal cells as at upload; with
 // `upd` set, the stage updates of this step are accumulated (one atomic
@@ -770,14 +799,6 @@ __device__ void move_image(const MoveArgs &a, int j) {
 __device__ void move_commit(const MoveArgs &a, int cb, int nblk) {
   const int ep = a.epoch[0];
   const int n = a.nw + a.ngw;
-  // every in->out cell must be a layer-1 cell (InfectNewIBCells
-  // Bodymove.cpp:575-650, abort 314: the move step is too large for the
-  // incremental update)
-  for (int r = cb * blockDim.x + threadIdx.x; r < min(a.cnt[2], a.cap_io);
-       r += nblk * blockDim.x) {
-    const int c = a.iolist[r];
-    if (a.mk1[c] != ep) report_error(a.err, c, 314);
-  }
   int cnt = 0;
   for (int t = cb * blockDim.x + threadIdx.x; t < n; t += nblk * blockDim.x) {
     const int i = set_cell(a, t, a.nw);
@@ -817,16 +838,44 @@ __device__ void move_commit(const MoveArgs &a, int cb, int nblk) {
   }
 }
 
-// Image search and commit in one launch: workgroups [0, nimage) search
-// (they read only the new arrays, not the committed ones), the others
-// commit.  The searches are a few thousand dependent chains of loads, the
-// commit a stream over every marched cell: dispatched first, the searches
-// run beside the stream instead of after it.
-__global__ void __launch_bounds__(256) k_move_finish(MoveArgs a, int nimage) {
+// Image search, in->out check and commit in one launch: workgroups
+// [0, nimage) search (they read only the new arrays, not the committed
+// ones), the others check and, with `commit`, commit.  The searches are a
+// few thousand dependent chains of loads, the commit a stream over every
+// marched cell: dispatched first, the searches run beside the stream
+// instead of after it.  A move run ahead (commit 0) is committed by
+// k_move_join.
+__global__ void __launch_bounds__(256) k_move_finish(MoveArgs a, int nimage, int commit) {
   if (wg_halted(a.err)) return;
-  const int b = (int)blockIdx.x;
-  if (b < nimage) move_image(a, b * blockDim.x + threadIdx.x);
-  else move_commit(a, b - nimage, (int)gridDim.x - nimage);
+  const int b = (int)blockIdx.x, nb = (int)gridDim.x - nimage;
+  if (b < nimage) {
+    move_image(a, b * blockDim.x + threadIdx.x);
+    return;
+  }
+  move_io_check(a, b - nimage, nb);
+  if (commit) move_commit(a, b - nimage, nb);
+}
+
+// The start of a step whose move ran ahead (err2: that move's error word):
+// the time advance of the step before, unless the march stopped in it (as
+// k_motion_advance); then the move's error, if it failed, becomes the
+// march's (the state stays the one after the step before, as when the
+// move fails in place); else the commit.  Only workgroup 0 writes the error
+// word, after reading it, so the advance sees the word the step before
+// left.
+__global__ void __launch_bounds__(256) k_move_join(MoveArgs a, const unsigned long long *err2,
+                                                   double *t) {
+  __shared__ int st;
+  if (threadIdx.x == 0) st = halted(a.err) ? 1 : (halted(err2) ? 2 : 0);
+  __syncthreads();
+  if (st == 1) return;
+  if (blockIdx.x == 0)
+    for (int k = threadIdx.x; k < a.ncases; k += blockDim.x) t[k] += a.dt[k];
+  if (st == 2) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicMin(a.err, *err2);
+    return;
+  }
+  move_commit(a, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // Per-level count of the cells the stage update touches.
