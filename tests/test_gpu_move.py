@@ -172,6 +172,50 @@ def test_prescribed_motion_march(tiny, mode):
     assert np.array_equal(eng.get_state(), out[0][0])
 
 
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_failed_move_run_ahead_equals_failed_move_in_place(tiny, use_graph):
+    """In a march of several steps each step's move by the law runs ahead,
+    beside the stages of the step before, with its own error word.  A law
+    whose second move is far too large (the section at rest at t0, then
+    tens of degrees within one step: the reference's abort) must end such a
+    march with the code, state and time of single-step marches, where the
+    move runs in place (the state the second step's exchange left, before
+    its move; the time advanced by the first step)."""
+    spec, m = tiny
+    print(f"failed move ahead: tiny case, graph {use_graph}", flush=True)
+    fv0 = perturbed_state(m)
+    e0 = ibgpu.Engine(m, mode="fp64", nstage=3)
+    e0.set_state(fv0)
+    e0.TimeMarching(0, 1, use_graph=False)
+    dt = e0.get_dt()[0]
+    e0.close()
+    # aoa(t) = aoa0 + amp sin(omega t), at rest at t0 on the mesh's pose
+    amp, omega = 30.0, 1.0 / dt
+    t0 = 0.5 * np.pi / omega
+    law = [(0.0, 0.0, spec.aoa - amp, amp, omega, 0.0)]
+
+    def fresh():
+        e = ibgpu.Engine(m, mode="fp64", nstage=3)
+        e.upload_surface()
+        e.set_motion(law, t0=t0)
+        e.set_state(fv0)
+        return e
+
+    a = fresh()
+    with pytest.raises(ibgpu.IBGError) as ea:
+        a.TimeMarching(0, 4, use_graph=use_graph)
+    b = fresh()
+    b.TimeMarching(0, 1, use_graph=False)
+    t1 = b.get_time()
+    with pytest.raises(ibgpu.IBGError) as eb:
+        b.TimeMarching(1, 1, use_graph=False)
+    assert ea.value.code == eb.value.code
+    assert ea.value.code in (263, 314, 1183)
+    assert np.array_equal(a.get_state(), b.get_state())
+    assert np.array_equal(a.get_time(), t1) and np.array_equal(b.get_time(), t1)
+    assert t1[0] > t0
+
+
 def test_regrid_during_prescribed_motion(tiny):
     """A regrid in the middle of a moving-body run: rebuild at the pose of the
     device time, state over by canonical keys (leaf totals conserved), the
