@@ -739,21 +739,22 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R, D> &a, int tile) {
       else if (j == NV) red[sl < 8 ? 10 : 11][wid] = r;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    // the tile's partials, one entry per thread: entries 0..4 the maxima of
+    // (roe, u, v, w, e) (in 2D w is 0 and e the fourth variable), 5..9 the
+    // sums of squares, 10 and 11 the working variable's
+    if (threadIdx.x < kRpart) {
       const int nw = (blockDim.x + 63) >> 6;
-      double *o = a.rpart + (size_t)tile * kRpart;
-      for (int k = 0; k < kRpart; ++k) o[k] = 0.0;
-      for (int k = 0; k < NV; ++k) {
-        double m = 0.0, q = 0.0;
-        for (int w = 0; w < nw; ++w) { m = fmax(m, red[k][w]); q += red[5 + k][w]; }
-        const int kk = (D == 3) ? k : (k == 3 ? 4 : k);
-        o[kk] = m;
-        o[5 + kk] = q;
+      const int j = threadIdx.x, jv = j < 10 ? j % 5 : 0;
+      const int k = (j >= 10) ? j : ((D == 3) ? jv : (jv == 4 ? 3 : (jv == 3 ? -1 : jv)));
+      double r = 0.0;
+      if (k >= 0) {
+        const int row = (j >= 10) ? j : (j < 5 ? k : 5 + k);
+        const bool is_max = (j < 5) || (j == 10);
+        for (int w = 0; w < nw; ++w) r = is_max ? fmax(r, red[row][w]) : r + red[row][w];
       }
-      double m = 0.0, q = 0.0;
-      for (int w = 0; w < nw; ++w) { m = fmax(m, red[10][w]); q += red[11][w]; }
-      o[10] = m;
-      o[11] = q;
+      a.rpart[(size_t)tile * kRpart + j] = r;
+    }
+    if (threadIdx.x == kRpart) {
       if constexpr (ML) {
         if (a.mlcount) {
           unsigned long long *slot = a.mlcount + (size_t)(blockIdx.x % kMlSlots) * kMlSlotStride;
