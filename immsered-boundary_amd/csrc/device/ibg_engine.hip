@@ -254,11 +254,13 @@ size_t ml_resident_offset(const ibg_ctx *c) {
 
 // Host loops over cells or faces (mesh upload, the state boundary): the
 // range [0, n) split into contiguous chunks over up to 16 threads; f(lo, hi)
-// must touch only its own entries.
+// must touch only its own entries.  Ranges shorter than serial_below run on
+// the calling thread (per-cell loops: threads cost more than they save below
+// ~64 K items; per-tile loops, 256 cells an item, pass a smaller bound).
 template <typename F>
-void host_parallel(int n, F f) {
+void host_parallel(int n, F f, int serial_below = 1 << 16) {
   const int nt = std::max(1, std::min<int>(16, (int)std::thread::hardware_concurrency()));
-  if (n < (1 << 16) || nt == 1) { f(0, n); return; }
+  if (n < serial_below || nt == 1) { f(0, n); return; }
   std::vector<std::thread> th;
   const int chunk = (n + nt - 1) / nt;
   for (int t = 0; t < nt; ++t) {
@@ -1011,21 +1013,33 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
     // they are the Box::neib neighbours ComptMutSource reads (NS_Solver.cpp:
     // 662-666), and that the wall model has its normals
     if (m->nib > 0 && !m->ib_nv) { c->errmsg = "SA closure needs ib_nv"; return IBG_ERR_ARG; }
-    for (int b = 0; b < m->ncases * m->nlevels; ++b)
-      for (int g = m->blk_start[b]; g < m->blk_start[b] + m->blk_nnormal[b]; ++g)
-        for (int d = 0; d < m->dim; ++d) {
-          int e[3] = {1, 1, 1};
-          e[d] = 0;
-          const int lo = m->nb[(size_t)g * 27 + e[0] * 9 + e[1] * 3 + e[2]];
-          e[d] = 2;
-          const int hi = m->nb[(size_t)g * 27 + e[0] * 9 + e[1] * 3 + e[2]];
-          const int flo = m->cell_faces[(size_t)g * 6 + d * 2], fhi = m->cell_faces[(size_t)g * 6 + d * 2 + 1];
-          if (m->face_cells[2 * flo] != lo || m->face_cells[2 * fhi + 1] != hi) {
-            c->errmsg = "face list and neighbour table disagree";
-            fprintf(stderr, "ibgpu: mesh rejected: %s\n", c->errmsg.c_str());
-            return IBG_ERR_ARG;
-          }
-        }
+    const int nb0 = m->ncases * m->nlevels;
+    const char *msg = nullptr;
+    first_failure(m->ncells, [&](int g) -> const char * {
+      // the block holding g (the last with blk_start <= g); ghosts pass
+      int lo = 0, hi = nb0;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (m->blk_start[mid] <= g) lo = mid; else hi = mid;
+      }
+      if (g >= m->blk_start[lo] + m->blk_nnormal[lo]) return nullptr;
+      for (int d = 0; d < m->dim; ++d) {
+        int e[3] = {1, 1, 1};
+        e[d] = 0;
+        const int nlo = m->nb[(size_t)g * 27 + e[0] * 9 + e[1] * 3 + e[2]];
+        e[d] = 2;
+        const int nhi = m->nb[(size_t)g * 27 + e[0] * 9 + e[1] * 3 + e[2]];
+        const int flo = m->cell_faces[(size_t)g * 6 + d * 2], fhi = m->cell_faces[(size_t)g * 6 + d * 2 + 1];
+        if (m->face_cells[2 * flo] != nlo || m->face_cells[2 * fhi + 1] != nhi)
+          return "face list and neighbour table disagree";
+      }
+      return nullptr;
+    }, &msg);
+    if (msg) {
+      c->errmsg = msg;
+      fprintf(stderr, "ibgpu: mesh rejected: %s\n", c->errmsg.c_str());
+      return IBG_ERR_ARG;
+    }
   }
   (void)hipStreamSynchronize(c->s);
   free_all(c);
@@ -1246,7 +1260,7 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
       }
       for (int h : touched) hkey[h] = -1;
     }
-  });
+  }, 64);
   int maxf = 1;
   {
     int off = 0;
@@ -1288,7 +1302,7 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
         fcount[f].fetch_add(1, std::memory_order_relaxed);
       }
     }
-  });
+  }, 64);
   tface.clear();
   tface.shrink_to_fit();
   // that first tile face, its tile's block, the tile count (-1 / 0 for a
