@@ -352,7 +352,10 @@ ibg_status ibg_move_bodies(ibg_ctx *ctx, const ibg_body_pose *pose);
 /* Move the bodies by law[ncases] at the start of every step of
  * ibg_time_march (the pose at each case's time t, t0 at this call, t += dt
  * after each step) with the law's wall velocity in the IB fill and, for
- * motion == NULL, in ibg_forces.  law == NULL stops it.                    */
+ * motion == NULL, in ibg_forces.  Inside a march each step's move runs
+ * ahead, beside the stages of the step before, on a second stream of the
+ * context; results and abort codes are those of the move in place.
+ * law == NULL stops it.                                                    */
 ibg_status ibg_set_motion(ibg_ctx *ctx, const ibg_pitch_heave *law, double t0);
 ibg_status ibg_get_time(ibg_ctx *ctx, double *t /* [ncases] */);
 /* The law's time of every case (carried over a regrid).                    */
