@@ -802,12 +802,24 @@ ibg_status validate_mesh(const ibg_mesh_soa *m) {
   if (first_failure(N, [&](int g) -> const char * {
         const int b = gblk[g];
         if (g >= m->blk_start[b] + m->blk_nnormal[b]) return nullptr;
-        for (int d = 0; d < D; ++d)
+        for (int d = 0; d < D; ++d) {
           for (int s2 = 0; s2 < 2; ++s2) {
             const int f = m->cell_faces[(size_t)g * 6 + d * 2 + s2];
             if (f < 0 || f >= m->nfaces) return "normal cell without face";
             if (m->face_cells[2 * f + (1 - s2)] != g) return "cell_faces and face_cells disagree";
           }
+          // the other cells of the two faces are the Box::neib neighbours
+          // (the SA source gathers them from the face list, NS_Solver.cpp:
+          // 662-666)
+          int e[3] = {1, 1, 1};
+          e[d] = 0;
+          const int lo = nb(g, e[0], e[1], e[2]);
+          e[d] = 2;
+          const int hi = nb(g, e[0], e[1], e[2]);
+          if (m->face_cells[2 * m->cell_faces[(size_t)g * 6 + d * 2]] != lo ||
+              m->face_cells[2 * m->cell_faces[(size_t)g * 6 + d * 2 + 1] + 1] != hi)
+            return "face list and neighbour table disagree";
+        }
         return nullptr;
       }, &msg) >= 0)
     return fail(msg);
@@ -1008,39 +1020,7 @@ ibg_status ibg_upload_mesh(ibg_ctx *c, const ibg_mesh_soa *m, const ibg_physics 
         fprintf(stderr, "ibgpu: mesh rejected: %s\n", c->errmsg.c_str());
         return IBG_ERR_ARG;
       }
-  if (c->turb) {
-    // the SA source takes its face neighbours from the face list: check that
-    // they are the Box::neib neighbours ComptMutSource reads (NS_Solver.cpp:
-    // 662-666), and that the wall model has its normals
-    if (m->nib > 0 && !m->ib_nv) { c->errmsg = "SA closure needs ib_nv"; return IBG_ERR_ARG; }
-    const int nb0 = m->ncases * m->nlevels;
-    const char *msg = nullptr;
-    first_failure(m->ncells, [&](int g) -> const char * {
-      // the block holding g (the last with blk_start <= g); ghosts pass
-      int lo = 0, hi = nb0;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) / 2;
-        if (m->blk_start[mid] <= g) lo = mid; else hi = mid;
-      }
-      if (g >= m->blk_start[lo] + m->blk_nnormal[lo]) return nullptr;
-      for (int d = 0; d < m->dim; ++d) {
-        int e[3] = {1, 1, 1};
-        e[d] = 0;
-        const int nlo = m->nb[(size_t)g * 27 + e[0] * 9 + e[1] * 3 + e[2]];
-        e[d] = 2;
-        const int nhi = m->nb[(size_t)g * 27 + e[0] * 9 + e[1] * 3 + e[2]];
-        const int flo = m->cell_faces[(size_t)g * 6 + d * 2], fhi = m->cell_faces[(size_t)g * 6 + d * 2 + 1];
-        if (m->face_cells[2 * flo] != nlo || m->face_cells[2 * fhi + 1] != nhi)
-          return "face list and neighbour table disagree";
-      }
-      return nullptr;
-    }, &msg);
-    if (msg) {
-      c->errmsg = msg;
-      fprintf(stderr, "ibgpu: mesh rejected: %s\n", c->errmsg.c_str());
-      return IBG_ERR_ARG;
-    }
-  }
+  if (c->turb && m->nib > 0 && !m->ib_nv) { c->errmsg = "SA closure needs ib_nv"; return IBG_ERR_ARG; }
   (void)hipStreamSynchronize(c->s);
   free_all(c);
   const int D = m->dim, NV = D + 2, L = m->nlevels, nc = m->ncases, nblk = nc * L;

@@ -85,7 +85,10 @@ def test_clean_mesh_passes(tiny_mesh):
 
 @pytest.mark.parametrize("what,expect", [
     ("face_cells", "face"),
-    ("nb_face_stencil", "face stencil without LL/RR"),
+    ("face_nb", "face list and neighbour table disagree"),
+    # (both outer stencil cells of a face missing: every face has a normal
+    # cell, whose neighbours the agreement check, run first, covers)
+    ("nb_face_stencil", "face list and neighbour table disagree"),
     ("res_fi", "restriction son missing"),
     ("pro_ci", "prolongation coarse cell"),
     ("ib_intp", "IB interpolation cell"),
@@ -100,6 +103,11 @@ def test_corrupted_mesh_rejected(tiny_mesh, what, expect):
     v = np.ctypeslib.as_array
     if what == "face_cells":
         arr, idx, bad = m.face_cells, (3, 1), m.ncells + 7
+    elif what == "face_nb":
+        # the -x neighbour of a normal cell replaced by its +x neighbour (a
+        # valid cell, so only the agreement with the face list fails)
+        g = 0
+        arr, idx, bad = m.nb, (g, 0 * 9 + 1 * 3 + 1), int(m.nb[g, 2 * 9 + 1 * 3 + 1])
     elif what == "nb_face_stencil":
         # both the LL neighbour of an x face's low cell and the RR neighbour
         # of its high cell missing (one missing is the reference's fallback)
